@@ -1,0 +1,1568 @@
+/*
+ * bdf_oracle.c -- CPU FP64 restatement of the reference's Dirac-Coulomb
+ * two-electron Fock build (bertha-scf.f).  TEST INFRASTRUCTURE ONLY: it is the
+ * checker for the HIP engine and the timed CPU baseline; the product never
+ * links it.
+ *
+ * What it restates (line numbers are bertha-scf.f):
+ *   CARTIND 2651-2703, FOCKIND 2582-2648, FACTRL 1084, GAMGEN 1037,
+ *   constants PHYSPRM 921-1034;
+ *   E-coefficients: E0LLGN 33378, E0SSGN 33511, EQLLMK 33930, EQSSMK 34270,
+ *   EQLL 34649-34934, EQSS 35928-36897, ESGTF 36900, EVRS 36981,
+ *   ESTEPLM 37105, ESTEPL 37253, ESTEPN 37569, RNORM1 37750;
+ *   sparsity lists E0LLCD0 17673, E0SSCD0 17833;
+ *   Boys/R: FUNFM 32465-32619, RMAKE 31900-32462;
+ *   ERIFAST 12269-13132 (E(AB) flags EPSAB=1e-16, R-column flags EPSRR=1e-10,
+ *   C1 reuse across the CD loop, C2 with the 16 mj-sign layout);
+ *   CLMFAST 11489-12266 loop nest, IFLG table, PAB/PCD phases and Hermitian
+ *   completion; digestion CLMMTZ 13135-13838 / CLMMTG 13841-14865.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   - every routine above against the reference itself, compiled from
+ *     bertha-scf.f by oracle/Makefile into oracle/_ref (ref_driver.f), on an
+ *     atom (ATOMIC, RACAH1 off), a diatomic on z (DIATOM) and a two-centre
+ *     system off the axes (NOSYMM), where the reference's parameters.h allows;
+ *   - FUNFM also against closed forms (F_m(0)=1/(2m+1), downward recursion
+ *     identity) and scipy's incomplete gamma;
+ *   - committed golden vectors in tests/golden/ written from oracle/_ref.
+ *   Three- and four-centre quartets run the same code as two-centre ones but
+ *   the reference's MCT=2 build cannot produce them: for those the pin is
+ *   translation/rotation invariance of the two-electron energy.
+ *
+ * Schwarz screening (SCHWRZ) and open-shell matrices are not restated: the
+ * reference hard-codes SCHWRZ=.FALSE. (5257) and the configs are closed-shell.
+ */
+#include "bdf_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zc;
+
+/* PHYSPRM (bertha-scf.f:968-973) */
+static const double PI = 3.1415926535897932;
+static const double PI12 = 1.7724538509055160;
+static const double PI52 = 1.7493418327624863e1;
+static const double PILG = 1.1447298858494002;
+static const double TWLG = 0.6931471805599453;
+
+#define ML4 28
+#define MRC ((ML4 + 1) * (ML4 + 2) * (ML4 + 3) / 6)
+#define MKPX 13
+
+static int IABC[ML4 + 1][ML4 + 1][ML4 + 1];
+static int IAX[MRC + 1], IBX[MRC + 1], ICX[MRC + 1], ILAM[MRC + 1];
+static double RFACT[81], GAMLOG[301];
+static int g_init = 0;
+
+static int ntuv(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
+static int lval(int k) { return k < 0 ? -k - 1 : k; }
+static int isgn(int x) { return x >= 0 ? 1 : -1; }
+static double m1pow(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
+
+/* CARTIND 2651, FACTRL 1084, GAMGEN 1037 */
+static void init_tables(void) {
+  if (g_init) return;
+  int n = 0;
+  for (int lam = 0; lam <= ML4; ++lam)
+    for (int it = 0; it <= lam; ++it)
+      for (int iu = 0; iu <= lam; ++iu)
+        for (int iv = 0; iv <= lam; ++iv) {
+          if (it + iu + iv != lam) continue;
+          ++n;
+          IAX[n] = it; IBX[n] = iu; ICX[n] = iv; ILAM[n] = lam;
+          IABC[it][iu][iv] = n;
+        }
+  RFACT[0] = RFACT[1] = 1.0;
+  for (int i = 2; i <= 80; ++i) RFACT[i] = (double)i * RFACT[i - 1];
+  GAMLOG[1] = PILG * 0.5;
+  GAMLOG[2] = 0.0;
+  double f1 = 0.5, f2 = 1.0;
+  for (int k = 4; k <= 300; k += 2) {
+    GAMLOG[k - 1] = GAMLOG[k - 3] + log(f1);
+    GAMLOG[k] = GAMLOG[k - 2] + log(f2);
+    f1 += 1.0; f2 += 1.0;
+  }
+  g_init = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* system                                                                   */
+/* ------------------------------------------------------------------------ */
+struct orc_sys {
+  int ncnt, mbs, mel;
+  double *xyz;  /* [ncnt][3] */
+  int *nkap;    /* [ncnt] */
+  int *kapa;    /* [ncnt][MKPX+1], 1-based k index */
+  int *nfnc;    /* [ncnt][mel+1] */
+  double *bexl; /* [ncnt][mel+1][mbs] */
+  int ndim, nskp;
+  int *lrge;    /* [ncnt][MKPX+1][MKPX+2] */
+  int *labicn, *labkqn, *labmqn;
+  int isym, intsym, racah1;
+  /* E tables: plane c of n words each; address map iad[(a,b,ka,kb,ma,mb)] */
+  double *e0ll[4], *e0ss[4];
+  long n0ll, n0ss;
+  long *iad0ll, *iad0ss;
+};
+
+#define KAPA(s, ic, k) ((s)->kapa[(ic) * (MKPX + 1) + (k)])
+#define NFNC(s, ic, l) ((s)->nfnc[(ic) * ((s)->mel + 1) + (l)])
+#define BEXL(s, ic, l, i) ((s)->bexl[((ic) * ((s)->mel + 1) + (l)) * (s)->mbs + (i)])
+#define LRGE(s, ic, k, m) ((s)->lrge[((ic) * (MKPX + 1) + (k)) * (MKPX + 2) + (m)])
+
+static long iadx(const orc_sys *s, int a, int b, int ka, int kb, int ma, int mb) {
+  /* all arguments 1-based as IAD0LL(ICNTA,ICNTB,KA,KB,MA,MB) */
+  long n = s->ncnt;
+  return ((((((long)(a - 1) * n + (b - 1)) * MKPX + (ka - 1)) * MKPX + (kb - 1)) * MKPX +
+           (ma - 1)) * MKPX + (mb - 1));
+}
+
+/* FOCKIND 2582-2648 */
+static void fockind(orc_sys *s) {
+  int ilst = 0;
+  for (int iz = 0; iz < s->ncnt; ++iz) {
+    for (int imv = 0; imv <= s->mel; ++imv) {
+      int mqn = 2 * imv + 1;
+      for (int half = 0; half < 2; ++half) {
+        for (int ka = 1; ka <= s->nkap[iz]; ++ka) {
+          int kqn = KAPA(s, iz, ka);
+          int lqn = lval(kqn);
+          int nbas = NFNC(s, iz, lqn);
+          int mqmax = 2 * abs(kqn) - 1;
+          if (mqmax >= mqn) {
+            LRGE(s, iz, ka, mqn + half) = ilst;
+            for (int ib = 1; ib <= nbas; ++ib) {
+              s->labicn[ilst + ib - 1] = iz + 1;
+              s->labkqn[ilst + ib - 1] = kqn;
+              s->labmqn[ilst + ib - 1] = half ? mqn : -mqn;
+            }
+            ilst += nbas;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* E-coefficients                                                           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int maxab;
+  double *p, *p2, *p22, *rkab, *pax, *pay, *paz, *pbx, *pby, *pbz, *pa2, *pb2;
+  zc *etemp;
+  long ntemp;
+} mcmd_t;
+
+static void mcmd_alloc(mcmd_t *g, int maxab, int lmax) {
+  g->maxab = maxab;
+  double **arr[] = {&g->p, &g->p2, &g->p22, &g->rkab, &g->pax, &g->pay,
+                    &g->paz, &g->pbx, &g->pby, &g->pbz, &g->pa2, &g->pb2};
+  for (int i = 0; i < 12; ++i) *arr[i] = (double *)calloc(maxab, sizeof(double));
+  long tot = 0;
+  for (int l = 0; l <= lmax + 2; ++l) tot += ntuv(l);
+  g->ntemp = tot * maxab;
+  g->etemp = (zc *)calloc(g->ntemp, sizeof(zc));
+}
+static void mcmd_free(mcmd_t *g) {
+  double *arr[] = {g->p, g->p2, g->p22, g->rkab, g->pax, g->pay,
+                   g->paz, g->pbx, g->pby, g->pbz, g->pa2, g->pb2};
+  for (int i = 0; i < 12; ++i) free(arr[i]);
+  free(g->etemp);
+}
+
+/* Gaussian product data (EQLL 34747-34772) */
+static void mcmd_geom(mcmd_t *g, const double *ea, int na, const double *eb, int nb,
+                      const double *xa, const double *xb) {
+  double d0 = xa[0] - xb[0], d1 = xa[1] - xb[1], d2 = xa[2] - xb[2];
+  double ab2 = d0 * d0 + d1 * d1 + d2 * d2;
+  int m = 0;
+  for (int i = 0; i < na; ++i)
+    for (int j = 0; j < nb; ++j, ++m) {
+      double p = ea[i] + eb[j];
+      g->p[m] = p; g->p2[m] = 2.0 * p; g->p22[m] = g->p2[m] * g->p2[m];
+      double px = (ea[i] * xa[0] + eb[j] * xb[0]) / p;
+      double py = (ea[i] * xa[1] + eb[j] * xb[1]) / p;
+      double pz = (ea[i] * xa[2] + eb[j] * xb[2]) / p;
+      g->pax[m] = px - xa[0]; g->pay[m] = py - xa[1]; g->paz[m] = pz - xa[2];
+      g->pbx[m] = px - xb[0]; g->pby[m] = py - xb[1]; g->pbz[m] = pz - xb[2];
+      g->pa2[m] = g->pax[m] * g->pax[m] + g->pay[m] * g->pay[m] + g->paz[m] * g->paz[m];
+      g->pb2[m] = g->pbx[m] * g->pbx[m] + g->pby[m] * g->pby[m] + g->pbz[m] * g->pbz[m];
+      g->rkab[m] = exp(-(ea[i] * eb[j] * ab2) / p);
+    }
+}
+
+/* ESTEPLM 37105: E[l,l] -> E[l+1,+-(l+1)], Saunders Eq.(64) */
+static void esteplm(mcmd_t *g, int *lam, long *istart, int mqn, int iz) {
+  if (mqn == 0) return;
+  int maxab = g->maxab;
+  const double *px = iz == 1 ? g->pax : g->pbx;
+  const double *py = iz == 1 ? g->pay : g->pby;
+  double phm = (double)isgn(mqn);
+  zc *e = g->etemp;
+  for (int lqn = 0; lqn <= abs(mqn) - 1; ++lqn) {
+    double r2l1 = (double)(2 * lqn + 1);
+    long nt = ntuv(*lam);
+    long js = *istart, ks = js + nt * maxab;
+    for (int io = 0; io <= *lam; ++io)
+      for (int it = 0; it <= io; ++it)
+        for (int iu = 0; iu <= io - it; ++iu) {
+          int iv = io - it - iu;
+          long j0 = js + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k0 = ks + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k1 = ks + (long)(IABC[it + 1][iu][iv] - 1) * maxab;
+          long k2 = ks + (long)(IABC[it][iu + 1][iv] - 1) * maxab;
+          long k3 = it ? ks + (long)(IABC[it - 1][iu][iv] - 1) * maxab : 0;
+          long k4 = iu ? ks + (long)(IABC[it][iu - 1][iv] - 1) * maxab : 0;
+          for (int m = 0; m < maxab; ++m) {
+            double t1 = r2l1 / g->p2[m];
+            double tx = r2l1 * px[m], ty = r2l1 * py[m];
+            e[k0 + m] += tx * e[j0 + m] + phm * I * ty * e[j0 + m];
+            e[k1 + m] += t1 * e[j0 + m];
+            e[k2 + m] += phm * I * t1 * e[j0 + m];
+          }
+          if (it) {
+            double fc = r2l1 * it;
+            for (int m = 0; m < maxab; ++m) e[k3 + m] += fc * e[j0 + m];
+          }
+          if (iu) {
+            double fc = r2l1 * iu;
+            for (int m = 0; m < maxab; ++m) e[k4 + m] += phm * I * fc * e[j0 + m];
+          }
+        }
+    *istart += nt * maxab;
+    *lam += 1;
+  }
+}
+
+/* ESTEPL 37253: E[|m|,m] -> E[l,m] by the three-term recurrence */
+static void estepl(mcmd_t *g, int *lam, long *istart0, int lqn, int mqn, int iz) {
+  if (lqn <= abs(mqn)) return;
+  int maxab = g->maxab;
+  const double *pp = iz == 1 ? g->pa2 : g->pb2;
+  const double *px = iz == 1 ? g->pax : g->pbx;
+  const double *py = iz == 1 ? g->pay : g->pby;
+  const double *pz = iz == 1 ? g->paz : g->pbz;
+  zc *e = g->etemp;
+  long nt = ntuv(*lam);
+  long js = *istart0, ks = js + nt * maxab;
+  double rlm = (double)(2 * abs(mqn) + 1);
+  for (int io = 0; io <= *lam; ++io)
+    for (int it = 0; it <= io; ++it)
+      for (int iu = 0; iu <= io - it; ++iu) {
+        int iv = io - it - iu;
+        long j0 = js + (long)(IABC[it][iu][iv] - 1) * maxab;
+        long k0 = ks + (long)(IABC[it][iu][iv] - 1) * maxab;
+        long k6 = ks + (long)(IABC[it][iu][iv + 1] - 1) * maxab;
+        long k9 = iv ? ks + (long)(IABC[it][iu][iv - 1] - 1) * maxab : 0;
+        for (int m = 0; m < maxab; ++m) {
+          double tz = rlm * pz[m], tp = rlm / g->p2[m];
+          e[k0 + m] += tz * e[j0 + m];
+          e[k6 + m] += tp * e[j0 + m];
+        }
+        if (iv >= 1) {
+          double fc = rlm * iv;
+          for (int m = 0; m < maxab; ++m) e[k9 + m] += fc * e[j0 + m];
+        }
+      }
+  long istart = *istart0;
+  *istart0 += nt * maxab;
+  *lam += 1;
+  if (lqn == abs(mqn) + 1) return;
+  for (int lstp = abs(mqn) + 1; lstp <= lqn - 1; ++lstp) {
+    double rlm1 = (double)(2 * lstp + 1) / (double)(lstp - abs(mqn) + 1);
+    double rlm2 = -(double)(lstp + abs(mqn)) / (double)(lstp - abs(mqn) + 1);
+    nt = ntuv(*lam);
+    js = *istart0;
+    ks = *istart0 + (long)maxab * nt;
+    for (int io = 0; io <= *lam; ++io)
+      for (int it = 0; it <= io; ++it)
+        for (int iu = 0; iu <= io - it; ++iu) {
+          int iv = io - it - iu;
+          long j0 = js + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k0 = ks + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k6 = ks + (long)(IABC[it][iu][iv + 1] - 1) * maxab;
+          long k9 = iv ? ks + (long)(IABC[it][iu][iv - 1] - 1) * maxab : 0;
+          for (int m = 0; m < maxab; ++m) {
+            double tz = pz[m], tp = 1.0 / g->p2[m];
+            e[k0 + m] += tz * rlm1 * e[j0 + m];
+            e[k6 + m] += tp * rlm1 * e[j0 + m];
+          }
+          if (iv >= 1) {
+            double fc = (double)iv;
+            for (int m = 0; m < maxab; ++m) e[k9 + m] += fc * rlm1 * e[j0 + m];
+          }
+        }
+    for (int io = 0; io <= *lam - 1; ++io)
+      for (int it = 0; it <= io; ++it)
+        for (int iu = 0; iu <= io - it; ++iu) {
+          int iv = io - it - iu;
+          long i0 = istart + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k0 = ks + (long)(IABC[it][iu][iv] - 1) * maxab;
+          long k1 = ks + (long)(IABC[it + 2][iu][iv] - 1) * maxab;
+          long k2 = ks + (long)(IABC[it][iu + 2][iv] - 1) * maxab;
+          long k3 = ks + (long)(IABC[it][iu][iv + 2] - 1) * maxab;
+          long k4 = ks + (long)(IABC[it + 1][iu][iv] - 1) * maxab;
+          long k5 = ks + (long)(IABC[it][iu + 1][iv] - 1) * maxab;
+          long k6 = ks + (long)(IABC[it][iu][iv + 1] - 1) * maxab;
+          long k7 = it > 0 ? ks + (long)(IABC[it - 1][iu][iv] - 1) * maxab : 0;
+          long k8 = iu > 0 ? ks + (long)(IABC[it][iu - 1][iv] - 1) * maxab : 0;
+          long k9 = iv > 0 ? ks + (long)(IABC[it][iu][iv - 1] - 1) * maxab : 0;
+          long k10 = it > 1 ? ks + (long)(IABC[it - 2][iu][iv] - 1) * maxab : 0;
+          long k11 = iu > 1 ? ks + (long)(IABC[it][iu - 2][iv] - 1) * maxab : 0;
+          long k12 = iv > 1 ? ks + (long)(IABC[it][iu][iv - 2] - 1) * maxab : 0;
+          double ti = (double)(2 * (it + iu + iv) + 3);
+          for (int m = 0; m < maxab; ++m) {
+            double t1 = 1.0 / g->p22[m];
+            double tx = px[m] / g->p[m], ty = py[m] / g->p[m], tz = pz[m] / g->p[m];
+            double tt = pp[m] + ti / g->p2[m];
+            zc e0 = e[i0 + m];
+            e[k0 + m] += rlm2 * tt * e0;
+            e[k1 + m] += rlm2 * t1 * e0;
+            e[k2 + m] += rlm2 * t1 * e0;
+            e[k3 + m] += rlm2 * t1 * e0;
+            e[k4 + m] += rlm2 * tx * e0;
+            e[k5 + m] += rlm2 * ty * e0;
+            e[k6 + m] += rlm2 * tz * e0;
+          }
+          if (it >= 1)
+            for (int m = 0; m < maxab; ++m) e[k7 + m] += rlm2 * (2.0 * it * px[m]) * e[i0 + m];
+          if (iu >= 1)
+            for (int m = 0; m < maxab; ++m) e[k8 + m] += rlm2 * (2.0 * iu * py[m]) * e[i0 + m];
+          if (iv >= 1)
+            for (int m = 0; m < maxab; ++m) e[k9 + m] += rlm2 * (2.0 * iv * pz[m]) * e[i0 + m];
+          if (it >= 2) {
+            double tx = (double)(it * (it - 1));
+            for (int m = 0; m < maxab; ++m) e[k10 + m] += rlm2 * tx * e[i0 + m];
+          }
+          if (iu >= 2) {
+            double ty = (double)(iu * (iu - 1));
+            for (int m = 0; m < maxab; ++m) e[k11 + m] += rlm2 * ty * e[i0 + m];
+          }
+          if (iv >= 2) {
+            double tz = (double)(iv * (iv - 1));
+            for (int m = 0; m < maxab; ++m) e[k12 + m] += rlm2 * tz * e[i0 + m];
+          }
+        }
+    istart = *istart0;
+    *istart0 += (long)maxab * nt;
+    *lam += 1;
+  }
+}
+
+/* EVRS 36981: raw Hermite coefficients of S[la,ma](A) S[lb,mb](B), esg[tuv][m] */
+static void evrs(mcmd_t *g, zc *esg, int la, int ma, int lb, int mb) {
+  int maxab = g->maxab;
+  int lmax = la + lb;
+  long nt = ntuv(lmax);
+  for (long k = 0; k < nt * maxab; ++k) esg[k] = 0.0;
+  if (abs(ma) > la || abs(mb) > lb) return;
+  memset(g->etemp, 0, sizeof(zc) * g->ntemp);
+  for (int m = 0; m < maxab; ++m) g->etemp[m] = g->rkab[m];
+  long istart = 0;
+  int lam = 0;
+  esteplm(g, &lam, &istart, ma, 1);
+  estepl(g, &lam, &istart, la, ma, 1);
+  esteplm(g, &lam, &istart, mb, 2);
+  estepl(g, &lam, &istart, lb, mb, 2);
+  nt = ntuv(lam);
+  for (long k = 0; k < nt * maxab; ++k) esg[k] = g->etemp[istart + k];
+}
+
+/* ESGTF 36900: conj(Y[la,ma]) Y[lb,mb] in Condon-Shortley phase */
+static void esgtf(mcmd_t *g, zc *esg, int la, int ma, int lb, int mb) {
+  evrs(g, esg, la, -ma, lb, mb);
+  double phs1 = m1pow((ma + abs(ma)) / 2);
+  double phs2 = m1pow((mb + abs(mb)) / 2);
+  double pi4 = 0.25 / PI;
+  double dgl = (double)((2 * la + 1) * (2 * lb + 1));
+  double cg1 = RFACT[la - abs(ma)] / RFACT[la + abs(ma)];
+  double cg2 = RFACT[lb - abs(mb)] / RFACT[lb + abs(mb)];
+  double ang = phs1 * phs2 * pi4 * sqrt(dgl * cg1 * cg2);
+  long nt = ntuv(la + lb);
+  for (long k = 0; k < nt * g->maxab; ++k) esg[k] *= ang;
+}
+
+/* ESTEPN 37569: multiply by r_A^2 (iz=1) or r_B^2 (iz=2); out has lam+2 */
+static void estepn(mcmd_t *g, const zc *esg, zc *ensg, int lam, int iz) {
+  int maxab = g->maxab;
+  const double *pp = iz == 1 ? g->pa2 : g->pb2;
+  const double *px = iz == 1 ? g->pax : g->pbx;
+  const double *py = iz == 1 ? g->pay : g->pby;
+  const double *pz = iz == 1 ? g->paz : g->pbz;
+  long nt = ntuv(lam + 2);
+  for (long k = 0; k < nt * maxab; ++k) ensg[k] = 0.0;
+#define EN(kk) (ensg + (long)((kk) - 1) * maxab)
+#define ES(kk) (esg + (long)((kk) - 1) * maxab)
+  for (int io = 0; io <= lam; ++io)
+    for (int it = 0; it <= io; ++it)
+      for (int iu = 0; iu <= io - it; ++iu) {
+        int iv = io - it - iu;
+        int i0 = IABC[it][iu][iv];
+        double tt = (double)(2 * (it + iu + iv) + 3);
+        for (int m = 0; m < maxab; ++m) {
+          double t1 = 1.0 / g->p22[m];
+          double tx = px[m] / g->p[m], ty = py[m] / g->p[m], tz = pz[m] / g->p[m];
+          double tp = pp[m] + tt / g->p2[m];
+          zc e0 = ES(i0)[m];
+          EN(IABC[it][iu][iv])[m] += tp * e0;
+          EN(IABC[it + 2][iu][iv])[m] += t1 * e0;
+          EN(IABC[it][iu + 2][iv])[m] += t1 * e0;
+          EN(IABC[it][iu][iv + 2])[m] += t1 * e0;
+          EN(IABC[it + 1][iu][iv])[m] += tx * e0;
+          EN(IABC[it][iu + 1][iv])[m] += ty * e0;
+          EN(IABC[it][iu][iv + 1])[m] += tz * e0;
+        }
+        if (it >= 1)
+          for (int m = 0; m < maxab; ++m) EN(IABC[it - 1][iu][iv])[m] += (px[m] * (2.0 * it)) * ES(i0)[m];
+        if (iu >= 1)
+          for (int m = 0; m < maxab; ++m) EN(IABC[it][iu - 1][iv])[m] += (py[m] * (2.0 * iu)) * ES(i0)[m];
+        if (iv >= 1)
+          for (int m = 0; m < maxab; ++m) EN(IABC[it][iu][iv - 1])[m] += (pz[m] * (2.0 * iv)) * ES(i0)[m];
+        if (it >= 2) {
+          double r = (double)(it * (it - 1));
+          for (int m = 0; m < maxab; ++m) EN(IABC[it - 2][iu][iv])[m] += r * ES(i0)[m];
+        }
+        if (iu >= 2) {
+          double r = (double)(iu * (iu - 1));
+          for (int m = 0; m < maxab; ++m) EN(IABC[it][iu - 2][iv])[m] += r * ES(i0)[m];
+        }
+        if (iv >= 2) {
+          double r = (double)(iv * (iv - 1));
+          for (int m = 0; m < maxab; ++m) EN(IABC[it][iu][iv - 2])[m] += r * ES(i0)[m];
+        }
+      }
+#undef EN
+#undef ES
+}
+
+/* RNORM1 37750: itt 1 = LL, 4 = SS (2/3 mixed) */
+static void rnorm1(double *rn, const double *ea, int na, const double *eb, int nb,
+                   int la, int lb, int itt) {
+  double rn1[64], rn2[64];
+  double gl = TWLG - GAMLOG[2 * la + 3], gs = TWLG - GAMLOG[2 * la + 5];
+  double ul = la + 1.5, us = la + 0.5;
+  for (int i = 0; i < na; ++i) {
+    double elog = log(2.0 * ea[i]);
+    rn1[i] = (itt == 1 || itt == 2) ? exp(0.5 * (gl + ul * elog)) : exp(0.5 * (gs + us * elog));
+  }
+  gl = TWLG - GAMLOG[2 * lb + 3]; gs = TWLG - GAMLOG[2 * lb + 5];
+  ul = lb + 1.5; us = lb + 0.5;
+  for (int j = 0; j < nb; ++j) {
+    double elog = log(2.0 * eb[j]);
+    rn2[j] = (itt == 1 || itt == 3) ? exp(0.5 * (gl + ul * elog)) : exp(0.5 * (gs + us * elog));
+  }
+  for (int i = 0; i < na; ++i)
+    for (int j = 0; j < nb; ++j) rn[i * nb + j] = rn1[i] * rn2[j];
+}
+
+/* Clebsch-Gordan factors of EQLL (34722-34734); small component (EQSS
+   36012-36024) swaps the kappa-sign rows. */
+static void cgfac(int kqn, int mqn, int small, double *cu, double *cl) {
+  int jqn = 2 * abs(kqn) - 1;
+  int neg = kqn < 0;
+  if (small) neg = !neg;
+  if (neg) {
+    *cu = sqrt((double)(jqn + mqn) / (double)(2 * jqn));
+    *cl = sqrt((double)(jqn - mqn) / (double)(2 * jqn));
+  } else {
+    *cu = -sqrt((double)(jqn - mqn + 2) / (double)(2 * jqn + 4));
+    *cl = sqrt((double)(jqn + mqn + 2) / (double)(2 * jqn + 4));
+  }
+}
+
+/* EQLL 34649 (small=0) and EQSS 35928 (small=1), iq = 0 only (Coulomb):
+   spinor coefficients for signed mqn pair; out ell[tuv][m], ntuv(la+lb[+2]) */
+static void eq_spinor(mcmd_t *g, zc *ell, const double *ea, int na, const double *eb,
+                      int nb, const double *xa, const double *xb, int ka, int kb,
+                      int mqa, int mqb, int small) {
+  const double sens = 1.0e-14;
+  int la = lval(ka), lb = lval(kb);
+  int maxab = na * nb;
+  int lt = la + lb + (small ? 2 : 0);
+  long ntot = ntuv(lt);
+  for (long k = 0; k < ntot * maxab; ++k) ell[k] = 0.0;
+  double cau, cal, cbu, cbl;
+  cgfac(ka, mqa, small, &cau, &cal);
+  cgfac(kb, mqb, small, &cbu, &cbl);
+  mcmd_geom(g, ea, na, eb, nb, xa, xb);
+  long big = (long)ntuv(lt + 2) * maxab;
+  zc *esg = (zc *)calloc(big, sizeof(zc));
+  zc *ensg = (zc *)calloc(big, sizeof(zc));
+  zc *enn = (zc *)calloc(big, sizeof(zc));
+  double *t22 = NULL, *t20 = NULL, *t02 = NULL, *t00 = NULL;
+  if (small) {
+    t22 = malloc(sizeof(double) * maxab); t20 = malloc(sizeof(double) * maxab);
+    t02 = malloc(sizeof(double) * maxab); t00 = malloc(sizeof(double) * maxab);
+    double rl1 = 2 * la + 1, rl2 = 2 * lb + 1;
+    int m = 0;
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < nb; ++j, ++m) {
+        t22[m] = 4.0 * ea[i] * eb[j];
+        t20[m] = -2.0 * rl2 * ea[i];
+        t02[m] = -2.0 * rl1 * eb[j];
+        t00[m] = rl1 * rl2;
+      }
+  }
+  /* terms 11 (upper-upper, sign +1) and 22 (lower-lower, SIG=+1 for iq=0) */
+  for (int term = 0; term < 2; ++term) {
+    int mla = term == 0 ? (mqa - 1) / 2 : (mqa + 1) / 2;
+    int mlb = term == 0 ? (mqb - 1) / 2 : (mqb + 1) / 2;
+    double cab = term == 0 ? cau * cbu : cal * cbl;
+    if (fabs(cab) < sens) continue;
+    if (!small) {
+      int lam0 = la + lb;
+      esgtf(g, esg, la, mla, lb, mlb);
+      long n0 = ntuv(lam0);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += cab * esg[t * maxab + m];
+      continue;
+    }
+    /* EQSS cases 1-4 (35996-36860) */
+    int la2 = ka < 0 ? la + 1 : la - 1;
+    int lb2 = kb < 0 ? lb + 1 : lb - 1;
+    int lam0 = la2 + lb2;
+    esgtf(g, esg, la2, mla, lb2, mlb);
+    long n0 = ntuv(lam0), n2 = ntuv(lam0 + 2), n4 = ntuv(lam0 + 4);
+    if (ka < 0 && kb < 0) {
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (cab * t22[m]) * esg[t * maxab + m];
+    } else if (ka < 0 && kb > 0) {
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (cab * t20[m]) * esg[t * maxab + m];
+      estepn(g, esg, ensg, lam0, 2);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n2; ++t) ell[t * maxab + m] += (cab * t22[m]) * ensg[t * maxab + m];
+    } else if (ka > 0 && kb < 0) {
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (cab * t02[m]) * esg[t * maxab + m];
+      estepn(g, esg, ensg, lam0, 1);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n2; ++t) ell[t * maxab + m] += (cab * t22[m]) * ensg[t * maxab + m];
+    } else {
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (cab * t00[m]) * esg[t * maxab + m];
+      estepn(g, esg, ensg, lam0, 1);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n2; ++t) ell[t * maxab + m] += (cab * t20[m]) * ensg[t * maxab + m];
+      estepn(g, esg, ensg, lam0, 2);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n2; ++t) ell[t * maxab + m] += (cab * t02[m]) * ensg[t * maxab + m];
+      estepn(g, ensg, enn, lam0 + 2, 1);
+      for (int m = 0; m < maxab; ++m)
+        for (long t = 0; t < n4; ++t) ell[t * maxab + m] += (cab * t22[m]) * enn[t * maxab + m];
+    }
+  }
+  double rn[64 * 64];
+  rnorm1(rn, ea, na, eb, nb, la, lb, small ? 4 : 1);
+  long nn = ntuv(small ? la + lb + 4 : la + lb);
+  if (nn > ntot) nn = ntot;
+  for (int m = 0; m < maxab; ++m)
+    for (long t = 0; t < ntot; ++t) ell[t * maxab + m] *= rn[m];
+  free(esg); free(ensg); free(enn);
+  free(t22); free(t20); free(t02); free(t00);
+}
+
+/* E0LLGN 33378 / E0SSGN 33511 with EQLLMK/EQSSMK (IPHS=+1, IQ=0) */
+static void build_etables(orc_sys *s) {
+  for (int pass = 0; pass < 2; ++pass) {
+    int small = pass == 1;
+    long count = 0;
+    /* size first */
+    for (int a = 0; a < s->ncnt; ++a)
+      for (int b = 0; b < s->ncnt; ++b)
+        for (int ka = 1; ka <= s->nkap[a]; ++ka)
+          for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+            int kqa = KAPA(s, a, ka), kqb = KAPA(s, b, kb);
+            int la = lval(kqa), lb = lval(kqb);
+            long maxab = (long)NFNC(s, a, la) * NFNC(s, b, lb);
+            count += (long)abs(kqa) * abs(kqb) * ntuv(la + lb + (small ? 2 : 0)) * maxab;
+          }
+    double **pl = small ? s->e0ss : s->e0ll;
+    for (int c = 0; c < 4; ++c) pl[c] = (double *)calloc(count > 0 ? count : 1, sizeof(double));
+    if (small) s->n0ss = count; else s->n0ll = count;
+    long *iad = small ? s->iad0ss : s->iad0ll;
+    long icount = 0;
+    for (int a = 0; a < s->ncnt; ++a)
+      for (int b = 0; b < s->ncnt; ++b)
+        for (int ka = 1; ka <= s->nkap[a]; ++ka) {
+          int kqa = KAPA(s, a, ka), la = lval(kqa);
+          int na = NFNC(s, a, la);
+          for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+            int kqb = KAPA(s, b, kb), lb = lval(kqb);
+            int nb = NFNC(s, b, lb);
+            int maxab = na * nb;
+            int lam = la + lb + (small ? 2 : 0);
+            long nt = ntuv(lam);
+            mcmd_t g;
+            mcmd_alloc(&g, maxab, lam + 4);
+            zc *e11 = (zc *)calloc((long)ntuv(lam + 4) * maxab, sizeof(zc));
+            zc *e21 = (zc *)calloc((long)ntuv(lam + 4) * maxab, sizeof(zc));
+            const double *ea = &BEXL(s, a, la, 0), *eb = &BEXL(s, b, lb, 0);
+            for (int ma = 1; ma <= abs(kqa); ++ma)
+              for (int mb = 1; mb <= abs(kqb); ++mb) {
+                int mja = 2 * ma - 1, mjb = 2 * mb - 1;
+                iad[iadx(s, a + 1, b + 1, ka, kb, ma, mb)] = icount;
+                eq_spinor(&g, e11, ea, na, eb, nb, s->xyz + 3 * a, s->xyz + 3 * b, kqa, kqb,
+                          -mja, -mjb, small);
+                eq_spinor(&g, e21, ea, na, eb, nb, s->xyz + 3 * a, s->xyz + 3 * b, kqa, kqb,
+                          mja, -mjb, small);
+                for (long t = 0; t < nt; ++t) {
+                  long iad0 = icount + t * maxab;
+                  for (int m = 0; m < maxab; ++m) {
+                    pl[0][iad0 + m] = creal(e11[t * maxab + m]);
+                    pl[1][iad0 + m] = cimag(e11[t * maxab + m]);
+                    pl[2][iad0 + m] = creal(e21[t * maxab + m]);
+                    pl[3][iad0 + m] = cimag(e21[t * maxab + m]);
+                  }
+                }
+                icount += nt * maxab;
+              }
+            free(e11); free(e21);
+            mcmd_free(&g);
+          }
+        }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boys function and R-integrals                                            */
+/* ------------------------------------------------------------------------ */
+/* FUNFM 32465-32619 */
+void orc_funfm(const double *t, int n, int lam, int itype, double *fm) {
+#define FM(m, k) fm[(long)(k) * n + (m)]
+  static const double A0 = 4.994501191201870e-1, B0 = 4.551838436668326e-1;
+  if (itype == 1) {
+    for (int k = 0; k <= lam; ++k)
+      for (int m = 0; m < n; ++m) FM(m, k) = 1.0 / (double)(2 * k + 1);
+  } else if (itype == 2) {
+    for (int m = 0; m < n; ++m) {
+      double txp = exp(-t[m]), tt2 = 2.0 * t[m], tl = 1.0, f = 1.0;
+      for (int k = 1; k <= 60; ++k) {
+        double dlam = (double)(2 * lam + 2 * k + 1);
+        tl = tl * tt2 / dlam;
+        f = f + tl;
+      }
+      FM(m, lam) = f * txp / (double)(2 * lam + 1);
+      for (int i = 1; i <= lam; ++i) {
+        int mind = lam - i; /* 0-based */
+        double coeff = (double)(2 * mind + 1);
+        FM(m, mind) = (tt2 * FM(m, mind + 1) + txp) / coeff;
+      }
+    }
+  } else if (itype == 3) {
+    for (int m = 0; m < n; ++m) {
+      double txp = exp(-t[m]), tt2 = 2.0 * t[m], trt = sqrt(t[m]);
+      double f0 = A0 / (B0 + t[m]);
+      FM(m, 0) = 0.5 * PI12 / trt - txp * f0;
+      for (int mind = 0; mind < lam; ++mind) {
+        double coeff = (double)(2 * mind + 1);
+        FM(m, mind + 1) = (coeff * FM(m, mind) - txp) / tt2;
+      }
+    }
+  } else {
+    for (int m = 0; m < n; ++m) {
+      double tt2 = 2.0 * t[m];
+      FM(m, 0) = 0.5 * PI12 / sqrt(t[m]);
+      for (int mind = 0; mind < lam; ++mind) {
+        double coeff = (double)(2 * mind + 1);
+        FM(m, mind + 1) = coeff * FM(m, mind) / tt2;
+      }
+    }
+  }
+#undef FM
+}
+
+/* one RMAKE sweep: writes dst from src for all (t,u,v) with t+u+v<=ituv */
+static void rsweep(double *dst, const double *src, const double *qp, int n, int ituv) {
+#define RD(k) (dst + (long)((k) - 1) * n)
+#define RS(k) (src + (long)((k) - 1) * n)
+  const double *qx = qp, *qy = qp + n, *qz = qp + 2 * n;
+  for (int it = 0; it <= ituv; ++it)
+    for (int iu = 0; iu <= ituv - it; ++iu)
+      for (int iv = 0; iv <= ituv - it - iu; ++iv) {
+        double rit = it, riu = iu, riv = iv;
+        int n1 = IABC[it + 1][iu][iv], n2 = IABC[it][iu + 1][iv], n3 = IABC[it][iu][iv + 1];
+        int k1 = IABC[it][iu][iv];
+        int m1 = it ? IABC[it - 1][iu][iv] : 0;
+        int m2 = iu ? IABC[it][iu - 1][iv] : 0;
+        int m3 = iv ? IABC[it][iu][iv - 1] : 0;
+        for (int m = 0; m < n; ++m) {
+          double r = RS(k1)[m];
+          RD(n1)[m] = -qx[m] * r + (it ? rit * RS(m1)[m] : 0.0);
+          RD(n2)[m] = -qy[m] * r + (iu ? riu * RS(m2)[m] : 0.0);
+          RD(n3)[m] = -qz[m] * r + (iv ? riv * RS(m3)[m] : 0.0);
+        }
+      }
+#undef RD
+#undef RS
+}
+
+/* RMAKE 31900-32462: rc(n, ntuv(lam)), qp(n,3) = Q-P; scratch rc2 */
+static void rmake_ws(double *rc, double *rc2, double *fs, const double *qp, const double *aph,
+                     int n, int lam) {
+  const double epszr = 1.0e-11, epspl = 1.7e1, epsas = 3.0e1;
+  double t1[1];
+  double *f1 = (double *)malloc(sizeof(double) * (lam + 1));
+  for (int m = 0; m < n; ++m) {
+    double x = qp[m] * qp[m] + qp[n + m] * qp[n + m] + qp[2 * n + m] * qp[2 * n + m];
+    x = x * aph[m];
+    int ity = x <= epszr ? 1 : (x <= epspl ? 2 : (x <= epsas ? 3 : 4));
+    t1[0] = x;
+    orc_funfm(t1, 1, lam, ity, f1);
+    for (int k = 0; k <= lam; ++k) fs[(long)k * n + m] = f1[k];
+  }
+  free(f1);
+  for (int m = 0; m < n; ++m) rc[m] = pow(-2.0 * aph[m], lam) * fs[(long)lam * n + m];
+  int ituvmin = (lam % 2 == 0) ? 1 : 2;
+  int ituv = -1;
+  for (int ilv = lam - 1; ilv >= ituvmin; ilv -= 2) {
+    ++ituv;
+    rsweep(rc2, rc, qp, n, ituv);
+    for (int m = 0; m < n; ++m) rc2[m] = pow(-2.0 * aph[m], ilv) * fs[(long)ilv * n + m];
+    ++ituv;
+    rsweep(rc, rc2, qp, n, ituv);
+    for (int m = 0; m < n; ++m) rc[m] = pow(-2.0 * aph[m], ilv - 1) * fs[(long)(ilv - 1) * n + m];
+  }
+  if (lam % 2 == 1) {
+    ++ituv;
+    rsweep(rc2, rc, qp, n, ituv);
+    for (int m = 0; m < n; ++m) rc2[m] = fs[m];
+    long nt = ntuv(lam);
+    memcpy(rc, rc2, sizeof(double) * nt * n);
+  }
+}
+
+void orc_rmake(const double *qp, const double *aph, int n, int lam, double *rc) {
+  init_tables();
+  long nt = ntuv(lam);
+  double *rc2 = (double *)calloc(nt * n, sizeof(double));
+  double *fs = (double *)calloc((long)(lam + 1) * n, sizeof(double));
+  double *w = (double *)calloc(nt * n, sizeof(double));
+  rmake_ws(w, rc2, fs, qp, aph, n, lam);
+  memcpy(rc, w, sizeof(double) * nt * n);
+  free(rc2); free(fs); free(w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ERIFAST state (COMMON /IRCM/, /ICDC/ and ERIFAST's saved locals)         */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int ieab, igab, ncd;
+  int nab_max;         /* size of irij */
+  unsigned char *irij; /* [ibas][jbas] */
+  /* /ICDC/ flags, [itn 0:LL 1:SS][icd] and [itn][ncd][mc][md][icd] */
+  int nte;             /* allocated icd extent */
+  int *icdr11, *icdi11, *icdr21, *icdi21;
+  int *km[4];          /* r11, i11, r21, i21 */
+  /* E(AB) flags [itn][ij][iab] */
+  unsigned char *iab[4];
+  long iab_cap;
+  /* saved R batches (RCTTFL/IRCTTFL) */
+  double *rctt;
+  unsigned char *irctt;
+  long rctt_cap, irctt_cap;
+  /* G(AB|  ) contraction storage */
+  double *gcd[4];
+  long gcd_cap;
+  /* RMAKE scratch */
+  double *rc, *rc2, *fs, *pq, *aph, *pre;
+  long rc_cap, pq_cap;
+} eri_ws;
+
+#define KMIDX(itn, ncd, mc, md, icd, nte) \
+  (((((long)(itn) * 4 + (ncd)) * MKPX + (mc)) * MKPX + (md)) * (nte) + (icd))
+
+static void ws_init(eri_ws *w) { memset(w, 0, sizeof(*w)); }
+static void ws_free(eri_ws *w) {
+  free(w->irij); free(w->icdr11); free(w->icdi11); free(w->icdr21); free(w->icdi21);
+  for (int c = 0; c < 4; ++c) { free(w->km[c]); free(w->iab[c]); free(w->gcd[c]); }
+  free(w->rctt); free(w->irctt);
+  free(w->rc); free(w->rc2); free(w->fs); free(w->pq); free(w->aph); free(w->pre);
+}
+
+static void ws_prepare(eri_ws *w, int nab, int lmx_cd) {
+  if (w->nab_max < nab) {
+    free(w->irij);
+    w->irij = calloc((long)nab * nab, 1);
+    w->nab_max = nab;
+  }
+  int nte = ntuv(lmx_cd);
+  if (w->nte < nte) {
+    free(w->icdr11); free(w->icdi11); free(w->icdr21); free(w->icdi21);
+    w->icdr11 = calloc(2L * nte, sizeof(int)); w->icdi11 = calloc(2L * nte, sizeof(int));
+    w->icdr21 = calloc(2L * nte, sizeof(int)); w->icdi21 = calloc(2L * nte, sizeof(int));
+    for (int c = 0; c < 4; ++c) {
+      free(w->km[c]);
+      w->km[c] = calloc(2L * 4 * MKPX * MKPX * nte, sizeof(int));
+    }
+    w->nte = nte;
+  }
+}
+
+/* size every per-class buffer for primitive count maxn and top l maxl */
+static void ws_size(eri_ws *w, int maxn, int maxl, const int *nmax) {
+  ws_prepare(w, maxn, 2 * maxl + 2);
+  long n2 = (long)maxn * maxn;
+  /* largest R batch of one class: MAXAB*MAXCD*NTUV(sum l + 4) */
+  long capr = 0, capc = 0, capf = 0;
+  for (int a = 0; a <= maxl; ++a)
+    for (int b = 0; b <= maxl; ++b)
+      for (int c = 0; c <= maxl; ++c)
+        for (int d = 0; d <= maxl; ++d) {
+          long nf = ntuv(a + b + c + d + 4);
+          long ab = (long)nmax[a] * nmax[b], cd = (long)nmax[c] * nmax[d];
+          if (ab * cd * nf > capr) capr = ab * cd * nf;
+          if (cd * nf > capc) capc = cd * nf;
+          if (ab * nf > capf) capf = ab * nf;
+        }
+  if (w->rctt_cap < capr) {
+    free(w->rctt); w->rctt = calloc(capr, sizeof(double)); w->rctt_cap = capr;
+  }
+  if (w->irctt_cap < capf) {
+    free(w->irctt); w->irctt = calloc(capf, 1); w->irctt_cap = capf;
+  }
+  if (w->rc_cap < capc) {
+    free(w->rc); free(w->rc2);
+    w->rc = calloc(capc, sizeof(double)); w->rc2 = calloc(capc, sizeof(double));
+    w->rc_cap = capc;
+  }
+  if (w->pq_cap < n2) {
+    free(w->fs); free(w->pq); free(w->aph); free(w->pre);
+    w->fs = calloc(n2 * (ML4 + 1), sizeof(double));
+    w->pq = calloc(3 * n2, sizeof(double));
+    w->aph = calloc(n2, sizeof(double)); w->pre = calloc(n2, sizeof(double));
+    w->pq_cap = n2;
+  }
+}
+
+/* E0LLCD0 17673 (itn=0) / E0SSCD0 17833 (itn=1) */
+static void ecd0(orc_sys *s, eri_ws *w, const int icnt[4], const int lqn[4], const int nbas[4],
+                 int itnsel) {
+  const double epscd = 1.0e-10;
+  int maxcd = nbas[2] * nbas[3];
+  int lamcd = lqn[2] + lqn[3] + (itnsel ? 2 : 0);
+  int ntcd = ntuv(lamcd);
+  int nte = w->nte;
+  int *agg[4] = {w->icdr11 + itnsel * nte, w->icdi11 + itnsel * nte, w->icdr21 + itnsel * nte,
+                 w->icdi21 + itnsel * nte};
+  for (int c = 0; c < 4; ++c)
+    for (int icd = 0; icd < ntcd; ++icd) agg[c][icd] = 0;
+  double **pl = itnsel ? s->e0ss : s->e0ll;
+  long *iad = itnsel ? s->iad0ss : s->iad0ll;
+  for (int nc = (lqn[2] == 0); nc <= 1; ++nc) {
+    int kc = 2 * lqn[2] + nc;
+    for (int nd = (lqn[3] == 0); nd <= 1; ++nd) {
+      int kd = 2 * lqn[3] + nd;
+      int ncd = 2 * nc + nd; /* 0-based NCD */
+      for (int mc = 1; mc <= lqn[2] + nc; ++mc)
+        for (int md = 1; md <= lqn[3] + nd; ++md) {
+          long icdtt = iad[iadx(s, icnt[2], icnt[3], kc, kd, mc, md)];
+          for (int icd = 0; icd < ntcd; ++icd) {
+            long mcd = icdtt + (long)icd * maxcd;
+            for (int c = 0; c < 4; ++c) {
+              double sum = 0.0;
+              int flag = 0;
+              for (int m = 0; m < maxcd; ++m) {
+                sum += fabs(pl[c][mcd + m]);
+                if (sum > epscd) { flag = 1; break; }
+              }
+              w->km[c][KMIDX(itnsel, ncd, mc - 1, md - 1, icd, nte)] = flag;
+              agg[c][icd] += flag;
+            }
+          }
+        }
+    }
+  }
+}
+
+typedef struct {
+  int icnt[4], kqn[4], mqn[4], nbas[4], lqn[4];
+  const double *exl[4];
+  const double *xyz[4];
+  long iabll, iabss, icdll, icdss;
+  int ilev, mcnt;
+} quartet_t;
+
+static int ncntrs(int a, int b, int c, int d) {
+  int n = 1;
+  if (b != a) ++n;
+  if (c != a && c != b) ++n;
+  if (d != a && d != b && d != c) ++n;
+  return n;
+}
+
+/* ERIFAST 12269-13132 for (ibas,jbas) (1-based), rr planes [16][maxcd] */
+static void erifast(orc_sys *s, eri_ws *w, const quartet_t *q, int ibas, int jbas,
+                    const int itn[2], zc *rr) {
+  const double epsab = 1.0e-16, epsrr = 1.0e-10;
+  int isym = s->isym;
+  int it1 = itn[0] == 4, it2 = itn[1] == 4;
+  long iabtt = it1 ? q->iabss : q->iabll;
+  long icdtt = it2 ? q->icdss : q->icdll;
+  double **plab = it1 ? s->e0ss : s->e0ll;
+  double **plcd = it2 ? s->e0ss : s->e0ll;
+  int maxab = q->nbas[0] * q->nbas[1];
+  int maxcd = q->nbas[2] * q->nbas[3];
+  int lamab = q->lqn[0] + q->lqn[1] + (it1 ? 2 : 0);
+  int lamcd = q->lqn[2] + q->lqn[3] + (it2 ? 2 : 0);
+  int lsum = q->lqn[0] + q->lqn[1] + q->lqn[2] + q->lqn[3];
+  int lamfl;
+  if (q->mcnt == 1) lamfl = lsum + 4;
+  else if (q->ilev == 1) lamfl = lsum;
+  else if (q->ilev == 2) lamfl = lsum + 2;
+  else lamfl = lsum + 4;
+  int ntfl = ntuv(lamfl);
+  int ntab = ntuv(lamab), ntcd = ntuv(lamcd);
+  int ij = (ibas - 1) * q->nbas[1] + jbas; /* 1-based */
+  double eij = q->exl[0][ibas - 1] + q->exl[1][jbas - 1];
+  int nte = w->nte;
+
+  /* E(AB) screening flags (12436-12554) */
+  long iabsz = (long)maxab * ntab;
+  if (w->ieab) {
+    if (w->iab_cap < iabsz) {
+      for (int c = 0; c < 4; ++c) { free(w->iab[c]); w->iab[c] = calloc(iabsz, 1); }
+      w->iab_cap = iabsz;
+    }
+    for (int iab = 0; iab < ntab; ++iab) {
+      long mab = iabtt + (long)iab * maxab;
+      for (int c = 0; c < 4; ++c)
+        for (int m = 0; m < maxab; ++m)
+          w->iab[c][(long)m * ntab + iab] = fabs(plab[c][mab + m]) > epsab;
+    }
+    w->ieab = 0;
+  }
+
+  /* R batch (12568-12658) */
+  long rsz = (long)maxcd * ntfl;
+  if (w->irij[(ibas - 1) * w->nab_max + (jbas - 1)]) {
+    double px = (q->xyz[0][0] * q->exl[0][ibas - 1] + q->xyz[1][0] * q->exl[1][jbas - 1]) / eij;
+    double py = (q->xyz[0][1] * q->exl[0][ibas - 1] + q->xyz[1][1] * q->exl[1][jbas - 1]) / eij;
+    double pz = (q->xyz[0][2] * q->exl[0][ibas - 1] + q->xyz[1][2] * q->exl[1][jbas - 1]) / eij;
+    if (w->rc_cap < rsz || w->pq_cap < maxcd) return; /* sized by ws_size */
+    int m = 0;
+    for (int kb = 0; kb < q->nbas[2]; ++kb)
+      for (int lb = 0; lb < q->nbas[3]; ++lb, ++m) {
+        double ekl = q->exl[2][kb] + q->exl[3][lb];
+        double qx = (q->xyz[2][0] * q->exl[2][kb] + q->xyz[3][0] * q->exl[3][lb]) / ekl;
+        double qy = (q->xyz[2][1] * q->exl[2][kb] + q->xyz[3][1] * q->exl[3][lb]) / ekl;
+        double qz = (q->xyz[2][2] * q->exl[2][kb] + q->xyz[3][2] * q->exl[3][lb]) / ekl;
+        w->pq[m] = qx - px; w->pq[maxcd + m] = qy - py; w->pq[2 * maxcd + m] = qz - pz;
+        w->aph[m] = eij * ekl / (eij + ekl);
+        double emx = sqrt(eij + ekl) * eij * ekl;
+        w->pre[m] = 2.0 * PI52 / emx;
+      }
+    rmake_ws(w->rc, w->rc2, w->fs, w->pq, w->aph, maxcd, lamfl);
+    long base = (long)(ij - 1) * rsz;
+    long fbase = (long)(ij - 1) * ntfl;
+    for (int col = 0; col < ntfl; ++col) {
+      double sum = 0.0;
+      int flag = 0;
+      for (int n = 0; n < maxcd; ++n) {
+        sum += fabs(w->rc[(long)col * maxcd + n]);
+        if (sum > epsrr) { flag = 1; break; }
+      }
+      w->irctt[fbase + col] = flag;
+      for (int n = 0; n < maxcd; ++n)
+        w->rctt[base + (long)col * maxcd + n] = w->pre[n] * w->rc[(long)col * maxcd + n];
+    }
+    w->irij[(ibas - 1) * w->nab_max + (jbas - 1)] = 0;
+  }
+
+  /* first contraction (12678-12794) */
+  long gsz = (long)maxcd * ntcd;
+  if (w->igab) {
+    if (w->gcd_cap < gsz) {
+      for (int c = 0; c < 4; ++c) { free(w->gcd[c]); w->gcd[c] = calloc(gsz, sizeof(double)); }
+      w->gcd_cap = gsz;
+    }
+    int *ag[4] = {w->icdr11 + it2 * nte, w->icdi11 + it2 * nte, w->icdr21 + it2 * nte,
+                  w->icdi21 + it2 * nte};
+    for (int icd = 1; icd <= ntcd; ++icd) {
+      for (int c = 0; c < 4; ++c)
+        for (int n = 0; n < maxcd; ++n) w->gcd[c][(long)(icd - 1) * maxcd + n] = 0.0;
+      int icdall = ag[0][icd - 1] + ag[1][icd - 1] + ag[2][icd - 1] + ag[3][icd - 1];
+      if (icdall == 0) continue;
+      for (int iab = 1; iab <= ntab; ++iab) {
+        long mab = iabtt + (long)(iab - 1) * maxab;
+        long ff = (long)(ij - 1) * ntab + (iab - 1);
+        int iaball = w->iab[0][ff] + w->iab[1][ff] + w->iab[2][ff] + w->iab[3][ff];
+        if (iaball == 0) continue;
+        int irabcd = IABC[IAX[iab] + IAX[icd]][IBX[iab] + IBX[icd]][ICX[iab] + ICX[icd]];
+        long mabcd = (long)(ij - 1) * rsz + (long)maxcd * (irabcd - 1);
+        if (w->irctt[(long)(ij - 1) * ntfl + irabcd - 1] == 0) continue;
+        for (int c = 0; c < 4; ++c) {
+          if (ag[c][icd - 1] == 0 && isym > 1) continue;
+          if (w->iab[c][ff] == 0) continue;
+          double e = plab[c][mab + ij - 1];
+          double *gg = w->gcd[c] + (long)(icd - 1) * maxcd;
+          const double *rr0 = w->rctt + mabcd;
+          for (int n = 0; n < maxcd; ++n) gg[n] += e * rr0[n];
+        }
+      }
+    }
+    w->igab = 0;
+  }
+
+  /* second contraction (12809-13119) */
+  int pab = isgn(q->kqn[0] * q->kqn[1]) * (int)m1pow(abs(q->mqn[0] - q->mqn[1]) / 2);
+  int pcd = isgn(q->kqn[2] * q->kqn[3]) * (int)m1pow(abs(q->mqn[2] - q->mqn[3]) / 2);
+  double pabcd = (double)(pab * pcd);
+  int mc = (q->mqn[2] + 1) / 2, md = (q->mqn[3] + 1) / 2;
+  int ncd = w->ncd;
+#define KM(c, icd) w->km[c][KMIDX(it2, ncd, mc - 1, md - 1, (icd) - 1, nte)]
+#define ECD(c, icd, n) plcd[c][icdtt + (long)((icd) - 1) * maxcd + (n)]
+#define GC(c, icd, n) w->gcd[c][(long)((icd) - 1) * maxcd + (n)]
+#define RR(n, k) rr[(long)((k) - 1) * maxcd + (n)]
+  double *qr1 = malloc(sizeof(double) * maxcd), *qr2 = malloc(sizeof(double) * maxcd);
+  double *qi1 = malloc(sizeof(double) * maxcd), *qi2 = malloc(sizeof(double) * maxcd);
+  if (isym == 1 || isym == 2) {
+    for (int set = 0; set < 2; ++set) {
+      /* set 0: (--|--) from E11 x G11 ; set 1: (+-|+-) from E21 x G21 */
+      int cr = set == 0 ? 0 : 2, ci = set == 0 ? 1 : 3;
+      for (int n = 0; n < maxcd; ++n) qr1[n] = qr2[n] = 0.0;
+      for (int icd = 1; icd <= ntcd; ++icd) {
+        double z = m1pow(ILAM[icd]);
+        if (KM(cr, icd) == 1)
+          for (int n = 0; n < maxcd; ++n) qr1[n] += z * ECD(cr, icd, n) * GC(cr, icd, n);
+        if (KM(ci, icd) == 1)
+          for (int n = 0; n < maxcd; ++n) qr2[n] -= z * ECD(ci, icd, n) * GC(ci, icd, n);
+      }
+      for (int n = 0; n < maxcd; ++n) {
+        if (set == 0) {
+          RR(n, 1) = qr1[n] + qr2[n];
+          RR(n, 13) = pab * (qr1[n] - qr2[n]);
+          RR(n, 16) = pabcd * conj(RR(n, 1));
+          RR(n, 4) = pabcd * conj(RR(n, 13));
+        } else {
+          RR(n, 11) = qr1[n] + qr2[n];
+          RR(n, 7) = -pab * (qr1[n] - qr2[n]);
+          RR(n, 6) = pabcd * conj(RR(n, 11));
+          RR(n, 10) = pabcd * conj(RR(n, 7));
+        }
+      }
+    }
+  } else {
+    /* sets: (ECD pair, G pair, out indices, signs) */
+    for (int set = 0; set < 4; ++set) {
+      int ce = (set == 0 || set == 2) ? 0 : 2; /* E(CD) --: comps 0/1 ; +-: 2/3 */
+      int cg = (set == 0 || set == 1) ? 0 : 2; /* G(AB) --: comps 0/1 ; +-: 2/3 */
+      for (int n = 0; n < maxcd; ++n) qr1[n] = qr2[n] = qi1[n] = qi2[n] = 0.0;
+      for (int icd = 1; icd <= ntcd; ++icd) {
+        double z = m1pow(ILAM[icd]);
+        if (KM(ce, icd) == 1)
+          for (int n = 0; n < maxcd; ++n) {
+            qr1[n] += z * ECD(ce, icd, n) * GC(cg, icd, n);
+            qi2[n] += z * ECD(ce, icd, n) * GC(cg + 1, icd, n);
+          }
+        if (KM(ce + 1, icd) == 1)
+          for (int n = 0; n < maxcd; ++n) {
+            qi1[n] += z * ECD(ce + 1, icd, n) * GC(cg, icd, n);
+            qr2[n] -= z * ECD(ce + 1, icd, n) * GC(cg + 1, icd, n);
+          }
+      }
+      for (int n = 0; n < maxcd; ++n) {
+        zc a = (qr1[n] + qr2[n]) + I * (qi1[n] + qi2[n]);
+        zc b = (qr1[n] - qr2[n]) + I * (qi1[n] - qi2[n]);
+        if (set == 0) {
+          RR(n, 1) = a; RR(n, 13) = pab * b;
+          RR(n, 16) = pabcd * conj(RR(n, 1)); RR(n, 4) = pabcd * conj(RR(n, 13));
+        } else if (set == 1) {
+          RR(n, 3) = a; RR(n, 15) = pab * b;
+          RR(n, 14) = -pabcd * conj(RR(n, 3)); RR(n, 2) = -pabcd * conj(RR(n, 15));
+        } else if (set == 2) {
+          RR(n, 9) = a; RR(n, 5) = -pab * b;
+          RR(n, 8) = -pabcd * conj(RR(n, 9)); RR(n, 12) = -pabcd * conj(RR(n, 5));
+        } else {
+          RR(n, 11) = a; RR(n, 7) = -pab * b;
+          RR(n, 6) = pabcd * conj(RR(n, 11)); RR(n, 10) = pabcd * conj(RR(n, 7));
+        }
+      }
+    }
+  }
+  free(qr1); free(qr2); free(qi1); free(qi2);
+#undef KM
+#undef ECD
+#undef GC
+#undef RR
+}
+
+/* ------------------------------------------------------------------------ */
+/* digestion: CLMMTG 13841-14865 and CLMMTZ 13135-13838                     */
+/* ------------------------------------------------------------------------ */
+/* Each CLMMTG batch adds, for every mj-sign combination (sA,sB,sC,sD), one
+   term RR(n)*phase*DENT(den) into GDIR/GXCH(out) with n = 1+8sA+4sB+2sC+sD
+   (s = 0 for -|m|, 1 for +|m|).  The batches differ only in which labels form
+   the output and density indices; sign slot flips (1-s) mark the labels that
+   enter through a permutation of the integral (comments at 13164-13166).
+   CLMMTZ keeps exactly the terms whose output and density blocks are both
+   sign-diagonal and uses real parts only. */
+typedef struct {
+  int xch;          /* 0 = GDIR, 1 = GXCH */
+  int orow, ocol;   /* label 0..3 = A,B,C,D */
+  int oflr, oflc;   /* 1 = sign slot flipped */
+  int drow, dcol;
+  int dflr, dflc;
+  int pab, pcd;     /* phase factors used */
+} batch_t;
+
+static const batch_t BATCH[11] = {
+    {0, 0, 1, 0, 0, 2, 3, 0, 0, 0, 0}, /* 1  G(A,B) += (AB|CD) D(C,D)          */
+    {0, 0, 1, 0, 0, 3, 2, 1, 1, 0, 1}, /* 2  G(A,B) += PCD (AB|CD) D(D',C')     */
+    {0, 2, 3, 0, 0, 0, 1, 0, 0, 0, 0}, /* 3  G(C,D) += (AB|CD) D(A,B)          */
+    {0, 2, 3, 0, 0, 1, 0, 1, 1, 1, 0}, /* 4  G(C,D) += PAB (AB|CD) D(B',A')     */
+    {1, 0, 3, 0, 0, 2, 1, 0, 0, 0, 0}, /* 5  X(A,D) += (AB|CD) D(C,B)          */
+    {1, 0, 2, 0, 1, 3, 1, 1, 0, 0, 1}, /* 6  X(A,C') += PCD (AB|CD) D(D',B)    */
+    {1, 1, 3, 1, 0, 2, 0, 0, 1, 1, 0}, /* 7  X(B',D) += PAB (AB|CD) D(C,A')    */
+    {1, 1, 2, 1, 1, 3, 0, 1, 1, 1, 1}, /* 8  X(B',C') += PAB PCD D(D',A')      */
+    {1, 2, 1, 0, 0, 0, 3, 0, 0, 0, 0}, /* 9  X(C,B) += (AB|CD) D(A,D)          */
+    {1, 2, 0, 0, 1, 1, 3, 1, 0, 1, 0}, /* 10 X(C,A') += PAB (AB|CD) D(B',D)    */
+    {1, 3, 1, 1, 0, 0, 2, 0, 1, 0, 1}, /* 11 X(D',B) += PCD (AB|CD) D(A,C')    */
+};
+
+typedef struct {
+  int nlab[4][2]; /* Fock offsets per label and sign slot (NA1/NA2 ...) */
+  int ibas, jbas;
+  double pab[2], pcd[2]; /* [0] = PAB1 (same slot), [1] = PAB2 */
+  int nbas[4];
+} digest_t;
+
+static void digest(const orc_sys *s, const digest_t *d, const int iflg[11], const zc *rr,
+                   int maxcd, const zc *dent, zc *gdir, zc *gxch, int ndim) {
+  int isym = s->isym;
+  for (int b = 0; b < 11; ++b) {
+    if (iflg[b] != 1) continue;
+    if (!s->intsym && b != 0 && b != 4) continue;
+    const batch_t *bt = &BATCH[b];
+    zc *g = bt->xch ? gxch : gdir;
+    int m = 0;
+    for (int kb = 1; kb <= d->nbas[2]; ++kb)
+      for (int lb = 1; lb <= d->nbas[3]; ++lb, ++m) {
+        int idx[4] = {d->ibas, d->jbas, kb, lb};
+        for (int sa = 0; sa < 2; ++sa)
+          for (int sb = 0; sb < 2; ++sb)
+            for (int sc = 0; sc < 2; ++sc)
+              for (int sd = 0; sd < 2; ++sd) {
+                int sl[4] = {sa, sb, sc, sd};
+                int n = 1 + 8 * sa + 4 * sb + 2 * sc + sd;
+                int orow_s = bt->oflr ? 1 - sl[bt->orow] : sl[bt->orow];
+                int ocol_s = bt->oflc ? 1 - sl[bt->ocol] : sl[bt->ocol];
+                int drow_s = bt->dflr ? 1 - sl[bt->drow] : sl[bt->drow];
+                int dcol_s = bt->dflc ? 1 - sl[bt->dcol] : sl[bt->dcol];
+                if (isym != 0 && (orow_s != ocol_s || drow_s != dcol_s)) continue;
+                double ph = 1.0;
+                if (bt->pab) ph *= d->pab[sa != sb];
+                if (bt->pcd) ph *= d->pcd[sc != sd];
+                int orow = d->nlab[bt->orow][orow_s] + idx[bt->orow] - 1;
+                int ocol = d->nlab[bt->ocol][ocol_s] + idx[bt->ocol] - 1;
+                int drow = d->nlab[bt->drow][drow_s] + idx[bt->drow] - 1;
+                int dcol = d->nlab[bt->dcol][dcol_s] + idx[bt->dcol] - 1;
+                zc r = rr[(long)(n - 1) * maxcd + m];
+                zc den = dent[(long)dcol * ndim + drow];
+                if (isym != 0)
+                  g[(long)ocol * ndim + orow] += ph * creal(r) * creal(den);
+                else
+                  g[(long)ocol * ndim + orow] += ph * r * den;
+              }
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CLMFAST                                                                  */
+/* ------------------------------------------------------------------------ */
+/* ISCF table (11555-11561): rows ITSCF 1..6, columns batch 1..11 */
+static const int ISCF[6][11] = {
+    {1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0}, {1, 1, 0, 0, 1, 1, 1, 0, 0, 0, 0},
+    {1, 0, 1, 1, 1, 0, 1, 0, 0, 0, 0}, {1, 1, 1, 0, 1, 1, 0, 0, 0, 0, 0},
+    {1, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0}, {1, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0}};
+static const int MAPTTTT[4][4] = {{1, 0, 0, 2}, {0, 5, 6, 0}, {0, 7, 8, 0}, {3, 0, 0, 4}};
+
+static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gxch, int a_lo,
+                          int a_hi, int b_sel, eri_ws *w) {
+  int ndim = s->ndim;
+  int isym = s->isym;
+  int maxl = 0, maxn = 0, nmax[MKPX] = {0};
+  for (int c = 0; c < s->ncnt; ++c)
+    for (int l = 0; l <= s->mel; ++l) {
+      if (NFNC(s, c, l) > maxn) maxn = NFNC(s, c, l);
+      if (NFNC(s, c, l) > nmax[l]) nmax[l] = NFNC(s, c, l);
+      if (NFNC(s, c, l) > 0 && l > maxl) maxl = l;
+    }
+  ws_size(w, maxn, maxl, nmax);
+  zc *rr = (zc *)calloc(16L * maxn * maxn, sizeof(zc));
+
+  /* INDEX(ICT,KAPPA,MJ) ordering (11606-11626) */
+  int *index = calloc((long)s->ncnt * 2 * MKPX * (2 * MKPX + 1), sizeof(int));
+#define INDEXQ(ct, kap, mj) index[(((long)(ct) * (2 * MKPX + 1)) + ((kap) + MKPX)) * (2 * MKPX) + (mj)]
+  int icount = 0;
+  for (int ct = 0; ct < s->ncnt; ++ct)
+    for (int kn = 1; kn <= s->nkap[ct]; ++kn) {
+      int kappa = KAPA(s, ct, kn);
+      for (int mj = 1; mj <= 2 * abs(kappa) - 1; mj += 2) INDEXQ(ct, kappa, mj) = ++icount;
+    }
+
+  for (int ca = a_lo; ca < a_hi; ++ca) {
+    int ncb = s->intsym ? ca + 1 : s->ncnt;
+    for (int cb = 0; cb < ncb; ++cb) {
+      if (b_sel >= 0 && cb != b_sel) continue;
+      for (int cc = 0; cc < s->ncnt; ++cc)
+        for (int cd = 0; cd < s->ncnt; ++cd) {
+          quartet_t q;
+          int ic[4] = {ca, cb, cc, cd};
+          for (int k = 0; k < 4; ++k) { q.icnt[k] = ic[k] + 1; q.xyz[k] = s->xyz + 3 * ic[k]; }
+          q.mcnt = ncntrs(ca, cb, cc, cd);
+          q.ilev = ilev;
+          if (q.mcnt == 1 && s->racah1) continue;
+          for (int la = 0; la <= (s->nkap[ca] - 1) / 2; ++la)
+          for (int lb = 0; lb <= (s->nkap[cb] - 1) / 2; ++lb)
+          for (int lc = 0; lc <= (s->nkap[cc] - 1) / 2; ++lc)
+          for (int ld = 0; ld <= (s->nkap[cd] - 1) / 2; ++ld) {
+            int lq[4] = {la, lb, lc, ld};
+            for (int k = 0; k < 4; ++k) {
+              q.lqn[k] = lq[k];
+              q.nbas[k] = NFNC(s, ic[k], lq[k]);
+              q.exl[k] = &BEXL(s, ic[k], lq[k], 0);
+            }
+            for (int i = 0; i < q.nbas[0]; ++i)
+              for (int j = 0; j < q.nbas[1]; ++j) w->irij[i * w->nab_max + j] = 1;
+            ecd0(s, w, q.icnt, q.lqn, q.nbas, 0);
+            if (ilev != 1) ecd0(s, w, q.icnt, q.lqn, q.nbas, 1);
+            if (q.mcnt == 1 && ((la + lb) % 2) != ((lc + ld) % 2)) continue;
+            for (int na = (la == 0); na <= 1; ++na) {
+              int ka = 2 * la + na;
+              q.kqn[0] = KAPA(s, ca, ka);
+              int jqa = 2 * abs(q.kqn[0]) - 1;
+              for (int nb = (lb == 0); nb <= 1; ++nb) {
+                int kb = 2 * lb + nb;
+                q.kqn[1] = KAPA(s, cb, kb);
+                int jqb = 2 * abs(q.kqn[1]) - 1;
+                for (int ma = 1; ma <= abs(q.kqn[0]); ++ma) {
+                  q.mqn[0] = 2 * ma - 1;
+                  for (int mb = 1; mb <= abs(q.kqn[1]); ++mb) {
+                    q.mqn[1] = 2 * mb - 1;
+                    q.iabll = s->iad0ll[iadx(s, q.icnt[0], q.icnt[1], ka, kb, ma, mb)];
+                    q.iabss = s->iad0ss[iadx(s, q.icnt[0], q.icnt[1], ka, kb, ma, mb)];
+                    for (int it1 = 4; it1 >= 1; it1 -= 3) {
+                      int naddab = it1 == 1 ? 0 : s->nskp;
+                      digest_t dg;
+                      dg.nlab[0][0] = LRGE(s, ca, ka, 2 * ma - 1) + naddab;
+                      dg.nlab[0][1] = LRGE(s, ca, ka, 2 * ma) + naddab;
+                      dg.nlab[1][0] = LRGE(s, cb, kb, 2 * mb - 1) + naddab;
+                      dg.nlab[1][1] = LRGE(s, cb, kb, 2 * mb) + naddab;
+                      w->ieab = 1;
+                      for (int ibas = 1; ibas <= q.nbas[0]; ++ibas)
+                        for (int jbas = 1; jbas <= q.nbas[1]; ++jbas) {
+                          w->igab = 1;
+                          for (int nc = (lc == 0); nc <= 1; ++nc) {
+                            int kc = 2 * lc + nc;
+                            q.kqn[2] = KAPA(s, cc, kc);
+                            int jqc = 2 * abs(q.kqn[2]) - 1;
+                            for (int nd = (ld == 0); nd <= 1; ++nd) {
+                              int kd = 2 * ld + nd;
+                              q.kqn[3] = KAPA(s, cd, kd);
+                              int jqd = 2 * abs(q.kqn[3]) - 1;
+                              w->ncd = 2 * nc + nd;
+                              if (q.mcnt == 1) {
+                                int nui = abs(jqa - jqb) / 2;
+                                if (abs(jqc - jqd) / 2 > nui) nui = abs(jqc - jqd) / 2;
+                                int nuf = (jqa + jqb) / 2;
+                                if ((jqc + jqd) / 2 < nuf) nuf = (jqc + jqd) / 2;
+                                if (nui > nuf) continue;
+                                int iselk = 0;
+                                for (int nu = nui; nu <= nuf; ++nu)
+                                  if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) iselk = 1;
+                                if (!iselk) continue;
+                              }
+                              for (int mc = 1; mc <= abs(q.kqn[2]); ++mc) {
+                                q.mqn[2] = 2 * mc - 1;
+                                for (int md = 1; md <= abs(q.kqn[3]); ++md) {
+                                  q.mqn[3] = 2 * md - 1;
+                                  q.icdll = s->iad0ll[iadx(s, q.icnt[2], q.icnt[3], kc, kd, mc, md)];
+                                  q.icdss = s->iad0ss[iadx(s, q.icnt[2], q.icnt[3], kc, kd, mc, md)];
+                                  if (isym == 1 || isym == 2) {
+                                    int ok = (q.mqn[0] == q.mqn[1] && q.mqn[2] == q.mqn[3]) ||
+                                             (q.mqn[0] == q.mqn[2] && q.mqn[1] == q.mqn[3]) ||
+                                             (q.mqn[0] == q.mqn[3] && q.mqn[1] == q.mqn[2]);
+                                    if (!ok) continue;
+                                  }
+                                  if (q.mcnt == 1) {
+                                    int iselm = 0;
+                                    for (int mma = 1; mma <= 2; ++mma)
+                                      for (int mmb = 1; mmb <= 2; ++mmb)
+                                        for (int mmc = 1; mmc <= 2; ++mmc)
+                                          for (int mmd = 1; mmd <= 2; ++mmd) {
+                                            int a_ = q.mqn[0] * (int)m1pow(mma);
+                                            int b_ = q.mqn[1] * (int)m1pow(mmb);
+                                            int c_ = q.mqn[2] * (int)m1pow(mmc);
+                                            int d_ = q.mqn[3] * (int)m1pow(mmd);
+                                            if (a_ - b_ == d_ - c_) iselm = 1;
+                                          }
+                                    if (!iselm) continue;
+                                  }
+                                  int iq1 = INDEXQ(ca, q.kqn[0], q.mqn[0]);
+                                  int iq2 = INDEXQ(cb, q.kqn[1], q.mqn[1]);
+                                  int iq3 = INDEXQ(cc, q.kqn[2], q.mqn[2]);
+                                  int iq4 = INDEXQ(cd, q.kqn[3], q.mqn[3]);
+                                  int iql = (iq1 * (iq1 - 1)) / 2 + iq2;
+                                  int iqr = (iq3 * (iq3 - 1)) / 2 + iq4;
+                                  if (s->intsym && iq1 < iq2) continue;
+                                  if (s->intsym && iq3 < iq4) continue;
+                                  if (s->intsym && iql < iqr) continue;
+                                  int itscf;
+                                  if (iq1 > iq2 && iq3 > iq4 && iql > iqr) itscf = 1;
+                                  else if (iq1 > iq2 && iq3 > iq4 && iql == iqr) itscf = 2;
+                                  else if (iq1 > iq2 && iq3 == iq4 && iql > iqr) itscf = 3;
+                                  else if (iq1 == iq2 && iq3 > iq4 && iql > iqr) itscf = 4;
+                                  else if (iq1 == iq2 && iq3 == iq4 && iql > iqr) itscf = 5;
+                                  else if (iq1 == iq2 && iq3 == iq4 && iql == iqr) itscf = 6;
+                                  else {
+                                    if (s->intsym) continue;
+                                    itscf = 6;
+                                  }
+                                  int iflg[11];
+                                  for (int k = 0; k < 11; ++k) iflg[k] = ISCF[itscf - 1][k];
+                                  if (iq2 == iq3 && (itscf == 1 || itscf == 3 || itscf == 4)) iflg[8] = 1;
+                                  if (itscf == 1) {
+                                    if (iq1 == iq3) iflg[9] = 1;
+                                    else if (iq2 == iq4) iflg[10] = 1;
+                                  }
+                                  int sab = isgn(q.kqn[0] * q.kqn[1]);
+                                  int scd = isgn(q.kqn[2] * q.kqn[3]);
+                                  dg.pab[0] = sab * m1pow(abs(-q.mqn[0] + q.mqn[1]) / 2);
+                                  dg.pab[1] = sab * m1pow((q.mqn[0] + q.mqn[1]) / 2);
+                                  dg.pcd[0] = scd * m1pow(abs(-q.mqn[2] + q.mqn[3]) / 2);
+                                  dg.pcd[1] = scd * m1pow((q.mqn[2] + q.mqn[3]) / 2);
+                                  for (int it2 = 4; it2 >= 1; it2 -= 3) {
+                                    int naddcd = it2 == 1 ? 0 : s->nskp;
+                                    dg.nlab[2][0] = LRGE(s, cc, kc, 2 * mc - 1) + naddcd;
+                                    dg.nlab[2][1] = LRGE(s, cc, kc, 2 * mc) + naddcd;
+                                    dg.nlab[3][0] = LRGE(s, cd, kd, 2 * md - 1) + naddcd;
+                                    dg.nlab[3][1] = LRGE(s, cd, kd, 2 * md) + naddcd;
+                                    int itt = MAPTTTT[it1 - 1][it2 - 1];
+                                    if (ilev == 1 && itt > 1 && q.mcnt != 1) continue;
+                                    if (ilev == 2 && itt > 3 && q.mcnt != 1) continue;
+                                    int itn[2] = {it1, it2};
+                                    erifast(s, w, &q, ibas, jbas, itn, rr);
+                                    dg.ibas = ibas; dg.jbas = jbas;
+                                    for (int k = 0; k < 4; ++k) dg.nbas[k] = q.nbas[k];
+                                    digest(s, &dg, iflg, rr, q.nbas[2] * q.nbas[3], dent, gdir,
+                                           gxch, ndim);
+                                  }
+                                }
+                              }
+                            }
+                          }
+                        }
+                    }
+                  }
+                }
+              }
+            }
+          }
+        }
+    }
+  }
+#undef INDEXQ
+  free(index);
+  free(rr);
+}
+
+/* Hermitian completion (12166-12205), closed shell */
+static void complete(const orc_sys *s, zc *gdir, zc *gxch) {
+  int ndim = s->ndim, nskp = s->nskp;
+#define G(m, i, j) m[(long)(j) * ndim + (i)]
+  for (int j = 0; j < ndim - nskp; ++j)
+    for (int i = 0; i <= j; ++i) {
+      int k = i + nskp, l = j + nskp;
+      if (s->intsym && s->labicn[i] == s->labicn[j] && s->labkqn[i] == s->labkqn[j] &&
+          abs(s->labmqn[i]) == abs(s->labmqn[j]))
+        continue;
+      G(gdir, i, j) += conj(G(gdir, j, i)); G(gdir, j, i) = conj(G(gdir, i, j));
+      G(gxch, i, j) += conj(G(gxch, j, i)); G(gxch, j, i) = conj(G(gxch, i, j));
+      G(gdir, k, l) += conj(G(gdir, l, k)); G(gdir, l, k) = conj(G(gdir, k, l));
+      G(gxch, k, l) += conj(G(gxch, l, k)); G(gxch, l, k) = conj(G(gxch, k, l));
+      G(gxch, i, l) += conj(G(gxch, l, i)); G(gxch, l, i) = conj(G(gxch, i, l));
+      G(gxch, k, j) += conj(G(gxch, j, k)); G(gxch, j, k) = conj(G(gxch, k, j));
+    }
+#undef G
+}
+
+void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir, double *gxch) {
+  orc_clmfast_mt(s, ilev, dent, gdir, gxch, 1);
+}
+
+void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, double *gxch,
+                    int nthreads) {
+  int ndim = s->ndim;
+  long nn = (long)ndim * ndim;
+  memset(gdir, 0, sizeof(double) * 2 * nn);
+  memset(gxch, 0, sizeof(double) * 2 * nn);
+  const zc *d = (const zc *)dent;
+  /* work items: (A, B<=A) centre pairs, processed in the reference's order
+     when single-threaded */
+  int npair = 0;
+  int pa[4096], pb[4096];
+  for (int a = 0; a < s->ncnt; ++a) {
+    int nb = s->intsym ? a + 1 : s->ncnt;
+    for (int b = 0; b < nb; ++b) { pa[npair] = a; pb[npair] = b; ++npair; }
+  }
+  if (nthreads <= 1) {
+    eri_ws w;
+    ws_init(&w);
+    for (int p = 0; p < npair; ++p)
+      clmfast_range(s, ilev, d, (zc *)gdir, (zc *)gxch, pa[p], pa[p] + 1, pb[p], &w);
+    ws_free(&w);
+  } else {
+    zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
+#pragma omp parallel num_threads(nthreads)
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      gd[tid] = calloc(nn, sizeof(zc));
+      gx[tid] = calloc(nn, sizeof(zc));
+      eri_ws w;
+      ws_init(&w);
+#pragma omp for schedule(dynamic, 1)
+      for (int p = 0; p < npair; ++p)
+        clmfast_range(s, ilev, d, gd[tid], gx[tid], pa[p], pa[p] + 1, pb[p], &w);
+      ws_free(&w);
+    }
+    zc *G = (zc *)gdir, *X = (zc *)gxch;
+    for (int t = 0; t < nthreads; ++t) {
+      if (!gd[t]) continue;
+      for (long k = 0; k < nn; ++k) { G[k] += gd[t][k]; X[k] += gx[t][k]; }
+      free(gd[t]); free(gx[t]);
+    }
+    free(gd); free(gx);
+  }
+  complete(s, (zc *)gdir, (zc *)gxch);
+}
+
+/* ------------------------------------------------------------------------ */
+/* set-up and accessors                                                     */
+/* ------------------------------------------------------------------------ */
+orc_sys *orc_create(int ncnt, const double *xyz, const int *lmax, const int *nfnc, int lstride,
+                    const double *exps, int mbs, int shape, int intsym, int racah1) {
+  init_tables();
+  orc_sys *s = (orc_sys *)calloc(1, sizeof(orc_sys));
+  s->ncnt = ncnt;
+  s->mbs = mbs;
+  int mel = 0;
+  for (int c = 0; c < ncnt; ++c) if (lmax[c] > mel) mel = lmax[c];
+  s->mel = mel;
+  s->xyz = calloc(3L * ncnt, sizeof(double));
+  memcpy(s->xyz, xyz, sizeof(double) * 3 * ncnt);
+  s->nkap = calloc(ncnt, sizeof(int));
+  s->kapa = calloc((long)ncnt * (MKPX + 1), sizeof(int));
+  s->nfnc = calloc((long)ncnt * (mel + 1), sizeof(int));
+  s->bexl = calloc((long)ncnt * (mel + 1) * mbs, sizeof(double));
+  s->ndim = 0;
+  for (int c = 0; c < ncnt; ++c) {
+    s->nkap[c] = 2 * lmax[c] + 1;
+    for (int l = 0; l <= lmax[c]; ++l) {
+      NFNC(s, c, l) = nfnc[c * lstride + l];
+      for (int i = 0; i < NFNC(s, c, l); ++i)
+        BEXL(s, c, l, i) = exps[((long)c * lstride + l) * mbs + i];
+      s->ndim += 4 * (2 * l + 1) * NFNC(s, c, l);
+      if (l != 0) KAPA(s, c, 2 * l) = l;
+      KAPA(s, c, 2 * l + 1) = -l - 1;
+    }
+  }
+  s->nskp = s->ndim / 2;
+  s->lrge = calloc((long)ncnt * (MKPX + 1) * (MKPX + 2), sizeof(int));
+  s->labicn = calloc(s->nskp + 1, sizeof(int));
+  s->labkqn = calloc(s->nskp + 1, sizeof(int));
+  s->labmqn = calloc(s->nskp + 1, sizeof(int));
+  s->isym = shape;
+  s->intsym = intsym;
+  s->racah1 = racah1;
+  fockind(s);
+  long niad = (long)ncnt * ncnt * MKPX * MKPX * MKPX * MKPX;
+  s->iad0ll = calloc(niad, sizeof(long));
+  s->iad0ss = calloc(niad, sizeof(long));
+  build_etables(s);
+  return s;
+}
+
+void orc_destroy(orc_sys *s) {
+  if (!s) return;
+  free(s->xyz); free(s->nkap); free(s->kapa); free(s->nfnc); free(s->bexl);
+  free(s->lrge); free(s->labicn); free(s->labkqn); free(s->labmqn);
+  for (int c = 0; c < 4; ++c) { free(s->e0ll[c]); free(s->e0ss[c]); }
+  free(s->iad0ll); free(s->iad0ss);
+  free(s);
+}
+
+int orc_ndim(const orc_sys *s) { return s->ndim; }
+int orc_nskp(const orc_sys *s) { return s->nskp; }
+
+void orc_labels(const orc_sys *s, int *icn, int *kqn, int *mqn) {
+  for (int i = 0; i < s->nskp; ++i) { icn[i] = s->labicn[i]; kqn[i] = s->labkqn[i]; mqn[i] = s->labmqn[i]; }
+}
+
+long orc_etable(const orc_sys *s, int itt, int ica, int icb, int ka, int kb, int ma, int mb,
+                double *out, long cap) {
+  int la = lval(KAPA(s, ica - 1, ka)), lb = lval(KAPA(s, icb - 1, kb));
+  long maxab = (long)NFNC(s, ica - 1, la) * NFNC(s, icb - 1, lb);
+  int lam = la + lb + (itt == 4 ? 2 : 0);
+  long n = ntuv(lam) * maxab;
+  if (cap < 4 * n) return -1;
+  long iad = (itt == 4 ? s->iad0ss : s->iad0ll)[iadx(s, ica, icb, ka, kb, ma, mb)];
+  double *const *pl = itt == 4 ? s->e0ss : s->e0ll;
+  for (int c = 0; c < 4; ++c) memcpy(out + c * n, pl[c] + iad, sizeof(double) * n);
+  return n;
+}
+
+int orc_erifast(orc_sys *s, const int icnt[4], const int kidx[4], const int midx[4], int ibas,
+                int jbas, const int itn[2], int ilev, double *rr_re, double *rr_im) {
+  quartet_t q;
+  eri_ws w;
+  ws_init(&w);
+  for (int k = 0; k < 4; ++k) {
+    int c = icnt[k] - 1;
+    q.icnt[k] = icnt[k];
+    q.kqn[k] = KAPA(s, c, kidx[k]);
+    q.lqn[k] = lval(q.kqn[k]);
+    q.mqn[k] = 2 * midx[k] - 1;
+    q.nbas[k] = NFNC(s, c, q.lqn[k]);
+    q.exl[k] = &BEXL(s, c, q.lqn[k], 0);
+    q.xyz[k] = s->xyz + 3 * c;
+  }
+  q.ilev = ilev;
+  q.mcnt = ncntrs(icnt[0], icnt[1], icnt[2], icnt[3]);
+  int maxl = 0, maxn = 0, nmax[MKPX] = {0};
+  for (int k = 0; k < 4; ++k) {
+    if (q.lqn[k] > maxl) maxl = q.lqn[k];
+    if (q.nbas[k] > maxn) maxn = q.nbas[k];
+  }
+  for (int l = 0; l <= maxl; ++l) nmax[l] = maxn;
+  ws_size(&w, maxn, maxl, nmax);
+  for (int i = 0; i < q.nbas[0]; ++i)
+    for (int j = 0; j < q.nbas[1]; ++j) w.irij[i * w.nab_max + j] = 1;
+  ecd0(s, &w, q.icnt, q.lqn, q.nbas, 0);
+  if (ilev != 1) ecd0(s, &w, q.icnt, q.lqn, q.nbas, 1);
+  q.iabll = s->iad0ll[iadx(s, icnt[0], icnt[1], kidx[0], kidx[1], midx[0], midx[1])];
+  q.iabss = s->iad0ss[iadx(s, icnt[0], icnt[1], kidx[0], kidx[1], midx[0], midx[1])];
+  q.icdll = s->iad0ll[iadx(s, icnt[2], icnt[3], kidx[2], kidx[3], midx[2], midx[3])];
+  q.icdss = s->iad0ss[iadx(s, icnt[2], icnt[3], kidx[2], kidx[3], midx[2], midx[3])];
+  w.ieab = 1;
+  w.igab = 1;
+  w.ncd = 2 * (kidx[2] - 2 * q.lqn[2]) + (kidx[3] - 2 * q.lqn[3]);
+  int maxcd = q.nbas[2] * q.nbas[3];
+  zc *rr = calloc(16L * maxcd, sizeof(zc));
+  erifast(s, &w, &q, ibas, jbas, itn, rr);
+  for (long k = 0; k < 16L * maxcd; ++k) { rr_re[k] = creal(rr[k]); rr_im[k] = cimag(rr[k]); }
+  free(rr);
+  ws_free(&w);
+  return maxcd;
+}

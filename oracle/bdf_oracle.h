@@ -1,0 +1,71 @@
+/*
+ * bdf_oracle.h -- CPU FP64 restatement of the reference's Dirac-Coulomb
+ * two-electron path (bertha-scf.f), used ONLY as the checker for the HIP
+ * engine: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * The product never links or calls anything declared here.
+ *
+ * Every function follows the reference routine named in its comment, in
+ * the reference's loop order, with the reference's screening thresholds and
+ * Boys-function branches.  Indices in comments are bertha-scf.f line numbers.
+ *
+ * Parity status of each part (how it is pinned) is recorded in the header
+ * comment of bdf_oracle.c.
+ */
+#ifndef BDF_ORACLE_H
+#define BDF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* shape codes, as the reference's SHAPE -> ISYM map (bertha-scf.f:11573-11579) */
+#define ORC_NOSYMM 0
+#define ORC_LINEAR 1
+#define ORC_ATOMIC 2
+
+typedef struct orc_sys orc_sys;
+
+/* basis: per centre lmax, nfnc[l] and exps[l][i] (row stride mbs).
+   nfnc is [ncnt][lstride], exps is [ncnt][lstride][mbs]. */
+orc_sys *orc_create(int ncnt, const double *xyz, const int *lmax,
+                    const int *nfnc, int lstride, const double *exps, int mbs,
+                    int shape, int intsym, int racah1);
+void orc_destroy(orc_sys *s);
+int orc_ndim(const orc_sys *s);
+int orc_nskp(const orc_sys *s);
+
+/* FOCKIND labels for the large block (1-based values, reference layout) */
+void orc_labels(const orc_sys *s, int *icn, int *kqn, int *mqn);
+
+/* FUNFM (bertha-scf.f:32465): F(n, lam+1) with n fastest */
+void orc_funfm(const double *t, int n, int lam, int itype, double *f);
+/* RMAKE (bertha-scf.f:31900): rc(n, ntuv(lam)), qp(n,3) = Q-P */
+void orc_rmake(const double *qp, const double *aph, int n, int lam, double *rc);
+
+/* E0LL (itt=1) or E0SS (itt=4) block; out[comp][tuv][ij]; returns ntuv*maxab
+   or -1 if out too small.  ka,kb are 1-based kappa indices, ma,mb 1-based |m|
+   indices, ica,icb 1-based centres (as IAD0LL's arguments). */
+long orc_etable(const orc_sys *s, int itt, int ica, int icb, int ka, int kb,
+                int ma, int mb, double *out, long cap);
+
+/* ERIFAST batch (bertha-scf.f:12269) after the CLMFAST set-up for a single
+   (ibas,jbas) and component pair itn = {1|4, 1|4}; rr[16][maxcd] complex
+   stored as re/im planes rr_re[16*maxcd], rr_im[16*maxcd]. */
+int orc_erifast(orc_sys *s, const int icnt[4], const int kidx[4],
+                const int midx[4], int ibas, int jbas, const int itn[2],
+                int ilev, double *rr_re, double *rr_im);
+
+/* CLMFAST (bertha-scf.f:11489): closed-shell Dirac-Coulomb G matrices.
+   dent, gdir, gxch are column-major ndim x ndim complex (re/im interleaved). */
+void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
+                 double *gxch);
+
+/* same but with OpenMP over the outer centre-pair loop when nthreads > 1
+   (digestion goes to thread-private matrices that are summed at the end) */
+void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,
+                    double *gxch, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
