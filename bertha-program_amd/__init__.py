@@ -1,0 +1,189 @@
+"""MI355X two-electron engine for four-component Dirac-Hartree-Fock.
+
+Host-side mirror of the reference's two-electron interface (bertha-scf.f):
+``Engine.__init__`` stands for EQSAVE (:32984), ``Engine.clmfast`` for
+CLMFAST (:11489) and ``Engine.erifast`` for ERIFAST (:12269).  All arithmetic
+runs in the HIP library ``libbertha_amd.so`` built from ``csrc/``; this module
+only marshals arrays through the C ABI declared in ``include/bertha_amd.h``.
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as _C
+import os as _os
+
+import numpy as _np
+
+from .systems import System, config_system  # noqa: F401
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIBPATH = _os.path.join(_HERE, "libbertha_amd.so")
+
+BDF_OK = 0
+BDF_EINVAL = -1
+BDF_ENOMEM = -2
+BDF_EDEVICE = -3
+BDF_ELIMIT = -4
+BDF_EUNSUPP = -5
+
+SHAPES = {"NOSYMM": 0, "LINEAR": 1, "DIATOM": 1, "ATOMIC": 2}
+HMLTS = {"DHFR": 1, "DHFB": 2}
+
+
+class BerthaError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bdf error {code}: {msg}")
+        self.code = code
+
+
+class _System(_C.Structure):
+    _fields_ = [
+        ("ncnt", _C.c_int),
+        ("xyz", _C.POINTER(_C.c_double)),
+        ("lmax", _C.POINTER(_C.c_int)),
+        ("nfnc", _C.POINTER(_C.c_int)),
+        ("exps", _C.POINTER(_C.c_double)),
+        ("lstride", _C.c_int),
+        ("mbs", _C.c_int),
+        ("shape", _C.c_int),
+        ("hmlt", _C.c_int),
+    ]
+
+
+class _Options(_C.Structure):
+    _fields_ = [
+        ("intsym", _C.c_int),
+        ("racah1", _C.c_int),
+        ("precision", _C.c_int),
+        ("split_threshold", _C.c_double),
+        ("device", _C.c_int),
+    ]
+
+
+_lib = None
+
+
+def library() -> _C.CDLL:
+    """Load libbertha_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _os.path.exists(LIBPATH):
+            raise RuntimeError(
+                f"{LIBPATH} is missing: build it with `make -C bertha-program_amd` "
+                "(or __graft_entry__.build())")
+        lib = _C.CDLL(LIBPATH)
+        P = _C.c_void_p
+        lib.bdf_last_error.restype = _C.c_char_p
+        lib.bdf_create.argtypes = [_C.POINTER(_System), _C.POINTER(_Options),
+                                   _C.POINTER(P), _C.POINTER(_C.c_int), _C.POINTER(_C.c_int)]
+        lib.bdf_coulomb.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
+        lib.bdf_coulomb_device.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_eri_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int, P, _C.c_int]
+        lib.bdf_etable.argtypes = [P, _C.c_int, _C.c_int, _C.c_int, _C.c_int, _C.c_int,
+                                   _C.c_int, _C.c_int, P, _C.c_long, _C.POINTER(_C.c_long)]
+        lib.bdf_last_stats.argtypes = [P, _C.POINTER(_C.c_double), _C.POINTER(_C.c_double)]
+        lib.bdf_stream.argtypes = [P]
+        lib.bdf_stream.restype = P
+        lib.bdf_destroy.argtypes = [P]
+        lib.bdf_destroy.restype = None
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != BDF_OK:
+        raise BerthaError(rc, library().bdf_last_error().decode())
+
+
+class Engine:
+    """One system's device tables and Fock-build plans.
+
+    Construction builds the E0LL/E0SS tables on the GPU (the reference's
+    EQSAVE).  ``clmfast`` returns (GDIR, GXCH) for a density, matching the
+    reference's CLMFAST outputs element by element.
+    """
+
+    def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
+                 device: int = 0):
+        lib = library()
+        self.system = system
+        xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
+        self._keep = (xyz, lmax, nfnc, exps)
+        s = _System(system.ncnt, xyz.ctypes.data_as(_C.POINTER(_C.c_double)),
+                    lmax.ctypes.data_as(_C.POINTER(_C.c_int)),
+                    nfnc.ctypes.data_as(_C.POINTER(_C.c_int)),
+                    exps.ctypes.data_as(_C.POINTER(_C.c_double)), lstride, mbs,
+                    SHAPES[system.shape], HMLTS[system.hmlt])
+        o = _Options(1 if intsym else 0, 1 if racah1 else 0, 0, 0.0, device)
+        h = _C.c_void_p()
+        nd = _C.c_int()
+        ns = _C.c_int()
+        _check(lib.bdf_create(_C.byref(s), _C.byref(o), _C.byref(h), _C.byref(nd), _C.byref(ns)))
+        self._h = h
+        self.ndim = nd.value
+        self.nskp = ns.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            library().bdf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def clmfast(self, dent: _np.ndarray, ilev: int = 3):
+        """GDIR, GXCH for total density ``dent`` (ndim x ndim complex)."""
+        n = self.ndim
+        d = _np.asfortranarray(dent, dtype=_np.complex128)
+        if d.shape != (n, n):
+            raise ValueError(f"density must be {n}x{n}")
+        g = _np.zeros((n, n), dtype=_np.complex128, order="F")
+        x = _np.zeros((n, n), dtype=_np.complex128, order="F")
+        _check(library().bdf_coulomb(self._h, ilev, d.ctypes.data, None, n,
+                                     g.ctypes.data, x.ctypes.data, None, None))
+        return g, x
+
+    def clmfast_device(self, d_dent: int, d_gdir: int, d_gxch: int, ilev: int = 3) -> None:
+        """Enqueue a build on device pointers (e.g. torch tensors' data_ptr())."""
+        _check(library().bdf_coulomb_device(self._h, ilev, d_dent, d_gdir, d_gxch))
+
+    def erifast(self, icnt, kidx, midx, ibas: int, jbas: int, itn, ilev: int = 3):
+        """RR(N,1..16) of one ERIFAST batch, shape (16, maxcd) complex."""
+        sysm = self.system
+        lq = []
+        for c, k in zip(icnt[2:], kidx[2:]):
+            lq.append(k // 2)
+        maxcd = sysm.nfnc_of(icnt[2] - 1, lq[0]) * sysm.nfnc_of(icnt[3] - 1, lq[1])
+        rr = _np.zeros((16, maxcd), dtype=_np.complex128)
+        a = [_np.asarray(v, dtype=_np.int32) for v in (icnt, kidx, midx, itn)]
+        _check(library().bdf_eri_batch(self._h, a[0].ctypes.data, a[1].ctypes.data,
+                                       a[2].ctypes.data, ibas, jbas, a[3].ctypes.data, ilev,
+                                       rr.ctypes.data, maxcd))
+        return rr
+
+    def etable(self, itt: int, ica: int, icb: int, ka: int, kb: int, ma: int, mb: int):
+        """One E0LL (itt=1) / E0SS (itt=4) block as [4][ntuv*maxab]."""
+        cap = 4 * 400000
+        out = _np.zeros(cap)
+        n = _C.c_long()
+        _check(library().bdf_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data,
+                                    cap, _C.byref(n)))
+        return out[:4 * n.value].reshape(4, n.value)
+
+    def stats(self):
+        q = _C.c_double()
+        ms = _C.c_double()
+        _check(library().bdf_last_stats(self._h, _C.byref(q), _C.byref(ms)))
+        return q.value, ms.value

@@ -1,0 +1,439 @@
+// api.cpp -- the C ABI (include/bertha_amd.h): engine lifetime, device
+// tables, Fock builds and test hooks.  Every entry point returns an error code
+// instead of stopping the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bertha_amd.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace bdf;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t _e = (x);                                                              \
+    if (_e != hipSuccess)                                                             \
+      return fail(BDF_EDEVICE, std::string(#x) + ": " + hipGetErrorString(_e));      \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    n = count;
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T> &v, hipStream_t s) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+// device copy of one stage's plan
+struct DevPlan {
+  int built = 0;
+  Plan P;
+  DevBuf<ClassDesc> classes;
+  DevBuf<ABCombo> abs;
+  DevBuf<CDCall> cds;
+  DevBuf<uint8_t> masks;
+  DevBuf<WorkItem> items;
+};
+
+struct bdf_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Basis B;
+  Etabs T;
+  int isym = 0, intsym = 1, racah1 = 0;
+  std::vector<uint8_t> eflags;
+  DevBuf<double> exps, normL, normS;
+  DevBuf<EPair> pairs;
+  DevBuf<ETerm> terms;
+  DevBuf<Mono> monos;
+  DevBuf<int64_t> first, flag_off;
+  DevBuf<double4> E;
+  DevBuf<uint8_t> flags;
+  DevBuf<int> icn, kqn, mqn;
+  DevPlan plans[3];
+  DevBuf<int> counter;
+  DevBuf<double> slab;
+  int nblocks = 0;
+  int64_t slab_stride = 0;
+  int slab_ld = 0;
+  DevBuf<double2> dent, gdir, gxch;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0, last_quartets = 0.0;
+};
+
+static int make_plan(bdf_engine *e, int ilev) {
+  DevPlan &dp = e->plans[ilev - 1];
+  if (dp.built) return BDF_OK;
+  std::string err = build_plan(e->B, e->T, e->eflags, e->isym, e->intsym, e->racah1, ilev, dp.P);
+  if (!err.empty()) return fail(BDF_EINVAL, err);
+  HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
+  HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
+  HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
+  HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
+  HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  // per-workgroup scratch: R rows, Boys rows, G rows, each ld doubles
+  int ld = ((std::max(dp.P.max_maxcd, 1) + 63) / 64) * 64;
+  int64_t stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * ld;
+  if (ld > e->slab_ld || stride > e->slab_stride) {
+    e->slab_ld = std::max(ld, e->slab_ld);
+    e->slab_stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * e->slab_ld;
+    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  dp.built = 1;
+  return BDF_OK;
+}
+
+static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_gdir,
+                    double2 *d_gxch, double2 *rr_out, DevPlan *override_plan) {
+  DevPlan &dp = override_plan ? *override_plan : e->plans[ilev - 1];
+  int nd = e->B.ndim;
+  if (!rr_out) HIPCHK(launch_zero(d_gdir, d_gxch, (int64_t)nd * nd, e->counter.p, e->stream));
+  else HIPCHK(launch_zero(rr_out, nullptr, 0, e->counter.p, e->stream));
+  FockArgs a;
+  a.classes = dp.classes.p;
+  a.abs = dp.abs.p;
+  a.cds = dp.cds.p;
+  a.masks = dp.masks.p;
+  a.items = dp.items.p;
+  a.nitems = (int)dp.P.items.size();
+  a.counter = e->counter.p;
+  a.exps = e->exps.p;
+  a.E = e->E.p;
+  a.dent = d_dent;
+  a.gdir = d_gdir;
+  a.gxch = d_gxch;
+  a.ndim = nd;
+  a.slab = e->slab.p;
+  a.slab_stride = e->slab_stride;
+  a.ld = e->slab_ld;
+  a.isym = e->isym;
+  a.intsym = e->intsym;
+  a.rr_out = rr_out;
+  int nb = std::min(e->nblocks, std::max(1, a.nitems));
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream));
+  if (!rr_out) {
+    CompleteArgs c;
+    c.gdir = d_gdir;
+    c.gxch = d_gxch;
+    c.icn = e->icn.p;
+    c.kqn = e->kqn.p;
+    c.mqn = e->mqn.p;
+    c.ndim = nd;
+    c.nskp = e->B.nskp;
+    c.intsym = e->intsym;
+    HIPCHK(launch_complete(c, e->stream));
+  }
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  e->last_quartets = dp.P.quartets;
+  return BDF_OK;
+}
+
+extern "C" {
+
+const char *bdf_last_error(void) { return g_err.c_str(); }
+
+int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, int *ndim,
+               int *nskp) {
+  if (!sys || !out) return fail(BDF_EINVAL, "bdf_create: null argument");
+  *out = nullptr;
+  if (sys->hmlt != BDF_HMLT_DHFR && sys->hmlt != BDF_HMLT_DHFB)
+    return fail(BDF_EINVAL, "bdf_create: unknown hamiltonian");
+  if (sys->shape < 0 || sys->shape > 2) return fail(BDF_EINVAL, "bdf_create: unknown shape");
+  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0};
+  if (opt) o = *opt;
+  if (o.precision != BDF_PREC_FP64) return fail(BDF_EUNSUPP, "bdf_create: only FP64 precision is built");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(BDF_EDEVICE, "bdf_create: no HIP device");
+  if (o.device < 0 || o.device >= ndev) return fail(BDF_EINVAL, "bdf_create: bad device ordinal");
+  bdf_engine *e = new bdf_engine();
+  e->device = o.device;
+  e->isym = sys->shape;
+  e->intsym = o.intsym ? 1 : 0;
+  e->racah1 = o.racah1 ? 1 : 0;
+  std::string err = build_basis(*sys, e->B);
+  if (!err.empty()) {
+    delete e;
+    return fail(BDF_ELIMIT, "bdf_create: " + err);
+  }
+  int rc = [&]() -> int {
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&e->ev0));
+    HIPCHK(hipEventCreate(&e->ev1));
+    HIPCHK(upload_constants());
+    build_etables(e->B, e->T);
+    HIPCHK(e->exps.upload(e->B.exps, e->stream));
+    HIPCHK(e->normL.upload(e->B.normL, e->stream));
+    HIPCHK(e->normS.upload(e->B.normS, e->stream));
+    HIPCHK(e->pairs.upload(e->T.pairs, e->stream));
+    HIPCHK(e->terms.upload(e->T.terms, e->stream));
+    HIPCHK(e->monos.upload(e->T.monos, e->stream));
+    std::vector<int64_t> first(e->T.pairs.size());
+    int64_t acc = 0;
+    for (size_t i = 0; i < e->T.pairs.size(); ++i) {
+      first[i] = acc;
+      acc += (int64_t)e->T.pairs[i].nA * e->T.pairs[i].nB;
+    }
+    HIPCHK(e->first.upload(first, e->stream));
+    HIPCHK(e->flag_off.upload(e->T.flag_off, e->stream));
+    HIPCHK(e->E.alloc((size_t)e->T.total));
+    HIPCHK(e->flags.alloc((size_t)e->T.nflag));
+    EtabArgs ea;
+    ea.pairs = e->pairs.p;
+    ea.npairs = (int)e->T.pairs.size();
+    ea.first = e->first.p;
+    ea.nitems = acc;
+    ea.terms = e->terms.p;
+    ea.monos = e->monos.p;
+    ea.exps = e->exps.p;
+    ea.normL = e->normL.p;
+    ea.normS = e->normS.p;
+    ea.E = e->E.p;
+    HIPCHK(launch_etab(ea, e->stream));
+    EflagArgs fa;
+    fa.pairs = e->pairs.p;
+    fa.npairs = ea.npairs;
+    fa.flag_off = e->flag_off.p;
+    fa.E = e->E.p;
+    fa.flags = e->flags.p;
+    HIPCHK(launch_eflag(fa, e->stream));
+    e->eflags.resize(e->T.nflag);
+    HIPCHK(hipMemcpyAsync(e->eflags.data(), e->flags.p, e->T.nflag, hipMemcpyDeviceToHost,
+                          e->stream));
+    HIPCHK(e->icn.upload(e->B.labicn, e->stream));
+    HIPCHK(e->kqn.upload(e->B.labkqn, e->stream));
+    HIPCHK(e->mqn.upload(e->B.labmqn, e->stream));
+    HIPCHK(e->counter.alloc(1));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, e->device));
+    e->nblocks = prop.multiProcessorCount * 8;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return BDF_OK;
+  }();
+  if (rc != BDF_OK) {
+    bdf_destroy(e);
+    return rc;
+  }
+  *out = e;
+  if (ndim) *ndim = e->B.ndim;
+  if (nskp) *nskp = e->B.nskp;
+  return BDF_OK;
+}
+
+int bdf_coulomb_device(bdf_engine *e, int ilev, const void *d_dent, void *d_gdir, void *d_gxch) {
+  if (!e || !d_dent || !d_gdir || !d_gxch) return fail(BDF_EINVAL, "bdf_coulomb_device: null argument");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_coulomb_device: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  return run_fock(e, ilev, (const double2 *)d_dent, (double2 *)d_gdir, (double2 *)d_gxch, nullptr,
+                  nullptr);
+}
+
+int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent, const double _Complex *deno,
+                int ld, double _Complex *gdir, double _Complex *gxch, double _Complex *qdir,
+                double _Complex *qxch) {
+  if (!e || !dent || !gdir || !gxch) return fail(BDF_EINVAL, "bdf_coulomb: null argument");
+  if (deno || qdir || qxch) return fail(BDF_EUNSUPP, "bdf_coulomb: open shells are not supported");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_coulomb: ld < ndim");
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(e->dent.alloc((size_t)nd * nd));
+  HIPCHK(e->gdir.alloc((size_t)nd * nd));
+  HIPCHK(e->gxch.alloc((size_t)nd * nd));
+  HIPCHK(hipMemcpy2DAsync(e->dent.p, nd * sizeof(double2), dent, ld * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyHostToDevice, e->stream));
+  int rc = bdf_coulomb_device(e, ilev, e->dent.p, e->gdir.p, e->gxch.p);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpy2DAsync(gdir, ld * sizeof(double2), e->gdir.p, nd * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpy2DAsync(gxch, ld * sizeof(double2), e->gxch.p, nd * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->last_ms = ms;
+  return BDF_OK;
+}
+
+int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int midx[4], int ibas,
+                  int jbas, const int itn[2], int ilev, double _Complex *rr, int maxcd) {
+  if (!e || !icnt || !kidx || !midx || !itn || !rr) return fail(BDF_EINVAL, "bdf_eri_batch: null argument");
+  const Basis &B = e->B;
+  int lq[4], kq[4], ic[4];
+  for (int k = 0; k < 4; ++k) {
+    ic[k] = icnt[k] - 1;
+    if (ic[k] < 0 || ic[k] >= B.ncnt) return fail(BDF_EINVAL, "bdf_eri_batch: bad centre");
+    if (kidx[k] < 1 || kidx[k] > 2 * B.lmax[ic[k]] + 1) return fail(BDF_EINVAL, "bdf_eri_batch: bad kappa index");
+    kq[k] = B.kappa_of(ic[k], kidx[k]);
+    lq[k] = kq[k] < 0 ? -kq[k] - 1 : kq[k];
+    if (midx[k] < 1 || midx[k] > std::abs(kq[k])) return fail(BDF_EINVAL, "bdf_eri_batch: bad |mj| index");
+  }
+  if ((itn[0] != 1 && itn[0] != 4) || (itn[1] != 1 && itn[1] != 4))
+    return fail(BDF_EINVAL, "bdf_eri_batch: itn must be 1 or 4");
+  int nA = B.nfn(ic[0], lq[0]), nBb = B.nfn(ic[1], lq[1]);
+  if (ibas < 1 || ibas > nA || jbas < 1 || jbas > nBb) return fail(BDF_EINVAL, "bdf_eri_batch: bad ibas/jbas");
+  int mcd = B.nfn(ic[2], lq[2]) * B.nfn(ic[3], lq[3]);
+  if (maxcd < mcd) return fail(BDF_EINVAL, "bdf_eri_batch: rr too small");
+  HIPCHK(hipSetDevice(e->device));
+  // a one-item plan: one class, one AB combination, one call
+  DevPlan dp;
+  ClassDesc cl;
+  for (int k = 0; k < 4; ++k) {
+    for (int x = 0; x < 3; ++x) cl.xyz[k][x] = B.xyz[3 * ic[k] + x];
+    cl.exp[k] = B.expo(ic[k], lq[k]);
+    cl.nbas[k] = B.nfn(ic[k], lq[k]);
+  }
+  int mcnt = 1;
+  if (ic[1] != ic[0]) ++mcnt;
+  if (ic[2] != ic[0] && ic[2] != ic[1]) ++mcnt;
+  if (ic[3] != ic[0] && ic[3] != ic[1] && ic[3] != ic[2]) ++mcnt;
+  int lsum = lq[0] + lq[1] + lq[2] + lq[3];
+  cl.lamfl = mcnt == 1 ? lsum + 4 : (ilev == 1 ? lsum : (ilev == 2 ? lsum + 2 : lsum + 4));
+  cl.maxcd = mcd;
+  cl.mcnt = mcnt;
+  cl.pad = 0;
+  cl.ab_begin = 0;
+  cl.ab_end = 1;
+  ABCombo ab;
+  int pab = e->T.pair_index.at(Etabs::key(ic[0], ic[1], kidx[0], kidx[1], midx[0], midx[1], itn[0] == 4));
+  int pcd = e->T.pair_index.at(Etabs::key(ic[2], ic[3], kidx[2], kidx[3], midx[2], midx[3], itn[1] == 4));
+  ab.eoff = e->T.pairs[pab].off;
+  ab.ntab = ntuv(lq[0] + lq[1] + (itn[0] == 4 ? 2 : 0));
+  int ntcd = ntuv(lq[2] + lq[3] + (itn[1] == 4 ? 2 : 0));
+  ab.ntg = ntcd;
+  for (int s = 0; s < 2; ++s) ab.fa[s] = ab.fb[s] = 0;
+  ab.nb = cl.nbas[1];
+  ab.cd_begin = 0;
+  ab.cd_end = 1;
+  ab.pad = 0;
+  // G mask: aggregated column flags of the (C,D,lc,ld) class for this itn
+  std::vector<int> agg(4 * ntcd, 0);
+  int lc = lq[2], ld = lq[3];
+  for (int nc = (lc == 0); nc <= 1; ++nc)
+    for (int nd = (ld == 0); nd <= 1; ++nd)
+      for (int mc = 1; mc <= lc + nc; ++mc)
+        for (int md = 1; md <= ld + nd; ++md) {
+          int pr = e->T.pair_index.at(Etabs::key(ic[2], ic[3], 2 * lc + nc, 2 * ld + nd, mc, md, itn[1] == 4));
+          for (int t = 0; t < ntcd; ++t)
+            for (int c = 0; c < 4; ++c) agg[4 * t + c] += e->eflags[e->T.flag_off[pr] + 4 * t + c];
+        }
+  ab.gmask = 0;
+  for (int t = 0; t < ntcd; ++t) {
+    int all = agg[4 * t] + agg[4 * t + 1] + agg[4 * t + 2] + agg[4 * t + 3];
+    for (int c = 0; c < 4; ++c)
+      dp.P.masks.push_back((uint8_t)(all != 0 && (e->isym <= 1 || agg[4 * t + c] != 0)));
+  }
+  CDCall cd;
+  cd.eoff = e->T.pairs[pcd].off;
+  cd.ntcd = ntcd;
+  cd.kmask = (int)dp.P.masks.size();
+  for (int t = 0; t < ntcd; ++t)
+    for (int c = 0; c < 4; ++c) dp.P.masks.push_back(e->eflags[e->T.flag_off[pcd] + 4 * t + c]);
+  for (int s = 0; s < 2; ++s) cd.fc[s] = cd.fd[s] = 0;
+  cd.iflg = 0;
+  int mq[4];
+  for (int k = 0; k < 4; ++k) mq[k] = 2 * midx[k] - 1;
+  int sab = (kq[0] * kq[1]) >= 0 ? 1 : -1, scd = (kq[2] * kq[3]) >= 0 ? 1 : -1;
+  cd.pab[0] = (int8_t)(sab * ((std::abs(mq[1] - mq[0]) / 2) % 2 ? -1 : 1));
+  cd.pab[1] = (int8_t)(sab * (((mq[0] + mq[1]) / 2) % 2 ? -1 : 1));
+  cd.pcd[0] = (int8_t)(scd * ((std::abs(mq[3] - mq[2]) / 2) % 2 ? -1 : 1));
+  cd.pcd[1] = (int8_t)(scd * (((mq[2] + mq[3]) / 2) % 2 ? -1 : 1));
+  dp.P.classes.push_back(cl);
+  dp.P.abs.push_back(ab);
+  dp.P.cds.push_back(cd);
+  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1)});
+  dp.P.max_maxcd = mcd;
+  HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
+  HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
+  HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
+  HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
+  HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  int ldr = ((mcd + 63) / 64) * 64;
+  if (ldr > e->slab_ld || !e->slab.p) {
+    e->slab_ld = std::max(ldr, e->slab_ld);
+    e->slab_stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * e->slab_ld;
+    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
+  }
+  DevBuf<double2> out;
+  HIPCHK(out.alloc((size_t)16 * mcd));
+  int rc = run_fock(e, ilev, nullptr, nullptr, nullptr, out.p, &dp);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpyAsync(rr, out.p, sizeof(double2) * 16 * mcd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb, int ma, int mb, double *out,
+               long cap, long *n) {
+  if (!e || !out || !n) return fail(BDF_EINVAL, "bdf_etable: null argument");
+  auto it = e->T.pair_index.find(Etabs::key(ica - 1, icb - 1, ka, kb, ma, mb, itt == 4));
+  if (it == e->T.pair_index.end()) return fail(BDF_EINVAL, "bdf_etable: no such block");
+  const EPair &P = e->T.pairs[it->second];
+  long cnt = (long)ntuv(P.lam) * P.nA * P.nB;
+  *n = cnt;
+  if (cap < 4 * cnt) return fail(BDF_EINVAL, "bdf_etable: out too small");
+  std::vector<double4> h(cnt);
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipMemcpy(h.data(), e->E.p + P.off, sizeof(double4) * cnt, hipMemcpyDeviceToHost));
+  for (long k = 0; k < cnt; ++k) {
+    out[k] = h[k].x;
+    out[cnt + k] = h[k].y;
+    out[2 * cnt + k] = h[k].z;
+    out[3 * cnt + k] = h[k].w;
+  }
+  return BDF_OK;
+}
+
+void *bdf_stream(bdf_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms) {
+  if (!e) return fail(BDF_EINVAL, "bdf_last_stats: null engine");
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->last_ms = ms;
+  if (quartets) *quartets = e->last_quartets;
+  if (kernel_ms) *kernel_ms = e->last_ms;
+  return BDF_OK;
+}
+
+void bdf_destroy(bdf_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"

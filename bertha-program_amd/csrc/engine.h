@@ -1,0 +1,104 @@
+// engine.h -- internal data layout of the MI355X two-electron engine.
+//
+// Host code (plan.cpp) turns a basis into flat tables; kernels.hip reads them.
+// Everything here is plain data so the same structs are valid on both sides.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bdf {
+
+constexpr int kMaxL = 3;                  // largest shell l (f)
+constexpr int kMaxPoly = kMaxL + 1;       // largest polynomial degree (SS of f)
+constexpr int kMaxLab = 2 * kMaxL + 2;    // largest pair Hermite order (SS f-f)
+constexpr int kMaxLfl = 4 * kMaxL + 4;    // largest R order (stage 3)
+constexpr int kMaxPrim = 32;              // primitives per shell
+constexpr int kWave = 64;
+
+__host__ __device__ constexpr inline int ntuv(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
+
+// ---- E-coefficient generation --------------------------------------------
+// A polynomial in (x,y,z) relative to one centre, stored as monomials.
+struct Mono {
+  double re, im;
+  int8_t i, j, k, pad;
+};
+
+// One term of a spinor-pair overlap density: w * a^pa * b^pb *
+// polyA(r-A) * polyB(r-B), where polyA/polyB index into the monomial table.
+struct ETerm {
+  double w;
+  int16_t pa, pb;
+  int32_t polyA, polyB;  // [begin, end) ranges packed as begin*65536 + count
+};
+
+// One (A-block, B-block) pair table: E11 and E21 for one component type.
+struct EPair {
+  double ax, ay, az, bx, by, bz;
+  int expA, expB;       // offsets into the exponent array
+  int nA, nB;
+  int lam;              // Hermite order of the stored expansion
+  int small;            // 0 = LL, 1 = SS
+  int64_t off;          // double4 offset of [tuv][ij]
+  int term11, nterm11;  // term ranges for E11 and E21
+  int term21, nterm21;
+  int la, lb;           // large-component l of A and B (normalisation)
+};
+
+// ---- Fock build plan -----------------------------------------------------
+// A class fixes the centres and l of all four labels; its R batch is shared
+// by every kappa/|m|/component combination inside it (the reference caches
+// it in RCTTFL, bertha-scf.f:12637-12645).
+struct ClassDesc {
+  double xyz[4][3];
+  int exp[4];       // exponent offsets
+  int nbas[4];
+  int lamfl;        // R order (ERIFAST LAMABCDFL)
+  int maxcd;
+  int ab_begin, ab_end;
+  int mcnt;
+  int pad;
+};
+
+// One (kappa_A, kappa_B, |m_A|, |m_B|, IT1) combination of a class.
+struct ABCombo {
+  int64_t eoff;     // E(AB) table double4 offset for IT1's component type
+  int ntab;         // N(L_AB)
+  int ntg;          // N(L_CD) of the first CD call: G is built that far
+  int gmask;        // byte offset of G mask [ntg][4]
+  int fa[2], fb[2]; // Fock offsets of A, B for sign slots (0 = -|m|)
+  int cd_begin, cd_end;
+  int nb;           // primitives of B (ij -> i, j)
+  int pad;
+};
+
+// One ERIFAST call (CD combination and IT2) with its digestion flags.
+struct CDCall {
+  int64_t eoff;     // E(CD) double4 offset
+  int ntcd;
+  int kmask;        // byte offset of the E(CD) column mask [ntcd][4]
+  int fc[2], fd[2]; // Fock offsets of C, D for sign slots
+  int iflg;         // bit b set: batch b+1 active
+  int8_t pab[2], pcd[2];  // [0] same-slot phase (PAB1), [1] PAB2
+};
+
+struct WorkItem {
+  int cls;
+  int ij;
+};
+
+// everything the host plan produces
+struct Plan {
+  int ndim = 0, nskp = 0;
+  int isym = 0, intsym = 1, racah1 = 0, ilev = 3;
+  std::vector<ClassDesc> classes;
+  std::vector<ABCombo> abs;
+  std::vector<CDCall> cds;
+  std::vector<uint8_t> masks;
+  std::vector<WorkItem> items;
+  int max_lfl = 0, max_maxcd = 0, max_ntg = 0;
+  double quartets = 0.0;  // primitive quartet count (ij,kl) x calls
+};
+
+}  // namespace bdf

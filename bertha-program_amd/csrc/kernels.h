@@ -1,0 +1,61 @@
+// kernels.h -- argument blocks and launchers of kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+
+namespace bdf {
+
+struct EtabArgs {
+  const EPair *pairs;
+  int npairs;
+  const int64_t *first;  // [npairs] first flat item of each pair
+  int64_t nitems;
+  const ETerm *terms;
+  const Mono *monos;
+  const double *exps, *normL, *normS;
+  double4 *E;
+};
+
+struct EflagArgs {
+  const EPair *pairs;
+  int npairs;
+  const int64_t *flag_off;
+  const double4 *E;
+  uint8_t *flags;
+};
+
+struct FockArgs {
+  const ClassDesc *classes;
+  const ABCombo *abs;
+  const CDCall *cds;
+  const uint8_t *masks;
+  const WorkItem *items;
+  int nitems;
+  int *counter;
+  const double *exps;
+  const double4 *E;
+  const double2 *dent;
+  double2 *gdir, *gxch;
+  int ndim;
+  double *slab;
+  int64_t slab_stride;  // doubles per workgroup
+  int ld;               // row length of the slab (>= largest maxcd)
+  int isym, intsym;
+  double2 *rr_out;      // test hook: write RR[16][maxcd] instead of digesting
+};
+
+struct CompleteArgs {
+  double2 *gdir, *gxch;
+  const int *icn, *kqn, *mqn;
+  int ndim, nskp, intsym;
+};
+
+hipError_t upload_constants();
+hipError_t launch_etab(const EtabArgs &a, hipStream_t s);
+hipError_t launch_eflag(const EflagArgs &a, hipStream_t s);
+hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter, hipStream_t s);
+hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
+
+}  // namespace bdf

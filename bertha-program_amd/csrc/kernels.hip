@@ -1,0 +1,582 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the two-electron engine.
+//
+//   etab_kernel   spinor-pair Hermite E-coefficients (replaces EQSAVE's
+//                 E0LLGN/E0SSGN, bertha-scf.f:33378, 33511)
+//   eflag_kernel  E(CD) column masks (E0LLCD0/E0SSCD0, :17673, :17833)
+//   fock_kernel   persistent work-queue kernel: Boys function and
+//                 R-integrals (FUNFM/RMAKE, :32465/:31900), the two E.R.E
+//                 contractions (ERIFAST, :12269) and J/K digestion
+//                 (CLMMTZ/CLMMTG, :13135/:13841)
+//   complete_kernel Hermitian completion (CLMFAST :12166-12205)
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace bdf {
+
+__constant__ uint16_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits
+__constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
+
+__device__ __forceinline__ int hidx(int t, int u, int v) {
+  int L = t + u + v;
+  return L * (L + 1) * (L + 2) / 6 + t * (L + 1) - t * (t - 1) / 2 + u;
+}
+__device__ __forceinline__ void dec(int n, int &t, int &u, int &v) {
+  uint16_t x = c_tuv[n];
+  t = x >> 10;
+  u = (x >> 5) & 31;
+  v = x & 31;
+}
+
+// ---------------------------------------------------------------------------
+// E-coefficients
+// ---------------------------------------------------------------------------
+// 1D Hermite expansion coefficients of x_A^i x_B^j exp(-p x_P^2):
+// E[i+1][j][t] = E[i][j][t-1]/(2p) + X_PA E[i][j][t] + (t+1) E[i][j][t+1]
+// (the same multiplication rule the reference applies in ESTEPL/ESTEPN).
+struct Herm1 {
+  double e[kMaxPoly + 1][kMaxPoly + 1][2 * kMaxPoly + 1];
+};
+__device__ void herm1(Herm1 &h, double pa, double pb, double inv2p) {
+  for (int i = 0; i <= kMaxPoly; ++i)
+    for (int j = 0; j <= kMaxPoly; ++j)
+      for (int t = 0; t <= 2 * kMaxPoly; ++t) h.e[i][j][t] = 0.0;
+  h.e[0][0][0] = 1.0;
+  for (int i = 0; i <= kMaxPoly; ++i) {
+    if (i > 0)
+      for (int t = 0; t <= i; ++t) {
+        double v = pa * h.e[i - 1][0][t];
+        if (t > 0) v += inv2p * h.e[i - 1][0][t - 1];
+        if (t + 1 <= i - 1) v += (t + 1) * h.e[i - 1][0][t + 1];
+        h.e[i][0][t] = v;
+      }
+    for (int j = 1; j <= kMaxPoly; ++j)
+      for (int t = 0; t <= i + j; ++t) {
+        double v = pb * h.e[i][j - 1][t];
+        if (t > 0) v += inv2p * h.e[i][j - 1][t - 1];
+        if (t + 1 <= i + j - 1) v += (t + 1) * h.e[i][j - 1][t + 1];
+        h.e[i][j][t] = v;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(128) etab_kernel(EtabArgs a) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nitems) return;
+  int lo = 0, hi = a.npairs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.first[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const EPair P = a.pairs[lo];
+  int ij = (int)(g - a.first[lo]);
+  int i = ij / P.nB, j = ij % P.nB;
+  double ea = a.exps[P.expA + i], eb = a.exps[P.expB + j];
+  double p = ea + eb;
+  double px = (ea * P.ax + eb * P.bx) / p, py = (ea * P.ay + eb * P.by) / p,
+         pz = (ea * P.az + eb * P.bz) / p;
+  double dx = P.ax - P.bx, dy = P.ay - P.by, dz = P.az - P.bz;
+  double kab = exp(-(ea * eb * (dx * dx + dy * dy + dz * dz)) / p);
+  double inv2p = 1.0 / (2.0 * p);
+  Herm1 hx, hy, hz;
+  herm1(hx, px - P.ax, px - P.bx, inv2p);
+  herm1(hy, py - P.ay, py - P.by, inv2p);
+  herm1(hz, pz - P.az, pz - P.bz, inv2p);
+  constexpr int NT = ntuv(kMaxLab);
+  double acc[2][2][NT];
+  int nt = ntuv(P.lam);
+  for (int s = 0; s < 2; ++s)
+    for (int n = 0; n < nt; ++n) acc[s][0][n] = acc[s][1][n] = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    int tb = s == 0 ? P.term11 : P.term21;
+    int tn = s == 0 ? P.nterm11 : P.nterm21;
+    for (int q = tb; q < tb + tn; ++q) {
+      ETerm T = a.terms[q];
+      double w = T.w;
+      if (T.pa) w *= ea;
+      if (T.pb) w *= eb;
+      int ab0 = T.polyA >> 16, abn = T.polyA & 0xffff;
+      int bb0 = T.polyB >> 16, bbn = T.polyB & 0xffff;
+      for (int ma = ab0; ma < ab0 + abn; ++ma) {
+        Mono A = a.monos[ma];
+        for (int mb = bb0; mb < bb0 + bbn; ++mb) {
+          Mono B = a.monos[mb];
+          double cr = w * (A.re * B.re - A.im * B.im);
+          double ci = w * (A.re * B.im + A.im * B.re);
+          for (int t = 0; t <= A.i + B.i; ++t) {
+            double ex = hx.e[A.i][B.i][t];
+            if (ex == 0.0) continue;
+            for (int u = 0; u <= A.j + B.j; ++u) {
+              double exy = ex * hy.e[A.j][B.j][u];
+              if (exy == 0.0) continue;
+              for (int v = 0; v <= A.k + B.k; ++v) {
+                double e3 = exy * hz.e[A.k][B.k][v];
+                int n = hidx(t, u, v);
+                acc[s][0][n] += cr * e3;
+                acc[s][1][n] += ci * e3;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  double nrm = P.small ? a.normS[P.expA + i] * a.normS[P.expB + j]
+                       : a.normL[P.expA + i] * a.normL[P.expB + j];
+  double sc = kab * nrm;
+  int maxab = P.nA * P.nB;
+  for (int n = 0; n < nt; ++n)
+    a.E[P.off + (int64_t)n * maxab + ij] =
+        make_double4(sc * acc[0][0][n], sc * acc[0][1][n], sc * acc[1][0][n], sc * acc[1][1][n]);
+}
+
+// per (pair, tuv, component): sum over ij of |E| > 1e-10 (E0LLCD0 EPSCD)
+__global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
+  int p = blockIdx.x;
+  const EPair P = a.pairs[p];
+  int nt = ntuv(P.lam), maxab = P.nA * P.nB;
+  for (int n = threadIdx.x; n < nt; n += blockDim.x) {
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int ij = 0; ij < maxab; ++ij) {
+      double4 e = a.E[P.off + (int64_t)n * maxab + ij];
+      s0 += fabs(e.x); s1 += fabs(e.y); s2 += fabs(e.z); s3 += fabs(e.w);
+    }
+    uint8_t *f = a.flags + a.flag_off[p] + 4 * n;
+    f[0] = s0 > 1.0e-10; f[1] = s1 > 1.0e-10; f[2] = s2 > 1.0e-10; f[3] = s3 > 1.0e-10;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fock build
+// ---------------------------------------------------------------------------
+// Boys function F_0..F_L of the reference's four branches (FUNFM, RMAKE's
+// thresholds 1e-11 / 17 / 30, bertha-scf.f:31927-31985, 32495-32605),
+// written to f[n*ld].
+__device__ void boys_ref(double T, int L, double *f, int ld) {
+  const double kA0 = 4.994501191201870e-1, kB0 = 4.551838436668326e-1;
+  const double kSqrtPiHalf = 0.5 * 1.7724538509055160;
+  if (T <= 1.0e-11) {
+    for (int m = 0; m <= L; ++m) f[m * ld] = c_rcp[m];
+  } else if (T <= 17.0) {
+    double txp = exp(-T), tt2 = 2.0 * T, tl = 1.0, s = 1.0;
+    for (int k = 1; k <= 60; ++k) {
+      tl = tl * tt2 * c_rcp[L + k];
+      s += tl;
+    }
+    double fm = s * txp * c_rcp[L];
+    f[L * ld] = fm;
+    for (int m = L - 1; m >= 0; --m) {
+      fm = (tt2 * fm + txp) * c_rcp[m];
+      f[m * ld] = fm;
+    }
+  } else if (T <= 30.0) {
+    double txp = exp(-T), rt2 = 1.0 / (2.0 * T);
+    double fm = kSqrtPiHalf / sqrt(T) - txp * (kA0 / (kB0 + T));
+    f[0] = fm;
+    for (int m = 0; m < L; ++m) {
+      fm = ((2 * m + 1) * fm - txp) * rt2;
+      f[(m + 1) * ld] = fm;
+    }
+  } else {
+    double rt2 = 1.0 / (2.0 * T);
+    double fm = kSqrtPiHalf / sqrt(T);
+    f[0] = fm;
+    for (int m = 0; m < L; ++m) {
+      fm = (2 * m + 1) * fm * rt2;
+      f[(m + 1) * ld] = fm;
+    }
+  }
+}
+
+// R^(0)_{tuv}(P-Q) for t+u+v <= L in Hermite order, in place in r[n*ld]:
+// R^(n)_{000} = (-2a)^n F_n and R^(n)_{t+1,u,v} = X R^(n+1)_{tuv} +
+// t R^(n+1)_{t-1,u,v} (McMurchie-Davidson), descending levels so each slot is
+// overwritten only after the lower order has been read.
+__device__ void rints(double *r, const double *f, int ld, int L, double alpha, double X,
+                      double Y, double Z) {
+  double m2a = -2.0 * alpha;
+  double pw = 1.0;
+  for (int k = 0; k < L; ++k) pw *= m2a;
+  r[0] = pw * f[L * ld];
+  for (int n = L - 1; n >= 0; --n) {
+    for (int lev = L - n; lev >= 1; --lev) {
+      for (int t = lev; t >= 0; --t)
+        for (int u = lev - t; u >= 0; --u) {
+          int v = lev - t - u;
+          double val;
+          if (t > 0) {
+            val = X * r[hidx(t - 1, u, v) * ld];
+            if (t > 1) val += (t - 1) * r[hidx(t - 2, u, v) * ld];
+          } else if (u > 0) {
+            val = Y * r[hidx(t, u - 1, v) * ld];
+            if (u > 1) val += (u - 1) * r[hidx(t, u - 2, v) * ld];
+          } else {
+            val = Z * r[hidx(t, u, v - 1) * ld];
+            if (v > 1) val += (v - 1) * r[hidx(t, u, v - 2) * ld];
+          }
+          r[hidx(t, u, v) * ld] = val;
+        }
+    }
+    pw = 1.0;
+    for (int k = 0; k < n; ++k) pw *= m2a;
+    r[0] = pw * f[n * ld];
+  }
+}
+
+// digestion batches of CLMMTG (bertha-scf.f:13902-14392): labels 0..3 =
+// A,B,C,D; each batch adds phase * RR(sA,sB,sC,sD) * D(drow,dcol) into
+// G(orow,ocol); *_f flags swap the sign slot of that label; pab/pcd select
+// the PAB/PCD permutation phases.
+struct Batch {
+  int8_t xch, orow, ocol, oflr, oflc, drow, dcol, dflr, dflc, pab, pcd;
+};
+__constant__ Batch c_batch[11] = {
+    {0, 0, 1, 0, 0, 2, 3, 0, 0, 0, 0}, {0, 0, 1, 0, 0, 3, 2, 1, 1, 0, 1},
+    {0, 2, 3, 0, 0, 0, 1, 0, 0, 0, 0}, {0, 2, 3, 0, 0, 1, 0, 1, 1, 1, 0},
+    {1, 0, 3, 0, 0, 2, 1, 0, 0, 0, 0}, {1, 0, 2, 0, 1, 3, 1, 1, 0, 0, 1},
+    {1, 1, 3, 1, 0, 2, 0, 0, 1, 1, 0}, {1, 1, 2, 1, 1, 3, 0, 1, 1, 1, 1},
+    {1, 2, 1, 0, 0, 0, 3, 0, 0, 0, 0}, {1, 2, 0, 0, 1, 1, 3, 1, 0, 1, 0},
+    {1, 3, 1, 1, 0, 0, 2, 0, 1, 0, 1}};
+
+__device__ __forceinline__ void atomic_add2(double2 *p, double re, double im) {
+  unsafeAtomicAdd(&p->x, re);
+  if (im != 0.0) unsafeAtomicAdd(&p->y, im);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
+  __shared__ double s_eab[ntuv(kMaxLab) * 4];
+  __shared__ uint8_t s_cflag[ntuv(kMaxLfl)];
+  __shared__ uint8_t s_gm[ntuv(kMaxLab) * 4];
+  __shared__ uint8_t s_km[ntuv(kMaxLab) * 4];
+  __shared__ int s_item;
+  const int lane = threadIdx.x;
+  const int ld = a.ld;
+  double *R = a.slab + (int64_t)blockIdx.x * a.slab_stride;  // [rows][ld]
+  double *F = R + (int64_t)ntuv(kMaxLfl) * ld;                // [kMaxLfl+1][ld]
+  double *G = F + (int64_t)(kMaxLfl + 1) * ld;                // [ntg*4][ld]
+  const double2 *Dm = a.dent;
+  const int nd = a.ndim;
+
+  for (;;) {
+    if (lane == 0) s_item = atomicAdd(a.counter, 1);
+    __syncthreads();
+    int idx = s_item;
+    __syncthreads();
+    if (idx >= a.nitems) break;
+    const WorkItem it = a.items[idx];
+    const ClassDesc &cl = a.classes[it.cls];
+    const int nB = cl.nbas[1], nD = cl.nbas[3];
+    const int ib = it.ij / nB, jb = it.ij % nB;
+    const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
+    const double p = ea + eb;
+    const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
+    const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
+    const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
+    const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
+    const int maxab = cl.nbas[0] * nB;
+
+    // R-integrals for every kl of this ij, then the reference's column test
+    // (sum over kl of |R| > 1e-10 before the prefactor, :12660-12675)
+    for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+      int kl = kl0 + lane;
+      if (kl < maxcd) {
+        int kb = kl / nD, lb = kl % nD;
+        double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
+        double q = ec + ed;
+        double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
+        double Qy = (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q;
+        double Qz = (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q;
+        double alpha = p * q / (p + q);
+        double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
+        double T = alpha * (X * X + Y * Y + Z * Z);
+        boys_ref(T, L, F + kl, ld);
+        rints(R + kl, F + kl, ld, L, alpha, X, Y, Z);
+      }
+    }
+    __syncthreads();
+    for (int col = lane; col < nt; col += 64) {
+      double s = 0.0;
+      const double *row = R + (int64_t)col * ld;
+      for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
+      s_cflag[col] = s > 1.0e-10;
+    }
+    __syncthreads();
+    for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+      int kl = kl0 + lane;
+      if (kl < maxcd) {
+        int kb = kl / nD, lb = kl % nD;
+        double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
+        double pre = 2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q);
+        for (int col = 0; col < nt; ++col) {
+          double *x = R + (int64_t)col * ld + kl;
+          *x = s_cflag[col] ? pre * *x : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+
+    for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+      const ABCombo ab = a.abs[abi];
+      for (int t = lane; t < ab.ntab; t += 64) {
+        double4 e = a.E[ab.eoff + (int64_t)t * maxab + it.ij];
+        // E(AB) entries of magnitude <= 1e-16 are skipped by ERIFAST (:12445)
+        s_eab[4 * t + 0] = fabs(e.x) > 1.0e-16 ? e.x : 0.0;
+        s_eab[4 * t + 1] = fabs(e.y) > 1.0e-16 ? e.y : 0.0;
+        s_eab[4 * t + 2] = fabs(e.z) > 1.0e-16 ? e.z : 0.0;
+        s_eab[4 * t + 3] = fabs(e.w) > 1.0e-16 ? e.w : 0.0;
+      }
+      for (int t = lane; t < 4 * ab.ntg; t += 64) s_gm[t] = a.masks[ab.gmask + t];
+      __syncthreads();
+      // first contraction G(kl,tuv') = sum_tuv E_AB(tuv) R(tuv+tuv')
+      for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+        int kl = kl0 + lane;
+        if (kl >= maxcd) continue;
+        for (int tp = 0; tp < ab.ntg; ++tp) {
+          double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+          if (s_gm[4 * tp] | s_gm[4 * tp + 1] | s_gm[4 * tp + 2] | s_gm[4 * tp + 3]) {
+            int t2, u2, v2;
+            dec(tp, t2, u2, v2);
+            for (int t = 0; t < ab.ntab; ++t) {
+              int t1, u1, v1;
+              dec(t, t1, u1, v1);
+              double r = R[(int64_t)hidx(t1 + t2, u1 + u2, v1 + v2) * ld + kl];
+              g0 += s_eab[4 * t + 0] * r;
+              g1 += s_eab[4 * t + 1] * r;
+              g2 += s_eab[4 * t + 2] * r;
+              g3 += s_eab[4 * t + 3] * r;
+            }
+          }
+          double *gp = G + (int64_t)(4 * tp) * ld + kl;
+          gp[0] = s_gm[4 * tp + 0] ? g0 : 0.0;
+          gp[ld] = s_gm[4 * tp + 1] ? g1 : 0.0;
+          gp[2 * ld] = s_gm[4 * tp + 2] ? g2 : 0.0;
+          gp[3 * ld] = s_gm[4 * tp + 3] ? g3 : 0.0;
+        }
+      }
+      double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
+      for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
+        const CDCall cd = a.cds[ci];
+        __syncthreads();
+        for (int t = lane; t < 4 * cd.ntcd; t += 64) s_km[t] = a.masks[cd.kmask + t];
+        __syncthreads();
+        const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+        const double pabcd = pab1 * pcd1;
+        for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+          int kl = kl0 + lane;
+          if (kl >= maxcd) continue;
+          // second contraction (ERIFAST :12809-13119): a_s = sum Z E_CD G,
+          // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
+          double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+          double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+          for (int tp = 0; tp < cd.ntcd; ++tp) {
+            double4 e = a.E[cd.eoff + (int64_t)tp * maxcd + kl];
+            int t2, u2, v2;
+            dec(tp, t2, u2, v2);
+            double z = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+            double e11r = s_km[4 * tp + 0] ? z * e.x : 0.0;
+            double e11i = s_km[4 * tp + 1] ? z * e.y : 0.0;
+            double e21r = s_km[4 * tp + 2] ? z * e.z : 0.0;
+            double e21i = s_km[4 * tp + 3] ? z * e.w : 0.0;
+            const double *gp = G + (int64_t)(4 * tp) * ld + kl;
+            double g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
+            // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
+            // set 3: E21cd x G21
+            ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
+            br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
+            ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
+            br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
+            ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
+            br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
+            ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
+            br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
+          }
+          // RR(n), n = 1 + 8 sA + 4 sB + 2 sC + sD (s = 1 for +|m|)
+          double rr_r[16], rr_i[16];
+          if (a.isym != 0) {
+            for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
+            rr_r[0] = ar[0];
+            rr_r[12] = pab1 * br[0];
+            rr_r[15] = pabcd * rr_r[0];
+            rr_r[3] = pabcd * rr_r[12];
+            rr_r[10] = ar[3];
+            rr_r[6] = -pab1 * br[3];
+            rr_r[5] = pabcd * rr_r[10];
+            rr_r[9] = pabcd * rr_r[6];
+          } else {
+            rr_r[0] = ar[0]; rr_i[0] = ai[0];
+            rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
+            rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
+            rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
+            rr_r[2] = ar[1]; rr_i[2] = ai[1];
+            rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
+            rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
+            rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
+            rr_r[8] = ar[2]; rr_i[8] = ai[2];
+            rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
+            rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
+            rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
+            rr_r[10] = ar[3]; rr_i[10] = ai[3];
+            rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
+            rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
+            rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
+          }
+          if (a.rr_out) {
+            for (int n = 0; n < 16; ++n)
+              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
+            continue;
+          }
+          // digestion (CLMMTG / CLMMTZ)
+          const int kb = kl / nD, lb = kl % nD;
+          const int bidx[4] = {ib, jb, kb, lb};
+          const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          for (int b = 0; b < 11; ++b) {
+            if (!((cd.iflg >> b) & 1)) continue;
+            if (!a.intsym && b != 0 && b != 4) continue;
+            const Batch bt = c_batch[b];
+            double acc[2][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
+            for (int n = 0; n < 16; ++n) {
+              int sl[4] = {(n >> 3) & 1, (n >> 2) & 1, (n >> 1) & 1, n & 1};
+              int ors = sl[bt.orow] ^ bt.oflr, ocs = sl[bt.ocol] ^ bt.oflc;
+              int drs = sl[bt.drow] ^ bt.dflr, dcs = sl[bt.dcol] ^ bt.dflc;
+              if (a.isym != 0 && (ors != ocs || drs != dcs)) continue;
+              double ph = 1.0;
+              if (bt.pab) ph *= (sl[0] != sl[1]) ? (double)cd.pab[1] : pab1;
+              if (bt.pcd) ph *= (sl[2] != sl[3]) ? (double)cd.pcd[1] : pcd1;
+              int dr = off[bt.drow][drs] + bidx[bt.drow];
+              int dc = off[bt.dcol][dcs] + bidx[bt.dcol];
+              double2 d = Dm[(int64_t)dc * nd + dr];
+              if (a.isym != 0) {
+                acc[ors][ocs][0] += ph * rr_r[n] * d.x;
+              } else {
+                acc[ors][ocs][0] += ph * (rr_r[n] * d.x - rr_i[n] * d.y);
+                acc[ors][ocs][1] += ph * (rr_r[n] * d.y + rr_i[n] * d.x);
+              }
+            }
+            if (b < 2) {
+              for (int s = 0; s < 4; ++s) {
+                accAB[s][0] += acc[s >> 1][s & 1][0];
+                accAB[s][1] += acc[s >> 1][s & 1][1];
+              }
+              continue;
+            }
+            double2 *Gm = bt.xch ? a.gxch : a.gdir;
+            for (int rs = 0; rs < 2; ++rs)
+              for (int cs = 0; cs < 2; ++cs) {
+                if (a.isym != 0 && rs != cs) continue;
+                int orr = off[bt.orow][rs] + bidx[bt.orow];
+                int occ = off[bt.ocol][cs] + bidx[bt.ocol];
+                atomic_add2(&Gm[(int64_t)occ * nd + orr], acc[rs][cs][0], acc[rs][cs][1]);
+              }
+          }
+        }
+      }
+      if (!a.rr_out) {
+        // batches 1 and 2 write G(A sA, B sB)(i,j): reduce over the wave
+        for (int s = 0; s < 4; ++s) {
+          double re = wave_sum(accAB[s][0]);
+          double im = wave_sum(accAB[s][1]);
+          if (lane == 0 && (re != 0.0 || im != 0.0)) {
+            int orr = ab.fa[s >> 1] + ib, occ = ab.fb[s & 1] + jb;
+            atomic_add2(&a.gdir[(int64_t)occ * nd + orr], re, im);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Hermitian completion of GDIR/GXCH (CLMFAST :12166-12205): thread per
+// (i,j), i <= j < nskp, skipping pairs inside one (centre, kappa, |mj|) block.
+__global__ void complete_kernel(CompleteArgs a) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int n = a.nskp;
+  if (g >= (int64_t)n * n) return;
+  int i = (int)(g % n), j = (int)(g / n);
+  if (i > j) return;
+  if (a.intsym && a.icn[i] == a.icn[j] && a.kqn[i] == a.kqn[j] && abs(a.mqn[i]) == abs(a.mqn[j]))
+    return;
+  int ndim = a.ndim;
+  int k = i + n, l = j + n;
+  auto comp = [&](double2 *m, int r, int c) {
+    double2 &x = m[(int64_t)c * ndim + r];
+    double2 &y = m[(int64_t)r * ndim + c];
+    x.x += y.x;
+    x.y -= y.y;
+    y.x = x.x;
+    y.y = -x.y;
+  };
+  comp(a.gdir, i, j);
+  comp(a.gxch, i, j);
+  comp(a.gdir, k, l);
+  comp(a.gxch, k, l);
+  comp(a.gxch, i, l);
+  comp(a.gxch, k, j);
+}
+
+__global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) {
+    a[g] = make_double2(0.0, 0.0);
+    if (b) b[g] = make_double2(0.0, 0.0);
+  }
+  if (g == 0 && counter) *counter = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+hipError_t upload_constants() {
+  uint16_t tuv[ntuv(kMaxLfl)];
+  // Hermite order: by L, then t ascending, then u ascending (CARTIND :2678-2700)
+  int n = 0;
+  for (int L = 0; L <= kMaxLfl; ++L)
+    for (int t = 0; t <= L; ++t)
+      for (int u = 0; u <= L - t; ++u) {
+        int v = L - t - u;
+        tuv[n++] = (uint16_t)((t << 10) | (u << 5) | v);
+      }
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tuv), tuv, sizeof(tuv));
+  if (e != hipSuccess) return e;
+  double rcp[2 * kMaxLfl + 2 * 60 + 4];
+  for (int k = 0; k < (int)(sizeof(rcp) / sizeof(double)); ++k) rcp[k] = 1.0 / (2.0 * k + 1.0);
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_rcp), rcp, sizeof(rcp));
+}
+
+hipError_t launch_etab(const EtabArgs &a, hipStream_t s) {
+  if (a.nitems == 0) return hipSuccess;
+  int bs = 128;
+  int64_t nb = (a.nitems + bs - 1) / bs;
+  hipLaunchKernelGGL(etab_kernel, dim3((unsigned)nb), dim3(bs), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_eflag(const EflagArgs &a, hipStream_t s) {
+  if (a.npairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(eflag_kernel, dim3(a.npairs), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter, hipStream_t s) {
+  int bs = 256;
+  int64_t nb = (n + bs - 1) / bs;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(bs), 0, s, x, n, y, counter);
+  return hipGetLastError();
+}
+hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(fock_kernel, dim3(nblocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_complete(const CompleteArgs &a, hipStream_t s) {
+  int64_t n = (int64_t)a.nskp * a.nskp;
+  int bs = 256;
+  hipLaunchKernelGGL(complete_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace bdf

@@ -1,0 +1,556 @@
+// plan.cpp -- host side of the engine: basis labels, spinor E-coefficient term
+// tables and the Fock-build plan (which block quartets are evaluated and how
+// each RR batch is digested).  The plan reproduces the decisions of the
+// reference's CLMFAST loop nest (bertha-scf.f:11630-12158); the arithmetic
+// happens in kernels.hip.
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <map>
+#include <tuple>
+
+namespace bdf {
+
+static int lval(int k) { return k < 0 ? -k - 1 : k; }
+static int isgn(int x) { return x >= 0 ? 1 : -1; }
+static double m1pow(int n) { return (std::abs(n) % 2 == 0) ? 1.0 : -1.0; }
+
+// ---------------------------------------------------------------------------
+// basis and labels
+// ---------------------------------------------------------------------------
+int Basis::kappa_of(int c, int k) const {  // k is 1-based KAPA index
+  int l = k / 2;
+  return (k % 2 == 0) ? l : -l - 1;
+}
+
+std::string build_basis(const bdf_system &sys, Basis &B) {
+  if (sys.ncnt <= 0 || !sys.xyz || !sys.lmax || !sys.nfnc || !sys.exps)
+    return "bdf_system: null or empty arrays";
+  B.ncnt = sys.ncnt;
+  B.xyz.assign(sys.xyz, sys.xyz + 3 * sys.ncnt);
+  B.lmax.assign(sys.lmax, sys.lmax + sys.ncnt);
+  B.nf.assign(sys.ncnt * (kMaxL + 1), 0);
+  B.expoff.assign(sys.ncnt * (kMaxL + 1), 0);
+  B.exps.clear();
+  B.ndim = 0;
+  for (int c = 0; c < sys.ncnt; ++c) {
+    if (sys.lmax[c] < 0 || sys.lmax[c] > kMaxL) return "lmax outside 0..3";
+    if (sys.lmax[c] >= sys.lstride) return "lmax >= lstride";
+    for (int l = 0; l <= sys.lmax[c]; ++l) {
+      int n = sys.nfnc[c * sys.lstride + l];
+      if (n <= 0 || n > kMaxPrim || n > sys.mbs) return "shell size outside 1..32";
+      B.nf[c * (kMaxL + 1) + l] = n;
+      B.expoff[c * (kMaxL + 1) + l] = (int)B.exps.size();
+      for (int i = 0; i < n; ++i) {
+        double e = sys.exps[((long)c * sys.lstride + l) * sys.mbs + i];
+        if (!(e > 0.0)) return "non-positive exponent";
+        B.exps.push_back(e);
+      }
+      B.ndim += 4 * (2 * l + 1) * n;
+    }
+  }
+  B.nskp = B.ndim / 2;
+  // FOCKIND (bertha-scf.f:2605-2645): per centre, per |mj| shell, first all
+  // -|mj| blocks in kappa order, then all +|mj| blocks.
+  B.lrge.assign(sys.ncnt * 16 * 16, -1);
+  B.labicn.assign(B.nskp, 0);
+  B.labkqn.assign(B.nskp, 0);
+  B.labmqn.assign(B.nskp, 0);
+  int ilst = 0;
+  for (int c = 0; c < sys.ncnt; ++c) {
+    int nkap = 2 * B.lmax[c] + 1;
+    for (int imv = 0; imv <= B.lmax[c]; ++imv) {
+      int mqn = 2 * imv + 1;
+      for (int half = 0; half < 2; ++half)
+        for (int k = 1; k <= nkap; ++k) {
+          int kq = B.kappa_of(c, k);
+          int n = B.nfn(c, lval(kq));
+          if (2 * std::abs(kq) - 1 >= mqn) {
+            B.lrge[(c * 16 + k) * 16 + mqn + half] = ilst;
+            for (int i = 0; i < n; ++i) {
+              B.labicn[ilst + i] = c + 1;
+              B.labkqn[ilst + i] = kq;
+              B.labmqn[ilst + i] = half ? mqn : -mqn;
+            }
+            ilst += n;
+          }
+        }
+    }
+  }
+  // normalisation of large and small radial functions (RNORM1,
+  // bertha-scf.f:37770-37800): N^2 = 2 (2a)^(l+3/2) / Gamma(l+3/2) and
+  // 2 (2a)^(l+1/2) / Gamma(l+5/2).
+  B.normL.assign(B.exps.size(), 0.0);
+  B.normS.assign(B.exps.size(), 0.0);
+  for (int c = 0; c < sys.ncnt; ++c)
+    for (int l = 0; l <= B.lmax[c]; ++l)
+      for (int i = 0; i < B.nfn(c, l); ++i) {
+        int o = B.expo(c, l) + i;
+        double le = std::log(2.0 * B.exps[o]);
+        B.normL[o] = std::exp(0.5 * (std::log(2.0) - std::lgamma(l + 1.5) + (l + 1.5) * le));
+        B.normS[o] = std::exp(0.5 * (std::log(2.0) - std::lgamma(l + 2.5) + (l + 0.5) * le));
+      }
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+// solid harmonics and spinor coupling
+// ---------------------------------------------------------------------------
+// Unnormalised solid harmonics with the Schiff phase, S[l,m](x,y,z), built by
+// the same recurrences the reference applies to Hermite expansions (Saunders
+// 1983 Eq. 64, bertha-scf.f:37105-37566): S[0,0] = 1,
+// S[l+1,+-(l+1)] = (2l+1)(x +- iy) S[l,+-l], S[|m|+1,m] = (2|m|+1) z S[|m|,m],
+// (l-|m|+1) S[l+1,m] = (2l+1) z S[l,m] - (l+|m|) r^2 S[l-1,m].
+namespace {
+constexpr int D = kMaxPoly + 3;
+struct Poly {
+  std::complex<double> c[D][D][D];
+  Poly() { clear(); }
+  void clear() {
+    for (auto &a : c) for (auto &b : a) for (auto &v : b) v = 0.0;
+  }
+};
+Poly mul_lin(const Poly &p, std::complex<double> cx, std::complex<double> cy,
+             std::complex<double> cz) {
+  Poly o;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) {
+        auto v = p.c[i][j][k];
+        if (v == 0.0) continue;
+        if (i + 1 < D) o.c[i + 1][j][k] += cx * v;
+        if (j + 1 < D) o.c[i][j + 1][k] += cy * v;
+        if (k + 1 < D) o.c[i][j][k + 1] += cz * v;
+      }
+  return o;
+}
+Poly mul_r2(const Poly &p) {
+  Poly o;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) {
+        auto v = p.c[i][j][k];
+        if (v == 0.0) continue;
+        if (i + 2 < D) o.c[i + 2][j][k] += v;
+        if (j + 2 < D) o.c[i][j + 2][k] += v;
+        if (k + 2 < D) o.c[i][j][k + 2] += v;
+      }
+  return o;
+}
+Poly axpy(double a, const Poly &x, double b, const Poly &y) {
+  Poly o;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) o.c[i][j][k] = a * x.c[i][j][k] + b * y.c[i][j][k];
+  return o;
+}
+Poly schiff(int l, int m) {
+  Poly p;
+  p.c[0][0][0] = 1.0;
+  double sg = m >= 0 ? 1.0 : -1.0;
+  for (int lq = 0; lq < std::abs(m); ++lq)
+    p = mul_lin(p, 2.0 * lq + 1.0, std::complex<double>(0.0, sg * (2.0 * lq + 1.0)), 0.0);
+  if (l <= std::abs(m)) return p;
+  int am = std::abs(m);
+  Poly prev = p;
+  Poly cur = mul_lin(p, 0.0, 0.0, 2.0 * am + 1.0);
+  for (int ls = am + 1; ls <= l - 1; ++ls) {
+    double r1 = (2.0 * ls + 1.0) / (ls - am + 1.0);
+    double r2 = -(double)(ls + am) / (ls - am + 1.0);
+    Poly nxt = axpy(r1, mul_lin(cur, 0.0, 0.0, 1.0), r2, mul_r2(prev));
+    prev = cur;
+    cur = nxt;
+  }
+  return cur;
+}
+}  // namespace
+
+int Etabs::poly_id(int l, int m, int nr2) {
+  auto key = std::make_tuple(l, m, nr2);
+  auto it = poly_index.find(key);
+  if (it != poly_index.end()) return it->second;
+  Poly p = schiff(l, m);
+  if (nr2) p = mul_r2(p);
+  int begin = (int)monos.size();
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) {
+        auto v = p.c[i][j][k];
+        if (std::abs(v) == 0.0) continue;
+        Mono mo;
+        mo.re = v.real(); mo.im = v.imag();
+        mo.i = (int8_t)i; mo.j = (int8_t)j; mo.k = (int8_t)k; mo.pad = 0;
+        monos.push_back(mo);
+      }
+  int id = begin * 65536 + ((int)monos.size() - begin);
+  poly_index[key] = id;
+  return id;
+}
+
+// Angular factor of conj(Y[l1,m1]) Y[l2,m2] with the Condon-Shortley phase
+// (ESGTF, bertha-scf.f:36948-36964).
+static double ang(int l1, int m1, int l2, int m2) {
+  double ph1 = m1pow((m1 + std::abs(m1)) / 2), ph2 = m1pow((m2 + std::abs(m2)) / 2);
+  double f1 = std::tgamma(l1 - std::abs(m1) + 1.0) / std::tgamma(l1 + std::abs(m1) + 1.0);
+  double f2 = std::tgamma(l2 - std::abs(m2) + 1.0) / std::tgamma(l2 + std::abs(m2) + 1.0);
+  return ph1 * ph2 * (0.25 / M_PI) * std::sqrt((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0) * f1 * f2);
+}
+
+// Clebsch-Gordan weights of the two spin components of a 2-spinor with
+// kappa kq and 2*mj = mq (EQLL bertha-scf.f:34722-34734); the small
+// component of kappa has the coefficients of -kappa (EQSS 36012-36024).
+static void cg(int kq, int mq, bool small, double &up, double &lo) {
+  int jq = 2 * std::abs(kq) - 1;
+  bool neg = kq < 0;
+  if (small) neg = !neg;
+  if (neg) {
+    up = std::sqrt((double)(jq + mq) / (2.0 * jq));
+    lo = std::sqrt((double)(jq - mq) / (2.0 * jq));
+  } else {
+    up = -std::sqrt((double)(jq - mq + 2) / (2.0 * jq + 4));
+    lo = std::sqrt((double)(jq + mq + 2) / (2.0 * jq + 4));
+  }
+}
+
+// Terms of the overlap density of spinors (ka, mqa) and (kb, mqb) in
+// polynomial form, for the large (small=false) or kinetically balanced small
+// (small=true) components.  Small components carry the radial factors of
+// sigma.p acting on r^l exp(-a r^2): -2a r^(l+1) for kappa<0 and
+// (2l+1) r^(l-1) - 2a r^(l+1) for kappa>0 (EQSS cases 1-4, 35996-36860).
+void Etabs::spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begin,
+                         int &count) {
+  auto key = std::make_tuple(ka, mqa, kb, mqb, (int)small);
+  auto it = term_index.find(key);
+  if (it != term_index.end()) {
+    begin = it->second.first;
+    count = it->second.second;
+    return;
+  }
+  begin = (int)terms.size();
+  int la = lval(ka), lb = lval(kb);
+  double cau, cal, cbu, cbl;
+  cg(ka, mqa, small, cau, cal);
+  cg(kb, mqb, small, cbu, cbl);
+  for (int s = 0; s < 2; ++s) {
+    int mla = s == 0 ? (mqa - 1) / 2 : (mqa + 1) / 2;
+    int mlb = s == 0 ? (mqb - 1) / 2 : (mqb + 1) / 2;
+    double cab = s == 0 ? cau * cbu : cal * cbl;
+    if (std::fabs(cab) < 1.0e-14) continue;
+    int l1 = la, l2 = lb;
+    if (small) {
+      l1 = ka < 0 ? la + 1 : la - 1;
+      l2 = kb < 0 ? lb + 1 : lb - 1;
+    }
+    if (std::abs(mla) > l1 || std::abs(mlb) > l2) continue;
+    double w0 = cab * ang(l1, mla, l2, mlb);
+    auto add = [&](double w, int pa, int pb, int ra, int rb) {
+      ETerm t;
+      t.w = w0 * w;
+      t.pa = (int16_t)pa;
+      t.pb = (int16_t)pb;
+      t.polyA = poly_id(l1, -mla, ra);
+      t.polyB = poly_id(l2, mlb, rb);
+      terms.push_back(t);
+    };
+    if (!small) {
+      add(1.0, 0, 0, 0, 0);
+    } else {
+      double t20 = -2.0 * (2 * lb + 1), t02 = -2.0 * (2 * la + 1), t00 = (2.0 * la + 1) * (2.0 * lb + 1);
+      if (ka < 0 && kb < 0) {
+        add(4.0, 1, 1, 0, 0);
+      } else if (ka < 0 && kb > 0) {
+        add(t20, 1, 0, 0, 0);
+        add(4.0, 1, 1, 0, 1);
+      } else if (ka > 0 && kb < 0) {
+        add(t02, 0, 1, 0, 0);
+        add(4.0, 1, 1, 1, 0);
+      } else {
+        add(t00, 0, 0, 0, 0);
+        add(t20, 1, 0, 1, 0);
+        add(t02, 0, 1, 0, 1);
+        add(4.0, 1, 1, 1, 1);
+      }
+    }
+  }
+  count = (int)terms.size() - begin;
+  term_index[key] = {begin, count};
+}
+
+// E0LLGN/E0SSGN (bertha-scf.f:33378, 33511): one table per ordered pair of
+// (centre, kappa, |mj|) blocks and component type; E11 = (-|mA|,-|mB|),
+// E21 = (+|mA|,-|mB|) (EQLLMK 34003-34034).
+void build_etables(const Basis &B, Etabs &T) {
+  T.pairs.clear();
+  T.pair_index.clear();
+  int64_t off = 0;
+  for (int small = 0; small < 2; ++small)
+    for (int a = 0; a < B.ncnt; ++a)
+      for (int b = 0; b < B.ncnt; ++b)
+        for (int ka = 1; ka <= 2 * B.lmax[a] + 1; ++ka)
+          for (int kb = 1; kb <= 2 * B.lmax[b] + 1; ++kb) {
+            int kqa = B.kappa_of(a, ka), kqb = B.kappa_of(b, kb);
+            int la = lval(kqa), lb = lval(kqb);
+            for (int ma = 1; ma <= std::abs(kqa); ++ma)
+              for (int mb = 1; mb <= std::abs(kqb); ++mb) {
+                EPair p;
+                p.ax = B.xyz[3 * a]; p.ay = B.xyz[3 * a + 1]; p.az = B.xyz[3 * a + 2];
+                p.bx = B.xyz[3 * b]; p.by = B.xyz[3 * b + 1]; p.bz = B.xyz[3 * b + 2];
+                p.expA = B.expo(a, la); p.expB = B.expo(b, lb);
+                p.nA = B.nfn(a, la); p.nB = B.nfn(b, lb);
+                p.lam = la + lb + (small ? 2 : 0);
+                p.small = small;
+                p.la = la; p.lb = lb;
+                p.off = off;
+                T.spinor_terms(kqa, -(2 * ma - 1), kqb, -(2 * mb - 1), small, p.term11, p.nterm11);
+                T.spinor_terms(kqa, (2 * ma - 1), kqb, -(2 * mb - 1), small, p.term21, p.nterm21);
+                off += (int64_t)ntuv(p.lam) * p.nA * p.nB;
+                T.pair_index[Etabs::key(a, b, ka, kb, ma, mb, small)] = (int)T.pairs.size();
+                T.pairs.push_back(p);
+              }
+          }
+  T.total = off;
+  // E(CD) column masks: 4 bytes per tuv per pair
+  T.flag_off.assign(T.pairs.size(), 0);
+  int64_t fo = 0;
+  for (size_t i = 0; i < T.pairs.size(); ++i) {
+    T.flag_off[i] = fo;
+    fo += 4 * ntuv(T.pairs[i].lam);
+  }
+  T.nflag = fo;
+}
+
+// ---------------------------------------------------------------------------
+// the Fock-build plan: CLMFAST's loop nest (bertha-scf.f:11630-12158)
+// ---------------------------------------------------------------------------
+// ISCF (bertha-scf.f:11555-11561) as bit masks over batches 1..11
+static const int kISCF[6] = {
+    0b00011111111, 0b00001110011, 0b00001011101, 0b00000110111, 0b00000010101, 0b00000010001};
+static const int kMAPTTTT[4][4] = {{1, 0, 0, 2}, {0, 5, 6, 0}, {0, 7, 8, 0}, {3, 0, 0, 4}};
+
+static int ncntrs(int a, int b, int c, int d) {
+  int n = 1;
+  if (b != a) ++n;
+  if (c != a && c != b) ++n;
+  if (d != a && d != b && d != c) ++n;
+  return n;
+}
+
+std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
+                       int isym, int intsym, int racah1, int ilev, Plan &P) {
+  P = Plan();
+  P.ndim = B.ndim;
+  P.nskp = B.nskp;
+  P.isym = isym;
+  P.intsym = intsym;
+  P.racah1 = racah1;
+  P.ilev = ilev;
+  // IQ block index (CLMFAST :11606-11626)
+  std::map<std::tuple<int, int, int>, int> iqidx;
+  int cnt = 0;
+  for (int c = 0; c < B.ncnt; ++c)
+    for (int k = 1; k <= 2 * B.lmax[c] + 1; ++k) {
+      int kq = B.kappa_of(c, k);
+      for (int mj = 1; mj <= 2 * std::abs(kq) - 1; mj += 2) iqidx[{c, kq, mj}] = ++cnt;
+    }
+  auto kmask_at = [&](int pair, int tuv, int comp) -> int {
+    return eflags[T.flag_off[pair] + 4 * tuv + comp];
+  };
+  for (int ca = 0; ca < B.ncnt; ++ca) {
+    int ncb = intsym ? ca + 1 : B.ncnt;
+    for (int cb = 0; cb < ncb; ++cb)
+      for (int cc = 0; cc < B.ncnt; ++cc)
+        for (int cd = 0; cd < B.ncnt; ++cd) {
+          int mcnt = ncntrs(ca, cb, cc, cd);
+          if (mcnt == 1 && racah1) continue;
+          int ic[4] = {ca, cb, cc, cd};
+          for (int la = 0; la <= B.lmax[ca]; ++la)
+          for (int lb = 0; lb <= B.lmax[cb]; ++lb)
+          for (int lc = 0; lc <= B.lmax[cc]; ++lc)
+          for (int ld = 0; ld <= B.lmax[cd]; ++ld) {
+            if (mcnt == 1 && ((la + lb) % 2) != ((lc + ld) % 2)) continue;
+            int lq[4] = {la, lb, lc, ld};
+            ClassDesc cl;
+            for (int k = 0; k < 4; ++k) {
+              for (int x = 0; x < 3; ++x) cl.xyz[k][x] = B.xyz[3 * ic[k] + x];
+              cl.exp[k] = B.expo(ic[k], lq[k]);
+              cl.nbas[k] = B.nfn(ic[k], lq[k]);
+            }
+            int lsum = la + lb + lc + ld;
+            if (mcnt == 1) cl.lamfl = lsum + 4;
+            else cl.lamfl = ilev == 1 ? lsum : (ilev == 2 ? lsum + 2 : lsum + 4);
+            cl.maxcd = cl.nbas[2] * cl.nbas[3];
+            cl.mcnt = mcnt;
+            cl.pad = 0;
+            cl.ab_begin = (int)P.abs.size();
+            // aggregated E(CD) flags of the (C,D,lc,ld) class (E0LLCD0/E0SSCD0)
+            int ntss = ntuv(lc + ld + 2);
+            std::vector<int> agg[2];
+            for (int s = 0; s < 2; ++s) {
+              agg[s].assign(4 * ntss, 0);
+              // at stage 1 the reference never refreshes the SS lists
+              // (E0SSCD0 is skipped, bertha-scf.f:11742): a fresh run sees
+              // them zero, so one-centre G built from SS flags vanish
+              if (s == 1 && ilev == 1) continue;
+              for (int nc = (lc == 0); nc <= 1; ++nc)
+                for (int nd = (ld == 0); nd <= 1; ++nd) {
+                  int kc = 2 * lc + nc, kd = 2 * ld + nd;
+                  for (int mc = 1; mc <= lc + nc; ++mc)
+                    for (int md = 1; md <= ld + nd; ++md) {
+                      int pr = T.pair_index.at(Etabs::key(cc, cd, kc, kd, mc, md, s));
+                      int nt = ntuv(lc + ld + (s ? 2 : 0));
+                      for (int t = 0; t < nt; ++t)
+                        for (int c = 0; c < 4; ++c) agg[s][4 * t + c] += kmask_at(pr, t, c);
+                    }
+                }
+            }
+            for (int na = (la == 0); na <= 1; ++na) {
+              int ka = 2 * la + na, kqa = B.kappa_of(ca, ka), jqa = 2 * std::abs(kqa) - 1;
+              for (int nb = (lb == 0); nb <= 1; ++nb) {
+                int kb = 2 * lb + nb, kqb = B.kappa_of(cb, kb), jqb = 2 * std::abs(kqb) - 1;
+                for (int ma = 1; ma <= std::abs(kqa); ++ma)
+                  for (int mb = 1; mb <= std::abs(kqb); ++mb) {
+                    int mqa = 2 * ma - 1, mqb = 2 * mb - 1;
+                    for (int it1 = 4; it1 >= 1; it1 -= 3) {
+                      int naddab = it1 == 1 ? 0 : B.nskp;
+                      ABCombo ab;
+                      int pab_pair = T.pair_index.at(Etabs::key(ca, cb, ka, kb, ma, mb, it1 == 4));
+                      ab.eoff = T.pairs[pab_pair].off;
+                      ab.ntab = ntuv(la + lb + (it1 == 4 ? 2 : 0));
+                      ab.fa[0] = B.lrge_at(ca, ka, 2 * ma - 1) + naddab;
+                      ab.fa[1] = B.lrge_at(ca, ka, 2 * ma) + naddab;
+                      ab.fb[0] = B.lrge_at(cb, kb, 2 * mb - 1) + naddab;
+                      ab.fb[1] = B.lrge_at(cb, kb, 2 * mb) + naddab;
+                      ab.nb = cl.nbas[1];
+                      ab.cd_begin = (int)P.cds.size();
+                      ab.pad = 0;
+                      int first_it2 = 0;
+                      for (int nc = (lc == 0); nc <= 1; ++nc) {
+                        int kc = 2 * lc + nc, kqc = B.kappa_of(cc, kc), jqc = 2 * std::abs(kqc) - 1;
+                        for (int nd = (ld == 0); nd <= 1; ++nd) {
+                          int kd = 2 * ld + nd, kqd = B.kappa_of(cd, kd), jqd = 2 * std::abs(kqd) - 1;
+                          if (mcnt == 1) {
+                            int nui = std::max(std::abs(jqa - jqb) / 2, std::abs(jqc - jqd) / 2);
+                            int nuf = std::min((jqa + jqb) / 2, (jqc + jqd) / 2);
+                            if (nui > nuf) continue;
+                            bool selk = false;
+                            for (int nu = nui; nu <= nuf; ++nu)
+                              if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) selk = true;
+                            if (!selk) continue;
+                          }
+                          for (int mc = 1; mc <= std::abs(kqc); ++mc)
+                            for (int md = 1; md <= std::abs(kqd); ++md) {
+                              int mqc = 2 * mc - 1, mqd = 2 * md - 1;
+                              if (isym == 1 || isym == 2) {
+                                bool ok = (mqa == mqb && mqc == mqd) || (mqa == mqc && mqb == mqd) ||
+                                          (mqa == mqd && mqb == mqc);
+                                if (!ok) continue;
+                              }
+                              if (mcnt == 1) {
+                                bool selm = false;
+                                for (int sa = -1; sa <= 1; sa += 2)
+                                  for (int sb = -1; sb <= 1; sb += 2)
+                                    for (int sc = -1; sc <= 1; sc += 2)
+                                      for (int sd = -1; sd <= 1; sd += 2)
+                                        if (sa * mqa - sb * mqb == sd * mqd - sc * mqc) selm = true;
+                                if (!selm) continue;
+                              }
+                              int iq1 = iqidx.at({ca, kqa, mqa}), iq2 = iqidx.at({cb, kqb, mqb});
+                              int iq3 = iqidx.at({cc, kqc, mqc}), iq4 = iqidx.at({cd, kqd, mqd});
+                              int iql = iq1 * (iq1 - 1) / 2 + iq2, iqr = iq3 * (iq3 - 1) / 2 + iq4;
+                              if (intsym && (iq1 < iq2 || iq3 < iq4 || iql < iqr)) continue;
+                              int itscf;
+                              if (iq1 > iq2 && iq3 > iq4 && iql > iqr) itscf = 1;
+                              else if (iq1 > iq2 && iq3 > iq4 && iql == iqr) itscf = 2;
+                              else if (iq1 > iq2 && iq3 == iq4 && iql > iqr) itscf = 3;
+                              else if (iq1 == iq2 && iq3 > iq4 && iql > iqr) itscf = 4;
+                              else if (iq1 == iq2 && iq3 == iq4 && iql > iqr) itscf = 5;
+                              else if (iq1 == iq2 && iq3 == iq4 && iql == iqr) itscf = 6;
+                              else {
+                                if (intsym) continue;
+                                itscf = 6;
+                              }
+                              int iflg = kISCF[itscf - 1];
+                              if (iq2 == iq3 && (itscf == 1 || itscf == 3 || itscf == 4)) iflg |= 1 << 8;
+                              if (itscf == 1) {
+                                if (iq1 == iq3) iflg |= 1 << 9;
+                                else if (iq2 == iq4) iflg |= 1 << 10;
+                              }
+                              int sab = isgn(kqa * kqb), scd = isgn(kqc * kqd);
+                              for (int it2 = 4; it2 >= 1; it2 -= 3) {
+                                int itt = kMAPTTTT[it1 - 1][it2 - 1];
+                                if (ilev == 1 && itt > 1 && mcnt != 1) continue;
+                                if (ilev == 2 && itt > 3 && mcnt != 1) continue;
+                                int naddcd = it2 == 1 ? 0 : B.nskp;
+                                CDCall call;
+                                int pcd_pair = T.pair_index.at(Etabs::key(cc, cd, kc, kd, mc, md, it2 == 4));
+                                call.eoff = T.pairs[pcd_pair].off;
+                                call.ntcd = ntuv(lc + ld + (it2 == 4 ? 2 : 0));
+                                call.kmask = (int)P.masks.size();
+                                for (int t = 0; t < call.ntcd; ++t)
+                                  for (int c = 0; c < 4; ++c) P.masks.push_back((uint8_t)kmask_at(pcd_pair, t, c));
+                                call.fc[0] = B.lrge_at(cc, kc, 2 * mc - 1) + naddcd;
+                                call.fc[1] = B.lrge_at(cc, kc, 2 * mc) + naddcd;
+                                call.fd[0] = B.lrge_at(cd, kd, 2 * md - 1) + naddcd;
+                                call.fd[1] = B.lrge_at(cd, kd, 2 * md) + naddcd;
+                                call.iflg = iflg;
+                                call.pab[0] = (int8_t)(sab * m1pow((-mqa + mqb) / 2));
+                                call.pab[1] = (int8_t)(sab * m1pow((mqa + mqb) / 2));
+                                call.pcd[0] = (int8_t)(scd * m1pow((-mqc + mqd) / 2));
+                                call.pcd[1] = (int8_t)(scd * m1pow((mqc + mqd) / 2));
+                                if (!first_it2) first_it2 = it2;
+                                P.cds.push_back(call);
+                                P.quartets += (double)cl.nbas[0] * cl.nbas[1] * cl.maxcd;
+                              }
+                            }
+                        }
+                      }
+                      ab.cd_end = (int)P.cds.size();
+                      if (ab.cd_end == ab.cd_begin) continue;
+                      // G is built once per (ij) with the flags of the first
+                      // call's component type (ERIFAST IGAB, :12674-12799)
+                      int gs = first_it2 == 4 ? 1 : 0;
+                      ab.ntg = ntuv(lc + ld + (gs ? 2 : 0));
+                      ab.gmask = (int)P.masks.size();
+                      for (int t = 0; t < ab.ntg; ++t) {
+                        const int *g = &agg[gs][4 * t];
+                        int all = g[0] + g[1] + g[2] + g[3];
+                        for (int c = 0; c < 4; ++c)
+                          P.masks.push_back((uint8_t)(all != 0 && (isym <= 1 || g[c] != 0)));
+                      }
+                      P.max_ntg = std::max(P.max_ntg, ab.ntg);
+                      P.abs.push_back(ab);
+                    }
+                  }
+              }
+            }
+            cl.ab_end = (int)P.abs.size();
+            if (cl.ab_end == cl.ab_begin) continue;
+            int cidx = (int)P.classes.size();
+            P.classes.push_back(cl);
+            P.max_lfl = std::max(P.max_lfl, cl.lamfl);
+            P.max_maxcd = std::max(P.max_maxcd, cl.maxcd);
+            for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij) P.items.push_back({cidx, ij});
+          }
+        }
+  }
+  // largest work first, so the persistent kernel's queue drains evenly
+  std::vector<double> cost(P.classes.size(), 0.0);
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    double w = (double)ntuv(cl.lamfl) * cl.maxcd;
+    for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
+      const ABCombo &ab = P.abs[a];
+      w += (double)ab.ntab * ab.ntg * cl.maxcd;
+      for (int k = ab.cd_begin; k < ab.cd_end; ++k) w += 8.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
+    }
+    cost[c] = w;
+  }
+  std::stable_sort(P.items.begin(), P.items.end(),
+                   [&](const WorkItem &x, const WorkItem &y) { return cost[x.cls] > cost[y.cls]; });
+  return "";
+}
+
+}  // namespace bdf

@@ -1,0 +1,51 @@
+// plan.h -- host-side basis, E-coefficient term tables and Fock-build plan.
+#pragma once
+#include <map>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../../include/bertha_amd.h"
+#include "engine.h"
+
+namespace bdf {
+
+struct Basis {
+  int ncnt = 0, ndim = 0, nskp = 0;
+  std::vector<double> xyz;
+  std::vector<int> lmax;
+  std::vector<int> nf;      // [c][kMaxL+1]
+  std::vector<int> expoff;  // [c][kMaxL+1]
+  std::vector<double> exps, normL, normS;
+  std::vector<int> lrge;    // [c][16][16]: FOCKIND block offsets
+  std::vector<int> labicn, labkqn, labmqn;
+  int kappa_of(int c, int k) const;
+  int nfn(int c, int l) const { return nf[c * (kMaxL + 1) + l]; }
+  int expo(int c, int l) const { return expoff[c * (kMaxL + 1) + l]; }
+  int lrge_at(int c, int k, int m) const { return lrge[(c * 16 + k) * 16 + m]; }
+};
+
+struct Etabs {
+  std::vector<Mono> monos;
+  std::vector<ETerm> terms;
+  std::vector<EPair> pairs;
+  std::vector<int64_t> flag_off;
+  int64_t total = 0;  // double4 words
+  int64_t nflag = 0;  // bytes
+  std::map<std::tuple<int, int, int>, int> poly_index;
+  std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> term_index;
+  std::map<long, int> pair_index;
+  static long key(int a, int b, int ka, int kb, int ma, int mb, int small) {
+    return ((((((long)a * 64 + b) * 16 + ka) * 16 + kb) * 8 + ma) * 8 + mb) * 2 + small;
+  }
+  int poly_id(int l, int m, int nr2);
+  void spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begin, int &count);
+};
+
+std::string build_basis(const bdf_system &sys, Basis &B);
+void build_etables(const Basis &B, Etabs &T);
+std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
+                       int isym, int intsym, int racah1, int ilev, Plan &P);
+
+}  // namespace bdf

@@ -1,0 +1,135 @@
+/*
+ * bertha_amd.h -- C ABI of the MI355X two-electron engine.
+ *
+ * The reference has no plugin or FFI surface for this path: the SCF driver
+ * HFSCF calls two parameterless Fortran subroutines that talk through COMMON
+ * blocks (bertha-scf.f:5401 CALL CLMFAST, :5441 CALL BRTFAST, set up once by
+ * :5287 CALL EQSAVE).  Each entry point below replaces one of those calls with
+ * explicit arguments; include/bertha_amd_mod.f90 is the ISO_C_BINDING module a
+ * Fortran driver uses to reach them.
+ *
+ * Conventions
+ *   - Complex matrices are double _Complex (re,im interleaved), column-major,
+ *     leading dimension ld >= ndim, in the reference's FOCKIND basis order
+ *     (bertha-scf.f:2582-2648): large-component block rows 1..nskp, small
+ *     component rows nskp+1..ndim.
+ *   - Every call returns 0 on success or a negative BDF_E* code; it never
+ *     exits the process (the reference STOPs, bertha-scf.f:12348-12352).
+ *     bdf_last_error() describes the last failure on the calling thread.
+ *   - The engine owns its device tables; it never keeps a host pointer after
+ *     a call returns.  One engine is driven by one host thread at a time.
+ */
+#ifndef BERTHA_AMD_H
+#define BERTHA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BDF_OK 0
+#define BDF_EINVAL -1   /* bad argument */
+#define BDF_ENOMEM -2   /* device or host allocation failed */
+#define BDF_EDEVICE -3  /* HIP runtime / kernel failure */
+#define BDF_ELIMIT -4   /* outside the engine's compiled limits */
+#define BDF_EUNSUPP -5  /* option not implemented by this engine */
+
+/* molecular shape -> integral symmetry class, as CLMFAST's SHAPE test
+   (bertha-scf.f:11573-11579): ATOMIC=2, DIATOM/LINEAR=1, otherwise 0 */
+#define BDF_SHAPE_NOSYMM 0
+#define BDF_SHAPE_LINEAR 1
+#define BDF_SHAPE_ATOMIC 2
+
+/* Hamiltonians, as CARDIN's HMLT (bertha-scf.f:218-225) */
+#define BDF_HMLT_DHFR 1 /* Dirac-Coulomb */
+#define BDF_HMLT_DHFB 2 /* Dirac-Coulomb-Breit */
+
+/* precision modes */
+#define BDF_PREC_FP64 0  /* every quartet in FP64 */
+#define BDF_PREC_MIXED 1 /* Schwarz-split FP32/FP64 (see bdf_options) */
+
+/* Basis and geometry, as CARDIN fills COMMON /BSET/ (bertha-scf.f:180-182):
+   centre c has angular momenta 0..lmax[c]; nfnc[c*lstride + l] exponents for
+   shell l, stored at exps[(c*lstride + l)*mbs + i]. */
+typedef struct {
+  int ncnt;
+  const double *xyz;   /* [ncnt][3] bohr */
+  const int *lmax;     /* [ncnt] */
+  const int *nfnc;     /* [ncnt][lstride] */
+  const double *exps;  /* [ncnt][lstride][mbs] */
+  int lstride;
+  int mbs;
+  int shape;           /* BDF_SHAPE_* */
+  int hmlt;            /* BDF_HMLT_* */
+} bdf_system;
+
+/* Toggles the reference hard-codes in HFSCF (bertha-scf.f:5215-5257) plus
+   the engine's own options. */
+typedef struct {
+  int intsym;          /* INTSYM: permutational symmetry of RR batches (1) */
+  int racah1;          /* RACAH1: skip one-centre quartets (reference default 1;
+                          this engine computes them by MD when 0) */
+  int precision;       /* BDF_PREC_* */
+  double split_threshold; /* MIXED: quartets with Schwarz bound below it in FP32 */
+  int device;          /* HIP device ordinal */
+} bdf_options;
+
+typedef struct bdf_engine bdf_engine;
+
+/* Replaces EQSAVE (bertha-scf.f:32984) plus the one-time label set-up FOCKIND
+   and CARTIND (bertha-scf.f:108-111): builds the E0LL/E0SS coefficient tables
+   and the quartet plan on the device.  ndim/nskp receive NDIM and NSKP. */
+int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out,
+               int *ndim, int *nskp);
+
+/* Replaces CALL CLMFAST (bertha-scf.f:5401; routine at :11489): closed-shell
+   Dirac-Coulomb direct (GDIR) and exchange (GXCH) matrices for the total
+   density DENT at integral-inclusion stage ilev (1..3, COMMON /LSHF/).
+   Outputs are overwritten (zeroed then filled, :11592-11599) and
+   Hermitian-completed as at :12166-12205.  Host arrays, ld >= ndim.
+   deno/qdir/qxch are the open-shell counterparts and must be NULL: only
+   closed shells are supported (BDF_EUNSUPP otherwise). */
+int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent,
+                const double _Complex *deno, int ld, double _Complex *gdir,
+                double _Complex *gxch, double _Complex *qdir,
+                double _Complex *qxch);
+
+/* Same computation on device-resident arrays (ld = ndim), enqueued on the
+   engine's stream and not synchronised: for callers that keep D, GDIR and
+   GXCH in HBM (the SCF loop and the serving path).  No counterpart in the
+   reference, whose matrices live in host COMMON /DENS/ and /MTRX/. */
+int bdf_coulomb_device(bdf_engine *e, int ilev, const void *d_dent,
+                       void *d_gdir, void *d_gxch);
+
+/* Test hook mirroring ERIFAST's output (bertha-scf.f:12269-13132) for one
+   block quartet: centres icnt[4] (1-based), kappa indices kidx[4] (1-based
+   position in KAPA, as KA/KB/KC/KD in CLMFAST :11771-11856), |mj| indices
+   midx[4] (1..|kappa|), primitive pair (ibas,jbas) (1-based), component pair
+   itn = {1|4, 1|4} (LL=1, SS=4) and stage ilev.  rr receives RR(N,1..16) for
+   all maxcd = nC*nD pairs, column-major [16][maxcd]. */
+int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
+                  const int midx[4], int ibas, int jbas, const int itn[2],
+                  int ilev, double _Complex *rr, int maxcd);
+
+/* Copy of one E0LL (itt=1) or E0SS (itt=4) block as the reference stores it
+   (bertha-scf.f:33485-33494): out[comp][tuv][ij], comp = Re E11, Im E11,
+   Re E21, Im E21.  Returns the element count per component in *n. */
+int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb,
+               int ma, int mb, double *out, long cap, long *n);
+
+/* Engine's HIP stream (hipStream_t) for callers that order their own work
+   after bdf_coulomb_device. */
+void *bdf_stream(bdf_engine *e);
+
+/* Per-build counters: number of primitive quartets evaluated and kernel time
+   of the last build in milliseconds (device events). */
+int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms);
+
+void bdf_destroy(bdf_engine *e);
+const char *bdf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

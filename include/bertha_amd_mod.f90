@@ -1,0 +1,90 @@
+! bertha_amd_mod.f90 -- ISO_C_BINDING interfaces to include/bertha_amd.h so a
+! Fortran SCF driver (the reference's HFSCF, bertha-scf.f:5097) can call the
+! engine in place of CALL EQSAVE / CALL CLMFAST without changing its own data:
+! COMPLEX*16 (MDM,MDM) arrays pass straight through with LD = MDM.
+module bertha_amd
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: BDF_OK = 0
+  integer(c_int), parameter :: BDF_SHAPE_NOSYMM = 0, BDF_SHAPE_LINEAR = 1, BDF_SHAPE_ATOMIC = 2
+  integer(c_int), parameter :: BDF_HMLT_DHFR = 1, BDF_HMLT_DHFB = 2
+
+  type, bind(c) :: bdf_system
+    integer(c_int) :: ncnt
+    type(c_ptr) :: xyz, lmax, nfnc, exps
+    integer(c_int) :: lstride, mbs, shape, hmlt
+  end type bdf_system
+
+  type, bind(c) :: bdf_options
+    integer(c_int) :: intsym, racah1, precision
+    real(c_double) :: split_threshold
+    integer(c_int) :: device
+  end type bdf_options
+
+  interface
+    integer(c_int) function bdf_create(sys, opt, eng, ndim, nskp) bind(c, name='bdf_create')
+      import :: c_int, c_ptr, bdf_system, bdf_options
+      type(bdf_system), intent(in) :: sys
+      type(bdf_options), intent(in) :: opt
+      type(c_ptr), intent(out) :: eng
+      integer(c_int), intent(out) :: ndim, nskp
+    end function
+
+    integer(c_int) function bdf_coulomb(eng, ilev, dent, deno, ld, gdir, gxch, qdir, qxch) &
+        bind(c, name='bdf_coulomb')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev, ld
+      complex(c_double_complex), intent(in) :: dent(ld, *)
+      type(c_ptr), value :: deno, qdir, qxch
+      complex(c_double_complex), intent(out) :: gdir(ld, *), gxch(ld, *)
+    end function
+
+    integer(c_int) function bdf_coulomb_device(eng, ilev, d_dent, d_gdir, d_gxch) &
+        bind(c, name='bdf_coulomb_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, d_dent, d_gdir, d_gxch
+      integer(c_int), value :: ilev
+    end function
+
+    integer(c_int) function bdf_eri_batch(eng, icnt, kidx, midx, ibas, jbas, itn, ilev, rr, maxcd) &
+        bind(c, name='bdf_eri_batch')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), intent(in) :: icnt(4), kidx(4), midx(4), itn(2)
+      integer(c_int), value :: ibas, jbas, ilev, maxcd
+      complex(c_double_complex), intent(out) :: rr(maxcd, 16)
+    end function
+
+    integer(c_int) function bdf_etable(eng, itt, ica, icb, ka, kb, ma, mb, out, cap, n) &
+        bind(c, name='bdf_etable')
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: eng
+      integer(c_int), value :: itt, ica, icb, ka, kb, ma, mb
+      real(c_double), intent(out) :: out(*)
+      integer(c_long), value :: cap
+      integer(c_long), intent(out) :: n
+    end function
+
+    type(c_ptr) function bdf_stream(eng) bind(c, name='bdf_stream')
+      import :: c_ptr
+      type(c_ptr), value :: eng
+    end function
+
+    integer(c_int) function bdf_last_stats(eng, quartets, kernel_ms) bind(c, name='bdf_last_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: eng
+      real(c_double), intent(out) :: quartets, kernel_ms
+    end function
+
+    subroutine bdf_destroy(eng) bind(c, name='bdf_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: eng
+    end subroutine
+
+    type(c_ptr) function bdf_last_error() bind(c, name='bdf_last_error')
+      import :: c_ptr
+    end function
+  end interface
+end module bertha_amd

@@ -704,6 +704,18 @@ static void rsweep(double *dst, const double *src, const double *qp, int n, int 
 #undef RS
 }
 
+/* integer power by repeated squaring, as Fortran's X**N with integer N */
+static double powi(double a, int b) {
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return r;
+}
+
 /* RMAKE 31900-32462: rc(n, ntuv(lam)), qp(n,3) = Q-P; scratch rc2 */
 static void rmake_ws(double *rc, double *rc2, double *fs, const double *qp, const double *aph,
                      int n, int lam) {
@@ -719,16 +731,16 @@ static void rmake_ws(double *rc, double *rc2, double *fs, const double *qp, cons
     for (int k = 0; k <= lam; ++k) fs[(long)k * n + m] = f1[k];
   }
   free(f1);
-  for (int m = 0; m < n; ++m) rc[m] = pow(-2.0 * aph[m], lam) * fs[(long)lam * n + m];
+  for (int m = 0; m < n; ++m) rc[m] = powi(-2.0 * aph[m], lam) * fs[(long)lam * n + m];
   int ituvmin = (lam % 2 == 0) ? 1 : 2;
   int ituv = -1;
   for (int ilv = lam - 1; ilv >= ituvmin; ilv -= 2) {
     ++ituv;
     rsweep(rc2, rc, qp, n, ituv);
-    for (int m = 0; m < n; ++m) rc2[m] = pow(-2.0 * aph[m], ilv) * fs[(long)ilv * n + m];
+    for (int m = 0; m < n; ++m) rc2[m] = powi(-2.0 * aph[m], ilv) * fs[(long)ilv * n + m];
     ++ituv;
     rsweep(rc, rc2, qp, n, ituv);
-    for (int m = 0; m < n; ++m) rc[m] = pow(-2.0 * aph[m], ilv - 1) * fs[(long)(ilv - 1) * n + m];
+    for (int m = 0; m < n; ++m) rc[m] = powi(-2.0 * aph[m], ilv - 1) * fs[(long)(ilv - 1) * n + m];
   }
   if (lam % 2 == 1) {
     ++ituv;

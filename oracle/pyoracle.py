@@ -1,0 +1,232 @@
+"""ctypes front ends of the checker libraries (test infrastructure only).
+
+``Oracle``    -- the C restatement in oracle/liboracle.so (bdf_oracle.c).
+``Reference`` -- the reference's own routines, compiled from bertha-scf.f into
+                 oracle/_ref/libbertha_ref.so by oracle/Makefile (only where
+                 the reference sources exist; its compiled limits apply:
+                 at most 2 centres, 26 exponents per shell).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this module; the engine never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libbertha_ref.so")
+
+SHAPE = {"NOSYMM": 0, "LINEAR": 1, "DIATOM": 1, "ATOMIC": 2}
+
+_olib = None
+
+
+def _oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: run make -C oracle liboracle.so")
+        lib = C.CDLL(ORACLE_LIB)
+        P = C.c_void_p
+        lib.orc_create.restype = P
+        lib.orc_create.argtypes = [C.c_int, P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int]
+        lib.orc_destroy.argtypes = [P]
+        lib.orc_ndim.argtypes = [P]
+        lib.orc_nskp.argtypes = [P]
+        lib.orc_labels.argtypes = [P, P, P, P]
+        lib.orc_funfm.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
+        lib.orc_rmake.argtypes = [P, P, C.c_int, C.c_int, P]
+        lib.orc_etable.restype = C.c_long
+        lib.orc_etable.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, P, C.c_long]
+        lib.orc_erifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, C.c_int, P, P]
+        lib.orc_clmfast.argtypes = [P, C.c_int, P, P, P]
+        lib.orc_clmfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
+        _olib = lib
+    return _olib
+
+
+def funfm(t, lam: int, itype: int) -> np.ndarray:
+    """FUNFM branch itype for arguments t: returns F[m, n] for m = 0..lam."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros((lam + 1, t.size))
+    _oracle_lib().orc_funfm(t.ctypes.data, t.size, lam, itype, out.ctypes.data)
+    return out
+
+
+def rmake(qp, aph, lam: int) -> np.ndarray:
+    """RMAKE: R[tuv, n] for Q-P vectors qp (n,3) and exponents aph (n)."""
+    qp = np.ascontiguousarray(np.asarray(qp, dtype=np.float64).T)
+    aph = np.ascontiguousarray(aph, dtype=np.float64)
+    n = aph.size
+    nt = (lam + 1) * (lam + 2) * (lam + 3) // 6
+    out = np.zeros((nt, n))
+    _oracle_lib().orc_rmake(qp.ctypes.data, aph.ctypes.data, n, lam, out.ctypes.data)
+    return out
+
+
+class Oracle:
+    def __init__(self, system, intsym: bool = True, racah1: bool = False):
+        xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
+        self._keep = (xyz, lmax, nfnc, exps)
+        lib = _oracle_lib()
+        self._h = lib.orc_create(system.ncnt, xyz.ctypes.data, lmax.ctypes.data, nfnc.ctypes.data,
+                                 lstride, exps.ctypes.data, mbs, SHAPE[system.shape],
+                                 1 if intsym else 0, 1 if racah1 else 0)
+        self.ndim = lib.orc_ndim(self._h)
+        self.nskp = lib.orc_nskp(self._h)
+        self.system = system
+
+    def close(self):
+        if self._h:
+            _oracle_lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def labels(self):
+        n = self.nskp
+        a = [np.zeros(n, dtype=np.int32) for _ in range(3)]
+        _oracle_lib().orc_labels(self._h, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data)
+        return a
+
+    def etable(self, itt, ica, icb, ka, kb, ma, mb):
+        cap = 4 * 400000
+        out = np.zeros(cap)
+        n = _oracle_lib().orc_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data, cap)
+        if n < 0:
+            raise ValueError("etable capacity")
+        return out[:4 * n].reshape(4, n)
+
+    def erifast(self, icnt, kidx, midx, ibas, jbas, itn, ilev=3):
+        lc, ld = kidx[2] // 2, kidx[3] // 2
+        maxcd = self.system.nfnc_of(icnt[2] - 1, lc) * self.system.nfnc_of(icnt[3] - 1, ld)
+        re = np.zeros(16 * maxcd)
+        im = np.zeros(16 * maxcd)
+        a = [np.asarray(v, dtype=np.int32) for v in (icnt, kidx, midx, itn)]
+        _oracle_lib().orc_erifast(self._h, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                  ibas, jbas, a[3].ctypes.data, ilev, re.ctypes.data,
+                                  im.ctypes.data)
+        return (re + 1j * im).reshape(16, maxcd)
+
+    def clmfast(self, dent, ilev=3, nthreads=1):
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        g = np.zeros((n, n), dtype=np.complex128, order="F")
+        x = np.zeros((n, n), dtype=np.complex128, order="F")
+        if nthreads > 1:
+            _oracle_lib().orc_clmfast_mt(self._h, ilev, d.ctypes.data, g.ctypes.data,
+                                         x.ctypes.data, nthreads)
+        else:
+            _oracle_lib().orc_clmfast(self._h, ilev, d.ctypes.data, g.ctypes.data, x.ctypes.data)
+        return g, x
+
+
+_rlib = None
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def _ref_lib():
+    global _rlib
+    if _rlib is None:
+        _rlib = C.CDLL(REF_LIB)
+    return _rlib
+
+
+class Reference:
+    """The reference's CLMFAST/ERIFAST/EQSAVE, driven through ref_driver.f.
+    One system at a time (the routines keep their state in COMMON)."""
+    MBS, MEL = 26, 6
+
+    def __init__(self, system, intsym: bool = True, racah1: bool = False):
+        lib = _ref_lib()
+        n = system.ncnt
+        if n > 2:
+            raise ValueError("the reference build holds at most 2 centres (parameters.h MCT)")
+        xyz = np.ascontiguousarray(system.xyz, dtype=np.float64)
+        lmax = np.array([system.lmax(c) for c in range(n)], dtype=np.int32)
+        nf = np.zeros((n, self.MEL + 1), dtype=np.int32)
+        ex = np.zeros((n, self.MEL + 1, self.MBS))
+        for c in range(n):
+            for l, e in enumerate(system.shells[c]):
+                if len(e) > self.MBS:
+                    raise ValueError("more than 26 exponents in a shell (parameters.h MBS)")
+                nf[c, l] = len(e)
+                ex[c, l, :len(e)] = e
+        nd, ns = C.c_int(), C.c_int()
+        lib.ref_setup(C.c_int(n), C.c_void_p(xyz.ctypes.data), C.c_void_p(lmax.ctypes.data),
+                      C.c_void_p(nf.ctypes.data), C.c_void_p(ex.ctypes.data),
+                      C.c_int(SHAPE[system.shape]), C.c_int(1 if system.hmlt == "DHFR" else 2),
+                      C.c_int(1 if racah1 else 0), C.c_int(1 if intsym else 0),
+                      C.byref(nd), C.byref(ns))
+        self.ndim, self.nskp, self.system = nd.value, ns.value, system
+
+    def labels(self):
+        n = self.nskp
+        a = [np.zeros(n, dtype=np.int32) for _ in range(3)]
+        lrge = np.zeros(2 * 13 * 14, dtype=np.int32)
+        _ref_lib().ref_labels(C.c_void_p(a[0].ctypes.data), C.c_void_p(a[1].ctypes.data),
+                              C.c_void_p(a[2].ctypes.data), C.c_void_p(lrge.ctypes.data))
+        return a
+
+    def etable(self, itt, ica, icb, ka, kb, ma, mb):
+        cap = 4 * 400000
+        out = np.zeros(cap)
+        nt, mx = C.c_int(), C.c_int()
+        _ref_lib().ref_etable(C.c_int(itt), C.c_int(ica), C.c_int(icb), C.c_int(ka), C.c_int(kb),
+                              C.c_int(ma), C.c_int(mb), C.c_int(cap), C.c_void_p(out.ctypes.data),
+                              C.byref(nt), C.byref(mx))
+        n = nt.value * mx.value
+        return out[:4 * n].reshape(4, n)
+
+    def erifast(self, icnt, kidx, midx, ibas, jbas, itn, ilev=3):
+        a = [np.asarray(v, dtype=np.int32) for v in (icnt, kidx, midx, itn)]
+        lc, ld = kidx[2] // 2, kidx[3] // 2
+        maxcd = self.system.nfnc_of(icnt[2] - 1, lc) * self.system.nfnc_of(icnt[3] - 1, ld)
+        re = np.zeros(16 * maxcd)
+        im = np.zeros(16 * maxcd)
+        mo = C.c_int()
+        _ref_lib().ref_erifast(*[C.c_void_p(x.ctypes.data) for x in a[:3]], C.c_int(ibas),
+                               C.c_int(jbas), C.c_void_p(a[3].ctypes.data), C.c_int(ilev),
+                               C.c_void_p(re.ctypes.data), C.c_void_p(im.ctypes.data),
+                               C.byref(mo))
+        return (re + 1j * im).reshape(16, maxcd)
+
+    def clmfast(self, dent, ilev=3):
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        dre = np.asfortranarray(d.real)
+        dim = np.asfortranarray(d.imag)
+        out = [np.zeros((n, n), order="F") for _ in range(4)]
+        _ref_lib().ref_clmfast(C.c_int(ilev), C.c_void_p(dre.ctypes.data),
+                               C.c_void_p(dim.ctypes.data),
+                               *[C.c_void_p(o.ctypes.data) for o in out])
+        return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+    def funfm(self, t, lam, itype):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        out = np.zeros((lam + 1) * t.size)
+        _ref_lib().ref_funfm(C.c_void_p(t.ctypes.data), C.c_int(t.size), C.c_int(lam),
+                             C.c_int(itype), C.c_void_p(out.ctypes.data))
+        return out.reshape(lam + 1, t.size)
+
+    def rmake(self, qp, aph, lam):
+        qp = np.asfortranarray(np.asarray(qp, dtype=np.float64))
+        aph = np.ascontiguousarray(aph, dtype=np.float64)
+        n = aph.size
+        nt = (lam + 1) * (lam + 2) * (lam + 3) // 6
+        out = np.zeros(nt * n)
+        _ref_lib().ref_rmake(C.c_void_p(qp.ctypes.data), C.c_void_p(aph.ctypes.data),
+                             C.c_int(n), C.c_int(lam), C.c_void_p(out.ctypes.data))
+        return out.reshape(nt, n)

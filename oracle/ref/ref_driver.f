@@ -30,17 +30,33 @@ C======================================================================C
       COMMON/BSET/BEXL(MBS,0:MEL,MCT),BXYZ(3,MCT),LRGE(MCT,MKP,MKP+1),
      &            KAPA(MKP,MCT),NFNC(0:MEL,MCT),NKAP(MCT),IQNC(MCT),NCNT
       COMMON/SHLL/ACFF,BCFF,FOPN,ICLS(MDM),IOPN(MDM),NCLS,NOPN,NOELEC
+      DIMENSION ICDR11(MEQ,4),ICDI11(MEQ,4),ICDR21(MEQ,4),ICDI21(MEQ,4)
+      DIMENSION ICDR11KM(4,MKP,MKP,MEQ,4),ICDI11KM(4,MKP,MKP,MEQ,4),
+     &          ICDR21KM(4,MKP,MKP,MEQ,4),ICDI21KM(4,MKP,MKP,MEQ,4)
+      COMMON/ICDC/ICDR11KM,ICDR11,ICDI11KM,ICDI11,
+     &            ICDR21KM,ICDR21,ICDI21KM,ICDI21
       LOGICAL OPND
 C
 C     SILENCE THE REFERENCE'S UNIT 6/7 REPORTS
       INQUIRE(UNIT=7,OPENED=OPND)
       IF(.NOT.OPND) OPEN(UNIT=7,FILE='/dev/null',STATUS='UNKNOWN')
 C     (UNIT 7 ALREADY HOLDS /dev/null, SO UNIT 6 TAKES /dev/zero)
-      OPEN(UNIT=6,FILE='/dev/zero',STATUS='UNKNOWN')
+      INQUIRE(FILE='/dev/zero',OPENED=OPND)
+      IF(.NOT.OPND) OPEN(UNIT=6,FILE='/dev/zero',STATUS='UNKNOWN')
 C
       CALL FACTRL
       CALL GAMGEN
       CALL CARTIND
+C
+C     START FROM THE ZEROED E(CD) SPARSITY LISTS OF A FRESH PROCESS
+      ICDR11 = 0
+      ICDI11 = 0
+      ICDR21 = 0
+      ICDI21 = 0
+      ICDR11KM = 0
+      ICDI11KM = 0
+      ICDR21KM = 0
+      ICDI21KM = 0
 C
       IF(IHMLT.EQ.2) THEN
         HMLT = 'DHFB'

@@ -1,0 +1,47 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
+
+
+@pytest.fixture(scope="session")
+def bdf():
+    from __graft_entry__ import load_package
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def orc():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    return pyoracle
+
+
+def small_systems(bdf):
+    """Systems small enough for the oracle to finish in about a second."""
+    S = bdf.systems
+    ev = S.even_tempered
+    ne = S.ne_atom()
+    diat = S.System("diatomic-z", np.array([[0, 0, 0.0], [0, 0, 2.1]]),
+                    [[ev(0.1, 2.7, 6), ev(0.15, 2.6, 4), ev(0.3, 2.5, 2)],
+                     [ev(0.05, 3.0, 4), ev(0.2, 2.5, 2)]], shape="DIATOM")
+    off = S.System("two-centre-off-axis", np.array([[0.1, -0.2, 0.3], [1.2, 0.9, -1.1]]),
+                   [[ev(0.1, 2.7, 5), ev(0.15, 2.6, 3), ev(0.3, 2.5, 2)],
+                    [ev(0.05, 3.0, 4), ev(0.2, 2.5, 2)]], shape="NOSYMM")
+    tri = S.System("three-centre", np.array([[0.0, 0.0, 0.0], [1.4, 0.3, 0.9], [-1.1, 0.8, 0.5]]),
+                   [[ev(0.1, 2.8, 4), ev(0.2, 2.6, 2), ev(0.5, 2.5, 1)],
+                    [ev(0.06, 3.0, 3), ev(0.3, 2.5, 1)],
+                    [ev(0.06, 3.0, 3)]], shape="NOSYMM")
+    fsh = S.System("f-shell-pair", np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 1.9]]),
+                   [[ev(0.2, 3.0, 2), ev(0.3, 3.0, 2), ev(0.4, 3.0, 1), ev(0.5, 3.0, 1)],
+                    [ev(0.1, 3.0, 2), ev(0.3, 3.0, 1)]], shape="NOSYMM")
+    return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh}
