@@ -85,6 +85,8 @@ def library() -> _C.CDLL:
         lib.bdf_etable.argtypes = [P, _C.c_int, _C.c_int, _C.c_int, _C.c_int, _C.c_int,
                                    _C.c_int, _C.c_int, P, _C.c_long, _C.POINTER(_C.c_long)]
         lib.bdf_last_stats.argtypes = [P, _C.POINTER(_C.c_double), _C.POINTER(_C.c_double)]
+        lib.bdf_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
+                                      _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -181,6 +183,17 @@ class Engine:
         _check(library().bdf_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data,
                                     cap, _C.byref(n)))
         return out[:4 * n.value].reshape(4, n.value)
+
+    def plan_info(self, ilev: int = 3):
+        """(algorithmic flops, primitive quartets, work items) of one build."""
+        f = _C.c_double()
+        q = _C.c_double()
+        n = _C.c_long()
+        _check(library().bdf_plan_info(self._h, ilev, _C.byref(f), _C.byref(q), _C.byref(n)))
+        return f.value, q.value, n.value
+
+    def stream(self) -> int:
+        return int(library().bdf_stream(self._h) or 0)
 
     def stats(self):
         q = _C.c_double()

@@ -85,7 +85,7 @@ struct bdf_engine {
   int64_t slab_stride = 0;
   int slab_ld = 0;
   DevBuf<double2> dent, gdir, gxch;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_ms = 0.0, last_quartets = 0.0;
 };
 
@@ -141,6 +141,7 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   int nb = std::min(e->nblocks, std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream));
+  HIPCHK(hipEventRecord(e->ev2, e->stream));
   if (!rr_out) {
     CompleteArgs c;
     c.gdir = d_gdir;
@@ -191,6 +192,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
+    HIPCHK(hipEventCreate(&e->ev2));
     HIPCHK(upload_constants());
     build_etables(e->B, e->T);
     HIPCHK(e->exps.upload(e->B.exps, e->stream));
@@ -281,8 +283,6 @@ int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent, const doub
   HIPCHK(hipMemcpy2DAsync(gxch, ld * sizeof(double2), e->gxch.p, nd * sizeof(double2),
                           nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->last_ms = ms;
   return BDF_OK;
 }
 
@@ -419,10 +419,25 @@ void *bdf_stream(bdf_engine *e) { return e ? (void *)e->stream : nullptr; }
 
 int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms) {
   if (!e) return fail(BDF_EINVAL, "bdf_last_stats: null engine");
+  HIPCHK(hipEventSynchronize(e->ev1));
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->last_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev2));
+  e->last_ms = ms;
   if (quartets) *quartets = e->last_quartets;
   if (kernel_ms) *kernel_ms = e->last_ms;
+  return BDF_OK;
+}
+
+int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items) {
+  if (!e) return fail(BDF_EINVAL, "bdf_plan_info: null engine");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_plan_info: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  const Plan &P = e->plans[ilev - 1].P;
+  if (flops) *flops = P.flops;
+  if (quartets) *quartets = P.quartets;
+  if (items) *items = (long)P.items.size();
   return BDF_OK;
 }
 
@@ -432,6 +447,7 @@ void bdf_destroy(bdf_engine *e) {
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->ev2) (void)hipEventDestroy(e->ev2);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

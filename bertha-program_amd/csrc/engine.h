@@ -99,6 +99,7 @@ struct Plan {
   std::vector<WorkItem> items;
   int max_lfl = 0, max_maxcd = 0, max_ntg = 0;
   double quartets = 0.0;  // primitive quartet count (ij,kl) x calls
+  double flops = 0.0;     // algorithmic FP64 work of one build (see plan.cpp)
 };
 
 }  // namespace bdf

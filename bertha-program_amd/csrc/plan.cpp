@@ -536,6 +536,27 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
           }
         }
   }
+  // Algorithmic FP64 work of one build, counted per unit of the reference's
+  // loop nest: R batch per (ij, class) MAXCD*(120 + 2L + 6*C(L+3,4) + 3N(L));
+  // first contraction per (ij, AB combination) 8*N(L_AB)*N(L_CD)*MAXCD;
+  // second contraction per (ij, call) 32*N(L_CD)*MAXCD (8 on the real
+  // linear/atomic path) and digestion 1408 (88 real) flops per RR batch
+  // element.
+  for (const ClassDesc &cl : P.classes) {
+    double maxab = (double)cl.nbas[0] * cl.nbas[1], maxcd = cl.maxcd;
+    double L = cl.lamfl;
+    double c4 = (L + 3) * (L + 2) * (L + 1) * L / 24.0;
+    P.flops += maxab * maxcd * (120.0 + 2.0 * L + 6.0 * c4 + 3.0 * ntuv(cl.lamfl));
+    for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
+      const ABCombo &ab = P.abs[a];
+      P.flops += maxab * 8.0 * ab.ntab * ab.ntg * maxcd;
+      for (int k = ab.cd_begin; k < ab.cd_end; ++k) {
+        double c2 = (isym ? 8.0 : 32.0) * P.cds[k].ntcd;
+        double dig = isym ? 88.0 : 1408.0;
+        P.flops += maxab * maxcd * (c2 + dig);
+      }
+    }
+  }
   // largest work first, so the persistent kernel's queue drains evenly
   std::vector<double> cost(P.classes.size(), 0.0);
   for (size_t c = 0; c < P.classes.size(); ++c) {

@@ -122,9 +122,16 @@ int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb,
    after bdf_coulomb_device. */
 void *bdf_stream(bdf_engine *e);
 
-/* Per-build counters: number of primitive quartets evaluated and kernel time
-   of the last build in milliseconds (device events). */
+/* Per-build counters: number of primitive quartets evaluated and the
+   duration of the last build's quartet kernel in milliseconds (HIP events on
+   the engine's stream).  No counterpart in the reference, whose timers are
+   the SYSTEM_CLOCK labour tables (bertha-scf.f:5941-6000). */
 int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms);
+
+/* Size of the stage-ilev build: algorithmic FP64 flops (the reference's loop
+   nest counted per R batch, contraction and digestion), primitive quartets
+   and work items.  No counterpart in the reference. */
+int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items);
 
 void bdf_destroy(bdf_engine *e);
 const char *bdf_last_error(void);

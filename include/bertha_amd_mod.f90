@@ -78,6 +78,15 @@ module bertha_amd
       real(c_double), intent(out) :: quartets, kernel_ms
     end function
 
+    integer(c_int) function bdf_plan_info(eng, ilev, flops, quartets, items) &
+        bind(c, name='bdf_plan_info')
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev
+      real(c_double), intent(out) :: flops, quartets
+      integer(c_long), intent(out) :: items
+    end function
+
     subroutine bdf_destroy(eng) bind(c, name='bdf_destroy')
       import :: c_ptr
       type(c_ptr), value :: eng

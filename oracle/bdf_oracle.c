@@ -1198,7 +1198,7 @@ static const int ISCF[6][11] = {
 static const int MAPTTTT[4][4] = {{1, 0, 0, 2}, {0, 5, 6, 0}, {0, 7, 8, 0}, {3, 0, 0, 4}};
 
 static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gxch, int a_lo,
-                          int a_hi, int b_sel, eri_ws *w) {
+                          int a_hi, int b_sel, int c_sel, int d_sel, eri_ws *w) {
   int ndim = s->ndim;
   int isym = s->isym;
   int maxl = 0, maxn = 0, nmax[MKPX] = {0};
@@ -1227,6 +1227,8 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
       if (b_sel >= 0 && cb != b_sel) continue;
       for (int cc = 0; cc < s->ncnt; ++cc)
         for (int cd = 0; cd < s->ncnt; ++cd) {
+          if (c_sel >= 0 && cc != c_sel) continue;
+          if (d_sel >= 0 && cd != d_sel) continue;
           quartet_t q;
           int ic[4] = {ca, cb, cc, cd};
           for (int k = 0; k < 4; ++k) { q.icnt[k] = ic[k] + 1; q.xyz[k] = s->xyz + 3 * ic[k]; }
@@ -1430,9 +1432,12 @@ void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, doub
     eri_ws w;
     ws_init(&w);
     for (int p = 0; p < npair; ++p)
-      clmfast_range(s, ilev, d, (zc *)gdir, (zc *)gxch, pa[p], pa[p] + 1, pb[p], &w);
+      clmfast_range(s, ilev, d, (zc *)gdir, (zc *)gxch, pa[p], pa[p] + 1, pb[p], -1, -1, &w);
     ws_free(&w);
   } else {
+    /* work items are (A,B,C,D) centre quadruples: every class lies inside
+       one, so the per-class ERIFAST state never crosses threads */
+    long nq = (long)npair * s->ncnt * s->ncnt;
     zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
 #pragma omp parallel num_threads(nthreads)
     {
@@ -1445,8 +1450,11 @@ void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, doub
       eri_ws w;
       ws_init(&w);
 #pragma omp for schedule(dynamic, 1)
-      for (int p = 0; p < npair; ++p)
-        clmfast_range(s, ilev, d, gd[tid], gx[tid], pa[p], pa[p] + 1, pb[p], &w);
+      for (long q = 0; q < nq; ++q) {
+        int p = (int)(q / ((long)s->ncnt * s->ncnt));
+        int c = (int)((q / s->ncnt) % s->ncnt), dd = (int)(q % s->ncnt);
+        clmfast_range(s, ilev, d, gd[tid], gx[tid], pa[p], pa[p] + 1, pb[p], c, dd, &w);
+      }
       ws_free(&w);
     }
     zc *G = (zc *)gdir, *X = (zc *)gxch;
