@@ -248,12 +248,119 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+constexpr Batch kBatchTab[11] = {
+    {0, 0, 1, 0, 0, 2, 3, 0, 0, 0, 0}, {0, 0, 1, 0, 0, 3, 2, 1, 1, 0, 1},
+    {0, 2, 3, 0, 0, 0, 1, 0, 0, 0, 0}, {0, 2, 3, 0, 0, 1, 0, 1, 1, 1, 0},
+    {1, 0, 3, 0, 0, 2, 1, 0, 0, 0, 0}, {1, 0, 2, 0, 1, 3, 1, 1, 0, 0, 1},
+    {1, 1, 3, 1, 0, 2, 0, 0, 1, 1, 0}, {1, 1, 2, 1, 1, 3, 0, 1, 1, 1, 1},
+    {1, 2, 1, 0, 0, 0, 3, 0, 0, 0, 0}, {1, 2, 0, 0, 1, 1, 3, 1, 0, 1, 0},
+    {1, 3, 1, 1, 0, 0, 2, 0, 1, 0, 1}};
+
+// One digestion batch for one (ij,kl): out[2*rowslot+colslot] accumulates
+// the four output sub-blocks.  Everything indexed by the batch table folds to
+// constants after unrolling, so the 16 terms stay in registers.
+template <int B, bool REAL>
+__device__ __forceinline__ void batch_terms(const double (&rr)[16], const double (&ri)[16],
+                                            const int (&off)[4][2], const int (&idx)[4],
+                                            const double2 *__restrict__ Dm, int nd,
+                                            double pab1, double pab2, double pcd1, double pcd2,
+                                            double (&out)[4][2]) {
+  constexpr Batch bt = kBatchTab[B];
+  double2 d[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int rs = s >> 1, cs = s & 1;
+    if (REAL && rs != cs) {
+      d[s] = make_double2(0.0, 0.0);
+      continue;
+    }
+    d[s] = Dm[(int64_t)(off[bt.dcol][cs] + idx[bt.dcol]) * nd + off[bt.drow][rs] + idx[bt.drow]];
+  }
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int sl[4] = {(n >> 3) & 1, (n >> 2) & 1, (n >> 1) & 1, n & 1};
+    const int ors = sl[bt.orow] ^ bt.oflr, ocs = sl[bt.ocol] ^ bt.oflc;
+    const int drs = sl[bt.drow] ^ bt.dflr, dcs = sl[bt.dcol] ^ bt.dflc;
+    if (REAL && (ors != ocs || drs != dcs)) continue;
+    double ph = 1.0;
+    if (bt.pab) ph *= (sl[0] != sl[1]) ? pab2 : pab1;
+    if (bt.pcd) ph *= (sl[2] != sl[3]) ? pcd2 : pcd1;
+    const double2 dv = d[2 * drs + dcs];
+    if (REAL) {
+      out[2 * ors + ocs][0] += ph * rr[n] * dv.x;
+    } else {
+      out[2 * ors + ocs][0] += ph * (rr[n] * dv.x - ri[n] * dv.y);
+      out[2 * ors + ocs][1] += ph * (rr[n] * dv.y + ri[n] * dv.x);
+    }
+  }
+}
+
+// LDS accumulators of exchange batches whose output varies only with l
+// (batches 5, 7, 11) or only with k (6, 8, 9, 10) inside one call
+struct XAcc {
+  double l[3][4][kMaxPrim][2];
+  double k[4][4][kMaxPrim][2];
+};
+
+template <bool REAL>
+__device__ __forceinline__ void lds_add4(double (&dst)[4][kMaxPrim][2], int pos,
+                                         const double (&o)[4][2]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (REAL && (s == 1 || s == 2)) continue;
+    if (o[s][0] != 0.0) unsafeAtomicAdd(&dst[s][pos][0], o[s][0]);
+    if (!REAL && o[s][1] != 0.0) unsafeAtomicAdd(&dst[s][pos][1], o[s][1]);
+  }
+}
+
+// digestion of one RR batch element (CLMMTG :13902-14392, CLMMTZ
+// :13196-13514 when REAL): batches 1-2 into registers (G(A,B) is fixed by
+// ij), 3-4 straight to global memory, exchange batches into LDS
+template <bool REAL>
+__device__ __forceinline__ void digest(const double (&rr)[16], const double (&ri)[16],
+                                       const int (&off)[4][2], const int (&idx)[4],
+                                       const double2 *__restrict__ Dm, int nd, int iflg,
+                                       int intsym, double pab1, double pab2, double pcd1,
+                                       double pcd2, double (&accAB)[4][2], XAcc &xa,
+                                       double2 *gdir) {
+  if (iflg & 1) batch_terms<0, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
+  if (!intsym) iflg &= 0x11;
+  if (iflg & 2) batch_terms<1, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
+  if (iflg & 12) {
+    double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (iflg & 4) batch_terms<2, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
+    if (iflg & 8) batch_terms<3, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int rs = s >> 1, cs = s & 1;
+      if (REAL && rs != cs) continue;
+      atomic_add2(&gdir[(int64_t)(off[3][cs] + idx[3]) * nd + off[2][rs] + idx[2]], o[s][0],
+                  o[s][1]);
+    }
+  }
+#define XB(B, ARR, POS)                                                                     \
+  if (iflg & (1 << B)) {                                                                    \
+    double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};                                      \
+    batch_terms<B, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);              \
+    lds_add4<REAL>(ARR, POS, o);                                                            \
+  }
+  XB(4, xa.l[0], idx[3])
+  XB(5, xa.k[0], idx[2])
+  XB(6, xa.l[1], idx[3])
+  XB(7, xa.k[1], idx[2])
+  XB(8, xa.k[2], idx[2])
+  XB(9, xa.k[3], idx[2])
+  XB(10, xa.l[2], idx[3])
+#undef XB
+}
+
 __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
   __shared__ double s_eab[ntuv(kMaxLab) * 4];
   __shared__ uint8_t s_cflag[ntuv(kMaxLfl)];
   __shared__ uint8_t s_gm[ntuv(kMaxLab) * 4];
   __shared__ uint8_t s_km[ntuv(kMaxLab) * 4];
   __shared__ int s_item;
+  __shared__ XAcc s_x;
   const int lane = threadIdx.x;
   const int ld = a.ld;
   double *R = a.slab + (int64_t)blockIdx.x * a.slab_stride;  // [rows][ld]
@@ -261,6 +368,11 @@ __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
   double *G = F + (int64_t)(kMaxLfl + 1) * ld;                // [ntg*4][ld]
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
+  {
+    double *z = &s_x.l[0][0][0][0];
+    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
+  }
+  __syncthreads();
 
   for (;;) {
     if (lane == 0) s_item = atomicAdd(a.counter, 1);
@@ -431,49 +543,48 @@ __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
             continue;
           }
           // digestion (CLMMTG / CLMMTZ)
-          const int kb = kl / nD, lb = kl % nD;
-          const int bidx[4] = {ib, jb, kb, lb};
+          const int idx[4] = {ib, jb, kl / nD, kl % nD};
           const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-          for (int b = 0; b < 11; ++b) {
-            if (!((cd.iflg >> b) & 1)) continue;
-            if (!a.intsym && b != 0 && b != 4) continue;
-            const Batch bt = c_batch[b];
-            double acc[2][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
-            for (int n = 0; n < 16; ++n) {
-              int sl[4] = {(n >> 3) & 1, (n >> 2) & 1, (n >> 1) & 1, n & 1};
-              int ors = sl[bt.orow] ^ bt.oflr, ocs = sl[bt.ocol] ^ bt.oflc;
-              int drs = sl[bt.drow] ^ bt.dflr, dcs = sl[bt.dcol] ^ bt.dflc;
-              if (a.isym != 0 && (ors != ocs || drs != dcs)) continue;
-              double ph = 1.0;
-              if (bt.pab) ph *= (sl[0] != sl[1]) ? (double)cd.pab[1] : pab1;
-              if (bt.pcd) ph *= (sl[2] != sl[3]) ? (double)cd.pcd[1] : pcd1;
-              int dr = off[bt.drow][drs] + bidx[bt.drow];
-              int dc = off[bt.dcol][dcs] + bidx[bt.dcol];
-              double2 d = Dm[(int64_t)dc * nd + dr];
-              if (a.isym != 0) {
-                acc[ors][ocs][0] += ph * rr_r[n] * d.x;
-              } else {
-                acc[ors][ocs][0] += ph * (rr_r[n] * d.x - rr_i[n] * d.y);
-                acc[ors][ocs][1] += ph * (rr_r[n] * d.y + rr_i[n] * d.x);
-              }
+          if (a.isym != 0)
+            digest<true>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+                         cd.pcd[1], accAB, s_x, a.gdir);
+          else
+            digest<false>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1],
+                          pcd1, cd.pcd[1], accAB, s_x, a.gdir);
+        }
+        if (!a.rr_out) {
+          // flush the exchange accumulators of this call (one atomic per
+          // output element instead of one per quartet), then clear them
+          __syncthreads();
+          const int nC = cl.nbas[2];
+          const int flg = a.intsym ? cd.iflg : (cd.iflg & 0x11);
+          const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
+            const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
+            const int b = 4 + bsel;  // batch index 4..10 (5..11)
+            if (!((flg >> b) & 1)) continue;
+            const bool isl = (b == 4 || b == 6 || b == 10);
+            if (pos >= (isl ? nD : nC)) continue;
+            double *v;
+            int row, col;
+            const int rs = s >> 1, cs = s & 1;
+            switch (b) {
+              case 4: v = s_x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
+              case 5: v = s_x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
+              case 6: v = s_x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
+              case 7: v = s_x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
+              case 8: v = s_x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
+              case 9: v = s_x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
+              default: v = s_x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
             }
-            if (b < 2) {
-              for (int s = 0; s < 4; ++s) {
-                accAB[s][0] += acc[s >> 1][s & 1][0];
-                accAB[s][1] += acc[s >> 1][s & 1][1];
-              }
-              continue;
-            }
-            double2 *Gm = bt.xch ? a.gxch : a.gdir;
-            for (int rs = 0; rs < 2; ++rs)
-              for (int cs = 0; cs < 2; ++cs) {
-                if (a.isym != 0 && rs != cs) continue;
-                int orr = off[bt.orow][rs] + bidx[bt.orow];
-                int occ = off[bt.ocol][cs] + bidx[bt.ocol];
-                atomic_add2(&Gm[(int64_t)occ * nd + orr], acc[rs][cs][0], acc[rs][cs][1]);
-              }
+            if (v[0] != 0.0 || v[1] != 0.0)
+              atomic_add2(&a.gxch[(int64_t)col * nd + row], v[0], v[1]);
+            v[0] = 0.0;
+            v[1] = 0.0;
           }
+          __syncthreads();
         }
       }
       if (!a.rr_out) {
