@@ -60,6 +60,8 @@ class _Options(_C.Structure):
         ("precision", _C.c_int),
         ("split_threshold", _C.c_double),
         ("device", _C.c_int),
+        ("schwarz", _C.c_int),
+        ("sens", _C.c_double),
     ]
 
 
@@ -87,6 +89,7 @@ def library() -> _C.CDLL:
         lib.bdf_last_stats.argtypes = [P, _C.POINTER(_C.c_double), _C.POINTER(_C.c_double)]
         lib.bdf_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
                                       _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
+        lib.bdf_schwarz_tables.argtypes = [P, P, P, _C.c_int]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -109,7 +112,8 @@ class Engine:
     """
 
     def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
-                 device: int = 0):
+                 device: int = 0, schwarz: bool = False, sens: float = 0.0,
+                 precision: str = "fp64", split_threshold: float = 0.0):
         lib = library()
         self.system = system
         xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
@@ -119,7 +123,9 @@ class Engine:
                     nfnc.ctypes.data_as(_C.POINTER(_C.c_int)),
                     exps.ctypes.data_as(_C.POINTER(_C.c_double)), lstride, mbs,
                     SHAPES[system.shape], HMLTS[system.hmlt])
-        o = _Options(1 if intsym else 0, 1 if racah1 else 0, 0, 0.0, device)
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        o = _Options(1 if intsym else 0, 1 if racah1 else 0, prec, split_threshold, device,
+                     1 if schwarz else 0, sens)
         h = _C.c_void_p()
         nd = _C.c_int()
         ns = _C.c_int()
@@ -183,6 +189,14 @@ class Engine:
         _check(library().bdf_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data,
                                     cap, _C.byref(n)))
         return out[:4 * n.value].reshape(4, n.value)
+
+    def schwarz_tables(self):
+        """(GDSC of CLMDIAG, sqrt|(ab|ba)| bound table), ndim x ndim real."""
+        n = self.ndim
+        g = _np.zeros((n, n), order="F")
+        q = _np.zeros((n, n), order="F")
+        _check(library().bdf_schwarz_tables(self._h, g.ctypes.data, q.ctypes.data, n))
+        return g, q
 
     def plan_info(self, ilev: int = 3):
         """(algorithmic flops, primitive quartets, work items) of one build."""

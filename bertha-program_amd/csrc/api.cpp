@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,7 @@ struct DevBuf {
 struct DevPlan {
   int built = 0;
   Plan P;
+  DevBuf<double> qitem;  // Schwarz factor of each work item (mixed precision)
   DevBuf<ClassDesc> classes;
   DevBuf<ABCombo> abs;
   DevBuf<CDCall> cds;
@@ -85,9 +87,21 @@ struct bdf_engine {
   int64_t slab_stride = 0;
   int slab_ld = 0;
   DevBuf<double2> dent, gdir, gxch;
+  // Schwarz tables (GDSC of CLMDIAG and the (ab|ba) bound), built on demand
+  int schwarz = 0, have_tables = 0;
+  double sens = 1.0e-12;
+  DevBuf<double> gdsc, qbnd;
+  std::vector<double> qbnd_host;
+  // FP32/FP64 split
+  int mixed = 0;
+  double split = 1.0e-5;
+  DevBuf<float4> E32;
+  DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_ms = 0.0, last_quartets = 0.0;
 };
+
+static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double> &q);
 
 static int make_plan(bdf_engine *e, int ilev) {
   DevPlan &dp = e->plans[ilev - 1];
@@ -99,6 +113,11 @@ static int make_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  if (e->mixed) {
+    std::vector<double> q;
+    item_factors(e, dp.P, q);
+    HIPCHK(dp.qitem.upload(q, e->stream));
+  }
   // per-workgroup scratch: R rows, Boys rows, G rows, each ld doubles
   int ld = ((std::max(dp.P.max_maxcd, 1) + 63) / 64) * 64;
   int64_t stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * ld;
@@ -112,12 +131,129 @@ static int make_plan(bdf_engine *e, int ilev) {
   return BDF_OK;
 }
 
+// CLMDIAG (bertha-scf.f:38126) and the (ab|ba) bound table on the device:
+// one DiagJob per (A, B, kappa_A, kappa_B, |m_A|, |m_B|) and component pair.
+static int make_tables(bdf_engine *e) {
+  if (e->have_tables) return BDF_OK;
+  const Basis &B = e->B;
+  const Etabs &T = e->T;
+  std::vector<DiagJob> jobs;
+  std::vector<int64_t> first;
+  int64_t acc = 0;
+  try {
+  for (int a = 0; a < B.ncnt; ++a)
+    for (int b = 0; b < B.ncnt; ++b)
+      for (int ka = 1; ka <= 2 * B.lmax[a] + 1; ++ka)
+        for (int kb = 1; kb <= 2 * B.lmax[b] + 1; ++kb) {
+          int kqa = B.kappa_of(a, ka), kqb = B.kappa_of(b, kb);
+          int la = kqa < 0 ? -kqa - 1 : kqa, lb = kqb < 0 ? -kqb - 1 : kqb;
+          for (int ma = 1; ma <= std::abs(kqa); ++ma)
+            for (int mb = 1; mb <= std::abs(kqb); ++mb)
+              for (int job = 0; job < 6; ++job) {
+                // jobs 0-3: GDSC for (T1,T2) = (S,S),(S,L),(L,S),(L,L); 4-5: bound, T = S, L
+                int t1 = job < 4 ? (job < 2) : (job == 4);
+                int t2 = job < 4 ? (job % 2 == 0) : t1;
+                DiagJob J;
+                const EPair &pab = T.pairs[T.pair_index.at(Etabs::key(a, b, ka, kb, ma, mb, t1))];
+                const EPair &pcd = job < 4
+                    ? T.pairs[T.pair_index.at(Etabs::key(a, b, ka, kb, ma, mb, t2))]
+                    : T.pairs[T.pair_index.at(Etabs::key(b, a, kb, ka, mb, ma, t2))];
+                J.eab = pab.off;
+                J.ecd = pcd.off;
+                J.ntab = ntuv(la + lb + (t1 ? 2 : 0));
+                J.ntcd = ntuv(la + lb + (t2 ? 2 : 0));
+                J.nA = B.nfn(a, la);
+                J.nB = B.nfn(b, lb);
+                J.expA = B.expo(a, la);
+                J.expB = B.expo(b, lb);
+                J.row[0] = B.lrge_at(a, ka, 2 * ma - 1) + (t1 ? B.nskp : 0);
+                J.row[1] = B.lrge_at(a, ka, 2 * ma) + (t1 ? B.nskp : 0);
+                J.col[0] = B.lrge_at(b, kb, 2 * mb - 1) + (t2 ? B.nskp : 0);
+                J.col[1] = B.lrge_at(b, kb, 2 * mb) + (t2 ? B.nskp : 0);
+                J.swap = job >= 4;
+                J.realpart = job < 4 && e->isym != 0;
+                J.out = job >= 4;
+                J.pad = 0;
+                first.push_back(acc);
+                acc += (int64_t)J.nA * J.nB;
+                jobs.push_back(J);
+              }
+        }
+  } catch (const std::out_of_range &) {
+    return fail(BDF_EINVAL, "Schwarz tables: missing E-coefficient block");
+  }
+  size_t nn = (size_t)B.ndim * B.ndim;
+  HIPCHK(e->gdsc.alloc(nn));
+  HIPCHK(e->qbnd.alloc(nn));
+  HIPCHK(hipMemsetAsync(e->gdsc.p, 0, nn * sizeof(double), e->stream));
+  HIPCHK(hipMemsetAsync(e->qbnd.p, 0, nn * sizeof(double), e->stream));
+  DevBuf<DiagJob> djobs;
+  DevBuf<int64_t> dfirst;
+  HIPCHK(djobs.upload(jobs, e->stream));
+  HIPCHK(dfirst.upload(first, e->stream));
+  DiagArgs da;
+  da.jobs = djobs.p;
+  da.njobs = (int)jobs.size();
+  da.first = dfirst.p;
+  da.nitems = acc;
+  da.exps = e->exps.p;
+  da.E = e->E.p;
+  da.gdsc = e->gdsc.p;
+  da.qbnd = e->qbnd.p;
+  da.ndim = B.ndim;
+  HIPCHK(launch_diag(da, e->stream));
+  e->qbnd_host.resize(nn);
+  HIPCHK(hipMemcpyAsync(e->qbnd_host.data(), e->qbnd.p, nn * sizeof(double),
+                        hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->have_tables = 1;
+  return BDF_OK;
+}
+
+// Schwarz factor of every work item: max over the item's AB combinations and
+// sign slots of Q(a,b) times the largest Q(c,d) of its class, so that every
+// quartet (ij|kl) of the item obeys |(ij|kl)| <= qitem.
+static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double> &q) {
+  const int nd = e->B.ndim;
+  const double *Q = e->qbnd_host.data();
+  auto at = [&](int r, int c) { return Q[(int64_t)c * nd + r]; };
+  std::vector<double> qcd(P.classes.size(), 0.0);
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    double m = 0.0;
+    for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi)
+      for (int ci = P.abs[abi].cd_begin; ci < P.abs[abi].cd_end; ++ci) {
+        const CDCall &cd = P.cds[ci];
+        for (int k = 0; k < cl.nbas[2]; ++k)
+          for (int l = 0; l < cl.nbas[3]; ++l)
+            for (int s = 0; s < 4; ++s) m = std::max(m, at(cd.fc[s >> 1] + k, cd.fd[s & 1] + l));
+      }
+    qcd[c] = m;
+  }
+  q.resize(P.items.size());
+  for (size_t i = 0; i < P.items.size(); ++i) {
+    const WorkItem &it = P.items[i];
+    const ClassDesc &cl = P.classes[it.cls];
+    int ib = it.ij / cl.nbas[1], jb = it.ij % cl.nbas[1];
+    double m = 0.0;
+    for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+      const ABCombo &ab = P.abs[abi];
+      for (int s = 0; s < 4; ++s) m = std::max(m, at(ab.fa[s >> 1] + ib, ab.fb[s & 1] + jb));
+    }
+    q[i] = m * qcd[it.cls];
+  }
+}
+
 static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_gdir,
                     double2 *d_gxch, double2 *rr_out, DevPlan *override_plan) {
   DevPlan &dp = override_plan ? *override_plan : e->plans[ilev - 1];
   int nd = e->B.ndim;
-  if (!rr_out) HIPCHK(launch_zero(d_gdir, d_gxch, (int64_t)nd * nd, e->counter.p, e->stream));
-  else HIPCHK(launch_zero(rr_out, nullptr, 0, e->counter.p, e->stream));
+  const bool mixed = e->mixed && !rr_out && !override_plan;
+  if (!rr_out)
+    HIPCHK(launch_zero(d_gdir, d_gxch, (int64_t)nd * nd, e->counter.p, e->dmax.p, e->stream));
+  else
+    HIPCHK(launch_zero(rr_out, nullptr, 0, e->counter.p, e->dmax.p, e->stream));
+  if (mixed) HIPCHK(launch_dmax(d_dent, (int64_t)nd * nd, e->dmax.p, e->stream));
   FockArgs a;
   a.classes = dp.classes.p;
   a.abs = dp.abs.p;
@@ -138,6 +274,15 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.isym = e->isym;
   a.intsym = e->intsym;
   a.rr_out = rr_out;
+  a.gdsc = e->gdsc.p;
+  a.sens = e->sens;
+  a.schwarz = (e->schwarz && !rr_out) ? 1 : 0;
+  a.nskp = e->B.nskp;
+  a.mixed = mixed ? 1 : 0;
+  a.split = e->split;
+  a.qitem = mixed ? dp.qitem.p : nullptr;
+  a.dmax_bits = e->dmax.p;
+  a.E32 = e->E32.p;
   int nb = std::min(e->nblocks, std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream));
@@ -170,9 +315,10 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   if (sys->hmlt != BDF_HMLT_DHFR && sys->hmlt != BDF_HMLT_DHFB)
     return fail(BDF_EINVAL, "bdf_create: unknown hamiltonian");
   if (sys->shape < 0 || sys->shape > 2) return fail(BDF_EINVAL, "bdf_create: unknown shape");
-  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0};
+  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0, 0, 0.0};
   if (opt) o = *opt;
-  if (o.precision != BDF_PREC_FP64) return fail(BDF_EUNSUPP, "bdf_create: only FP64 precision is built");
+  if (o.precision != BDF_PREC_FP64 && o.precision != BDF_PREC_MIXED)
+    return fail(BDF_EINVAL, "bdf_create: unknown precision mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(BDF_EDEVICE, "bdf_create: no HIP device");
@@ -182,6 +328,10 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   e->isym = sys->shape;
   e->intsym = o.intsym ? 1 : 0;
   e->racah1 = o.racah1 ? 1 : 0;
+  e->schwarz = o.schwarz ? 1 : 0;
+  e->sens = o.sens > 0.0 ? o.sens : 1.0e-12;
+  e->mixed = o.precision == BDF_PREC_MIXED;
+  if (o.split_threshold > 0.0) e->split = o.split_threshold;
   std::string err = build_basis(*sys, e->B);
   if (!err.empty()) {
     delete e;
@@ -237,10 +387,16 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(e->kqn.upload(e->B.labkqn, e->stream));
     HIPCHK(e->mqn.upload(e->B.labmqn, e->stream));
     HIPCHK(e->counter.alloc(1));
+    HIPCHK(e->dmax.alloc(1));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
     e->nblocks = prop.multiProcessorCount * 8;
     HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->mixed) {
+      HIPCHK(e->E32.alloc((size_t)e->T.total));
+      HIPCHK(launch_to_f32(e->E.p, e->E32.p, e->T.total, e->stream));
+    }
+    if (e->schwarz || e->mixed) return make_tables(e);
     return BDF_OK;
   }();
   if (rc != BDF_OK) {
@@ -412,6 +568,22 @@ int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb, int ma,
     out[2 * cnt + k] = h[k].z;
     out[3 * cnt + k] = h[k].w;
   }
+  return BDF_OK;
+}
+
+int bdf_schwarz_tables(bdf_engine *e, double *gdsc, double *qbnd, int ld) {
+  if (!e) return fail(BDF_EINVAL, "bdf_schwarz_tables: null engine");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_schwarz_tables: ld < ndim");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_tables(e);
+  if (rc != BDF_OK) return rc;
+  if (gdsc)
+    HIPCHK(hipMemcpy2D(gdsc, ld * sizeof(double), e->gdsc.p, nd * sizeof(double),
+                       nd * sizeof(double), nd, hipMemcpyDeviceToHost));
+  if (qbnd)
+    HIPCHK(hipMemcpy2D(qbnd, ld * sizeof(double), e->qbnd.p, nd * sizeof(double),
+                       nd * sizeof(double), nd, hipMemcpyDeviceToHost));
   return BDF_OK;
 }
 

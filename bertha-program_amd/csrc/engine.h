@@ -83,6 +83,21 @@ struct CDCall {
   int8_t pab[2], pcd[2];  // [0] same-slot phase (PAB1), [1] PAB2
 };
 
+// One block pair of the Schwarz tables: the self-pair integrals
+// (A B | A B) of CLMDIAG (swap = 0, CD side = the same pair, kl = ij) or the
+// bound integrals (A B | B A) (swap = 1, CD side = the (B,A) pair, kl = ji).
+struct DiagJob {
+  int64_t eab, ecd;  // double4 offsets of the AB- and CD-side E tables
+  int ntab, ntcd;
+  int nA, nB;
+  int expA, expB;
+  int row[2], col[2];  // matrix offsets of the A and B sign slots
+  int swap;
+  int realpart;        // keep real parts (ERI's path for ISYM 1, 2)
+  int out;             // 0: GDSC, 1: bound table
+  int pad;
+};
+
 struct WorkItem {
   int cls;
   int ij;

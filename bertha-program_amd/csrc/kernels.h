@@ -43,6 +43,27 @@ struct FockArgs {
   int ld;               // row length of the slab (>= largest maxcd)
   int isym, intsym;
   double2 *rr_out;      // test hook: write RR[16][maxcd] instead of digesting
+  // SCHWARZ masks (SCHWRZ): GDSC table, sensitivity, large-block size
+  const double *gdsc;
+  double sens;
+  int schwarz, nskp;
+  // Schwarz split: items with qitem * max|D| < split run in FP32 on E32
+  int mixed;
+  double split;
+  const double *qitem;
+  const unsigned long long *dmax_bits;
+  const float4 *E32;
+};
+
+struct DiagArgs {
+  const DiagJob *jobs;
+  int njobs;
+  const int64_t *first;  // [njobs] first flat thread of each job
+  int64_t nitems;
+  const double *exps;
+  const double4 *E;
+  double *gdsc, *qbnd;   // ndim x ndim column-major
+  int ndim;
 };
 
 struct CompleteArgs {
@@ -54,8 +75,12 @@ struct CompleteArgs {
 hipError_t upload_constants();
 hipError_t launch_etab(const EtabArgs &a, hipStream_t s);
 hipError_t launch_eflag(const EflagArgs &a, hipStream_t s);
-hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter, hipStream_t s);
+hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
+                       unsigned long long *dmax, hipStream_t s);
+hipError_t launch_to_f32(const double4 *src, float4 *dst, int64_t n, hipStream_t s);
+hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hipStream_t s);
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
+hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
 
 }  // namespace bdf

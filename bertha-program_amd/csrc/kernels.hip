@@ -16,6 +16,7 @@ namespace bdf {
 
 __constant__ uint16_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits
 __constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
+__constant__ float c_rcpf[2 * kMaxLfl + 2 * 60 + 4];
 
 __device__ __forceinline__ int hidx(int t, int u, int v) {
   int L = t + u + v;
@@ -147,39 +148,132 @@ __global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Schwarz tables
+// ---------------------------------------------------------------------------
+// R^(0)_{tuv} at P = Q: (-2a)^k (t-1)!!(u-1)!!(v-1)!!/(2k+1) for even t,u,v
+// (k = (t+u+v)/2), zero otherwise -- the Taylor coefficients of F_0(a r^2).
+__device__ __forceinline__ double r_origin(int t, int u, int v, double m2a) {
+  if ((t | u | v) & 1) return 0.0;
+  int k = (t + u + v) >> 1;
+  double x = c_rcp[k];
+  for (int i = 0; i < k; ++i) x *= m2a;
+  for (int i = t - 1; i > 1; i -= 2) x *= i;
+  for (int i = u - 1; i > 1; i -= 2) x *= i;
+  for (int i = v - 1; i > 1; i -= 2) x *= i;
+  return x;
+}
+
+// One primitive pair (i,j) of one block pair: the two sign-class integrals
+// s0 = sum Z E11cd G11 and s1 = sum Z E21cd G21 (conj(G21) for the bound
+// table), G = E_AB . R(P=Q); the four sign slots take sqrt|.| (CLMDIAG
+// :38399-38404 for GDSC).
+__global__ void __launch_bounds__(128) diag_kernel(DiagArgs a) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nitems) return;
+  int lo = 0, hi = a.njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.first[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const DiagJob J = a.jobs[lo];
+  int ij = (int)(g - a.first[lo]);
+  int i = ij / J.nB, j = ij % J.nB;
+  int maxab = J.nA * J.nB;
+  int kl = J.swap ? j * J.nA + i : ij;
+  double p = a.exps[J.expA + i] + a.exps[J.expB + j];
+  double m2a = -p;  // -2 alpha, alpha = p q / (p + q) = p / 2
+  double pre = 2.0 * 1.7493418327624863e1 / (sqrt(2.0 * p) * p * p);
+  double sr[2] = {0.0, 0.0}, si[2] = {0.0, 0.0};
+  for (int tp = 0; tp < J.ntcd; ++tp) {
+    int t2, u2, v2;
+    dec(tp, t2, u2, v2);
+    double z = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+    double g11r = 0, g11i = 0, g21r = 0, g21i = 0;
+    for (int t = 0; t < J.ntab; ++t) {
+      int t1, u1, v1;
+      dec(t, t1, u1, v1);
+      if (((t1 + t2) | (u1 + u2) | (v1 + v2)) & 1) continue;
+      double r = r_origin(t1 + t2, u1 + u2, v1 + v2, m2a);
+      double4 e = a.E[J.eab + (int64_t)t * maxab + ij];
+      g11r += e.x * r; g11i += e.y * r; g21r += e.z * r; g21i += e.w * r;
+    }
+    double4 e = a.E[J.ecd + (int64_t)tp * maxab + kl];
+    sr[0] += z * (e.x * g11r - e.y * g11i);
+    si[0] += z * (e.x * g11i + e.y * g11r);
+    if (J.swap) {
+      sr[1] += z * (e.z * g21r + e.w * g21i);
+      si[1] += z * (e.w * g21r - e.z * g21i);
+    } else {
+      sr[1] += z * (e.z * g21r - e.w * g21i);
+      si[1] += z * (e.z * g21i + e.w * g21r);
+    }
+  }
+  double q[2];
+  for (int s = 0; s < 2; ++s) {
+    double m = J.realpart ? fabs(sr[s]) : sqrt(sr[s] * sr[s] + si[s] * si[s]);
+    q[s] = sqrt(pre * m);
+  }
+  double *out = J.out ? a.qbnd : a.gdsc;
+  int nd = a.ndim;
+  out[(int64_t)(J.col[0] + j) * nd + J.row[0] + i] = q[0];
+  out[(int64_t)(J.col[1] + j) * nd + J.row[1] + i] = q[0];
+  out[(int64_t)(J.col[1] + j) * nd + J.row[0] + i] = q[1];
+  out[(int64_t)(J.col[0] + j) * nd + J.row[1] + i] = q[1];
+}
+
+// ---------------------------------------------------------------------------
 // Fock build
 // ---------------------------------------------------------------------------
+// Arithmetic of one precision: the FP64 path and the FP32 path of the
+// Schwarz split share the code below.
+template <typename T>
+struct Arith;
+template <>
+struct Arith<double> {
+  __device__ static double ex(double x) { return exp(x); }
+  __device__ static double sq(double x) { return sqrt(x); }
+  __device__ static double rcp(int k) { return c_rcp[k]; }
+};
+template <>
+struct Arith<float> {
+  __device__ static float ex(float x) { return expf(x); }
+  __device__ static float sq(float x) { return sqrtf(x); }
+  __device__ static float rcp(int k) { return c_rcpf[k]; }
+};
+
 // Boys function F_0..F_L of the reference's four branches (FUNFM, RMAKE's
 // thresholds 1e-11 / 17 / 30, bertha-scf.f:31927-31985, 32495-32605),
 // written to f[n*ld].
-__device__ void boys_ref(double T, int L, double *f, int ld) {
-  const double kA0 = 4.994501191201870e-1, kB0 = 4.551838436668326e-1;
-  const double kSqrtPiHalf = 0.5 * 1.7724538509055160;
-  if (T <= 1.0e-11) {
-    for (int m = 0; m <= L; ++m) f[m * ld] = c_rcp[m];
-  } else if (T <= 17.0) {
-    double txp = exp(-T), tt2 = 2.0 * T, tl = 1.0, s = 1.0;
+template <typename T>
+__device__ void boys_ref(T t, int L, T *f, int ld) {
+  using A = Arith<T>;
+  const T kA0 = (T)4.994501191201870e-1, kB0 = (T)4.551838436668326e-1;
+  const T kSqrtPiHalf = (T)(0.5 * 1.7724538509055160);
+  if (t <= (T)1.0e-11) {
+    for (int m = 0; m <= L; ++m) f[m * ld] = A::rcp(m);
+  } else if (t <= (T)17.0) {
+    T txp = A::ex(-t), tt2 = 2 * t, tl = 1, s = 1;
     for (int k = 1; k <= 60; ++k) {
-      tl = tl * tt2 * c_rcp[L + k];
+      tl = tl * tt2 * A::rcp(L + k);
       s += tl;
     }
-    double fm = s * txp * c_rcp[L];
+    T fm = s * txp * A::rcp(L);
     f[L * ld] = fm;
     for (int m = L - 1; m >= 0; --m) {
-      fm = (tt2 * fm + txp) * c_rcp[m];
+      fm = (tt2 * fm + txp) * A::rcp(m);
       f[m * ld] = fm;
     }
-  } else if (T <= 30.0) {
-    double txp = exp(-T), rt2 = 1.0 / (2.0 * T);
-    double fm = kSqrtPiHalf / sqrt(T) - txp * (kA0 / (kB0 + T));
+  } else if (t <= (T)30.0) {
+    T txp = A::ex(-t), rt2 = 1 / (2 * t);
+    T fm = kSqrtPiHalf / A::sq(t) - txp * (kA0 / (kB0 + t));
     f[0] = fm;
     for (int m = 0; m < L; ++m) {
       fm = ((2 * m + 1) * fm - txp) * rt2;
       f[(m + 1) * ld] = fm;
     }
   } else {
-    double rt2 = 1.0 / (2.0 * T);
-    double fm = kSqrtPiHalf / sqrt(T);
+    T rt2 = 1 / (2 * t);
+    T fm = kSqrtPiHalf / A::sq(t);
     f[0] = fm;
     for (int m = 0; m < L; ++m) {
       fm = (2 * m + 1) * fm * rt2;
@@ -192,10 +286,10 @@ __device__ void boys_ref(double T, int L, double *f, int ld) {
 // R^(n)_{000} = (-2a)^n F_n and R^(n)_{t+1,u,v} = X R^(n+1)_{tuv} +
 // t R^(n+1)_{t-1,u,v} (McMurchie-Davidson), descending levels so each slot is
 // overwritten only after the lower order has been read.
-__device__ void rints(double *r, const double *f, int ld, int L, double alpha, double X,
-                      double Y, double Z) {
-  double m2a = -2.0 * alpha;
-  double pw = 1.0;
+template <typename T>
+__device__ void rints(T *r, const T *f, int ld, int L, T alpha, T X, T Y, T Z) {
+  T m2a = -2 * alpha;
+  T pw = 1;
   for (int k = 0; k < L; ++k) pw *= m2a;
   r[0] = pw * f[L * ld];
   for (int n = L - 1; n >= 0; --n) {
@@ -203,7 +297,7 @@ __device__ void rints(double *r, const double *f, int ld, int L, double alpha, d
       for (int t = lev; t >= 0; --t)
         for (int u = lev - t; u >= 0; --u) {
           int v = lev - t - u;
-          double val;
+          T val;
           if (t > 0) {
             val = X * r[hidx(t - 1, u, v) * ld];
             if (t > 1) val += (t - 1) * r[hidx(t - 2, u, v) * ld];
@@ -217,7 +311,7 @@ __device__ void rints(double *r, const double *f, int ld, int L, double alpha, d
           r[hidx(t, u, v) * ld] = val;
         }
     }
-    pw = 1.0;
+    pw = 1;
     for (int k = 0; k < n; ++k) pw *= m2a;
     r[0] = pw * f[n * ld];
   }
@@ -354,26 +448,331 @@ __device__ __forceinline__ void digest(const double (&rr)[16], const double (&ri
 #undef XB
 }
 
-__global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
-  __shared__ double s_eab[ntuv(kMaxLab) * 4];
-  __shared__ uint8_t s_cflag[ntuv(kMaxLfl)];
-  __shared__ uint8_t s_gm[ntuv(kMaxLab) * 4];
-  __shared__ uint8_t s_km[ntuv(kMaxLab) * 4];
-  __shared__ int s_item;
-  __shared__ XAcc s_x;
-  const int lane = threadIdx.x;
+// SCHWARZ (bertha-scf.f:37889-38123) for one kl of one ERIFAST call: bit b
+// keeps digestion batch b+1 (IMTX), 0 drops the kl (ISCR = 0).  Bound groups
+// in the reference's order: direct (1,2) with D(C,D), (3,4) with D(A,B);
+// exchange (8,9) with D(A,D), (5) with D(B,C), (7,11) with D(A,C), (6,10)
+// with D(B,D); the first group above SENS ends the test.
+__device__ __forceinline__ double slot_max(const double *G, int nd, const int (&r)[2], int ir,
+                                           const int (&c)[2], int ic) {
+  double x = 0.0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) x = fmax(x, G[(int64_t)(c[w] + ic) * nd + r[u] + ir]);
+  return x;
+}
+__device__ __forceinline__ double slot_absmax(const double2 *D, int nd, const int (&r)[2], int ir,
+                                              const int (&c)[2], int ic) {
+  double x = 0.0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      double2 d = D[(int64_t)(c[w] + ic) * nd + r[u] + ir];
+      x = fmax(x, hypot(d.x, d.y));
+    }
+  return x;
+}
+__device__ int schwarz_mask(const double *G, const double2 *D, int nd, const int (&off)[4][2],
+                            const int (&idx)[4], double sens) {
+  const double gdir = slot_max(G, nd, off[0], idx[0], off[1], idx[1]) *
+                      slot_max(G, nd, off[2], idx[2], off[3], idx[3]);
+  const double gx1 = slot_max(G, nd, off[1], idx[1], off[2], idx[2]) *
+                     slot_max(G, nd, off[0], idx[0], off[3], idx[3]);
+  const double gx2 = slot_max(G, nd, off[1], idx[1], off[3], idx[3]) *
+                     slot_max(G, nd, off[0], idx[0], off[2], idx[2]);
+  int mask = 0x7ff;
+  if (gdir * slot_absmax(D, nd, off[2], idx[2], off[3], idx[3]) > sens) return mask;
+  mask &= ~0x003;
+  if (gdir * slot_absmax(D, nd, off[0], idx[0], off[1], idx[1]) > sens) return mask;
+  mask &= ~0x00c;
+  if (gx1 * slot_absmax(D, nd, off[0], idx[0], off[3], idx[3]) > sens) return mask;
+  mask &= ~0x180;
+  if (gx1 * slot_absmax(D, nd, off[1], idx[1], off[2], idx[2]) > sens) return mask;
+  mask &= ~0x010;
+  if (gx2 * slot_absmax(D, nd, off[0], idx[0], off[2], idx[2]) > sens) return mask;
+  mask &= ~0x440;
+  if (gx2 * slot_absmax(D, nd, off[1], idx[1], off[3], idx[3]) > sens) return mask;
+  return 0;
+}
+
+// E-table rows in the working precision: the FP64 table, or its FP32 copy
+__device__ __forceinline__ void lde(const FockArgs &a, int64_t i, double &x, double &y, double &z,
+                                    double &w) {
+  double4 e = a.E[i];
+  x = e.x; y = e.y; z = e.z; w = e.w;
+}
+__device__ __forceinline__ void lde(const FockArgs &a, int64_t i, float &x, float &y, float &z,
+                                    float &w) {
+  float4 e = a.E32[i];
+  x = e.x; y = e.y; z = e.z; w = e.w;
+}
+
+// Shared-memory state of one workgroup
+struct FockShared {
+  double eab[ntuv(kMaxLab) * 4];
+  uint8_t cflag[ntuv(kMaxLfl)];
+  uint8_t gm[ntuv(kMaxLab) * 4];
+  uint8_t km[ntuv(kMaxLab) * 4];
+  XAcc x;
+};
+
+// One work item (class, ij) in precision T: R-integrals for every kl, the two
+// E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
+// accumulates in FP64.
+template <typename T>
+__device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, FockShared &sh,
+                          int lane) {
   const int ld = a.ld;
-  double *R = a.slab + (int64_t)blockIdx.x * a.slab_stride;  // [rows][ld]
-  double *F = R + (int64_t)ntuv(kMaxLfl) * ld;                // [kMaxLfl+1][ld]
-  double *G = F + (int64_t)(kMaxLfl + 1) * ld;                // [ntg*4][ld]
+  T *R = reinterpret_cast<T *>(slab);            // [rows][ld]
+  T *F = R + (int64_t)ntuv(kMaxLfl) * ld;         // [kMaxLfl+1][ld]
+  T *G = F + (int64_t)(kMaxLfl + 1) * ld;         // [ntg*4][ld]
+  T *s_eab = reinterpret_cast<T *>(sh.eab);
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
-  {
-    double *z = &s_x.l[0][0][0][0];
-    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
+  const ClassDesc &cl = a.classes[it.cls];
+  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int ib = it.ij / nB, jb = it.ij % nB;
+  const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
+  const double p = ea + eb;
+  const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
+  const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
+  const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
+  const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
+  const int maxab = cl.nbas[0] * nB;
+
+  // R-integrals for every kl of this ij, then the reference's column test
+  // (sum over kl of |R| > 1e-10 before the prefactor, :12660-12675)
+  for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+    int kl = kl0 + lane;
+    if (kl < maxcd) {
+      int kb = kl / nD, lb = kl % nD;
+      double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
+      double q = ec + ed;
+      double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
+      double Qy = (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q;
+      double Qz = (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q;
+      double alpha = p * q / (p + q);
+      double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
+      double Targ = alpha * (X * X + Y * Y + Z * Z);
+      boys_ref<T>((T)Targ, L, F + kl, ld);
+      rints<T>(R + kl, F + kl, ld, L, (T)alpha, (T)X, (T)Y, (T)Z);
+    }
+  }
+  __syncthreads();
+  for (int col = lane; col < nt; col += 64) {
+    T s = 0;
+    const T *row = R + (int64_t)col * ld;
+    for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
+    sh.cflag[col] = s > (T)1.0e-10;
+  }
+  __syncthreads();
+  for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+    int kl = kl0 + lane;
+    if (kl < maxcd) {
+      int kb = kl / nD, lb = kl % nD;
+      double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
+      T pre = (T)(2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
+      for (int col = 0; col < nt; ++col) {
+        T *x = R + (int64_t)col * ld + kl;
+        *x = sh.cflag[col] ? pre * *x : (T)0;
+      }
+    }
   }
   __syncthreads();
 
+  for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+    const ABCombo ab = a.abs[abi];
+    for (int t = lane; t < ab.ntab; t += 64) {
+      T ex, ey, ez, ew;
+      lde(a, ab.eoff + (int64_t)t * maxab + it.ij, ex, ey, ez, ew);
+      // E(AB) entries of magnitude <= 1e-16 are skipped by ERIFAST (:12445)
+      s_eab[4 * t + 0] = fabs(ex) > (T)1.0e-16 ? ex : (T)0;
+      s_eab[4 * t + 1] = fabs(ey) > (T)1.0e-16 ? ey : (T)0;
+      s_eab[4 * t + 2] = fabs(ez) > (T)1.0e-16 ? ez : (T)0;
+      s_eab[4 * t + 3] = fabs(ew) > (T)1.0e-16 ? ew : (T)0;
+    }
+    for (int t = lane; t < 4 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
+    __syncthreads();
+    // first contraction G(kl,tuv') = sum_tuv E_AB(tuv) R(tuv+tuv')
+    for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+      int kl = kl0 + lane;
+      if (kl >= maxcd) continue;
+      for (int tp = 0; tp < ab.ntg; ++tp) {
+        T g0 = 0, g1 = 0, g2 = 0, g3 = 0;
+        if (sh.gm[4 * tp] | sh.gm[4 * tp + 1] | sh.gm[4 * tp + 2] | sh.gm[4 * tp + 3]) {
+          int t2, u2, v2;
+          dec(tp, t2, u2, v2);
+          for (int t = 0; t < ab.ntab; ++t) {
+            int t1, u1, v1;
+            dec(t, t1, u1, v1);
+            T r = R[(int64_t)hidx(t1 + t2, u1 + u2, v1 + v2) * ld + kl];
+            g0 += s_eab[4 * t + 0] * r;
+            g1 += s_eab[4 * t + 1] * r;
+            g2 += s_eab[4 * t + 2] * r;
+            g3 += s_eab[4 * t + 3] * r;
+          }
+        }
+        T *gp = G + (int64_t)(4 * tp) * ld + kl;
+        gp[0] = sh.gm[4 * tp + 0] ? g0 : (T)0;
+        gp[ld] = sh.gm[4 * tp + 1] ? g1 : (T)0;
+        gp[2 * ld] = sh.gm[4 * tp + 2] ? g2 : (T)0;
+        gp[3 * ld] = sh.gm[4 * tp + 3] ? g3 : (T)0;
+      }
+    }
+    double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
+    for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
+      const CDCall cd = a.cds[ci];
+      __syncthreads();
+      for (int t = lane; t < 4 * cd.ntcd; t += 64) sh.km[t] = a.masks[cd.kmask + t];
+      __syncthreads();
+      const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+      const double pabcd = pab1 * pcd1;
+      for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+        int kl = kl0 + lane;
+        if (kl >= maxcd) continue;
+        // second contraction (ERIFAST :12809-13119): a_s = sum Z E_CD G,
+        // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
+        T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+        T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+        for (int tp = 0; tp < cd.ntcd; ++tp) {
+          T ex, ey, ez, ew;
+          lde(a, cd.eoff + (int64_t)tp * maxcd + kl, ex, ey, ez, ew);
+          int t2, u2, v2;
+          dec(tp, t2, u2, v2);
+          T z = ((t2 + u2 + v2) & 1) ? (T)-1 : (T)1;
+          T e11r = sh.km[4 * tp + 0] ? z * ex : (T)0;
+          T e11i = sh.km[4 * tp + 1] ? z * ey : (T)0;
+          T e21r = sh.km[4 * tp + 2] ? z * ez : (T)0;
+          T e21i = sh.km[4 * tp + 3] ? z * ew : (T)0;
+          const T *gp = G + (int64_t)(4 * tp) * ld + kl;
+          T g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
+          // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
+          // set 3: E21cd x G21
+          ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
+          br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
+          ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
+          br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
+          ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
+          br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
+          ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
+          br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
+        }
+        // RR(n), n = 1 + 8 sA + 4 sB + 2 sC + sD (s = 1 for +|m|), in FP64
+        double rr_r[16], rr_i[16];
+        if (a.isym != 0) {
+          for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
+          rr_r[0] = ar[0];
+          rr_r[12] = pab1 * br[0];
+          rr_r[15] = pabcd * rr_r[0];
+          rr_r[3] = pabcd * rr_r[12];
+          rr_r[10] = ar[3];
+          rr_r[6] = -pab1 * br[3];
+          rr_r[5] = pabcd * rr_r[10];
+          rr_r[9] = pabcd * rr_r[6];
+        } else {
+          rr_r[0] = ar[0]; rr_i[0] = ai[0];
+          rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
+          rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
+          rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
+          rr_r[2] = ar[1]; rr_i[2] = ai[1];
+          rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
+          rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
+          rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
+          rr_r[8] = ar[2]; rr_i[8] = ai[2];
+          rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
+          rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
+          rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
+          rr_r[10] = ar[3]; rr_i[10] = ai[3];
+          rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
+          rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
+          rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
+        }
+        if (a.rr_out) {
+          for (int n = 0; n < 16; ++n)
+            a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
+          continue;
+        }
+        // digestion (CLMMTG / CLMMTZ)
+        const int idx[4] = {ib, jb, kl / nD, kl % nD};
+        const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                               {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+        int flg = cd.iflg;
+        // ITOG: screening only when an SS pair is involved (ITT > 1, :12096)
+        if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
+          flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
+        if (a.isym != 0)
+          digest<true>(rr_r, rr_i, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+                       cd.pcd[1], accAB, sh.x, a.gdir);
+        else
+          digest<false>(rr_r, rr_i, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+                        cd.pcd[1], accAB, sh.x, a.gdir);
+      }
+      if (!a.rr_out) {
+        // flush the exchange accumulators of this call (one atomic per
+        // output element instead of one per quartet), then clear them
+        __syncthreads();
+        const int nC = cl.nbas[2];
+        const int flg = a.intsym ? cd.iflg : (cd.iflg & 0x11);
+        const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+        for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
+          const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
+          const int b = 4 + bsel;  // batch index 4..10 (5..11)
+          if (!((flg >> b) & 1)) continue;
+          const bool isl = (b == 4 || b == 6 || b == 10);
+          if (pos >= (isl ? nD : nC)) continue;
+          double *v;
+          int row, col;
+          const int rs = s >> 1, cs = s & 1;
+          switch (b) {
+            case 4: v = sh.x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
+            case 5: v = sh.x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
+            case 6: v = sh.x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
+            case 7: v = sh.x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
+            case 8: v = sh.x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
+            case 9: v = sh.x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
+            default: v = sh.x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
+          }
+          if (v[0] != 0.0 || v[1] != 0.0)
+            atomic_add2(&a.gxch[(int64_t)col * nd + row], v[0], v[1]);
+          v[0] = 0.0;
+          v[1] = 0.0;
+        }
+        __syncthreads();
+      }
+    }
+    if (!a.rr_out) {
+      // batches 1 and 2 write G(A sA, B sB)(i,j): reduce over the wave
+      for (int s = 0; s < 4; ++s) {
+        double re = wave_sum(accAB[s][0]);
+        double im = wave_sum(accAB[s][1]);
+        if (lane == 0 && (re != 0.0 || im != 0.0)) {
+          int orr = ab.fa[s >> 1] + ib, occ = ab.fb[s & 1] + jb;
+          atomic_add2(&a.gdir[(int64_t)occ * nd + orr], re, im);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Persistent work-queue kernel: each 64-lane workgroup takes (class, ij)
+// items from a global counter until the list is exhausted.  With the Schwarz
+// split on, an item whose largest possible quartet contribution
+// qitem * max|D| is below the split threshold runs in FP32.
+__global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
+  __shared__ FockShared sh;
+  __shared__ int s_item;
+  const int lane = threadIdx.x;
+  double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
+  {
+    double *z = &sh.x.l[0][0][0][0];
+    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
+  }
+  const double dmax = a.mixed ? __longlong_as_double(*a.dmax_bits) : 0.0;
+  __syncthreads();
   for (;;) {
     if (lane == 0) s_item = atomicAdd(a.counter, 1);
     __syncthreads();
@@ -381,226 +780,22 @@ __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
     __syncthreads();
     if (idx >= a.nitems) break;
     const WorkItem it = a.items[idx];
-    const ClassDesc &cl = a.classes[it.cls];
-    const int nB = cl.nbas[1], nD = cl.nbas[3];
-    const int ib = it.ij / nB, jb = it.ij % nB;
-    const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
-    const double p = ea + eb;
-    const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
-    const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
-    const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
-    const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
-    const int maxab = cl.nbas[0] * nB;
-
-    // R-integrals for every kl of this ij, then the reference's column test
-    // (sum over kl of |R| > 1e-10 before the prefactor, :12660-12675)
-    for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
-      int kl = kl0 + lane;
-      if (kl < maxcd) {
-        int kb = kl / nD, lb = kl % nD;
-        double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
-        double q = ec + ed;
-        double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
-        double Qy = (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q;
-        double Qz = (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q;
-        double alpha = p * q / (p + q);
-        double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
-        double T = alpha * (X * X + Y * Y + Z * Z);
-        boys_ref(T, L, F + kl, ld);
-        rints(R + kl, F + kl, ld, L, alpha, X, Y, Z);
-      }
-    }
-    __syncthreads();
-    for (int col = lane; col < nt; col += 64) {
-      double s = 0.0;
-      const double *row = R + (int64_t)col * ld;
-      for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
-      s_cflag[col] = s > 1.0e-10;
-    }
-    __syncthreads();
-    for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
-      int kl = kl0 + lane;
-      if (kl < maxcd) {
-        int kb = kl / nD, lb = kl % nD;
-        double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
-        double pre = 2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q);
-        for (int col = 0; col < nt; ++col) {
-          double *x = R + (int64_t)col * ld + kl;
-          *x = s_cflag[col] ? pre * *x : 0.0;
-        }
-      }
-    }
-    __syncthreads();
-
-    for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
-      const ABCombo ab = a.abs[abi];
-      for (int t = lane; t < ab.ntab; t += 64) {
-        double4 e = a.E[ab.eoff + (int64_t)t * maxab + it.ij];
-        // E(AB) entries of magnitude <= 1e-16 are skipped by ERIFAST (:12445)
-        s_eab[4 * t + 0] = fabs(e.x) > 1.0e-16 ? e.x : 0.0;
-        s_eab[4 * t + 1] = fabs(e.y) > 1.0e-16 ? e.y : 0.0;
-        s_eab[4 * t + 2] = fabs(e.z) > 1.0e-16 ? e.z : 0.0;
-        s_eab[4 * t + 3] = fabs(e.w) > 1.0e-16 ? e.w : 0.0;
-      }
-      for (int t = lane; t < 4 * ab.ntg; t += 64) s_gm[t] = a.masks[ab.gmask + t];
-      __syncthreads();
-      // first contraction G(kl,tuv') = sum_tuv E_AB(tuv) R(tuv+tuv')
-      for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
-        int kl = kl0 + lane;
-        if (kl >= maxcd) continue;
-        for (int tp = 0; tp < ab.ntg; ++tp) {
-          double g0 = 0, g1 = 0, g2 = 0, g3 = 0;
-          if (s_gm[4 * tp] | s_gm[4 * tp + 1] | s_gm[4 * tp + 2] | s_gm[4 * tp + 3]) {
-            int t2, u2, v2;
-            dec(tp, t2, u2, v2);
-            for (int t = 0; t < ab.ntab; ++t) {
-              int t1, u1, v1;
-              dec(t, t1, u1, v1);
-              double r = R[(int64_t)hidx(t1 + t2, u1 + u2, v1 + v2) * ld + kl];
-              g0 += s_eab[4 * t + 0] * r;
-              g1 += s_eab[4 * t + 1] * r;
-              g2 += s_eab[4 * t + 2] * r;
-              g3 += s_eab[4 * t + 3] * r;
-            }
-          }
-          double *gp = G + (int64_t)(4 * tp) * ld + kl;
-          gp[0] = s_gm[4 * tp + 0] ? g0 : 0.0;
-          gp[ld] = s_gm[4 * tp + 1] ? g1 : 0.0;
-          gp[2 * ld] = s_gm[4 * tp + 2] ? g2 : 0.0;
-          gp[3 * ld] = s_gm[4 * tp + 3] ? g3 : 0.0;
-        }
-      }
-      double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
-      for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
-        const CDCall cd = a.cds[ci];
-        __syncthreads();
-        for (int t = lane; t < 4 * cd.ntcd; t += 64) s_km[t] = a.masks[cd.kmask + t];
-        __syncthreads();
-        const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
-        const double pabcd = pab1 * pcd1;
-        for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
-          int kl = kl0 + lane;
-          if (kl >= maxcd) continue;
-          // second contraction (ERIFAST :12809-13119): a_s = sum Z E_CD G,
-          // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
-          double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-          double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
-          for (int tp = 0; tp < cd.ntcd; ++tp) {
-            double4 e = a.E[cd.eoff + (int64_t)tp * maxcd + kl];
-            int t2, u2, v2;
-            dec(tp, t2, u2, v2);
-            double z = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
-            double e11r = s_km[4 * tp + 0] ? z * e.x : 0.0;
-            double e11i = s_km[4 * tp + 1] ? z * e.y : 0.0;
-            double e21r = s_km[4 * tp + 2] ? z * e.z : 0.0;
-            double e21i = s_km[4 * tp + 3] ? z * e.w : 0.0;
-            const double *gp = G + (int64_t)(4 * tp) * ld + kl;
-            double g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
-            // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
-            // set 3: E21cd x G21
-            ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
-            br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
-            ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
-            br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
-            ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
-            br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
-            ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
-            br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
-          }
-          // RR(n), n = 1 + 8 sA + 4 sB + 2 sC + sD (s = 1 for +|m|)
-          double rr_r[16], rr_i[16];
-          if (a.isym != 0) {
-            for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
-            rr_r[0] = ar[0];
-            rr_r[12] = pab1 * br[0];
-            rr_r[15] = pabcd * rr_r[0];
-            rr_r[3] = pabcd * rr_r[12];
-            rr_r[10] = ar[3];
-            rr_r[6] = -pab1 * br[3];
-            rr_r[5] = pabcd * rr_r[10];
-            rr_r[9] = pabcd * rr_r[6];
-          } else {
-            rr_r[0] = ar[0]; rr_i[0] = ai[0];
-            rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
-            rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
-            rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
-            rr_r[2] = ar[1]; rr_i[2] = ai[1];
-            rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
-            rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
-            rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
-            rr_r[8] = ar[2]; rr_i[8] = ai[2];
-            rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
-            rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
-            rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
-            rr_r[10] = ar[3]; rr_i[10] = ai[3];
-            rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
-            rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
-            rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
-          }
-          if (a.rr_out) {
-            for (int n = 0; n < 16; ++n)
-              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
-            continue;
-          }
-          // digestion (CLMMTG / CLMMTZ)
-          const int idx[4] = {ib, jb, kl / nD, kl % nD};
-          const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-          if (a.isym != 0)
-            digest<true>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                         cd.pcd[1], accAB, s_x, a.gdir);
-          else
-            digest<false>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1],
-                          pcd1, cd.pcd[1], accAB, s_x, a.gdir);
-        }
-        if (!a.rr_out) {
-          // flush the exchange accumulators of this call (one atomic per
-          // output element instead of one per quartet), then clear them
-          __syncthreads();
-          const int nC = cl.nbas[2];
-          const int flg = a.intsym ? cd.iflg : (cd.iflg & 0x11);
-          const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-          for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
-            const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
-            const int b = 4 + bsel;  // batch index 4..10 (5..11)
-            if (!((flg >> b) & 1)) continue;
-            const bool isl = (b == 4 || b == 6 || b == 10);
-            if (pos >= (isl ? nD : nC)) continue;
-            double *v;
-            int row, col;
-            const int rs = s >> 1, cs = s & 1;
-            switch (b) {
-              case 4: v = s_x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
-              case 5: v = s_x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
-              case 6: v = s_x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
-              case 7: v = s_x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
-              case 8: v = s_x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
-              case 9: v = s_x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
-              default: v = s_x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
-            }
-            if (v[0] != 0.0 || v[1] != 0.0)
-              atomic_add2(&a.gxch[(int64_t)col * nd + row], v[0], v[1]);
-            v[0] = 0.0;
-            v[1] = 0.0;
-          }
-          __syncthreads();
-        }
-      }
-      if (!a.rr_out) {
-        // batches 1 and 2 write G(A sA, B sB)(i,j): reduce over the wave
-        for (int s = 0; s < 4; ++s) {
-          double re = wave_sum(accAB[s][0]);
-          double im = wave_sum(accAB[s][1]);
-          if (lane == 0 && (re != 0.0 || im != 0.0)) {
-            int orr = ab.fa[s >> 1] + ib, occ = ab.fb[s & 1] + jb;
-            atomic_add2(&a.gdir[(int64_t)occ * nd + orr], re, im);
-          }
-        }
-      }
-      __syncthreads();
-    }
+    if (a.mixed && a.qitem[idx] * dmax < a.split)
+      fock_item<float>(a, it, slab, sh, lane);
+    else
+      fock_item<double>(a, it, slab, sh, lane);
   }
+}
+
+// largest |D| of the density (non-negative doubles order like their bit
+// patterns, so an integer max is exact)
+__global__ void dmax_kernel(const double2 *d, int64_t n, unsigned long long *out) {
+  double m = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, hypot(d[g].x, d[g].y));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
 // Hermitian completion of GDIR/GXCH (CLMFAST :12166-12205): thread per
@@ -631,13 +826,15 @@ __global__ void complete_kernel(CompleteArgs a) {
   comp(a.gxch, k, j);
 }
 
-__global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter) {
+__global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter,
+                            unsigned long long *dmax) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g < n) {
     a[g] = make_double2(0.0, 0.0);
     if (b) b[g] = make_double2(0.0, 0.0);
   }
   if (g == 0 && counter) *counter = 0;
+  if (g == 0 && dmax) *dmax = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +853,13 @@ hipError_t upload_constants() {
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tuv), tuv, sizeof(tuv));
   if (e != hipSuccess) return e;
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
-  for (int k = 0; k < (int)(sizeof(rcp) / sizeof(double)); ++k) rcp[k] = 1.0 / (2.0 * k + 1.0);
+  float rcpf[2 * kMaxLfl + 2 * 60 + 4];
+  for (int k = 0; k < (int)(sizeof(rcp) / sizeof(double)); ++k) {
+    rcp[k] = 1.0 / (2.0 * k + 1.0);
+    rcpf[k] = (float)rcp[k];
+  }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_rcpf), rcpf, sizeof(rcpf));
+  if (e != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_rcp), rcp, sizeof(rcp));
 }
 
@@ -672,15 +875,41 @@ hipError_t launch_eflag(const EflagArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(eflag_kernel, dim3(a.npairs), dim3(64), 0, s, a);
   return hipGetLastError();
 }
-hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter, hipStream_t s) {
+hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
+                       unsigned long long *dmax, hipStream_t s) {
   int bs = 256;
   int64_t nb = (n + bs - 1) / bs;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(bs), 0, s, x, n, y, counter);
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(bs), 0, s, x, n, y, counter, dmax);
+  return hipGetLastError();
+}
+__global__ void to_f32_kernel(const double4 *src, float4 *dst, int64_t n) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) {
+    double4 v = src[g];
+    dst[g] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+  }
+}
+hipError_t launch_to_f32(const double4 *src, float4 *dst, int64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  int bs = 256;
+  hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, src, dst,
+                     n);
+  return hipGetLastError();
+}
+hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hipStream_t s) {
+  hipLaunchKernelGGL(dmax_kernel, dim3(256), dim3(256), 0, s, d, n, out);
   return hipGetLastError();
 }
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(fock_kernel, dim3(nblocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
+  if (a.nitems == 0) return hipSuccess;
+  int bs = 128;
+  int64_t nb = (a.nitems + bs - 1) / bs;
+  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)nb), dim3(bs), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s) {

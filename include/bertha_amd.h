@@ -71,8 +71,15 @@ typedef struct {
   int racah1;          /* RACAH1: skip one-centre quartets (reference default 1;
                           this engine computes them by MD when 0) */
   int precision;       /* BDF_PREC_* */
-  double split_threshold; /* MIXED: quartets with Schwarz bound below it in FP32 */
+  double split_threshold; /* MIXED: a work item (class, primitive pair ij) whose
+                             largest quartet contribution bound
+                             Q(ij) max Q(kl) max|D| is below it runs in FP32
+                             (<= 0: 1e-5); Q = sqrt|(ab|ba)| */
   int device;          /* HIP device ordinal */
+  int schwarz;         /* SCHWRZ: SCHWARZ masks from CLMDIAG bounds (reference
+                          default 0, bertha-scf.f:5257) */
+  double sens;         /* SENS of the masks; <= 0 selects the reference's 1e-12
+                          (bertha-scf.f:11567) */
 } bdf_options;
 
 typedef struct bdf_engine bdf_engine;
@@ -117,6 +124,17 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
    Re E21, Im E21.  Returns the element count per component in *n. */
 int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb,
                int ma, int mb, double *out, long cap, long *n);
+
+/* Schwarz tables, ndim x ndim real, column-major, leading dimension ld
+   (either pointer may be NULL):
+   gdsc -- CLMDIAG's GDSC (bertha-scf.f:38126-38445): sqrt|(a b|a b)| of the
+           self-pair integrals, row block by A's component type, column block
+           by the second pair's, sign slots (--|--)=(++|++), (-+|-+)=(+-|+-);
+           real parts only on the ISYM 1/2 path as ERI computes them.
+   qbnd -- sqrt|(a b|b a)|, the Cauchy-Schwarz factor of the Coulomb metric
+           (|(ab|cd)| <= qbnd(a,b) qbnd(c,d)); the engine's FP32/FP64 split
+           uses it.  No counterpart in the reference. */
+int bdf_schwarz_tables(bdf_engine *e, double *gdsc, double *qbnd, int ld);
 
 /* Engine's HIP stream (hipStream_t) for callers that order their own work
    after bdf_coulomb_device. */

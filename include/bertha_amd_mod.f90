@@ -19,7 +19,8 @@ module bertha_amd
   type, bind(c) :: bdf_options
     integer(c_int) :: intsym, racah1, precision
     real(c_double) :: split_threshold
-    integer(c_int) :: device
+    integer(c_int) :: device, schwarz
+    real(c_double) :: sens
   end type bdf_options
 
   interface
@@ -65,6 +66,13 @@ module bertha_amd
       real(c_double), intent(out) :: out(*)
       integer(c_long), value :: cap
       integer(c_long), intent(out) :: n
+    end function
+
+    integer(c_int) function bdf_schwarz_tables(eng, gdsc, qbnd, ld) &
+        bind(c, name='bdf_schwarz_tables')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, gdsc, qbnd
+      integer(c_int), value :: ld
     end function
 
     type(c_ptr) function bdf_stream(eng) bind(c, name='bdf_stream')

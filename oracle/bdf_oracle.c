@@ -29,8 +29,13 @@
  *   the reference's MCT=2 build cannot produce them: for those the pin is
  *   translation/rotation invariance of the two-electron energy.
  *
- * Schwarz screening (SCHWRZ) and open-shell matrices are not restated: the
- * reference hard-codes SCHWRZ=.FALSE. (5257) and the configs are closed-shell.
+ * Schwarz screening (SCHWRZ, off in the reference's HFSCF at 5257): CLMDIAG
+ * 38126 (GDSC from ERI's self-pair integrals) and SCHWARZ 37889 (per-kl
+ * ISCR/IMTX masks, SENS = 1e-12, only for ITT > 1).  One deliberate
+ * difference: with screening on, the reference's ERIFAST reuses G(AB|..)
+ * built for the first CD call's compacted kl list (IMAP) in later calls whose
+ * IMAP differs (IGAB, 12678); here every call sees its own kl's integrals, as
+ * the screening intends.  Open-shell matrices are not restated.
  */
 #include "bdf_oracle.h"
 
@@ -109,6 +114,12 @@ struct orc_sys {
   double *e0ll[4], *e0ss[4];
   long n0ll, n0ss;
   long *iad0ll, *iad0ss;
+  /* Schwarz screening (SCHWRZ): GDSC from CLMDIAG, SENS */
+  double *gdsc; /* [ndim][ndim] column-major, NULL until computed */
+  int schwarz;
+  double sens;
+  double drop_max, drop_count; /* largest screened digestion term, count */
+  int record;                  /* single-threaded build: keep the two above */
 };
 
 #define KAPA(s, ic, k) ((s)->kapa[(ic) * (MKPX + 1) + (k)])
@@ -1146,8 +1157,8 @@ typedef struct {
   int nbas[4];
 } digest_t;
 
-static void digest(const orc_sys *s, const digest_t *d, const int iflg[11], const zc *rr,
-                   int maxcd, const zc *dent, zc *gdir, zc *gxch, int ndim) {
+static void digest(orc_sys *s, const digest_t *d, const int iflg[11], const zc *rr,
+                   int maxcd, const zc *dent, zc *gdir, zc *gxch, int ndim, const int *msk) {
   int isym = s->isym;
   for (int b = 0; b < 11; ++b) {
     if (iflg[b] != 1) continue;
@@ -1157,6 +1168,7 @@ static void digest(const orc_sys *s, const digest_t *d, const int iflg[11], cons
     int m = 0;
     for (int kb = 1; kb <= d->nbas[2]; ++kb)
       for (int lb = 1; lb <= d->nbas[3]; ++lb, ++m) {
+        int drop = msk && !((msk[m] >> b) & 1); /* ISCR / IMTX (13221-13223) */
         int idx[4] = {d->ibas, d->jbas, kb, lb};
         for (int sa = 0; sa < 2; ++sa)
           for (int sb = 0; sb < 2; ++sb)
@@ -1178,6 +1190,14 @@ static void digest(const orc_sys *s, const digest_t *d, const int iflg[11], cons
                 int dcol = d->nlab[bt->dcol][dcol_s] + idx[bt->dcol] - 1;
                 zc r = rr[(long)(n - 1) * maxcd + m];
                 zc den = dent[(long)dcol * ndim + drow];
+                if (drop) {
+                  if (!s->record) continue;
+                  /* screened term: record its size for the bound check */
+                  double t = isym != 0 ? fabs(ph * creal(r) * creal(den)) : cabs(ph * r * den);
+                  if (t > s->drop_max) s->drop_max = t;
+                  s->drop_count += 1;
+                  continue;
+                }
                 if (isym != 0)
                   g[(long)ocol * ndim + orow] += ph * creal(r) * creal(den);
                 else
@@ -1185,6 +1205,188 @@ static void digest(const orc_sys *s, const digest_t *d, const int iflg[11], cons
               }
       }
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Schwarz screening: SCHWARZ 37889-38123 and CLMDIAG 38126-38445           */
+/* ------------------------------------------------------------------------ */
+/* largest of the four sign-slot entries M(r_s + ir, c_s + ic) */
+static double slot_max(const double *m, int n, const int r[2], int ir, const int c[2], int ic) {
+  double x = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      double v = m[(long)(c[b] + ic) * n + r[a] + ir];
+      if (v > x) x = v;
+    }
+  return x;
+}
+static double slot_absmax(const zc *m, int n, const int r[2], int ir, const int c[2], int ic) {
+  double x = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      double v = cabs(m[(long)(c[b] + ic) * n + r[a] + ir]);
+      if (v > x) x = v;
+    }
+  return x;
+}
+
+/* SCHWARZ for one ERIFAST call.  For each kl, msk[m] bit b keeps digestion
+   batch b+1 (IMTX(M,b+1)); msk[m] = 0 drops the kl (ISCR(M) = 0).  Bound
+   groups are tested in the reference's order -- direct (1,2) with D(C,D),
+   (3,4) with D(A,B); exchange (8,9) with D(A,D), (5) with D(B,C), (7,11) with
+   D(A,C), (6,10) with D(B,D) -- and the first group above SENS ends the test,
+   so the groups after it stay on. */
+static void schwarz_masks(const orc_sys *s, const digest_t *d, const zc *dent, int *msk) {
+  const int n = s->ndim;
+  const double *G = s->gdsc;
+  const double sens = s->sens;
+  const int i = d->ibas - 1, j = d->jbas - 1;
+  const int *A = d->nlab[0], *B = d->nlab[1], *C = d->nlab[2], *D = d->nlab[3];
+  const double gab = slot_max(G, n, A, i, B, j);
+  int m = 0;
+  for (int k = 0; k < d->nbas[2]; ++k)
+    for (int l = 0; l < d->nbas[3]; ++l, ++m) {
+      const double gdir = gab * slot_max(G, n, C, k, D, l);
+      const double gx1 = slot_max(G, n, B, j, C, k) * slot_max(G, n, A, i, D, l);
+      const double gx2 = slot_max(G, n, B, j, D, l) * slot_max(G, n, A, i, C, k);
+      const double xl[6] = {gdir * slot_absmax(dent, n, C, k, D, l),
+                            gdir * slot_absmax(dent, n, A, i, B, j),
+                            gx1 * slot_absmax(dent, n, A, i, D, l),
+                            gx1 * slot_absmax(dent, n, B, j, C, k),
+                            gx2 * slot_absmax(dent, n, A, i, C, k),
+                            gx2 * slot_absmax(dent, n, B, j, D, l)};
+      static const int bits[6] = {0x003, 0x00c, 0x180, 0x010, 0x440, 0x220};
+      int mask = 0x7ff, keep = 0;
+      for (int g = 0; g < 6; ++g) {
+        if (xl[g] > sens) { keep = 1; break; }
+        mask &= ~bits[g];
+      }
+      msk[m] = keep ? mask : 0;
+    }
+}
+
+/* ERI (10569) restricted to the self-pair kl = ij of one block pair, as
+   CLMDIAG uses it: (A^T1 B^T1 | A^T2 B^T2) for sign combinations (--|--) and
+   (+-|+-).  With P = Q the R-integrals are RMAKE at PQ = 0, alpha = p/2.  On
+   the real path (ISYM 1, 2) ERI keeps real parts only (11263-11296). */
+static void diag_pair(const orc_sys *s, int a, int b, int ka, int kb, int ma, int mb, int it1,
+                      int it2, int i, int j, int swap, zc *rr1, zc *rr11) {
+  int la = lval(KAPA(s, a, ka)), lb = lval(KAPA(s, b, kb));
+  int na = NFNC(s, a, la), nb = NFNC(s, b, lb);
+  long maxab = (long)na * nb, ij = (long)i * nb + j;
+  long kl = swap ? (long)j * na + i : ij;
+  int lamab = la + lb + (it1 == 4 ? 2 : 0), lamcd = la + lb + (it2 == 4 ? 2 : 0);
+  int ntab = ntuv(lamab), ntcd = ntuv(lamcd), lam = lamab + lamcd;
+  long ix = iadx(s, a + 1, b + 1, ka, kb, ma, mb);
+  long ixcd = swap ? iadx(s, b + 1, a + 1, kb, ka, mb, ma) : ix;
+  long iab = (it1 == 4 ? s->iad0ss : s->iad0ll)[ix];
+  long icd = (it2 == 4 ? s->iad0ss : s->iad0ll)[ixcd];
+  double *const *pab = it1 == 4 ? s->e0ss : s->e0ll;
+  double *const *pcd = it2 == 4 ? s->e0ss : s->e0ll;
+  double p = BEXL(s, a, la, i) + BEXL(s, b, lb, j);
+  double qp[3] = {0.0, 0.0, 0.0}, aph = 0.5 * p;
+  double *rc = malloc(sizeof(double) * ntuv(lam));
+  orc_rmake(qp, &aph, 1, lam, rc);
+  double pre = 2.0 * PI52 / (sqrt(2.0 * p) * p * p);
+  zc acc[2] = {0.0, 0.0};
+  for (int set = 0; set < 2; ++set) {
+    int c0 = set == 0 ? 0 : 2; /* E11 planes 0/1, E21 planes 2/3 */
+    for (int tp = 1; tp <= ntcd; ++tp) {
+      zc g = 0.0;
+      for (int t = 1; t <= ntab; ++t) {
+        int h = IABC[IAX[t] + IAX[tp]][IBX[t] + IBX[tp]][ICX[t] + ICX[tp]];
+        zc e = pab[c0][iab + (t - 1) * maxab + ij] + I * pab[c0 + 1][iab + (t - 1) * maxab + ij];
+        g += e * rc[h - 1];
+      }
+      zc e = pcd[c0][icd + (tp - 1) * maxab + kl] + I * pcd[c0 + 1][icd + (tp - 1) * maxab + kl];
+      /* (+-|+-) for CLMDIAG; (-+|+-) = sum Z E21cd conj(G21) for the bound */
+      acc[set] += m1pow(ILAM[tp]) * e * ((swap && set == 1) ? conj(g) : g);
+    }
+    acc[set] *= pre;
+    if (!swap && (s->isym == 1 || s->isym == 2)) acc[set] = creal(acc[set]);
+  }
+  free(rc);
+  *rr1 = acc[0];
+  *rr11 = acc[1];
+}
+
+/* CLMDIAG 38126-38445: GDSC(A^T1 row, B^T2 column) = sqrt|(a b|a b)| over the
+   (T1,T2) component pairs and sign slots; (--|--) = (++|++) and
+   (-+|-+) = (+-|+-) in modulus (38399-38404). */
+void orc_clmdiag(orc_sys *s, double *out) {
+  int n = s->ndim;
+  if (!s->gdsc) s->gdsc = calloc((long)n * n, sizeof(double));
+  double *G = s->gdsc;
+  memset(G, 0, sizeof(double) * n * n);
+  for (int a = 0; a < s->ncnt; ++a)
+    for (int b = 0; b < s->ncnt; ++b)
+      for (int ka = 1; ka <= s->nkap[a]; ++ka)
+        for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+          int la = lval(KAPA(s, a, ka)), lb = lval(KAPA(s, b, kb));
+          for (int ma = 1; ma <= abs(KAPA(s, a, ka)); ++ma)
+            for (int mb = 1; mb <= abs(KAPA(s, b, kb)); ++mb)
+              for (int it1 = 4; it1 >= 1; it1 -= 3)
+                for (int it2 = 4; it2 >= 1; it2 -= 3) {
+                  int na1 = LRGE(s, a, ka, 2 * ma - 1) + (it1 == 4 ? s->nskp : 0);
+                  int na2 = LRGE(s, a, ka, 2 * ma) + (it1 == 4 ? s->nskp : 0);
+                  int nb1 = LRGE(s, b, kb, 2 * mb - 1) + (it2 == 4 ? s->nskp : 0);
+                  int nb2 = LRGE(s, b, kb, 2 * mb) + (it2 == 4 ? s->nskp : 0);
+                  for (int i = 0; i < NFNC(s, a, la); ++i)
+                    for (int j = 0; j < NFNC(s, b, lb); ++j) {
+                      zc r1, r11;
+                      diag_pair(s, a, b, ka, kb, ma, mb, it1, it2, i, j, 0, &r1, &r11);
+                      G[(long)(nb1 + j) * n + na1 + i] = sqrt(cabs(r1));
+                      G[(long)(nb2 + j) * n + na1 + i] = sqrt(cabs(r11));
+                      G[(long)(nb1 + j) * n + na2 + i] = sqrt(cabs(r11));
+                      G[(long)(nb2 + j) * n + na2 + i] = sqrt(cabs(r1));
+                    }
+                }
+        }
+  if (out) memcpy(out, G, sizeof(double) * n * n);
+}
+
+/* Cauchy-Schwarz factors of the Coulomb metric: Q(a sA, b sB) =
+   sqrt|(a sA b sB | b sB a sA)| for LL and SS pairs, so that
+   |(ab|cd)| <= Q(a,b) Q(c,d).  Not a reference routine: the bound the
+   engine's FP32/FP64 split uses. */
+void orc_pairbound(orc_sys *s, double *out) {
+  int n = s->ndim;
+  memset(out, 0, sizeof(double) * n * n);
+  for (int a = 0; a < s->ncnt; ++a)
+    for (int b = 0; b < s->ncnt; ++b)
+      for (int ka = 1; ka <= s->nkap[a]; ++ka)
+        for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+          int la = lval(KAPA(s, a, ka)), lb = lval(KAPA(s, b, kb));
+          for (int ma = 1; ma <= abs(KAPA(s, a, ka)); ++ma)
+            for (int mb = 1; mb <= abs(KAPA(s, b, kb)); ++mb)
+              for (int it = 4; it >= 1; it -= 3) {
+                int off = it == 4 ? s->nskp : 0;
+                int na1 = LRGE(s, a, ka, 2 * ma - 1) + off, na2 = LRGE(s, a, ka, 2 * ma) + off;
+                int nb1 = LRGE(s, b, kb, 2 * mb - 1) + off, nb2 = LRGE(s, b, kb, 2 * mb) + off;
+                for (int i = 0; i < NFNC(s, a, la); ++i)
+                  for (int j = 0; j < NFNC(s, b, lb); ++j) {
+                    zc r1, r7;
+                    diag_pair(s, a, b, ka, kb, ma, mb, it, it, i, j, 1, &r1, &r7);
+                    out[(long)(nb1 + j) * n + na1 + i] = sqrt(cabs(r1));
+                    out[(long)(nb2 + j) * n + na2 + i] = sqrt(cabs(r1));
+                    out[(long)(nb2 + j) * n + na1 + i] = sqrt(cabs(r7));
+                    out[(long)(nb1 + j) * n + na2 + i] = sqrt(cabs(r7));
+                  }
+              }
+        }
+}
+
+void orc_schwarz_stats(const orc_sys *s, double *drop_max, double *drop_count) {
+  *drop_max = s->drop_max;
+  *drop_count = s->drop_count;
+}
+
+void orc_set_schwarz(orc_sys *s, int on, double sens) {
+  s->drop_max = 0.0;
+  s->drop_count = 0.0;
+  s->schwarz = on;
+  s->sens = sens > 0.0 ? sens : 1.0e-12; /* CLMFAST DATA SENS (11567) */
+  if (on && !s->gdsc) orc_clmdiag(s, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1210,6 +1412,7 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
     }
   ws_size(w, maxn, maxl, nmax);
   zc *rr = (zc *)calloc(16L * maxn * maxn, sizeof(zc));
+  int *msk = (int *)calloc((long)maxn * maxn, sizeof(int));
 
   /* INDEX(ICT,KAPPA,MJ) ordering (11606-11626) */
   int *index = calloc((long)s->ncnt * 2 * MKPX * (2 * MKPX + 1), sizeof(int));
@@ -1367,8 +1570,11 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
                                     erifast(s, w, &q, ibas, jbas, itn, rr);
                                     dg.ibas = ibas; dg.jbas = jbas;
                                     for (int k = 0; k < 4; ++k) dg.nbas[k] = q.nbas[k];
+                                    /* ITOG: screening only for ITT > 1 (12096-12100) */
+                                    int screen = s->schwarz && itt > 1;
+                                    if (screen) schwarz_masks(s, &dg, dent, msk);
                                     digest(s, &dg, iflg, rr, q.nbas[2] * q.nbas[3], dent, gdir,
-                                           gxch, ndim);
+                                           gxch, ndim, screen ? msk : NULL);
                                   }
                                 }
                               }
@@ -1386,6 +1592,7 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
   }
 #undef INDEXQ
   free(index);
+  free(msk);
   free(rr);
 }
 
@@ -1417,6 +1624,7 @@ void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, doub
                     int nthreads) {
   int ndim = s->ndim;
   long nn = (long)ndim * ndim;
+  s->record = nthreads <= 1;
   memset(gdir, 0, sizeof(double) * 2 * nn);
   memset(gxch, 0, sizeof(double) * 2 * nn);
   const zc *d = (const zc *)dent;
@@ -1520,6 +1728,7 @@ void orc_destroy(orc_sys *s) {
   free(s->lrge); free(s->labicn); free(s->labkqn); free(s->labmqn);
   for (int c = 0; c < 4; ++c) { free(s->e0ll[c]); free(s->e0ss[c]); }
   free(s->iad0ll); free(s->iad0ss);
+  free(s->gdsc);
   free(s);
 }
 

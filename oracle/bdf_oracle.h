@@ -65,6 +65,22 @@ void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
 void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,
                     double *gxch, int nthreads);
 
+/* CLMDIAG (bertha-scf.f:38126): GDSC, ndim x ndim real column-major */
+void orc_clmdiag(orc_sys *s, double *gdsc);
+
+/* sqrt|(a b|b a)| for LL and SS pairs, ndim x ndim real column-major: the
+   Cauchy-Schwarz factors of the engine's FP32/FP64 split (not a reference
+   routine) */
+void orc_pairbound(orc_sys *s, double *out);
+
+/* SCHWRZ toggle for orc_clmfast*: SCHWARZ masks (bertha-scf.f:37889) with
+   sensitivity sens (<= 0: the reference's 1e-12) */
+void orc_set_schwarz(orc_sys *s, int on, double sens);
+
+/* largest |term| among the digestion terms the masks dropped since the last
+   orc_set_schwarz, and their number (single-threaded builds only) */
+void orc_schwarz_stats(const orc_sys *s, double *drop_max, double *drop_count);
+
 #ifdef __cplusplus
 }
 #endif

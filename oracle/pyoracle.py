@@ -46,6 +46,10 @@ def _oracle_lib():
         lib.orc_erifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, C.c_int, P, P]
         lib.orc_clmfast.argtypes = [P, C.c_int, P, P, P]
         lib.orc_clmfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
+        lib.orc_clmdiag.argtypes = [P, P]
+        lib.orc_pairbound.argtypes = [P, P]
+        lib.orc_set_schwarz.argtypes = [P, C.c_int, C.c_double]
+        lib.orc_schwarz_stats.argtypes = [P, P, P]
         _olib = lib
     return _olib
 
@@ -117,11 +121,31 @@ class Oracle:
                                   im.ctypes.data)
         return (re + 1j * im).reshape(16, maxcd)
 
-    def clmfast(self, dent, ilev=3, nthreads=1):
+    def clmdiag(self):
+        """CLMDIAG's GDSC (ndim x ndim real)."""
+        g = np.zeros((self.ndim, self.ndim), order="F")
+        _oracle_lib().orc_clmdiag(self._h, g.ctypes.data)
+        return g
+
+    def pairbound(self):
+        """sqrt|(ab|ba)| table (ndim x ndim real)."""
+        g = np.zeros((self.ndim, self.ndim), order="F")
+        _oracle_lib().orc_pairbound(self._h, g.ctypes.data)
+        return g
+
+    def schwarz_stats(self):
+        """(largest dropped digestion term, number dropped) of the last
+        single-threaded screened build."""
+        a, b = C.c_double(), C.c_double()
+        _oracle_lib().orc_schwarz_stats(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def clmfast(self, dent, ilev=3, nthreads=1, schwarz=False, sens=0.0):
         n = self.ndim
         d = np.asfortranarray(dent, dtype=np.complex128)
         g = np.zeros((n, n), dtype=np.complex128, order="F")
         x = np.zeros((n, n), dtype=np.complex128, order="F")
+        _oracle_lib().orc_set_schwarz(self._h, 1 if schwarz else 0, sens)
         if nthreads > 1:
             _oracle_lib().orc_clmfast_mt(self._h, ilev, d.ctypes.data, g.ctypes.data,
                                          x.ctypes.data, nthreads)
@@ -203,8 +227,15 @@ class Reference:
                                C.byref(mo))
         return (re + 1j * im).reshape(16, maxcd)
 
-    def clmfast(self, dent, ilev=3):
+    def clmdiag(self):
+        """CLMDIAG's GDSC (ndim x ndim real)."""
+        g = np.zeros(self.ndim * self.ndim)
+        _ref_lib().ref_clmdiag(C.c_void_p(g.ctypes.data))
+        return g.reshape(self.ndim, self.ndim, order="F")
+
+    def clmfast(self, dent, ilev=3, schwarz=False):
         n = self.ndim
+        _ref_lib().ref_schwarz(C.c_int(1 if schwarz else 0))
         d = np.asfortranarray(dent, dtype=np.complex128)
         dre = np.asfortranarray(d.real)
         dim = np.asfortranarray(d.imag)

@@ -187,6 +187,40 @@ C     DENSITY IN: ROW-FASTEST (FORTRAN ORDER), LEADING DIMENSION NDIM
       END
 C
 C
+C     CLMDIAG (SCHWARZ DIAGONAL BOUNDS) -> GDSC, ROW-FASTEST, LD = NDIM.
+C     CLMDIAG SWITCHES RCFILE OFF FOR ITS OWN ERI CALLS; IT IS SWITCHED
+C     BACK ON SO LATER ERIFAST CALLS RUN AS IN THE SCF.
+      SUBROUTINE REFCLMDIAG(GOUT) BIND(C,NAME='ref_clmdiag')
+      USE ISO_C_BINDING
+      IMPLICIT DOUBLE PRECISION (A-H,O-Z)
+      INCLUDE 'parameters.h'
+      INCLUDE 'scfoptions.h'
+      REAL(C_DOUBLE) :: GOUT(*)
+      COMMON/BDIM/NDIM,NSKP,NOCC,NVRT
+      COMMON/SWRZ/GDSC(MDM,MDM),BDSC(MDM,MDM)
+      CALL CLMDIAG
+      RCFILE = .TRUE.
+      DO J=1,NDIM
+        DO I=1,NDIM
+          GOUT((J-1)*NDIM+I) = GDSC(I,J)
+        ENDDO
+      ENDDO
+      RETURN
+      END
+C
+C
+C     SCHWRZ TOGGLE (HFSCF HARD-CODES .FALSE.)
+      SUBROUTINE REFSCHWARZ(ION) BIND(C,NAME='ref_schwarz')
+      USE ISO_C_BINDING
+      IMPLICIT DOUBLE PRECISION (A-H,O-Z)
+      INCLUDE 'parameters.h'
+      INCLUDE 'scfoptions.h'
+      INTEGER(C_INT), VALUE :: ION
+      SCHWRZ = ION.NE.0
+      RETURN
+      END
+C
+C
       SUBROUTINE REFERIFAST(ICNTIN,KIN,MIN,IBASIN,JBASIN,ITNIN,ILEVIN,
      &                      RRRE,RRIM,MAXCDO) BIND(C,NAME='ref_erifast')
 C     KIN(N)  = KAPPA INDEX (1..NKAP) OF BLOCK N

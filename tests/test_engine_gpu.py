@@ -114,3 +114,58 @@ def test_translation_invariance(bdf, systems):
         g1, x1 = e1.clmfast(d, 3)
         g2, x2 = e2.clmfast(d, 3)
     assert np.abs(g1 - g2).max() < 1e-9 and np.abs(x1 - x2).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", ["ne", "off", "tri"])
+def test_schwarz_tables_match_oracle(bdf, orc, systems, name):
+    s = systems[name]
+    print(f"[schwarz tables] {name}", flush=True)
+    with bdf.Engine(s) as eng:
+        g, q = eng.schwarz_tables()
+    O = orc.Oracle(s)
+    go, qo = O.clmdiag(), O.pairbound()
+    # compared as integrals (squares of the tables)
+    assert np.abs(g ** 2 - go ** 2).max() < 1e-12
+    assert np.abs(q ** 2 - qo ** 2).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["diat", "tri"])
+def test_screened_clmfast_matches_oracle(bdf, orc, systems, name):
+    s = systems[name]
+    print(f"[schwarz build] {name}", flush=True)
+    d = bdf.systems.random_density(s.ndim, seed=6, real=(s.shape != "NOSYMM"))
+    O = orc.Oracle(s)
+    with bdf.Engine(s, schwarz=True, sens=1e-6) as eng:
+        g, x = eng.clmfast(d, 3)
+    go, xo = O.clmfast(d, 3, schwarz=True, sens=1e-6)
+    assert O.schwarz_stats()[1] > 0
+    assert np.abs(g - go).max() < FOCK_ABS
+    assert np.abs(x - xo).max() < FOCK_ABS
+
+
+def _e2(d, g, x):
+    return 0.5 * np.real(np.sum(d.T * (g - x)))
+
+
+@pytest.mark.parametrize("name", ["off", "tri"])
+def test_mixed_precision_split(bdf, orc, systems, name):
+    """FP32 items below the split threshold: a zero threshold reproduces the
+    FP64 build exactly; a threshold above every bound runs everything in
+    FP32 (checks the FP32 path itself); a working threshold keeps the
+    two-electron energy within 1 microhartree of FP64."""
+    s = systems[name]
+    print(f"[mixed] {name}", flush=True)
+    d = bdf.systems.random_density(s.ndim, seed=8)
+    go, xo = orc.Oracle(s).clmfast(d, 3)
+    e64 = _e2(d, go, xo)
+    with bdf.Engine(s, precision="mixed", split_threshold=1e-300) as eng:
+        g, x = eng.clmfast(d, 3)
+    assert np.abs(g - go).max() < FOCK_ABS and np.abs(x - xo).max() < FOCK_ABS
+    with bdf.Engine(s, precision="mixed", split_threshold=1e300) as eng:
+        g, x = eng.clmfast(d, 3)
+    scale = np.abs(go).max()
+    assert np.abs(g - go).max() < 1e-4 * scale
+    assert np.abs(x - xo).max() < 1e-4 * scale
+    with bdf.Engine(s, precision="mixed", split_threshold=1e-5) as eng:
+        g, x = eng.clmfast(d, 3)
+    assert abs(_e2(d, g, x) - e64) < 1e-6
