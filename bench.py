@@ -41,6 +41,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--split", type=float, default=0.1,
+                    help="Schwarz split threshold of the mixed-precision leg (0: skip the leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU design (partial J/K "
@@ -99,6 +101,35 @@ def main() -> int:
     g = g_t.cpu().numpy().T
     x = x_t.cpu().numpy().T
 
+    # mixed-precision leg: same build with FP32 items below the Schwarz split
+    mixed = None
+    if args.split > 0:
+        eng_m = pkg.Engine(system, precision="mixed", split_threshold=args.split)
+        stream_m = torch.cuda.ExternalStream(eng_m.stream(), device=dev)
+        gm_t = torch.zeros_like(g_t)
+        xm_t = torch.zeros_like(x_t)
+
+        def build_m():
+            eng_m.clmfast_device(d_t.data_ptr(), gm_t.data_ptr(), xm_t.data_ptr(), 3)
+
+        for _ in range(args.warmup):
+            build_m()
+        torch.cuda.synchronize()
+        ev0.record(stream_m)
+        for _ in range(args.steps):
+            build_m()
+        ev1.record(stream_m)
+        torch.cuda.synchronize()
+        ms_m = ev0.elapsed_time(ev1) / args.steps
+        gm = gm_t.cpu().numpy().T
+        xm = xm_t.cpu().numpy().T
+        de_m = 0.5 * np.real(np.sum(dens.T * ((gm - xm) - (g - x))))
+        mixed = {"value": 1000.0 / ms_m, "ms_per_step": ms_m, "split_threshold": args.split,
+                 "abs_dE2e_vs_fp64_Eh": float(abs(de_m)),
+                 "max_abs_dG_vs_fp64": float(max(np.abs(gm - g).max(), np.abs(xm - x).max())),
+                 "speedup_vs_fp64": ms_per_step / ms_m}
+        eng_m.close()
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -140,6 +171,7 @@ def main() -> int:
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
         "parity": parity,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
     eng.close()

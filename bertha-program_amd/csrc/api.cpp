@@ -94,7 +94,7 @@ struct bdf_engine {
   std::vector<double> qbnd_host;
   // FP32/FP64 split
   int mixed = 0;
-  double split = 1.0e-5;
+  double split = 0.1;
   DevBuf<float4> E32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;

@@ -74,7 +74,7 @@ typedef struct {
   double split_threshold; /* MIXED: a work item (class, primitive pair ij) whose
                              largest quartet contribution bound
                              Q(ij) max Q(kl) max|D| is below it runs in FP32
-                             (<= 0: 1e-5); Q = sqrt|(ab|ba)| */
+                             (<= 0: 0.1); Q = sqrt|(ab|ba)| */
   int device;          /* HIP device ordinal */
   int schwarz;         /* SCHWRZ: SCHWARZ masks from CLMDIAG bounds (reference
                           default 0, bertha-scf.f:5257) */
