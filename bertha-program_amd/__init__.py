@@ -62,6 +62,7 @@ class _Options(_C.Structure):
         ("device", _C.c_int),
         ("schwarz", _C.c_int),
         ("sens", _C.c_double),
+        ("gaunt", _C.c_int),
     ]
 
 
@@ -90,6 +91,11 @@ def library() -> _C.CDLL:
         lib.bdf_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
                                       _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_schwarz_tables.argtypes = [P, P, P, _C.c_int]
+        lib.bdf_breit.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
+        lib.bdf_breit_device.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_bii_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int]
+        lib.bdf_breit_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
+                                            _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -113,7 +119,7 @@ class Engine:
 
     def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
                  device: int = 0, schwarz: bool = False, sens: float = 0.0,
-                 precision: str = "fp64", split_threshold: float = 0.0):
+                 precision: str = "fp64", split_threshold: float = 0.0, gaunt: bool = False):
         lib = library()
         self.system = system
         xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
@@ -125,7 +131,7 @@ class Engine:
                     SHAPES[system.shape], HMLTS[system.hmlt])
         prec = {"fp64": 0, "mixed": 1}[precision]
         o = _Options(1 if intsym else 0, 1 if racah1 else 0, prec, split_threshold, device,
-                     1 if schwarz else 0, sens)
+                     1 if schwarz else 0, sens, 1 if gaunt else 0)
         h = _C.c_void_p()
         nd = _C.c_int()
         ns = _C.c_int()
@@ -182,13 +188,48 @@ class Engine:
         return rr
 
     def etable(self, itt: int, ica: int, icb: int, ka: int, kb: int, ma: int, mb: int):
-        """One E0LL (itt=1) / E0SS (itt=4) block as [4][ntuv*maxab]."""
-        cap = 4 * 400000
+        """One E0LL (itt=1) / E0SS (itt=4) block as [4][ntuv*maxab]; EILS
+        (itt=2, DHFB engines) as [12][ntuv*maxab]."""
+        nc = 12 if itt == 2 else 4
+        cap = nc * 400000
         out = _np.zeros(cap)
         n = _C.c_long()
         _check(library().bdf_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data,
                                     cap, _C.byref(n)))
-        return out[:4 * n.value].reshape(4, n.value)
+        return out[:nc * n.value].reshape(nc, n.value)
+
+    def breit(self, dent: _np.ndarray, ilev: int = 3):
+        """BDIR, BXCH (BRTFAST) for total density ``dent``."""
+        n = self.ndim
+        d = _np.asfortranarray(dent, dtype=_np.complex128)
+        if d.shape != (n, n):
+            raise ValueError(f"density must be {n}x{n}")
+        b = _np.zeros((n, n), dtype=_np.complex128, order="F")
+        x = _np.zeros((n, n), dtype=_np.complex128, order="F")
+        _check(library().bdf_breit(self._h, ilev, d.ctypes.data, None, n, b.ctypes.data,
+                                   x.ctypes.data, None, None))
+        return b, x
+
+    def breit_device(self, d_dent: int, d_bdir: int, d_bxch: int, ilev: int = 3) -> None:
+        _check(library().bdf_breit_device(self._h, ilev, d_dent, d_bdir, d_bxch))
+
+    def biifast(self, icnt, kidx, midx, ibas: int, jbas: int):
+        """RR(N,1..16) of one BIIFAST batch, shape (16, maxcd) complex."""
+        sysm = self.system
+        maxcd = sysm.nfnc_of(icnt[2] - 1, kidx[2] // 2) * sysm.nfnc_of(icnt[3] - 1, kidx[3] // 2)
+        rr = _np.zeros((16, maxcd), dtype=_np.complex128)
+        a = [_np.asarray(v, dtype=_np.int32) for v in (icnt, kidx, midx)]
+        _check(library().bdf_bii_batch(self._h, a[0].ctypes.data, a[1].ctypes.data,
+                                       a[2].ctypes.data, ibas, jbas, rr.ctypes.data, maxcd))
+        return rr
+
+    def breit_plan_info(self, ilev: int = 3):
+        f = _C.c_double()
+        q = _C.c_double()
+        n = _C.c_long()
+        _check(library().bdf_breit_plan_info(self._h, ilev, _C.byref(f), _C.byref(q),
+                                             _C.byref(n)))
+        return f.value, q.value, n.value
 
     def schwarz_tables(self):
         """(GDSC of CLMDIAG, sqrt|(ab|ba)| bound table), ndim x ndim real."""

@@ -65,6 +65,17 @@ struct DevPlan {
   DevBuf<WorkItem> items;
 };
 
+struct DevBreitPlan {
+  int built = 0;
+  BreitPlan P;
+  DevBuf<ClassDesc> classes;
+  DevBuf<BrAB> abs;
+  DevBuf<BrCD> cds;
+  DevBuf<uint8_t> masks;
+  DevBuf<WorkItem> items;
+  DevBuf<uint8_t> stale;
+};
+
 struct bdf_engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -85,7 +96,7 @@ struct bdf_engine {
   DevBuf<double> slab;
   int nblocks = 0;
   int64_t slab_stride = 0;
-  int slab_ld = 0;
+  int slab_ld = 0, slab_rows = 0;
   DevBuf<double2> dent, gdir, gxch;
   // Schwarz tables (GDSC of CLMDIAG and the (ab|ba) bound), built on demand
   int schwarz = 0, have_tables = 0;
@@ -95,6 +106,9 @@ struct bdf_engine {
   // FP32/FP64 split
   int mixed = 0;
   double split = 0.1;
+  int breit = 0;  // DHFB: LS tables and the Breit plans
+  int gaunt = 0;  // GAUNT2
+  DevBreitPlan bplans[3];
   DevBuf<float4> E32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -102,6 +116,20 @@ struct bdf_engine {
 };
 
 static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double> &q);
+
+// per-workgroup scratch: R rows, Boys rows and g_rows G rows, each ld
+// doubles (ld = largest kl count rounded to the wavefront)
+static int ensure_slab(bdf_engine *e, int maxcd, int g_rows) {
+  int ld = ((std::max(maxcd, 1) + 63) / 64) * 64;
+  int rows = ntuv(kMaxLfl) + kMaxLfl + 1 + g_rows;
+  if (ld > e->slab_ld || rows > e->slab_rows || !e->slab.p) {
+    e->slab_ld = std::max(ld, e->slab_ld);
+    e->slab_rows = std::max(rows, e->slab_rows);
+    e->slab_stride = (int64_t)e->slab_rows * e->slab_ld;
+    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
+  }
+  return BDF_OK;
+}
 
 static int make_plan(bdf_engine *e, int ilev) {
   DevPlan &dp = e->plans[ilev - 1];
@@ -118,14 +146,8 @@ static int make_plan(bdf_engine *e, int ilev) {
     item_factors(e, dp.P, q);
     HIPCHK(dp.qitem.upload(q, e->stream));
   }
-  // per-workgroup scratch: R rows, Boys rows, G rows, each ld doubles
-  int ld = ((std::max(dp.P.max_maxcd, 1) + 63) / 64) * 64;
-  int64_t stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * ld;
-  if (ld > e->slab_ld || stride > e->slab_stride) {
-    e->slab_ld = std::max(ld, e->slab_ld);
-    e->slab_stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * e->slab_ld;
-    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
-  }
+  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * ntuv(kMaxLab));
+  if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
   return BDF_OK;
@@ -304,6 +326,69 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   return BDF_OK;
 }
 
+static int make_breit_plan(bdf_engine *e, int ilev) {
+  DevBreitPlan &dp = e->bplans[ilev - 1];
+  if (dp.built) return BDF_OK;
+  std::string err;
+  try {
+    err = build_breit_plan(e->B, e->T, e->eflags, e->isym, e->intsym, e->racah1, ilev, e->gaunt,
+                           dp.P);
+  } catch (const std::out_of_range &) {
+    err = "Breit plan: missing E-coefficient block";
+  }
+  if (!err.empty()) return fail(BDF_EINVAL, err);
+  HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
+  HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
+  HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
+  HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
+  HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  HIPCHK(dp.stale.alloc((size_t)std::max(dp.P.nflag, 1)));
+  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1));
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  dp.built = 1;
+  return BDF_OK;
+}
+
+static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, double2 *d_bdir,
+                     double2 *d_bxch, double2 *rr_out) {
+  int nd = e->B.ndim;
+  if (!rr_out)
+    HIPCHK(launch_zero(d_bdir, d_bxch, (int64_t)nd * nd, e->counter.p, nullptr, e->stream));
+  else
+    HIPCHK(launch_zero(rr_out, nullptr, 0, e->counter.p, nullptr, e->stream));
+  BreitArgs a;
+  a.classes = dp.classes.p;
+  a.nclasses = (int)dp.P.classes.size();
+  a.abs = dp.abs.p;
+  a.cds = dp.cds.p;
+  a.masks = dp.masks.p;
+  a.items = dp.items.p;
+  a.nitems = (int)dp.P.items.size();
+  a.counter = e->counter.p;
+  a.exps = e->exps.p;
+  a.E = e->E.p;
+  a.dent = d_dent;
+  a.bdir = d_bdir;
+  a.bxch = d_bxch;
+  a.ndim = nd;
+  a.nskp = e->B.nskp;
+  a.slab = e->slab.p;
+  a.slab_stride = e->slab_stride;
+  a.ld = e->slab_ld;
+  a.isym = e->isym;
+  a.intsym = e->intsym;
+  a.gaunt = e->gaunt;
+  a.stale = dp.stale.p;
+  a.rr_out = rr_out;
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  HIPCHK(launch_breit(a, e->nblocks, e->stream));
+  HIPCHK(hipEventRecord(e->ev2, e->stream));
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  e->last_quartets = dp.P.quartets;
+  return BDF_OK;
+}
+
 extern "C" {
 
 const char *bdf_last_error(void) { return g_err.c_str(); }
@@ -315,7 +400,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   if (sys->hmlt != BDF_HMLT_DHFR && sys->hmlt != BDF_HMLT_DHFB)
     return fail(BDF_EINVAL, "bdf_create: unknown hamiltonian");
   if (sys->shape < 0 || sys->shape > 2) return fail(BDF_EINVAL, "bdf_create: unknown shape");
-  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0, 0, 0.0};
+  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0, 0, 0.0, 0};
   if (opt) o = *opt;
   if (o.precision != BDF_PREC_FP64 && o.precision != BDF_PREC_MIXED)
     return fail(BDF_EINVAL, "bdf_create: unknown precision mode");
@@ -331,6 +416,8 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   e->schwarz = o.schwarz ? 1 : 0;
   e->sens = o.sens > 0.0 ? o.sens : 1.0e-12;
   e->mixed = o.precision == BDF_PREC_MIXED;
+  e->breit = sys->hmlt == BDF_HMLT_DHFB;
+  e->gaunt = o.gaunt ? 1 : 0;
   if (o.split_threshold > 0.0) e->split = o.split_threshold;
   std::string err = build_basis(*sys, e->B);
   if (!err.empty()) {
@@ -344,7 +431,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(hipEventCreate(&e->ev1));
     HIPCHK(hipEventCreate(&e->ev2));
     HIPCHK(upload_constants());
-    build_etables(e->B, e->T);
+    build_etables(e->B, e->T, e->breit);
     HIPCHK(e->exps.upload(e->B.exps, e->stream));
     HIPCHK(e->normL.upload(e->B.normL, e->stream));
     HIPCHK(e->normS.upload(e->B.normS, e->stream));
@@ -478,7 +565,7 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   cl.lamfl = mcnt == 1 ? lsum + 4 : (ilev == 1 ? lsum : (ilev == 2 ? lsum + 2 : lsum + 4));
   cl.maxcd = mcd;
   cl.mcnt = mcnt;
-  cl.pad = 0;
+  cl.flagoff = 0;
   cl.ab_begin = 0;
   cl.ab_end = 1;
   ABCombo ab;
@@ -535,12 +622,8 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
-  int ldr = ((mcd + 63) / 64) * 64;
-  if (ldr > e->slab_ld || !e->slab.p) {
-    e->slab_ld = std::max(ldr, e->slab_ld);
-    e->slab_stride = (int64_t)(ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * ntuv(kMaxLab)) * e->slab_ld;
-    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
-  }
+  int src = ensure_slab(e, mcd, 4 * ntuv(kMaxLab));
+  if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
   int rc = run_fock(e, ilev, nullptr, nullptr, nullptr, out.p, &dp);
@@ -553,6 +636,28 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
 int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb, int ma, int mb, double *out,
                long cap, long *n) {
   if (!e || !out || !n) return fail(BDF_EINVAL, "bdf_etable: null argument");
+  if (itt == 2) {
+    if (!e->breit) return fail(BDF_EINVAL, "bdf_etable: LS tables need a DHFB engine");
+    long cnt = -1;
+    HIPCHK(hipSetDevice(e->device));
+    for (int ix = 0; ix < 3; ++ix) {
+      auto it = e->T.pair_index.find(Etabs::key(ica - 1, icb - 1, ka, kb, ma, mb, kLSx + ix));
+      if (it == e->T.pair_index.end()) return fail(BDF_EINVAL, "bdf_etable: no such block");
+      const EPair &P = e->T.pairs[it->second];
+      cnt = (long)ntuv(P.lam) * P.nA * P.nB;
+      if (cap < 12 * cnt) return fail(BDF_EINVAL, "bdf_etable: out too small");
+      std::vector<double4> h(cnt);
+      HIPCHK(hipMemcpy(h.data(), e->E.p + P.off, sizeof(double4) * cnt, hipMemcpyDeviceToHost));
+      for (long k = 0; k < cnt; ++k) {
+        out[(4 * ix + 0) * cnt + k] = h[k].x;
+        out[(4 * ix + 1) * cnt + k] = h[k].y;
+        out[(4 * ix + 2) * cnt + k] = h[k].z;
+        out[(4 * ix + 3) * cnt + k] = h[k].w;
+      }
+    }
+    *n = cnt;
+    return BDF_OK;
+  }
   auto it = e->T.pair_index.find(Etabs::key(ica - 1, icb - 1, ka, kb, ma, mb, itt == 4));
   if (it == e->T.pair_index.end()) return fail(BDF_EINVAL, "bdf_etable: no such block");
   const EPair &P = e->T.pairs[it->second];
@@ -584,6 +689,167 @@ int bdf_schwarz_tables(bdf_engine *e, double *gdsc, double *qbnd, int ld) {
   if (qbnd)
     HIPCHK(hipMemcpy2D(qbnd, ld * sizeof(double), e->qbnd.p, nd * sizeof(double),
                        nd * sizeof(double), nd, hipMemcpyDeviceToHost));
+  return BDF_OK;
+}
+
+int bdf_breit_device(bdf_engine *e, int ilev, const void *d_dent, void *d_bdir, void *d_bxch) {
+  if (!e || !d_dent || !d_bdir || !d_bxch) return fail(BDF_EINVAL, "bdf_breit_device: null argument");
+  if (!e->breit) return fail(BDF_EINVAL, "bdf_breit_device: engine built without DHFB");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_breit_device: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_breit_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  return run_breit(e, e->bplans[ilev - 1], (const double2 *)d_dent, (double2 *)d_bdir,
+                   (double2 *)d_bxch, nullptr);
+}
+
+int bdf_breit(bdf_engine *e, int ilev, const double _Complex *dent, const double _Complex *deno,
+              int ld, double _Complex *bdir, double _Complex *bxch, double _Complex *wdir,
+              double _Complex *wxch) {
+  if (!e || !dent || !bdir || !bxch) return fail(BDF_EINVAL, "bdf_breit: null argument");
+  if (deno || wdir || wxch) return fail(BDF_EUNSUPP, "bdf_breit: open shells are not supported");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_breit: ld < ndim");
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(e->dent.alloc((size_t)nd * nd));
+  HIPCHK(e->gdir.alloc((size_t)nd * nd));
+  HIPCHK(e->gxch.alloc((size_t)nd * nd));
+  HIPCHK(hipMemcpy2DAsync(e->dent.p, nd * sizeof(double2), dent, ld * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyHostToDevice, e->stream));
+  int rc = bdf_breit_device(e, ilev, e->dent.p, e->gdir.p, e->gxch.p);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpy2DAsync(bdir, ld * sizeof(double2), e->gdir.p, nd * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpy2DAsync(bxch, ld * sizeof(double2), e->gxch.p, nd * sizeof(double2),
+                          nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int midx[4], int ibas,
+                  int jbas, double _Complex *rr, int maxcd) {
+  if (!e || !icnt || !kidx || !midx || !rr) return fail(BDF_EINVAL, "bdf_bii_batch: null argument");
+  if (!e->breit) return fail(BDF_EINVAL, "bdf_bii_batch: engine built without DHFB");
+  const Basis &B = e->B;
+  int lq[4], kq[4], ic[4];
+  for (int k = 0; k < 4; ++k) {
+    ic[k] = icnt[k] - 1;
+    if (ic[k] < 0 || ic[k] >= B.ncnt) return fail(BDF_EINVAL, "bdf_bii_batch: bad centre");
+    if (kidx[k] < 1 || kidx[k] > 2 * B.lmax[ic[k]] + 1) return fail(BDF_EINVAL, "bdf_bii_batch: bad kappa index");
+    kq[k] = B.kappa_of(ic[k], kidx[k]);
+    lq[k] = kq[k] < 0 ? -kq[k] - 1 : kq[k];
+    if (midx[k] < 1 || midx[k] > std::abs(kq[k])) return fail(BDF_EINVAL, "bdf_bii_batch: bad |mj| index");
+  }
+  int nA = B.nfn(ic[0], lq[0]), nBb = B.nfn(ic[1], lq[1]);
+  if (ibas < 1 || ibas > nA || jbas < 1 || jbas > nBb) return fail(BDF_EINVAL, "bdf_bii_batch: bad ibas/jbas");
+  int mcd = B.nfn(ic[2], lq[2]) * B.nfn(ic[3], lq[3]);
+  if (maxcd < mcd) return fail(BDF_EINVAL, "bdf_bii_batch: rr too small");
+  HIPCHK(hipSetDevice(e->device));
+  // a one-item plan: one class, one AB combination (the class's first, so
+  // the gauge term sees this pair's own R flags), one call
+  DevBreitPlan dp;
+  ClassDesc cl;
+  for (int k = 0; k < 4; ++k) {
+    for (int x = 0; x < 3; ++x) cl.xyz[k][x] = B.xyz[3 * ic[k] + x];
+    cl.exp[k] = B.expo(ic[k], lq[k]);
+    cl.nbas[k] = B.nfn(ic[k], lq[k]);
+  }
+  int lamab = lq[0] + lq[1] + 1, lamcd = lq[2] + lq[3] + 1;
+  cl.lamfl = e->gaunt ? lamab + lamcd : lamab + lamcd + 2;
+  cl.maxcd = mcd;
+  cl.mcnt = 0;
+  cl.flagoff = 0;
+  cl.ab_begin = 0;
+  cl.ab_end = 1;
+  int ntcd = ntuv(lamcd);
+  try {
+    BrAB ab;
+    for (int ix = 0; ix < 3; ++ix)
+      ab.eoff[ix] = e->T.pairs[e->T.pair_index.at(Etabs::key(ic[0], ic[1], kidx[0], kidx[1], midx[0], midx[1], kLSx + ix))].off;
+    ab.ntab = ntuv(lamab);
+    ab.ntg = ntcd;
+    for (int s = 0; s < 2; ++s) ab.fa[s] = ab.fb[s] = 0;
+    ab.cd_begin = 0;
+    ab.cd_end = 1;
+    ab.pad = 0;
+    // G mask: E(CD) flags summed over the (C,D,lc,ld) class (EILSCD0)
+    std::vector<int> agg(12 * ntcd, 0);
+    int lc = lq[2], ld = lq[3];
+    for (int nc = (lc == 0); nc <= 1; ++nc)
+      for (int ndd = (ld == 0); ndd <= 1; ++ndd)
+        for (int mc = 1; mc <= lc + nc; ++mc)
+          for (int md = 1; md <= ld + ndd; ++md)
+            for (int ix = 0; ix < 3; ++ix) {
+              int pr = e->T.pair_index.at(Etabs::key(ic[2], ic[3], 2 * lc + nc, 2 * ld + ndd, mc, md, kLSx + ix));
+              for (int t = 0; t < ntcd; ++t)
+                for (int c = 0; c < 4; ++c)
+                  agg[(t * 3 + ix) * 4 + c] += e->eflags[e->T.flag_off[pr] + 4 * t + c];
+            }
+    ab.gmask = 0;
+    for (int t = 0; t < ntcd; ++t)
+      for (int ix = 0; ix < 3; ++ix) {
+        const int *g = &agg[(t * 3 + ix) * 4];
+        int all = g[0] + g[1] + g[2] + g[3];
+        for (int c = 0; c < 4; ++c)
+          dp.P.masks.push_back((uint8_t)(all != 0 && (e->isym == 0 || g[c] != 0)));
+      }
+    BrCD cd;
+    cd.ntcd = ntcd;
+    cd.kmask = (int)dp.P.masks.size();
+    for (int ix = 0; ix < 3; ++ix)
+      cd.eoff[ix] = e->T.pairs[e->T.pair_index.at(Etabs::key(ic[2], ic[3], kidx[2], kidx[3], midx[2], midx[3], kLSx + ix))].off;
+    for (int t = 0; t < ntcd; ++t)
+      for (int ix = 0; ix < 3; ++ix) {
+        int pr = e->T.pair_index.at(Etabs::key(ic[2], ic[3], kidx[2], kidx[3], midx[2], midx[3], kLSx + ix));
+        for (int c = 0; c < 4; ++c) dp.P.masks.push_back(e->eflags[e->T.flag_off[pr] + 4 * t + c]);
+      }
+    for (int s = 0; s < 2; ++s) cd.fc[s] = cd.fd[s] = 0;
+    cd.iflg = 0;
+    int mq[4];
+    for (int k = 0; k < 4; ++k) mq[k] = 2 * midx[k] - 1;
+    int sab = (kq[0] * kq[1]) >= 0 ? 1 : -1, scd = (kq[2] * kq[3]) >= 0 ? 1 : -1;
+    cd.pab[0] = (int8_t)(-sab * ((std::abs(mq[1] - mq[0]) / 2) % 2 ? -1 : 1));
+    cd.pab[1] = (int8_t)(-sab * (((mq[0] + mq[1]) / 2) % 2 ? -1 : 1));
+    cd.pcd[0] = (int8_t)(-scd * ((std::abs(mq[3] - mq[2]) / 2) % 2 ? -1 : 1));
+    cd.pcd[1] = (int8_t)(-scd * (((mq[2] + mq[3]) / 2) % 2 ? -1 : 1));
+    dp.P.abs.push_back(ab);
+    dp.P.cds.push_back(cd);
+  } catch (const std::out_of_range &) {
+    return fail(BDF_EINVAL, "bdf_bii_batch: missing E-coefficient block");
+  }
+  dp.P.classes.push_back(cl);
+  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1)});
+  dp.P.nflag = ntuv(cl.lamfl);
+  dp.P.max_maxcd = mcd;
+  dp.P.max_ntg = ntcd;
+  HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
+  HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
+  HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
+  HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
+  HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  HIPCHK(dp.stale.alloc((size_t)dp.P.nflag));
+  int src = ensure_slab(e, mcd, 12 * ntcd);
+  if (src != BDF_OK) return src;
+  DevBuf<double2> out;
+  HIPCHK(out.alloc((size_t)16 * mcd));
+  int rc = run_breit(e, dp, nullptr, nullptr, nullptr, out.p);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpyAsync(rr, out.p, sizeof(double2) * 16 * mcd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_breit_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items) {
+  if (!e) return fail(BDF_EINVAL, "bdf_breit_plan_info: null engine");
+  if (!e->breit) return fail(BDF_EINVAL, "bdf_breit_plan_info: engine built without DHFB");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_breit_plan_info: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_breit_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  const BreitPlan &P = e->bplans[ilev - 1].P;
+  if (flops) *flops = P.flops;
+  if (quartets) *quartets = P.quartets;
+  if (items) *items = (long)P.items.size();
   return BDF_OK;
 }
 

@@ -33,17 +33,22 @@ struct ETerm {
   int32_t polyA, polyB;  // [begin, end) ranges packed as begin*65536 + count
 };
 
+// component kinds of the E tables
+constexpr int kLL = 0, kSS = 1, kLSx = 2, kLSy = 3, kLSz = 4;
+
 // One (A-block, B-block) pair table: E11 and E21 for one component type.
 struct EPair {
   double ax, ay, az, bx, by, bz;
   int expA, expB;       // offsets into the exponent array
   int nA, nB;
   int lam;              // Hermite order of the stored expansion
-  int small;            // 0 = LL, 1 = SS
+  int small;            // component kind kLL, kSS, kLSx..kLSz
   int64_t off;          // double4 offset of [tuv][ij]
   int term11, nterm11;  // term ranges for E11 and E21
   int term21, nterm21;
   int la, lb;           // large-component l of A and B (normalisation)
+  int phase;            // 0: none, 1: times i, 2: times -1 (i^2, sigma_y)
+  int pad;
 };
 
 // ---- Fock build plan -----------------------------------------------------
@@ -58,7 +63,7 @@ struct ClassDesc {
   int maxcd;
   int ab_begin, ab_end;
   int mcnt;
-  int pad;
+  int flagoff;      // Breit: offset of the class's gauge-term column flags
 };
 
 // One (kappa_A, kappa_B, |m_A|, |m_B|, IT1) combination of a class.
@@ -101,6 +106,39 @@ struct DiagJob {
 struct WorkItem {
   int cls;
   int ij;
+};
+
+// ---- Breit plan (BRTFAST, bertha-scf.f:17060) ------------------------------
+// One (kappa_A, kappa_B, |m_A|, |m_B|) combination: LS tables for sigma_x,y,z.
+struct BrAB {
+  int64_t eoff[3];
+  int ntab, ntg;
+  int gmask;        // bytes [ntg][3][4]: G^ix component built
+  int fa[2], fb[2]; // Fock offsets of A (large) and B (small) sign slots
+  int cd_begin, cd_end;
+  int pad;
+};
+
+// One BIIFAST call with its digestion batches (BRTMTG/BRTMTZ)
+struct BrCD {
+  int64_t eoff[3];
+  int ntcd;
+  int kmask;        // bytes [ntcd][3][4]: E(CD) column flags (EILSCD0)
+  int fc[2], fd[2];
+  int iflg;         // bit b set: exchange batch b+1 active
+  int8_t pab[2], pcd[2];
+};
+
+struct BreitPlan {
+  std::vector<ClassDesc> classes;
+  std::vector<BrAB> abs;
+  std::vector<BrCD> cds;
+  std::vector<uint8_t> masks;
+  std::vector<WorkItem> items;
+  int max_lfl = 0, max_maxcd = 0, max_ntg = 0;
+  int nflag = 0;          // gauge-term flag bytes over all classes
+  double quartets = 0.0;
+  double flops = 0.0;
 };
 
 // everything the host plan produces

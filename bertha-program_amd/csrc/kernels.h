@@ -66,6 +66,28 @@ struct DiagArgs {
   int ndim;
 };
 
+struct BreitArgs {
+  const ClassDesc *classes;
+  int nclasses;
+  const BrAB *abs;
+  const BrCD *cds;
+  const uint8_t *masks;
+  const WorkItem *items;
+  int nitems;
+  int *counter;
+  const double *exps;
+  const double4 *E;
+  const double2 *dent;
+  double2 *bdir, *bxch;
+  int ndim, nskp;
+  double *slab;
+  int64_t slab_stride;
+  int ld;
+  int isym, intsym, gaunt;
+  uint8_t *stale;   // per class: column flags of the last R batch (BIIFAST's IRC)
+  double2 *rr_out;  // test hook: RR[16][maxcd] of the first call instead of digesting
+};
+
 struct CompleteArgs {
   double2 *gdir, *gxch;
   const int *icn, *kqn, *mqn;
@@ -82,5 +104,6 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
+hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s);
 
 }  // namespace bdf

@@ -10,6 +10,8 @@
 //   complete_kernel Hermitian completion (CLMFAST :12166-12205)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace bdf {
@@ -122,13 +124,22 @@ __global__ void __launch_bounds__(128) etab_kernel(EtabArgs a) {
       }
     }
   }
-  double nrm = P.small ? a.normS[P.expA + i] * a.normS[P.expB + j]
-                       : a.normL[P.expA + i] * a.normL[P.expB + j];
+  // large component of A except for SS; small component of B except for LL
+  double nrm = (P.small == kSS ? a.normS[P.expA + i] : a.normL[P.expA + i]) *
+               (P.small == kLL ? a.normL[P.expB + j] : a.normS[P.expB + j]);
   double sc = kab * nrm;
   int maxab = P.nA * P.nB;
-  for (int n = 0; n < nt; ++n)
-    a.E[P.off + (int64_t)n * maxab + ij] =
-        make_double4(sc * acc[0][0][n], sc * acc[0][1][n], sc * acc[1][0][n], sc * acc[1][1][n]);
+  for (int n = 0; n < nt; ++n) {
+    double r0 = sc * acc[0][0][n], i0 = sc * acc[0][1][n];
+    double r1 = sc * acc[1][0][n], i1 = sc * acc[1][1][n];
+    if (P.phase == 1) {  // times i (EQLS :35410-35416)
+      double t0 = r0, t1 = r1;
+      r0 = -i0; i0 = t0; r1 = -i1; i1 = t1;
+    } else if (P.phase == 2) {  // times i twice for sigma_y (:35418-35425)
+      r0 = -r0; i0 = -i0; r1 = -r1; i1 = -i1;
+    }
+    a.E[P.off + (int64_t)n * maxab + ij] = make_double4(r0, i0, r1, i1);
+  }
 }
 
 // per (pair, tuv, component): sum over ij of |E| > 1e-10 (E0LLCD0 EPSCD)
@@ -509,6 +520,36 @@ __device__ __forceinline__ void lde(const FockArgs &a, int64_t i, float &x, floa
   x = e.x; y = e.y; z = e.z; w = e.w;
 }
 
+// Flush the exchange accumulators of one call into the exchange matrix (one
+// atomic per output element instead of one per quartet), then clear them.
+__device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib, int jb, int nC,
+                               int nD, double2 *out, int nd, int lane) {
+  __syncthreads();
+  for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
+    const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
+    const int b = 4 + bsel;  // batch index 4..10 (5..11)
+    if (!((flg >> b) & 1)) continue;
+    const bool isl = (b == 4 || b == 6 || b == 10);
+    if (pos >= (isl ? nD : nC)) continue;
+    double *v;
+    int row, col;
+    const int rs = s >> 1, cs = s & 1;
+    switch (b) {
+      case 4: v = x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
+      case 5: v = x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
+      case 6: v = x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
+      case 7: v = x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
+      case 8: v = x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
+      case 9: v = x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
+      default: v = x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
+    }
+    if (v[0] != 0.0 || v[1] != 0.0) atomic_add2(&out[(int64_t)col * nd + row], v[0], v[1]);
+    v[0] = 0.0;
+    v[1] = 0.0;
+  }
+  __syncthreads();
+}
+
 // Shared-memory state of one workgroup
 struct FockShared {
   double eab[ntuv(kMaxLab) * 4];
@@ -710,37 +751,10 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
                         cd.pcd[1], accAB, sh.x, a.gdir);
       }
       if (!a.rr_out) {
-        // flush the exchange accumulators of this call (one atomic per
-        // output element instead of one per quartet), then clear them
-        __syncthreads();
-        const int nC = cl.nbas[2];
-        const int flg = a.intsym ? cd.iflg : (cd.iflg & 0x11);
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
-          const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
-          const int b = 4 + bsel;  // batch index 4..10 (5..11)
-          if (!((flg >> b) & 1)) continue;
-          const bool isl = (b == 4 || b == 6 || b == 10);
-          if (pos >= (isl ? nD : nC)) continue;
-          double *v;
-          int row, col;
-          const int rs = s >> 1, cs = s & 1;
-          switch (b) {
-            case 4: v = sh.x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
-            case 5: v = sh.x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
-            case 6: v = sh.x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
-            case 7: v = sh.x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
-            case 8: v = sh.x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
-            case 9: v = sh.x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
-            default: v = sh.x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
-          }
-          if (v[0] != 0.0 || v[1] != 0.0)
-            atomic_add2(&a.gxch[(int64_t)col * nd + row], v[0], v[1]);
-          v[0] = 0.0;
-          v[1] = 0.0;
-        }
-        __syncthreads();
+        flush_exchange(sh.x, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, cl.nbas[2],
+                       nD, a.gxch, nd, lane);
       }
     }
     if (!a.rr_out) {
@@ -785,6 +799,304 @@ __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
     else
       fock_item<double>(a, it, slab, sh, lane);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Breit (BRTFAST / BIIFAST / BRTMTG / BRTMTZ)
+// ---------------------------------------------------------------------------
+// R-integrals R[col][kl] (before the prefactor) of primitive pair ij against
+// every kl of a class, to order L
+__device__ void r_columns(const ClassDesc &cl, const double *exps, int ij, double *R, double *F,
+                          int ld, int L, int lane) {
+  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int ib = ij / nB, jb = ij % nB;
+  const double ea = exps[cl.exp[0] + ib], eb = exps[cl.exp[1] + jb];
+  const double p = ea + eb;
+  const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
+  const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
+  const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
+  for (int kl = lane; kl < cl.maxcd; kl += 64) {
+    int kb = kl / nD, lb = kl % nD;
+    double ec = exps[cl.exp[2] + kb], ed = exps[cl.exp[3] + lb];
+    double q = ec + ed;
+    double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
+    double Qy = (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q;
+    double Qz = (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q;
+    double alpha = p * q / (p + q);
+    double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
+    boys_ref<double>(alpha * (X * X + Y * Y + Z * Z), L, F + kl, ld);
+    rints<double>(R + kl, F + kl, ld, L, alpha, X, Y, Z);
+  }
+}
+
+// column test of ERIFAST/BIIFAST: sum over kl of |R| > 1e-10 (:18465-18474)
+__device__ void r_flags(const double *R, int ld, int nt, int maxcd, uint8_t *flag, int lane) {
+  for (int col = lane; col < nt; col += 64) {
+    double s = 0.0;
+    const double *row = R + (int64_t)col * ld;
+    for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
+    flag[col] = s > 1.0e-10;
+  }
+}
+
+// BIIFAST's gauge term reads IRC, the column flags of the most recent R
+// batch.  Inside a class the first (kappa, |m|) combination generates the R
+// batch of every ij in turn, so every later combination sees the flags of
+// the class's last pair ij = MAXAB: computed here, one class per pass.
+__global__ void __launch_bounds__(64) breit_flags_kernel(BreitArgs a) {
+  const int lane = threadIdx.x;
+  const int ld = a.ld;
+  double *R = a.slab + (int64_t)blockIdx.x * a.slab_stride;
+  double *F = R + (int64_t)ntuv(kMaxLfl) * ld;
+  for (int c = blockIdx.x; c < a.nclasses; c += gridDim.x) {
+    const ClassDesc cl = a.classes[c];
+    r_columns(cl, a.exps, cl.nbas[0] * cl.nbas[1] - 1, R, F, ld, cl.lamfl, lane);
+    __syncthreads();
+    r_flags(R, ld, ntuv(cl.lamfl), cl.maxcd, a.stale + cl.flagoff, lane);
+    __syncthreads();
+  }
+}
+
+struct BreitShared {
+  double eab[3][ntuv(2 * kMaxL + 1) * 4];
+  uint8_t cflag[ntuv(kMaxLfl)];
+  uint8_t gflag[ntuv(kMaxLfl)];
+  uint8_t gm[ntuv(2 * kMaxL + 1) * 12];
+  uint8_t km[ntuv(2 * kMaxL + 1) * 12];
+  XAcc x;
+};
+
+__device__ __forceinline__ int any4(const double *e) {
+  return e[0] != 0.0 || e[1] != 0.0 || e[2] != 0.0 || e[3] != 0.0;
+}
+
+// One work item (class, ij) of the Breit build: R to order L_AB + L_CD + 2,
+// G^ix(kl, tuv') = sum_tuv [E^ix R(tuv+tuv') - sum_jx E^jx T_ix,jx] with the
+// gauge term T = R(+e_ix+e_jx)/(2 alpha) - PQ_jx R(+e_ix) +
+// (t+t')_jx R(+e_ix-e_jx) (BIIFAST :18492-18810), RR = sum_ix E^ix_CD G^ix
+// (:18830-19150) and the exchange digestion into BXCH.
+__device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
+                           BreitShared &sh, int lane) {
+  const int ld = a.ld;
+  double *R = slab;
+  double *F = R + (int64_t)ntuv(kMaxLfl) * ld;
+  double *G = F + (int64_t)(kMaxLfl + 1) * ld;
+  const double2 *Dm = a.dent;
+  const int nd = a.ndim;
+  const ClassDesc &cl = a.classes[it.cls];
+  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int ib = it.ij / nB, jb = it.ij % nB;
+  const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
+  const double p = ea + eb;
+  const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
+  const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
+  const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
+  const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
+  const int maxab = cl.nbas[0] * nB;
+  const double bfac = a.gaunt ? -1.0 : -0.5;
+
+  r_columns(cl, a.exps, it.ij, R, F, ld, L, lane);
+  __syncthreads();
+  r_flags(R, ld, nt, maxcd, sh.cflag, lane);
+  __syncthreads();
+  // RCTTFL = BFAC * PRE * R for every column; the flags are applied where
+  // the reference tests them
+  for (int kl = lane; kl < maxcd; kl += 64) {
+    int kb = kl / nD, lb = kl % nD;
+    double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
+    double pre = bfac * (2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
+    for (int col = 0; col < nt; ++col) R[(int64_t)col * ld + kl] *= pre;
+  }
+  __syncthreads();
+
+  for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+    const BrAB ab = a.abs[abi];
+    for (int w = lane; w < 3 * ab.ntab; w += 64) {
+      int ix = w / ab.ntab, t = w % ab.ntab;
+      double4 e = a.E[ab.eoff[ix] + (int64_t)t * maxab + it.ij];
+      // EPSAB = 1e-16 (:18436-18470)
+      sh.eab[ix][4 * t + 0] = fabs(e.x) > 1.0e-16 ? e.x : 0.0;
+      sh.eab[ix][4 * t + 1] = fabs(e.y) > 1.0e-16 ? e.y : 0.0;
+      sh.eab[ix][4 * t + 2] = fabs(e.z) > 1.0e-16 ? e.z : 0.0;
+      sh.eab[ix][4 * t + 3] = fabs(e.w) > 1.0e-16 ? e.w : 0.0;
+    }
+    for (int t = lane; t < 12 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
+    const uint8_t *gsrc = abi == cl.ab_begin ? sh.cflag : a.stale + cl.flagoff;
+    for (int t = lane; t < nt; t += 64) sh.gflag[t] = gsrc[t];
+    __syncthreads();
+    for (int kl = lane; kl < maxcd; kl += 64) {
+      int kb = kl / nD, lb = kl % nD;
+      double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
+      double q = ec + ed;
+      double pq[3] = {(cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q - Px,
+                      (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q - Py,
+                      (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q - Pz};
+      double half_ia = 0.5 / (p * q / (p + q));
+      for (int ix = 0; ix < 3; ++ix)
+        for (int tp = 0; tp < ab.ntg; ++tp) {
+          const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
+          double g[4] = {0.0, 0.0, 0.0, 0.0};
+          if (gm[0] | gm[1] | gm[2] | gm[3]) {
+            int t2, u2, v2;
+            dec(tp, t2, u2, v2);
+            for (int t = 0; t < ab.ntab; ++t) {
+              int t1, u1, v1;
+              dec(t, t1, u1, v1);
+              const int s3[3] = {t1 + t2, u1 + u2, v1 + v2};
+              const int h = hidx(s3[0], s3[1], s3[2]);
+              if (sh.cflag[h]) {
+                double r = R[(int64_t)h * ld + kl];
+                const double *e = &sh.eab[ix][4 * t];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) g[c] += e[c] * r;
+              }
+              if (a.gaunt) continue;
+              for (int jx = 0; jx < 3; ++jx) {
+                const double *e = &sh.eab[jx][4 * t];
+                if (!any4(e)) continue;
+                int e1[3] = {s3[0], s3[1], s3[2]};
+                e1[ix] += 1;
+                const int h2 = hidx(e1[0], e1[1], e1[2]);
+                int e3[3] = {e1[0], e1[1], e1[2]};
+                e1[jx] += 1;
+                e3[jx] -= 1;
+                const int h1 = hidx(e1[0], e1[1], e1[2]);
+                const bool ok3 = e3[0] >= 0 && e3[1] >= 0 && e3[2] >= 0;
+                const int h3 = ok3 ? hidx(e3[0], e3[1], e3[2]) : 0;
+                int fl = sh.gflag[h1] + sh.gflag[h2] + (ok3 ? sh.gflag[h3] : 0);
+                if (fl == 0) continue;
+                double tv = R[(int64_t)h1 * ld + kl] * half_ia - R[(int64_t)h2 * ld + kl] * pq[jx];
+                if (ok3) tv += R[(int64_t)h3 * ld + kl] * (double)s3[jx];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) g[c] -= e[c] * tv;
+              }
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            G[(int64_t)((ix * 4 + c) * ab.ntg + tp) * ld + kl] = gm[c] ? g[c] : 0.0;
+        }
+    }
+    for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
+      const BrCD cd = a.cds[ci];
+      __syncthreads();
+      for (int t = lane; t < 12 * cd.ntcd; t += 64) sh.km[t] = a.masks[cd.kmask + t];
+      __syncthreads();
+      const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+      const double pabcd = pab1 * pcd1;
+      for (int kl = lane; kl < maxcd; kl += 64) {
+        double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+        double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+        for (int ix = 0; ix < 3; ++ix)
+          for (int tp = 0; tp < cd.ntcd; ++tp) {
+            double4 e = a.E[cd.eoff[ix] + (int64_t)tp * maxcd + kl];
+            int t2, u2, v2;
+            dec(tp, t2, u2, v2);
+            double z = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
+            const uint8_t *km = &sh.km[(tp * 3 + ix) * 4];
+            double e11r = km[0] ? z * e.x : 0.0;
+            double e11i = km[1] ? z * e.y : 0.0;
+            double e21r = km[2] ? z * e.z : 0.0;
+            double e21i = km[3] ? z * e.w : 0.0;
+            const double *gp = G + (int64_t)(ix * 4 * ab.ntg + tp) * ld + kl;
+            const int64_t gs = (int64_t)ab.ntg * ld;
+            double g11r = gp[0], g11i = gp[gs], g21r = gp[2 * gs], g21i = gp[3 * gs];
+            ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
+            br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
+            ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
+            br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
+            ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
+            br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
+            ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
+            br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
+          }
+        double rr_r[16], rr_i[16];
+        if (a.isym != 0) {
+          for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
+          rr_r[0] = ar[0];
+          rr_r[12] = pab1 * br[0];
+          rr_r[15] = pabcd * rr_r[0];
+          rr_r[3] = pabcd * rr_r[12];
+          rr_r[10] = ar[3];
+          rr_r[6] = -pab1 * br[3];
+          rr_r[5] = pabcd * rr_r[10];
+          rr_r[9] = pabcd * rr_r[6];
+        } else {
+          rr_r[0] = ar[0]; rr_i[0] = ai[0];
+          rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
+          rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
+          rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
+          rr_r[2] = ar[1]; rr_i[2] = ai[1];
+          rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
+          rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
+          rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
+          rr_r[8] = ar[2]; rr_i[8] = ai[2];
+          rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
+          rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
+          rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
+          rr_r[10] = ar[3]; rr_i[10] = ai[3];
+          rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
+          rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
+          rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
+        }
+        if (a.rr_out) {
+          if (ci == 0)
+            for (int n = 0; n < 16; ++n)
+              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
+          continue;
+        }
+        const int idx[4] = {ib, jb, kl / nD, kl % nD};
+        const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                               {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+        double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        if (a.isym != 0)
+          digest<true>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+                       cd.pcd[1], acc_unused, sh.x, a.bdir);
+        else
+          digest<false>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+                        cd.pcd[1], acc_unused, sh.x, a.bdir);
+      }
+      if (!a.rr_out) {
+        const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+        flush_exchange(sh.x, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, cl.nbas[2], nD,
+                       a.bxch, nd, lane);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
+  __shared__ BreitShared sh;
+  __shared__ int s_item;
+  const int lane = threadIdx.x;
+  double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
+  {
+    double *z = &sh.x.l[0][0][0][0];
+    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
+  }
+  __syncthreads();
+  for (;;) {
+    if (lane == 0) s_item = atomicAdd(a.counter, 1);
+    __syncthreads();
+    int idx = s_item;
+    __syncthreads();
+    if (idx >= a.nitems) break;
+    breit_item(a, a.items[idx], slab, sh, lane);
+  }
+}
+
+// BRTFAST :17646-17664: the SL blocks are the conjugate transpose of LS
+__global__ void breit_complete_kernel(double2 *bdir, double2 *bxch, int ndim, int nskp) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t n = (int64_t)nskp * (ndim - nskp);
+  if (g >= n) return;
+  int i = (int)(g % nskp), j = nskp + (int)(g / nskp);
+  double2 v = bdir[(int64_t)j * ndim + i];
+  bdir[(int64_t)i * ndim + j] = make_double2(v.x, -v.y);
+  v = bxch[(int64_t)j * ndim + i];
+  bxch[(int64_t)i * ndim + j] = make_double2(v.x, -v.y);
 }
 
 // largest |D| of the density (non-negative doubles order like their bit
@@ -910,6 +1222,21 @@ hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
   int bs = 128;
   int64_t nb = (a.nitems + bs - 1) / bs;
   hipLaunchKernelGGL(diag_kernel, dim3((unsigned)nb), dim3(bs), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s) {
+  int nf = std::max(1, std::min(nblocks, a.nclasses));
+  if (a.nclasses > 0)
+    hipLaunchKernelGGL(breit_flags_kernel, dim3(nf), dim3(64), 0, s, a);
+  if (a.nitems > 0)
+    hipLaunchKernelGGL(breit_kernel, dim3(std::max(1, std::min(nblocks, a.nitems))), dim3(64), 0,
+                       s, a);
+  if (!a.rr_out) {
+    int64_t n = (int64_t)a.nskp * (a.ndim - a.nskp);
+    int bs = 256;
+    hipLaunchKernelGGL(breit_complete_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s,
+                       a.bdir, a.bxch, a.ndim, a.nskp);
+  }
   return hipGetLastError();
 }
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s) {

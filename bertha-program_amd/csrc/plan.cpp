@@ -220,9 +220,8 @@ static void cg(int kq, int mq, bool small, double &up, double &lo) {
 // (small=true) components.  Small components carry the radial factors of
 // sigma.p acting on r^l exp(-a r^2): -2a r^(l+1) for kappa<0 and
 // (2l+1) r^(l-1) - 2a r^(l+1) for kappa>0 (EQSS cases 1-4, 35996-36860).
-void Etabs::spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begin,
-                         int &count) {
-  auto key = std::make_tuple(ka, mqa, kb, mqb, (int)small);
+void Etabs::spinor_terms(int ka, int mqa, int kb, int mqb, int kind, int &begin, int &count) {
+  auto key = std::make_tuple(ka, mqa, kb, mqb, kind);
   auto it = term_index.find(key);
   if (it != term_index.end()) {
     begin = it->second.first;
@@ -230,6 +229,13 @@ void Etabs::spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begi
     return;
   }
   begin = (int)terms.size();
+  if (kind >= kLSx) {
+    ls_terms(ka, mqa, kb, mqb, kind - kLSx + 1);
+    count = (int)terms.size() - begin;
+    term_index[key] = {begin, count};
+    return;
+  }
+  const bool small = kind == kSS;
   int la = lval(ka), lb = lval(kb);
   double cau, cal, cbu, cbl;
   cg(ka, mqa, small, cau, cal);
@@ -279,14 +285,62 @@ void Etabs::spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begi
   term_index[key] = {begin, count};
 }
 
+// Large component of (ka, mqa) against the kinetically balanced small
+// component of (kb, mqb), coupled by the Pauli matrix sigma_iq (EQLS,
+// bertha-scf.f:34937-35430): spin terms 11/22 (z) or 12/21 (x, y) with the
+// reference's signs; radial factors of the small component -2b r^(l+1)
+// (kappa < 0) or (2l+1) r^(l-1) - 2b r^(l+1) (kappa > 0).  The factor i
+// (and i^2 for sigma_y) is applied by the table kernel.
+void Etabs::ls_terms(int ka, int mqa, int kb, int mqb, int iq) {
+  int la = lval(ka), lb = lval(kb);
+  double cau, cal, cbu, cbl;
+  cg(ka, mqa, false, cau, cal);
+  cg(kb, mqb, true, cbu, cbl);
+  double sig = iq == 1 ? 1.0 : -1.0;
+  int tsa[2], tsb[2];
+  double tw[2];
+  if (iq == 3) {
+    tsa[0] = 0; tsb[0] = 0; tw[0] = 1.0;
+    tsa[1] = 1; tsb[1] = 1; tw[1] = sig;
+  } else {
+    tsa[0] = 0; tsb[0] = 1; tw[0] = sig;
+    tsa[1] = 1; tsb[1] = 0; tw[1] = 1.0;
+  }
+  int l2 = kb < 0 ? lb + 1 : lb - 1;
+  for (int s = 0; s < 2; ++s) {
+    int mla = tsa[s] == 0 ? (mqa - 1) / 2 : (mqa + 1) / 2;
+    int mlb = tsb[s] == 0 ? (mqb - 1) / 2 : (mqb + 1) / 2;
+    double cab = (tsa[s] == 0 ? cau : cal) * (tsb[s] == 0 ? cbu : cbl);
+    if (std::fabs(cab) < 1.0e-14) continue;
+    if (std::abs(mla) > la || std::abs(mlb) > l2) continue;
+    double w0 = tw[s] * cab * ang(la, mla, l2, mlb);
+    auto add = [&](double w, int pb, int rb) {
+      ETerm t;
+      t.w = w0 * w;
+      t.pa = 0;
+      t.pb = (int16_t)pb;
+      t.polyA = poly_id(la, -mla, 0);
+      t.polyB = poly_id(l2, mlb, rb);
+      terms.push_back(t);
+    };
+    if (kb < 0) {
+      add(-2.0, 1, 0);
+    } else {
+      add(2.0 * lb + 1.0, 0, 0);
+      add(-2.0, 1, 1);
+    }
+  }
+}
+
 // E0LLGN/E0SSGN (bertha-scf.f:33378, 33511): one table per ordered pair of
 // (centre, kappa, |mj|) blocks and component type; E11 = (-|mA|,-|mB|),
 // E21 = (+|mA|,-|mB|) (EQLLMK 34003-34034).
-void build_etables(const Basis &B, Etabs &T) {
+void build_etables(const Basis &B, Etabs &T, bool with_ls) {
   T.pairs.clear();
   T.pair_index.clear();
   int64_t off = 0;
-  for (int small = 0; small < 2; ++small)
+  const int nkind = with_ls ? 5 : 2;
+  for (int small = 0; small < nkind; ++small)
     for (int a = 0; a < B.ncnt; ++a)
       for (int b = 0; b < B.ncnt; ++b)
         for (int ka = 1; ka <= 2 * B.lmax[a] + 1; ++ka)
@@ -300,8 +354,9 @@ void build_etables(const Basis &B, Etabs &T) {
                 p.bx = B.xyz[3 * b]; p.by = B.xyz[3 * b + 1]; p.bz = B.xyz[3 * b + 2];
                 p.expA = B.expo(a, la); p.expB = B.expo(b, lb);
                 p.nA = B.nfn(a, la); p.nB = B.nfn(b, lb);
-                p.lam = la + lb + (small ? 2 : 0);
+                p.lam = la + lb + (small == kSS ? 2 : (small >= kLSx ? 1 : 0));
                 p.small = small;
+                p.phase = small >= kLSx ? (small == kLSy ? 2 : 1) : 0;
                 p.la = la; p.lb = lb;
                 p.off = off;
                 T.spinor_terms(kqa, -(2 * ma - 1), kqb, -(2 * mb - 1), small, p.term11, p.nterm11);
@@ -383,7 +438,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
             else cl.lamfl = ilev == 1 ? lsum : (ilev == 2 ? lsum + 2 : lsum + 4);
             cl.maxcd = cl.nbas[2] * cl.nbas[3];
             cl.mcnt = mcnt;
-            cl.pad = 0;
+            cl.flagoff = 0;
             cl.ab_begin = (int)P.abs.size();
             // aggregated E(CD) flags of the (C,D,lc,ld) class (E0LLCD0/E0SSCD0)
             int ntss = ntuv(lc + ld + 2);
@@ -566,6 +621,215 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
       const ABCombo &ab = P.abs[a];
       w += (double)ab.ntab * ab.ntg * cl.maxcd;
       for (int k = ab.cd_begin; k < ab.cd_end; ++k) w += 8.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
+    }
+    cost[c] = w;
+  }
+  std::stable_sort(P.items.begin(), P.items.end(),
+                   [&](const WorkItem &x, const WorkItem &y) { return cost[x.cls] > cost[y.cls]; });
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+// the Breit plan: BRTFAST's loop nest (bertha-scf.f:17185-17600)
+// ---------------------------------------------------------------------------
+std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
+                             int isym, int intsym, int racah1, int ilev, int gaunt,
+                             BreitPlan &P) {
+  P = BreitPlan();
+  std::map<std::tuple<int, int, int>, int> iqidx;
+  int cnt = 0;
+  for (int c = 0; c < B.ncnt; ++c)
+    for (int k = 1; k <= 2 * B.lmax[c] + 1; ++k) {
+      int kq = B.kappa_of(c, k);
+      for (int mj = 1; mj <= 2 * std::abs(kq) - 1; mj += 2) iqidx[{c, kq, mj}] = ++cnt;
+    }
+  auto flag_at = [&](int pair, int tuv, int comp) -> int {
+    return eflags[T.flag_off[pair] + 4 * tuv + comp];
+  };
+  auto ls_pair = [&](int a, int b, int ka, int kb, int ma, int mb, int ix) {
+    return T.pair_index.at(Etabs::key(a, b, ka, kb, ma, mb, kLSx + ix));
+  };
+  for (int ca = 0; ca < B.ncnt; ++ca)
+    for (int cb = 0; cb < B.ncnt; ++cb)
+      for (int cc = 0; cc < B.ncnt; ++cc)
+        for (int cd = 0; cd < B.ncnt; ++cd) {
+          int mcnt = ncntrs(ca, cb, cc, cd);
+          if (mcnt == 1 && racah1) continue;
+          if (mcnt != 1 && ilev < 3) continue;
+          int ic[4] = {ca, cb, cc, cd};
+          for (int la = 0; la <= B.lmax[ca]; ++la)
+          for (int lb = 0; lb <= B.lmax[cb]; ++lb)
+          for (int lc = 0; lc <= B.lmax[cc]; ++lc)
+          for (int ld = 0; ld <= B.lmax[cd]; ++ld) {
+            if (mcnt == 1 && ((la + lb) % 2) != ((lc + ld) % 2)) continue;
+            int lq[4] = {la, lb, lc, ld};
+            ClassDesc cl;
+            for (int k = 0; k < 4; ++k) {
+              for (int x = 0; x < 3; ++x) cl.xyz[k][x] = B.xyz[3 * ic[k] + x];
+              cl.exp[k] = B.expo(ic[k], lq[k]);
+              cl.nbas[k] = B.nfn(ic[k], lq[k]);
+            }
+            int lamab = la + lb + 1, lamcd = lc + ld + 1;
+            cl.lamfl = gaunt ? lamab + lamcd : lamab + lamcd + 2;
+            cl.maxcd = cl.nbas[2] * cl.nbas[3];
+            cl.mcnt = mcnt;
+            cl.flagoff = P.nflag;
+            cl.ab_begin = (int)P.abs.size();
+            int ntcd = ntuv(lamcd);
+            // EILSCD0 (:17997): E(CD) flags summed over the class, [tuv][ix][c]
+            std::vector<int> agg(12 * ntcd, 0);
+            for (int nc = (lc == 0); nc <= 1; ++nc)
+              for (int nd = (ld == 0); nd <= 1; ++nd)
+                for (int mc = 1; mc <= lc + nc; ++mc)
+                  for (int md = 1; md <= ld + nd; ++md)
+                    for (int ix = 0; ix < 3; ++ix) {
+                      int pr = ls_pair(cc, cd, 2 * lc + nc, 2 * ld + nd, mc, md, ix);
+                      for (int t = 0; t < ntcd; ++t)
+                        for (int c = 0; c < 4; ++c) agg[(t * 3 + ix) * 4 + c] += flag_at(pr, t, c);
+                    }
+            for (int na = (la == 0); na <= 1; ++na) {
+              int ka = 2 * la + na, kqa = B.kappa_of(ca, ka), jqa = 2 * std::abs(kqa) - 1;
+              for (int nb = (lb == 0); nb <= 1; ++nb) {
+                int kb = 2 * lb + nb, kqb = B.kappa_of(cb, kb), jqb = 2 * std::abs(kqb) - 1;
+                for (int ma = 1; ma <= std::abs(kqa); ++ma)
+                  for (int mb = 1; mb <= std::abs(kqb); ++mb) {
+                    int mqa = 2 * ma - 1, mqb = 2 * mb - 1;
+                    BrAB ab;
+                    for (int ix = 0; ix < 3; ++ix)
+                      ab.eoff[ix] = T.pairs[ls_pair(ca, cb, ka, kb, ma, mb, ix)].off;
+                    ab.ntab = ntuv(lamab);
+                    ab.ntg = ntcd;
+                    // IT1 = 2: A large, B small (:17432-17440)
+                    ab.fa[0] = B.lrge_at(ca, ka, 2 * ma - 1);
+                    ab.fa[1] = B.lrge_at(ca, ka, 2 * ma);
+                    ab.fb[0] = B.lrge_at(cb, kb, 2 * mb - 1) + B.nskp;
+                    ab.fb[1] = B.lrge_at(cb, kb, 2 * mb) + B.nskp;
+                    ab.cd_begin = (int)P.cds.size();
+                    ab.pad = 0;
+                    int sab = isgn(kqa * kqb);
+                    for (int nc = (lc == 0); nc <= 1; ++nc) {
+                      int kc = 2 * lc + nc, kqc = B.kappa_of(cc, kc), jqc = 2 * std::abs(kqc) - 1;
+                      for (int nd = (ld == 0); nd <= 1; ++nd) {
+                        int kd = 2 * ld + nd, kqd = B.kappa_of(cd, kd), jqd = 2 * std::abs(kqd) - 1;
+                        if (mcnt == 1) {
+                          // Breit selection rule (:17484-17499)
+                          int nui = std::max(std::abs(jqa - jqb) / 2, std::abs(jqc - jqd) / 2);
+                          int nuf = std::min((jqa + jqb) / 2, (jqc + jqd) / 2);
+                          if (nui > nuf) continue;
+                          bool selk = false;
+                          for (int nu = nui; nu <= nuf; ++nu) {
+                            int pab_ = (la + lb + nu) % 2, pcd_ = (lc + ld + nu) % 2;
+                            if (pab_ == 1 && pcd_ == 1 && nu != 0) selk = true;
+                            if (pab_ == 0 && pcd_ == 0) selk = true;
+                          }
+                          if (!selk) continue;
+                        }
+                        for (int mc = 1; mc <= std::abs(kqc); ++mc)
+                          for (int md = 1; md <= std::abs(kqd); ++md) {
+                            int mqc = 2 * mc - 1, mqd = 2 * md - 1;
+                            if (isym == 1 || isym == 2) {
+                              bool ok = (mqa == mqb && mqc == mqd) || (mqa == mqc && mqb == mqd) ||
+                                        (mqa == mqd && mqb == mqc);
+                              if (!ok) continue;
+                            }
+                            if (mcnt == 1) {
+                              bool selm = false;
+                              for (int sa = -1; sa <= 1; sa += 2)
+                                for (int sb = -1; sb <= 1; sb += 2)
+                                  for (int sc = -1; sc <= 1; sc += 2)
+                                    for (int sd = -1; sd <= 1; sd += 2)
+                                      if (sa * mqa - sb * mqb == sd * mqd - sc * mqc) selm = true;
+                              if (!selm) continue;
+                            }
+                            int iq1 = iqidx.at({ca, kqa, mqa}), iq2 = iqidx.at({cb, kqb, mqb});
+                            int iq3 = iqidx.at({cc, kqc, mqc}), iq4 = iqidx.at({cd, kqd, mqd});
+                            int iql = iq1 * (iq1 - 1) / 2 + iq2, iqr = iq3 * (iq3 - 1) / 2 + iq4;
+                            if (intsym && iql < iqr) continue;
+                            int scd = isgn(kqc * kqd);
+                            BrCD call;
+                            for (int ix = 0; ix < 3; ++ix)
+                              call.eoff[ix] = T.pairs[ls_pair(cc, cd, kc, kd, mc, md, ix)].off;
+                            call.ntcd = ntcd;
+                            call.kmask = (int)P.masks.size();
+                            for (int t = 0; t < ntcd; ++t)
+                              for (int ix = 0; ix < 3; ++ix) {
+                                int pr = ls_pair(cc, cd, kc, kd, mc, md, ix);
+                                for (int c = 0; c < 4; ++c) P.masks.push_back((uint8_t)flag_at(pr, t, c));
+                              }
+                            // IT2 = 2: C large, D small
+                            call.fc[0] = B.lrge_at(cc, kc, 2 * mc - 1);
+                            call.fc[1] = B.lrge_at(cc, kc, 2 * mc);
+                            call.fd[0] = B.lrge_at(cd, kd, 2 * md - 1) + B.nskp;
+                            call.fd[1] = B.lrge_at(cd, kd, 2 * md) + B.nskp;
+                            // exchange batches 5, 6, 7 and, off the diagonal
+                            // IQL = IQR, 9, 10, 11 (BRTMTG :20043-20350)
+                            call.iflg = (1 << 4) | (1 << 5) | (1 << 6);
+                            if (iql != iqr) call.iflg |= (1 << 8) | (1 << 9) | (1 << 10);
+                            // Breit phases carry an extra minus (:17530-17533)
+                            call.pab[0] = (int8_t)(-sab * m1pow((-mqa + mqb) / 2));
+                            call.pab[1] = (int8_t)(-sab * m1pow((mqa + mqb) / 2));
+                            call.pcd[0] = (int8_t)(-scd * m1pow((-mqc + mqd) / 2));
+                            call.pcd[1] = (int8_t)(-scd * m1pow((mqc + mqd) / 2));
+                            P.cds.push_back(call);
+                            P.quartets += (double)cl.nbas[0] * cl.nbas[1] * cl.maxcd;
+                          }
+                      }
+                    }
+                    ab.cd_end = (int)P.cds.size();
+                    if (ab.cd_end == ab.cd_begin) continue;
+                    // BIIFAST builds G^ix only where some E(CD) column of the
+                    // class is non-negligible (:18505-18508, :18545-18548)
+                    ab.gmask = (int)P.masks.size();
+                    for (int t = 0; t < ntcd; ++t)
+                      for (int ix = 0; ix < 3; ++ix) {
+                        const int *g = &agg[(t * 3 + ix) * 4];
+                        int all = g[0] + g[1] + g[2] + g[3];
+                        for (int c = 0; c < 4; ++c)
+                          P.masks.push_back((uint8_t)(all != 0 && (isym == 0 || g[c] != 0)));
+                      }
+                    P.max_ntg = std::max(P.max_ntg, ab.ntg);
+                    P.abs.push_back(ab);
+                  }
+              }
+            }
+            cl.ab_end = (int)P.abs.size();
+            if (cl.ab_end == cl.ab_begin) continue;
+            int cidx = (int)P.classes.size();
+            P.classes.push_back(cl);
+            P.nflag += ntuv(cl.lamfl);
+            P.max_lfl = std::max(P.max_lfl, cl.lamfl);
+            P.max_maxcd = std::max(P.max_maxcd, cl.maxcd);
+            for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij) P.items.push_back({cidx, ij});
+          }
+        }
+  // Algorithmic FP64 work, counted as for the Coulomb build: R batch per
+  // (ij, class); first contraction per (ij, AB combination) 3 x (8 flops of
+  // the Gaunt term + 3 x 14 of the gauge term) per (tuv, tuv', kl); second
+  // contraction 3 x 32 (3 x 8 real) per (tuv', kl) and call; digestion 128
+  // (8 real) flops per active batch and element.
+  for (const ClassDesc &cl : P.classes) {
+    double maxab = (double)cl.nbas[0] * cl.nbas[1], maxcd = cl.maxcd;
+    double L = cl.lamfl;
+    double c4 = (L + 3) * (L + 2) * (L + 1) * L / 24.0;
+    P.flops += maxab * maxcd * (120.0 + 2.0 * L + 6.0 * c4 + 3.0 * ntuv(cl.lamfl));
+    for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
+      const BrAB &ab = P.abs[a];
+      P.flops += maxab * 3.0 * (gaunt ? 8.0 : 50.0) * ab.ntab * ab.ntg * maxcd;
+      for (int k = ab.cd_begin; k < ab.cd_end; ++k) {
+        double c2 = 3.0 * (isym ? 8.0 : 32.0) * P.cds[k].ntcd;
+        double dig = (isym ? 8.0 : 128.0) * __builtin_popcount(P.cds[k].iflg);
+        P.flops += maxab * maxcd * (c2 + dig);
+      }
+    }
+  }
+  std::vector<double> cost(P.classes.size(), 0.0);
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    double w = (double)ntuv(cl.lamfl) * cl.maxcd;
+    for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
+      const BrAB &ab = P.abs[a];
+      w += 30.0 * ab.ntab * ab.ntg * cl.maxcd;
+      for (int k = ab.cd_begin; k < ab.cd_end; ++k) w += 24.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
     }
     cost[c] = w;
   }

@@ -36,16 +36,21 @@ struct Etabs {
   std::map<std::tuple<int, int, int>, int> poly_index;
   std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> term_index;
   std::map<long, int> pair_index;
-  static long key(int a, int b, int ka, int kb, int ma, int mb, int small) {
-    return ((((((long)a * 64 + b) * 16 + ka) * 16 + kb) * 8 + ma) * 8 + mb) * 2 + small;
+  // kind: kLL, kSS or kLSx/kLSy/kLSz (large A, small B, sigma_x/y/z)
+  static long key(int a, int b, int ka, int kb, int ma, int mb, int kind) {
+    return ((((((long)a * 64 + b) * 16 + ka) * 16 + kb) * 8 + ma) * 8 + mb) * 8 + kind;
   }
   int poly_id(int l, int m, int nr2);
-  void spinor_terms(int ka, int mqa, int kb, int mqb, bool small, int &begin, int &count);
+  void spinor_terms(int ka, int mqa, int kb, int mqb, int kind, int &begin, int &count);
+  void ls_terms(int ka, int mqa, int kb, int mqb, int iq);
 };
 
 std::string build_basis(const bdf_system &sys, Basis &B);
-void build_etables(const Basis &B, Etabs &T);
+void build_etables(const Basis &B, Etabs &T, bool with_ls);
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                        int isym, int intsym, int racah1, int ilev, Plan &P);
+std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
+                             int isym, int intsym, int racah1, int ilev, int gaunt,
+                             BreitPlan &P);
 
 }  // namespace bdf

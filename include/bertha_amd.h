@@ -80,6 +80,8 @@ typedef struct {
                           default 0, bertha-scf.f:5257) */
   double sens;         /* SENS of the masks; <= 0 selects the reference's 1e-12
                           (bertha-scf.f:11567) */
+  int gaunt;           /* GAUNT2: Gaunt interaction instead of Breit in
+                          bdf_breit (reference default 0, bertha-scf.f:5246) */
 } bdf_options;
 
 typedef struct bdf_engine bdf_engine;
@@ -119,9 +121,38 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
                   const int midx[4], int ibas, int jbas, const int itn[2],
                   int ilev, double _Complex *rr, int maxcd);
 
+/* Replaces CALL BRTFAST (bertha-scf.f:5441; routine at :17060): closed-shell
+   Breit (or, with opt.gaunt, Gaunt) matrices for the total density DENT at
+   stage ilev (multi-centre quartets only at ilev 3, :17254).  BDIR stays
+   zero for closed shells; BXCH holds the LS blocks and their conjugate
+   transpose SL (:17646-17664).  Needs an engine created with
+   BDF_HMLT_DHFB.  deno/wdir/wxch (open shells) must be NULL. */
+int bdf_breit(bdf_engine *e, int ilev, const double _Complex *dent,
+              const double _Complex *deno, int ld, double _Complex *bdir,
+              double _Complex *bxch, double _Complex *wdir, double _Complex *wxch);
+
+/* Same on device-resident arrays (ld = ndim), enqueued on the engine's
+   stream and not synchronised. */
+int bdf_breit_device(bdf_engine *e, int ilev, const void *d_dent, void *d_bdir,
+                     void *d_bxch);
+
+/* Test hook mirroring BIIFAST's output (bertha-scf.f:18337-19169) for one
+   (LS|LS) block quartet at a fresh class (the gauge term sees this pair's
+   own R-column flags): arguments as bdf_eri_batch without itn/ilev;
+   rr[16][maxcd]. */
+int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
+                  const int midx[4], int ibas, int jbas, double _Complex *rr,
+                  int maxcd);
+
+/* Size of the stage-ilev Breit build (as bdf_plan_info). */
+int bdf_breit_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets,
+                        long *items);
+
 /* Copy of one E0LL (itt=1) or E0SS (itt=4) block as the reference stores it
    (bertha-scf.f:33485-33494): out[comp][tuv][ij], comp = Re E11, Im E11,
-   Re E21, Im E21.  Returns the element count per component in *n. */
+   Re E21, Im E21.  itt=2 returns the EILS block of a DHFB engine
+   (bertha-scf.f:33757-33768): 12 components, sigma_x, y, z times the four
+   above.  Returns the element count per component in *n. */
 int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb,
                int ma, int mb, double *out, long cap, long *n);
 

@@ -21,6 +21,7 @@ module bertha_amd
     real(c_double) :: split_threshold
     integer(c_int) :: device, schwarz
     real(c_double) :: sens
+    integer(c_int) :: gaunt
   end type bdf_options
 
   interface
@@ -66,6 +67,41 @@ module bertha_amd
       real(c_double), intent(out) :: out(*)
       integer(c_long), value :: cap
       integer(c_long), intent(out) :: n
+    end function
+
+    integer(c_int) function bdf_breit(eng, ilev, dent, deno, ld, bdir, bxch, wdir, wxch) &
+        bind(c, name='bdf_breit')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev, ld
+      complex(c_double_complex), intent(in) :: dent(ld, *)
+      type(c_ptr), value :: deno, wdir, wxch
+      complex(c_double_complex), intent(out) :: bdir(ld, *), bxch(ld, *)
+    end function
+
+    integer(c_int) function bdf_breit_device(eng, ilev, d_dent, d_bdir, d_bxch) &
+        bind(c, name='bdf_breit_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, d_dent, d_bdir, d_bxch
+      integer(c_int), value :: ilev
+    end function
+
+    integer(c_int) function bdf_bii_batch(eng, icnt, kidx, midx, ibas, jbas, rr, maxcd) &
+        bind(c, name='bdf_bii_batch')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), intent(in) :: icnt(4), kidx(4), midx(4)
+      integer(c_int), value :: ibas, jbas, maxcd
+      complex(c_double_complex), intent(out) :: rr(maxcd, 16)
+    end function
+
+    integer(c_int) function bdf_breit_plan_info(eng, ilev, flops, quartets, items) &
+        bind(c, name='bdf_breit_plan_info')
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev
+      real(c_double), intent(out) :: flops, quartets
+      integer(c_long), intent(out) :: items
     end function
 
     integer(c_int) function bdf_schwarz_tables(eng, gdsc, qbnd, ld) &

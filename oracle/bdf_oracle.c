@@ -118,8 +118,13 @@ struct orc_sys {
   double *gdsc; /* [ndim][ndim] column-major, NULL until computed */
   int schwarz;
   double sens;
+  /* EILS tables (EILSGN): plane 4*(ix-1)+c, c = Re/Im E11, Re/Im E21 */
+  double *eils[12];
+  long nls;
+  long *iadls;
   double drop_max, drop_count; /* largest screened digestion term, count */
   int record;                  /* single-threaded build: keep the two above */
+  int gaunt;                   /* GAUNT2: Gaunt instead of Breit (BGAUNT) */
 };
 
 #define KAPA(s, ic, k) ((s)->kapa[(ic) * (MKPX + 1) + (k)])
@@ -640,6 +645,143 @@ static void build_etables(orc_sys *s) {
           }
         }
   }
+}
+
+/* EQLS 34937-35430: large component of A against the kinetically balanced
+   small component of B, coupled by the Pauli matrix sigma_iq (1 x, 2 y,
+   3 z): spin terms 11/22 for z and 12/21 for x and y, with the reference's
+   signs (SIG = -1 for y and z on terms 22 and 12).
+   Radial factors of B's small component: -2b r^(l+1) for kappa < 0,
+   (2l+1) r^(l-1) - 2b r^(l+1) for kappa > 0.  The block is multiplied by
+   i (and by i again for sigma_y, 35418-35425).  out ell[tuv][m],
+   ntuv(la+lb+1). */
+static void eq_ls(mcmd_t *g, zc *ell, const double *ea, int na, const double *eb, int nb,
+                  const double *xa, const double *xb, int ka, int kb, int mqa, int mqb, int iq) {
+  const double sens = 1.0e-14;
+  int la = lval(ka), lb = lval(kb);
+  int maxab = na * nb;
+  int lt = la + lb + 1;
+  long ntot = ntuv(lt);
+  for (long k = 0; k < ntot * maxab; ++k) ell[k] = 0.0;
+  double sig = (iq == 0 || iq == 1) ? 1.0 : -1.0;
+  double cau, cal, cbu, cbl;
+  cgfac(ka, mqa, 0, &cau, &cal);
+  cgfac(kb, mqb, 1, &cbu, &cbl);
+  mcmd_geom(g, ea, na, eb, nb, xa, xb);
+  long big = (long)ntuv(lt + 2) * maxab;
+  zc *esg = (zc *)calloc(big, sizeof(zc));
+  zc *ensg = (zc *)calloc(big, sizeof(zc));
+  double *t2 = malloc(sizeof(double) * maxab);
+  double t0 = (double)(2 * lb + 1);
+  for (int i = 0, m = 0; i < na; ++i)
+    for (int j = 0; j < nb; ++j, ++m) t2[m] = -2.0 * eb[j];
+  int lb2 = kb < 0 ? lb + 1 : lb - 1;
+  /* spin terms (sa, sb, weight): z uses 11 and 22, x and y use 12 and 21 */
+  int tsa[2], tsb[2];
+  double tw[2];
+  if (iq == 0 || iq == 3) {
+    tsa[0] = 0; tsb[0] = 0; tw[0] = 1.0;
+    tsa[1] = 1; tsb[1] = 1; tw[1] = sig;
+  } else {
+    tsa[0] = 0; tsb[0] = 1; tw[0] = sig;
+    tsa[1] = 1; tsb[1] = 0; tw[1] = 1.0;
+  }
+  for (int term = 0; term < 2; ++term) {
+    int mla = tsa[term] == 0 ? (mqa - 1) / 2 : (mqa + 1) / 2;
+    int mlb = tsb[term] == 0 ? (mqb - 1) / 2 : (mqb + 1) / 2;
+    double cab = (tsa[term] == 0 ? cau : cal) * (tsb[term] == 0 ? cbu : cbl);
+    if (fabs(cab) < sens) continue;
+    int lam0 = la + lb2;
+    esgtf(g, esg, la, mla, lb2, mlb);
+    long n0 = ntuv(lam0), n2 = ntuv(lam0 + 2);
+    if (kb < 0) {
+      for (int m = 0; m < maxab; ++m) {
+        double tk = cab * t2[m];
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (tw[term] * tk) * esg[t * maxab + m];
+      }
+    } else {
+      for (int m = 0; m < maxab; ++m) {
+        double tk = cab * t0;
+        for (long t = 0; t < n0; ++t) ell[t * maxab + m] += (tw[term] * tk) * esg[t * maxab + m];
+      }
+      estepn(g, esg, ensg, lam0, 2);
+      for (int m = 0; m < maxab; ++m) {
+        double tk = cab * t2[m];
+        for (long t = 0; t < n2; ++t) ell[t * maxab + m] += (tw[term] * tk) * ensg[t * maxab + m];
+      }
+    }
+  }
+  double rn[64 * 64];
+  rnorm1(rn, ea, na, eb, nb, la, lb, 2);
+  for (int m = 0; m < maxab; ++m)
+    for (long t = 0; t < ntot; ++t) {
+      ell[t * maxab + m] = (I * rn[m]) * ell[t * maxab + m];
+      if (iq == 2) ell[t * maxab + m] = I * ell[t * maxab + m];
+    }
+  free(esg); free(ensg); free(t2);
+}
+
+/* EILSGN 33644-33784 with EQLSMK (IPHS = +1): LS tables for every ordered
+   block pair, built on first use */
+static void build_ls_tables(orc_sys *s) {
+  if (s->iadls) return;
+  long niad = (long)s->ncnt * s->ncnt * MKPX * MKPX * MKPX * MKPX;
+  s->iadls = calloc(niad, sizeof(long));
+  long count = 0;
+  for (int a = 0; a < s->ncnt; ++a)
+    for (int b = 0; b < s->ncnt; ++b)
+      for (int ka = 1; ka <= s->nkap[a]; ++ka)
+        for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+          int kqa = KAPA(s, a, ka), kqb = KAPA(s, b, kb);
+          int la = lval(kqa), lb = lval(kqb);
+          long maxab = (long)NFNC(s, a, la) * NFNC(s, b, lb);
+          count += (long)abs(kqa) * abs(kqb) * ntuv(la + lb + 1) * maxab;
+        }
+  for (int c = 0; c < 12; ++c) s->eils[c] = (double *)calloc(count > 0 ? count : 1, sizeof(double));
+  s->nls = count;
+  long icount = 0;
+  for (int a = 0; a < s->ncnt; ++a)
+    for (int b = 0; b < s->ncnt; ++b)
+      for (int ka = 1; ka <= s->nkap[a]; ++ka) {
+        int kqa = KAPA(s, a, ka), la = lval(kqa);
+        int na = NFNC(s, a, la);
+        for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+          int kqb = KAPA(s, b, kb), lb = lval(kqb);
+          int nb = NFNC(s, b, lb);
+          int maxab = na * nb;
+          int lam = la + lb + 1;
+          long nt = ntuv(lam);
+          mcmd_t g;
+          mcmd_alloc(&g, maxab, lam + 4);
+          zc *e11 = (zc *)calloc((long)ntuv(lam + 4) * maxab, sizeof(zc));
+          zc *e21 = (zc *)calloc((long)ntuv(lam + 4) * maxab, sizeof(zc));
+          const double *ea = &BEXL(s, a, la, 0), *eb = &BEXL(s, b, lb, 0);
+          for (int ma = 1; ma <= abs(kqa); ++ma)
+            for (int mb = 1; mb <= abs(kqb); ++mb) {
+              int mja = 2 * ma - 1, mjb = 2 * mb - 1;
+              s->iadls[iadx(s, a + 1, b + 1, ka, kb, ma, mb)] = icount;
+              for (int iq = 1; iq <= 3; ++iq) {
+                eq_ls(&g, e11, ea, na, eb, nb, s->xyz + 3 * a, s->xyz + 3 * b, kqa, kqb, -mja,
+                      -mjb, iq);
+                eq_ls(&g, e21, ea, na, eb, nb, s->xyz + 3 * a, s->xyz + 3 * b, kqa, kqb, mja,
+                      -mjb, iq);
+                double **pl = s->eils + 4 * (iq - 1);
+                for (long t = 0; t < nt; ++t) {
+                  long iad0 = icount + t * maxab;
+                  for (int m = 0; m < maxab; ++m) {
+                    pl[0][iad0 + m] = creal(e11[t * maxab + m]);
+                    pl[1][iad0 + m] = cimag(e11[t * maxab + m]);
+                    pl[2][iad0 + m] = creal(e21[t * maxab + m]);
+                    pl[3][iad0 + m] = cimag(e21[t * maxab + m]);
+                  }
+                }
+              }
+              icount += nt * maxab;
+            }
+          free(e11); free(e21);
+          mcmd_free(&g);
+        }
+      }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1677,6 +1819,561 @@ void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, doub
 }
 
 /* ------------------------------------------------------------------------ */
+/* Breit: BRTFAST 17060-17670, EILSCD0 17997, BIIFAST 18337-19169,          */
+/* BRTMTZ 19172 / BRTMTG 19815                                              */
+/* ------------------------------------------------------------------------ */
+/* per-class state of BRTFAST/BIIFAST (COMMON /IRCM/, /ICDB/ and BIIFAST's
+   saved locals RCTTFL, IRCTTFL, IRC) */
+typedef struct {
+  int maxab, maxcd, ntab, ntcd, ntabcd;
+  int ieab, igab, ncd;
+  unsigned char *irij;  /* [ij] */
+  double *rctt;         /* [ij][tuv][kl] */
+  unsigned char *irctt; /* [ij][tuv] */
+  unsigned char *irc;   /* IRC: column flags of the latest R batch */
+  unsigned char *iab;   /* [ix][c][ij][iab] */
+  double *gcd;          /* [ix][c][icd][kl] */
+  int *km;              /* [c][ncd][mc][md][icd][ix] */
+  int *agg;             /* [c][icd][ix] */
+  double *rc, *rc2, *fs, *pq, *aph, *pre, *tt;
+} brt_ws;
+
+#define BIAB(w, ix, c, ij, iab) (w)->iab[((((ix) * 4 + (c)) * (long)(w)->maxab + (ij)) * (w)->ntab) + (iab)]
+#define BGCD(w, ix, c, icd) ((w)->gcd + (((ix) * 4 + (c)) * (long)(w)->ntcd + (icd)) * (w)->maxcd)
+#define BKM(w, c, ncd, mc, md, icd, ix) \
+  (w)->km[(((((c) * 4 + (ncd)) * 4 + (mc)) * 4 + (md)) * (long)(w)->ntcd + (icd)) * 3 + (ix)]
+#define BAGG(w, c, icd, ix) (w)->agg[((c) * (w)->ntcd + (icd)) * 3 + (ix)]
+
+static void brt_ws_alloc(brt_ws *w, int maxab, int maxcd, int lamab, int lamcd, int lamabcd) {
+  memset(w, 0, sizeof(*w));
+  w->maxab = maxab; w->maxcd = maxcd;
+  w->ntab = ntuv(lamab); w->ntcd = ntuv(lamcd); w->ntabcd = ntuv(lamabcd);
+  w->irij = calloc(maxab, 1);
+  w->rctt = calloc((long)maxab * w->ntabcd * maxcd, sizeof(double));
+  w->irctt = calloc((long)maxab * w->ntabcd, 1);
+  w->irc = calloc(w->ntabcd, 1);
+  w->iab = calloc(12L * maxab * w->ntab, 1);
+  w->gcd = calloc(12L * w->ntcd * maxcd, sizeof(double));
+  w->km = calloc(4L * 4 * 4 * 4 * w->ntcd * 3, sizeof(int));
+  w->agg = calloc(4L * w->ntcd * 3, sizeof(int));
+  w->rc = calloc((long)w->ntabcd * maxcd, sizeof(double));
+  w->rc2 = calloc((long)w->ntabcd * maxcd, sizeof(double));
+  w->fs = calloc((long)(lamabcd + 1) * maxcd, sizeof(double));
+  w->pq = calloc(3L * maxcd, sizeof(double));
+  w->aph = calloc(maxcd, sizeof(double));
+  w->pre = calloc(maxcd, sizeof(double));
+  w->tt = calloc(maxcd, sizeof(double));
+}
+static void brt_ws_free(brt_ws *w) {
+  free(w->irij); free(w->rctt); free(w->irctt); free(w->irc); free(w->iab); free(w->gcd);
+  free(w->km); free(w->agg); free(w->rc); free(w->rc2); free(w->fs); free(w->pq); free(w->aph);
+  free(w->pre); free(w->tt);
+}
+
+typedef struct {
+  int icnt[4], kqn[4], mqn[4], nbas[4], lqn[4];
+  const double *exl[4];
+  const double *xyz[4];
+  long iabls, icdls;
+} bquartet_t;
+
+/* EILSCD0 17997-18166: E(CD) column flags of the LS tables for every
+   (kappa_C, kappa_D, |m_C|, |m_D|) of the class, per Cartesian index ix,
+   and their sums over the class */
+static void eilscd0(orc_sys *s, brt_ws *w, const int icnt[4], const int lqn[4]) {
+  const double epscd = 1.0e-10;
+  int maxcd = w->maxcd, ntcd = w->ntcd;
+  memset(w->agg, 0, sizeof(int) * 4 * ntcd * 3);
+  for (int nc = (lqn[2] == 0); nc <= 1; ++nc) {
+    int kc = 2 * lqn[2] + nc;
+    for (int nd = (lqn[3] == 0); nd <= 1; ++nd) {
+      int kd = 2 * lqn[3] + nd;
+      int ncd = 2 * nc + nd;
+      for (int mc = 1; mc <= lqn[2] + nc; ++mc)
+        for (int md = 1; md <= lqn[3] + nd; ++md) {
+          long icdtt = s->iadls[iadx(s, icnt[2], icnt[3], kc, kd, mc, md)];
+          for (int icd = 0; icd < ntcd; ++icd) {
+            long mcd = icdtt + (long)icd * maxcd;
+            for (int ix = 0; ix < 3; ++ix)
+              for (int c = 0; c < 4; ++c) {
+                double sum = 0.0;
+                int flag = 0;
+                for (int m = 0; m < maxcd; ++m) {
+                  sum += fabs(s->eils[4 * ix + c][mcd + m]);
+                  if (sum > epscd) { flag = 1; break; }
+                }
+                BKM(w, c, ncd, mc - 1, md - 1, icd, ix) = flag;
+                BAGG(w, c, icd, ix) += flag;
+              }
+          }
+        }
+    }
+  }
+}
+
+/* BIIFAST 18337-19169: one batch of Breit integrals (LS|LS) for (ibas,jbas)
+   against every kl, RR[16][maxcd].  Gaunt term sum_ix E^ix R E^ix and, for
+   the Breit (not Gaunt) interaction, the gauge term built from
+   R(+e_ix+e_jx)/(2 alpha) - PQ_jx R(+e_ix) + (t+t')_jx R(+e_ix-e_jx).  The
+   gauge term's skip test reads the column flags of the most recent R batch
+   (BIIFAST's local IRC), as the reference does. */
+static void biifast(orc_sys *s, brt_ws *w, const bquartet_t *q, int ibas, int jbas, zc *rr) {
+  const double epsab = 1.0e-16, epsrr = 1.0e-10;
+  int isym = s->isym;
+  int maxab = w->maxab, maxcd = w->maxcd, ntab = w->ntab, ntcd = w->ntcd, ntabcd = w->ntabcd;
+  int lamab = q->lqn[0] + q->lqn[1] + 1, lamcd = q->lqn[2] + q->lqn[3] + 1;
+  int lamabcd = s->gaunt ? lamab + lamcd : lamab + lamcd + 2;
+  int ij0 = (ibas - 1) * q->nbas[1] + (jbas - 1);
+  double eij = q->exl[0][ibas - 1] + q->exl[1][jbas - 1];
+
+  if (w->ieab) {
+    for (int ix = 0; ix < 3; ++ix)
+      for (int iab = 0; iab < ntab; ++iab) {
+        long mab = q->iabls + (long)iab * maxab;
+        for (int c = 0; c < 4; ++c)
+          for (int m = 0; m < maxab; ++m)
+            BIAB(w, ix, c, m, iab) = fabs(s->eils[4 * ix + c][mab + m]) > epsab;
+      }
+    w->ieab = 0;
+  }
+  double px = (q->xyz[0][0] * q->exl[0][ibas - 1] + q->xyz[1][0] * q->exl[1][jbas - 1]) / eij;
+  double py = (q->xyz[0][1] * q->exl[0][ibas - 1] + q->xyz[1][1] * q->exl[1][jbas - 1]) / eij;
+  double pz = (q->xyz[0][2] * q->exl[0][ibas - 1] + q->xyz[1][2] * q->exl[1][jbas - 1]) / eij;
+  {
+    int m = 0;
+    for (int kb = 0; kb < q->nbas[2]; ++kb)
+      for (int lb = 0; lb < q->nbas[3]; ++lb, ++m) {
+        double ekl = q->exl[2][kb] + q->exl[3][lb];
+        double qx = (q->xyz[2][0] * q->exl[2][kb] + q->xyz[3][0] * q->exl[3][lb]) / ekl;
+        double qy = (q->xyz[2][1] * q->exl[2][kb] + q->xyz[3][1] * q->exl[3][lb]) / ekl;
+        double qz = (q->xyz[2][2] * q->exl[2][kb] + q->xyz[3][2] * q->exl[3][lb]) / ekl;
+        w->pq[m] = qx - px; w->pq[maxcd + m] = qy - py; w->pq[2 * maxcd + m] = qz - pz;
+        w->aph[m] = eij * ekl / (eij + ekl);
+        double emx = sqrt(eij + ekl) * eij * ekl;
+        w->pre[m] = 2.0 * PI52 / emx;
+      }
+  }
+  double bfac = s->gaunt ? -1.0 : -0.5;
+  if (w->irij[ij0]) {
+    rmake_ws(w->rc, w->rc2, w->fs, w->pq, w->aph, maxcd, lamabcd);
+    for (int col = 0; col < ntabcd; ++col) {
+      double sum = 0.0;
+      int flag = 0;
+      for (int n = 0; n < maxcd; ++n) {
+        sum += fabs(w->rc[(long)col * maxcd + n]);
+        if (sum > epsrr) { flag = 1; break; }
+      }
+      w->irc[col] = flag;
+    }
+    w->irij[ij0] = 0;
+    double *rt = w->rctt + (long)ij0 * maxcd * ntabcd;
+    for (int col = 0; col < ntabcd; ++col)
+      for (int n = 0; n < maxcd; ++n)
+        rt[(long)col * maxcd + n] = bfac * w->pre[n] * w->rc[(long)col * maxcd + n];
+    for (int col = 0; col < ntabcd; ++col) w->irctt[(long)ij0 * ntabcd + col] = w->irc[col];
+  }
+
+  /* first contraction over ix (18492-18716) */
+  if (w->igab) {
+    const double *rt = w->rctt + (long)ij0 * maxcd * ntabcd;
+    const unsigned char *irt = w->irctt + (long)ij0 * ntabcd;
+    for (int ix = 0; ix < 3; ++ix)
+      for (int icd = 0; icd < ntcd; ++icd) {
+        for (int c = 0; c < 4; ++c) memset(BGCD(w, ix, c, icd), 0, sizeof(double) * maxcd);
+        int icdall = BAGG(w, 0, icd, ix) + BAGG(w, 1, icd, ix) + BAGG(w, 2, icd, ix) +
+                     BAGG(w, 3, icd, ix);
+        if (icdall == 0) continue;
+        int tc = IAX[icd + 1], uc = IBX[icd + 1], vc = ICX[icd + 1];
+        for (int iab = 0; iab < ntab; ++iab) {
+          long mab = q->iabls + (long)iab * maxab;
+          int ta = IAX[iab + 1], ua = IBX[iab + 1], va = ICX[iab + 1];
+          int irabcd = IABC[ta + tc][ua + uc][va + vc];
+          int iaball = BIAB(w, ix, 0, ij0, iab) + BIAB(w, ix, 1, ij0, iab) +
+                       BIAB(w, ix, 2, ij0, iab) + BIAB(w, ix, 3, ij0, iab);
+          if (iaball != 0 && irt[irabcd - 1] != 0) {
+            const double *r0 = rt + (long)maxcd * (irabcd - 1);
+            for (int c = 0; c < 4; ++c) {
+              if (BAGG(w, c, icd, ix) == 0 && isym >= 1) continue;
+              if (!BIAB(w, ix, c, ij0, iab)) continue;
+              double e = s->eils[4 * ix + c][mab + ij0];
+              double *g = BGCD(w, ix, c, icd);
+              for (int n = 0; n < maxcd; ++n) g[n] += e * r0[n];
+            }
+          }
+          if (s->gaunt) continue;
+          /* gauge term (18719-18810) */
+          for (int jx = 0; jx < 3; ++jx) {
+            if (BIAB(w, jx, 0, ij0, iab) + BIAB(w, jx, 1, ij0, iab) + BIAB(w, jx, 2, ij0, iab) +
+                    BIAB(w, jx, 3, ij0, iab) == 0)
+              continue;
+            int tuv[3] = {ta + tc, ua + uc, va + vc};
+            double rtp = (double)tuv[jx];
+            int e1[3] = {tuv[0], tuv[1], tuv[2]};
+            e1[ix] += 1;
+            int e2[3] = {e1[0], e1[1], e1[2]};
+            e1[jx] += 1;
+            int e3[3] = {e2[0], e2[1], e2[2]};
+            e3[jx] -= 1;
+            int iadr1 = IABC[e1[0]][e1[1]][e1[2]];
+            int iadr2 = IABC[e2[0]][e2[1]][e2[2]];
+            int ok3 = e3[0] >= 0 && e3[1] >= 0 && e3[2] >= 0;
+            int iadr3 = ok3 ? IABC[e3[0]][e3[1]][e3[2]] : 0;
+            if (iadr3 != 0) {
+              if (w->irc[iadr1 - 1] + w->irc[iadr2 - 1] + w->irc[iadr3 - 1] == 0) continue;
+            } else {
+              if (w->irc[iadr1 - 1] + w->irc[iadr2 - 1] == 0) continue;
+            }
+            const double *r1 = rt + (long)maxcd * (iadr1 - 1);
+            const double *r2 = rt + (long)maxcd * (iadr2 - 1);
+            const double *r3 = ok3 ? rt + (long)maxcd * (iadr3 - 1) : NULL;
+            const double *pqj = w->pq + (long)jx * maxcd;
+            for (int n = 0; n < maxcd; ++n) {
+              double t1 = r1[n] * 0.5 / w->aph[n];
+              double t2 = r2[n] * pqj[n];
+              double t3 = ok3 ? r3[n] * rtp : 0.0;
+              w->tt[n] = t1 - t2 + t3;
+            }
+            for (int c = 0; c < 4; ++c) {
+              if (BAGG(w, c, icd, ix) == 0 && isym >= 1) continue;
+              if (!BIAB(w, jx, c, ij0, iab)) continue;
+              double e = s->eils[4 * jx + c][mab + ij0];
+              double *g = BGCD(w, ix, c, icd);
+              for (int n = 0; n < maxcd; ++n) g[n] -= e * w->tt[n];
+            }
+          }
+        }
+      }
+    w->igab = 0;
+  }
+
+  /* second contraction, summed over ix (18830-19150) */
+  for (long k = 0; k < 16L * maxcd; ++k) rr[k] = 0.0;
+  int pab = -isgn(q->kqn[0] * q->kqn[1]) * (int)m1pow((q->mqn[0] - q->mqn[1]) / 2);
+  int pcd = -isgn(q->kqn[2] * q->kqn[3]) * (int)m1pow((q->mqn[2] - q->mqn[3]) / 2);
+  double pabcd = (double)(pab * pcd);
+  int mc = (q->mqn[2] + 1) / 2 - 1, md = (q->mqn[3] + 1) / 2 - 1;
+  int ncd = w->ncd;
+  double *qr1 = malloc(sizeof(double) * maxcd), *qr2 = malloc(sizeof(double) * maxcd);
+  double *qi1 = malloc(sizeof(double) * maxcd), *qi2 = malloc(sizeof(double) * maxcd);
+#define RRB(n, k) rr[(long)((k) - 1) * maxcd + (n)]
+#define ELS(c, icd, n) s->eils[4 * ix + (c)][q->icdls + (long)(icd) * maxcd + (n)]
+  for (int ix = 0; ix < 3; ++ix) {
+    if (isym == 1 || isym == 2) {
+      for (int set = 0; set < 2; ++set) {
+        int cr = set == 0 ? 0 : 2, ci = set == 0 ? 1 : 3;
+        for (int n = 0; n < maxcd; ++n) qr1[n] = qr2[n] = 0.0;
+        for (int icd = 0; icd < ntcd; ++icd) {
+          double z = m1pow(ILAM[icd + 1]);
+          const double *gr = BGCD(w, ix, cr, icd), *gi = BGCD(w, ix, ci, icd);
+          if (BKM(w, cr, ncd, mc, md, icd, ix) != 0)
+            for (int n = 0; n < maxcd; ++n) qr1[n] += z * ELS(cr, icd, n) * gr[n];
+          if (BKM(w, ci, ncd, mc, md, icd, ix) != 0)
+            for (int n = 0; n < maxcd; ++n) qr2[n] -= z * ELS(ci, icd, n) * gi[n];
+        }
+        for (int n = 0; n < maxcd; ++n) {
+          if (set == 0) {
+            RRB(n, 1) += (qr1[n] + qr2[n]);
+            RRB(n, 13) += pab * (qr1[n] - qr2[n]);
+          } else {
+            RRB(n, 11) += (qr1[n] + qr2[n]);
+            RRB(n, 7) -= pab * (qr1[n] - qr2[n]);
+          }
+        }
+      }
+    } else {
+      for (int set = 0; set < 4; ++set) {
+        int ce = (set == 0 || set == 2) ? 0 : 2; /* E(CD): 11 or 21 */
+        int cg = (set == 0 || set == 1) ? 0 : 2; /* G: 11 or 21 */
+        for (int n = 0; n < maxcd; ++n) qr1[n] = qr2[n] = qi1[n] = qi2[n] = 0.0;
+        for (int icd = 0; icd < ntcd; ++icd) {
+          double z = m1pow(ILAM[icd + 1]);
+          const double *gr = BGCD(w, ix, cg, icd), *gi = BGCD(w, ix, cg + 1, icd);
+          if (BKM(w, ce, ncd, mc, md, icd, ix) != 0)
+            for (int n = 0; n < maxcd; ++n) {
+              qr1[n] += z * ELS(ce, icd, n) * gr[n];
+              qi2[n] += z * ELS(ce, icd, n) * gi[n];
+            }
+          if (BKM(w, ce + 1, ncd, mc, md, icd, ix) != 0)
+            for (int n = 0; n < maxcd; ++n) {
+              qi1[n] += z * ELS(ce + 1, icd, n) * gr[n];
+              qr2[n] -= z * ELS(ce + 1, icd, n) * gi[n];
+            }
+        }
+        for (int n = 0; n < maxcd; ++n) {
+          zc a = (qr1[n] + qr2[n]) + I * (qi1[n] + qi2[n]);
+          zc b = (qr1[n] - qr2[n]) + I * (qi1[n] - qi2[n]);
+          if (set == 0) { RRB(n, 1) += a; RRB(n, 13) += pab * b; }
+          else if (set == 1) { RRB(n, 3) += a; RRB(n, 15) += pab * b; }
+          else if (set == 2) { RRB(n, 9) += a; RRB(n, 5) -= pab * b; }
+          else { RRB(n, 11) += a; RRB(n, 7) -= pab * b; }
+        }
+      }
+    }
+  }
+  for (int n = 0; n < maxcd; ++n) {
+    RRB(n, 16) = pabcd * conj(RRB(n, 1));
+    RRB(n, 4) = pabcd * conj(RRB(n, 13));
+    RRB(n, 14) = -pabcd * conj(RRB(n, 3));
+    RRB(n, 2) = -pabcd * conj(RRB(n, 15));
+    RRB(n, 8) = -pabcd * conj(RRB(n, 9));
+    RRB(n, 12) = -pabcd * conj(RRB(n, 5));
+    RRB(n, 6) = pabcd * conj(RRB(n, 11));
+    RRB(n, 10) = pabcd * conj(RRB(n, 7));
+  }
+#undef RRB
+#undef ELS
+  free(qr1); free(qr2); free(qi1); free(qi2);
+}
+
+/* BRTFAST loop nest for centre A in [a_lo, a_hi) and optional B, C, D
+   selections; digestion as BRTMTG/BRTMTZ: exchange batches 5, 6, 7 and,
+   for IQL != IQR, 9, 10, 11 into BXCH (closed shell: no direct term) */
+static void brtfast_range(orc_sys *s, int ilev, const zc *dent, zc *bdir, zc *bxch, int a_lo,
+                          int a_hi, int b_sel, int c_sel, int d_sel) {
+  int ndim = s->ndim;
+  int isym = s->isym;
+  int *index = calloc((long)s->ncnt * 2 * MKPX * (2 * MKPX + 1), sizeof(int));
+#define INDEXQ(ct, kap, mj) index[(((long)(ct) * (2 * MKPX + 1)) + ((kap) + MKPX)) * (2 * MKPX) + (mj)]
+  int icount = 0;
+  for (int ct = 0; ct < s->ncnt; ++ct)
+    for (int kn = 1; kn <= s->nkap[ct]; ++kn) {
+      int kappa = KAPA(s, ct, kn);
+      for (int mj = 1; mj <= 2 * abs(kappa) - 1; mj += 2) INDEXQ(ct, kappa, mj) = ++icount;
+    }
+  for (int ca = a_lo; ca < a_hi; ++ca)
+    for (int cb = 0; cb < s->ncnt; ++cb) {
+      if (b_sel >= 0 && cb != b_sel) continue;
+      for (int cc = 0; cc < s->ncnt; ++cc)
+        for (int cd = 0; cd < s->ncnt; ++cd) {
+          if (c_sel >= 0 && cc != c_sel) continue;
+          if (d_sel >= 0 && cd != d_sel) continue;
+          int ic[4] = {ca, cb, cc, cd};
+          int mcnt = ncntrs(ca, cb, cc, cd);
+          if (mcnt == 1 && s->racah1) continue;
+          if (mcnt != 1 && ilev < 3) continue;
+          bquartet_t q;
+          for (int k = 0; k < 4; ++k) { q.icnt[k] = ic[k] + 1; q.xyz[k] = s->xyz + 3 * ic[k]; }
+          for (int la = 0; la <= (s->nkap[ca] - 1) / 2; ++la)
+          for (int lb = 0; lb <= (s->nkap[cb] - 1) / 2; ++lb)
+          for (int lc = 0; lc <= (s->nkap[cc] - 1) / 2; ++lc)
+          for (int ld = 0; ld <= (s->nkap[cd] - 1) / 2; ++ld) {
+            int lq[4] = {la, lb, lc, ld};
+            for (int k = 0; k < 4; ++k) {
+              q.lqn[k] = lq[k];
+              q.nbas[k] = NFNC(s, ic[k], lq[k]);
+              q.exl[k] = &BEXL(s, ic[k], lq[k], 0);
+            }
+            int lamab = la + lb + 1, lamcd = lc + ld + 1;
+            brt_ws w;
+            brt_ws_alloc(&w, q.nbas[0] * q.nbas[1], q.nbas[2] * q.nbas[3], lamab, lamcd,
+                         s->gaunt ? lamab + lamcd : lamab + lamcd + 2);
+            for (int k = 0; k < w.maxab; ++k) w.irij[k] = 1;
+            eilscd0(s, &w, q.icnt, q.lqn);
+            if (mcnt == 1 && ((la + lb) % 2) != ((lc + ld) % 2)) { brt_ws_free(&w); continue; }
+            zc *rr = calloc(16L * w.maxcd, sizeof(zc));
+            for (int na = (la == 0); na <= 1; ++na) {
+              int ka = 2 * la + na;
+              q.kqn[0] = KAPA(s, ca, ka);
+              int jqa = 2 * abs(q.kqn[0]) - 1;
+              for (int nb = (lb == 0); nb <= 1; ++nb) {
+                int kb = 2 * lb + nb;
+                q.kqn[1] = KAPA(s, cb, kb);
+                int jqb = 2 * abs(q.kqn[1]) - 1;
+                for (int ma = 1; ma <= abs(q.kqn[0]); ++ma) {
+                  q.mqn[0] = 2 * ma - 1;
+                  for (int mb = 1; mb <= abs(q.kqn[1]); ++mb) {
+                    q.mqn[1] = 2 * mb - 1;
+                    w.ieab = 1;
+                    q.iabls = s->iadls[iadx(s, q.icnt[0], q.icnt[1], ka, kb, ma, mb)];
+                    digest_t dg;
+                    dg.nlab[0][0] = LRGE(s, ca, ka, 2 * ma - 1);
+                    dg.nlab[0][1] = LRGE(s, ca, ka, 2 * ma);
+                    dg.nlab[1][0] = LRGE(s, cb, kb, 2 * mb - 1) + s->nskp;
+                    dg.nlab[1][1] = LRGE(s, cb, kb, 2 * mb) + s->nskp;
+                    for (int ibas = 1; ibas <= q.nbas[0]; ++ibas)
+                      for (int jbas = 1; jbas <= q.nbas[1]; ++jbas) {
+                        w.igab = 1;
+                        for (int nc = (lc == 0); nc <= 1; ++nc) {
+                          int kc = 2 * lc + nc;
+                          q.kqn[2] = KAPA(s, cc, kc);
+                          int jqc = 2 * abs(q.kqn[2]) - 1;
+                          for (int nd = (ld == 0); nd <= 1; ++nd) {
+                            int kd = 2 * ld + nd;
+                            q.kqn[3] = KAPA(s, cd, kd);
+                            int jqd = 2 * abs(q.kqn[3]) - 1;
+                            w.ncd = 2 * nc + nd;
+                            if (mcnt == 1) {
+                              int nui = abs(jqa - jqb) / 2;
+                              if (abs(jqc - jqd) / 2 > nui) nui = abs(jqc - jqd) / 2;
+                              int nuf = (jqa + jqb) / 2;
+                              if ((jqc + jqd) / 2 < nuf) nuf = (jqc + jqd) / 2;
+                              if (nui > nuf) continue;
+                              int iselk = 0;
+                              for (int nu = nui; nu <= nuf; ++nu) {
+                                int pab_ = (la + lb + nu) % 2, pcd_ = (lc + ld + nu) % 2;
+                                if (pab_ == 1 && pcd_ == 1 && nu != 0) iselk = 1;
+                                if (pab_ == 0 && pcd_ == 0) iselk = 1;
+                              }
+                              if (!iselk) continue;
+                            }
+                            for (int mc = 1; mc <= abs(q.kqn[2]); ++mc) {
+                              q.mqn[2] = 2 * mc - 1;
+                              for (int md = 1; md <= abs(q.kqn[3]); ++md) {
+                                q.mqn[3] = 2 * md - 1;
+                                q.icdls = s->iadls[iadx(s, q.icnt[2], q.icnt[3], kc, kd, mc, md)];
+                                if (isym == 1 || isym == 2) {
+                                  int ok = (q.mqn[0] == q.mqn[1] && q.mqn[2] == q.mqn[3]) ||
+                                           (q.mqn[0] == q.mqn[2] && q.mqn[1] == q.mqn[3]) ||
+                                           (q.mqn[0] == q.mqn[3] && q.mqn[1] == q.mqn[2]);
+                                  if (!ok) continue;
+                                }
+                                if (mcnt == 1) {
+                                  int iselm = 0;
+                                  for (int mma = 1; mma <= 2; ++mma)
+                                    for (int mmb = 1; mmb <= 2; ++mmb)
+                                      for (int mmc = 1; mmc <= 2; ++mmc)
+                                        for (int mmd = 1; mmd <= 2; ++mmd) {
+                                          int a_ = q.mqn[0] * (int)m1pow(mma);
+                                          int b_ = q.mqn[1] * (int)m1pow(mmb);
+                                          int c_ = q.mqn[2] * (int)m1pow(mmc);
+                                          int d_ = q.mqn[3] * (int)m1pow(mmd);
+                                          if (a_ - b_ == d_ - c_) iselm = 1;
+                                        }
+                                  if (!iselm) continue;
+                                }
+                                int iq1 = INDEXQ(ca, q.kqn[0], q.mqn[0]);
+                                int iq2 = INDEXQ(cb, q.kqn[1], q.mqn[1]);
+                                int iq3 = INDEXQ(cc, q.kqn[2], q.mqn[2]);
+                                int iq4 = INDEXQ(cd, q.kqn[3], q.mqn[3]);
+                                int iql = (iq1 * (iq1 - 1)) / 2 + iq2;
+                                int iqr = (iq3 * (iq3 - 1)) / 2 + iq4;
+                                if (s->intsym && iql < iqr) continue;
+                                int sab = isgn(q.kqn[0] * q.kqn[1]);
+                                int scd = isgn(q.kqn[2] * q.kqn[3]);
+                                dg.pab[0] = -sab * m1pow(abs(-q.mqn[0] + q.mqn[1]) / 2);
+                                dg.pab[1] = -sab * m1pow((q.mqn[0] + q.mqn[1]) / 2);
+                                dg.pcd[0] = -scd * m1pow(abs(-q.mqn[2] + q.mqn[3]) / 2);
+                                dg.pcd[1] = -scd * m1pow((q.mqn[2] + q.mqn[3]) / 2);
+                                dg.nlab[2][0] = LRGE(s, cc, kc, 2 * mc - 1);
+                                dg.nlab[2][1] = LRGE(s, cc, kc, 2 * mc);
+                                dg.nlab[3][0] = LRGE(s, cd, kd, 2 * md - 1) + s->nskp;
+                                dg.nlab[3][1] = LRGE(s, cd, kd, 2 * md) + s->nskp;
+                                biifast(s, &w, &q, ibas, jbas, rr);
+                                int iflg[11] = {0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 0};
+                                if (iql != iqr) iflg[8] = iflg[9] = iflg[10] = 1;
+                                dg.ibas = ibas; dg.jbas = jbas;
+                                for (int k = 0; k < 4; ++k) dg.nbas[k] = q.nbas[k];
+                                digest(s, &dg, iflg, rr, w.maxcd, dent, bdir, bxch, ndim, NULL);
+                              }
+                            }
+                          }
+                        }
+                      }
+                  }
+                }
+              }
+            }
+            free(rr);
+            brt_ws_free(&w);
+          }
+        }
+    }
+#undef INDEXQ
+  free(index);
+}
+
+/* BRTFAST 17646-17664: SL blocks as the conjugate transpose of LS */
+static void complete_breit(const orc_sys *s, zc *bdir, zc *bxch) {
+  int ndim = s->ndim, nskp = s->nskp;
+  for (int i = 0; i < ndim - nskp; ++i)
+    for (int j = nskp; j < ndim; ++j) {
+      bdir[(long)i * ndim + j] = conj(bdir[(long)j * ndim + i]);
+      bxch[(long)i * ndim + j] = conj(bxch[(long)j * ndim + i]);
+    }
+}
+
+void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir, double *bxch,
+                    int nthreads) {
+  int ndim = s->ndim;
+  long nn = (long)ndim * ndim;
+  build_ls_tables(s);
+  memset(bdir, 0, sizeof(double) * 2 * nn);
+  memset(bxch, 0, sizeof(double) * 2 * nn);
+  const zc *d = (const zc *)dent;
+  if (nthreads <= 1) {
+    for (int a = 0; a < s->ncnt; ++a)
+      brtfast_range(s, ilev, d, (zc *)bdir, (zc *)bxch, a, a + 1, -1, -1, -1);
+  } else {
+    long nq = (long)s->ncnt * s->ncnt * s->ncnt * s->ncnt;
+    zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
+#pragma omp parallel num_threads(nthreads)
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+      gd[tid] = calloc(nn, sizeof(zc));
+      gx[tid] = calloc(nn, sizeof(zc));
+#pragma omp for schedule(dynamic, 1)
+      for (long k = 0; k < nq; ++k) {
+        int n = s->ncnt;
+        int a = (int)(k / ((long)n * n * n)), b = (int)((k / ((long)n * n)) % n);
+        int c = (int)((k / n) % n), dd = (int)(k % n);
+        brtfast_range(s, ilev, d, gd[tid], gx[tid], a, a + 1, b, c, dd);
+      }
+    }
+    zc *B = (zc *)bdir, *X = (zc *)bxch;
+    for (int t = 0; t < nthreads; ++t) {
+      if (!gd[t]) continue;
+      for (long k = 0; k < nn; ++k) { B[k] += gd[t][k]; X[k] += gx[t][k]; }
+      free(gd[t]); free(gx[t]);
+    }
+    free(gd); free(gx);
+  }
+  complete_breit(s, (zc *)bdir, (zc *)bxch);
+}
+
+void orc_brtfast(orc_sys *s, int ilev, const double *dent, double *bdir, double *bxch) {
+  orc_brtfast_mt(s, ilev, dent, bdir, bxch, 1);
+}
+
+void orc_set_gaunt(orc_sys *s, int on) { s->gaunt = on; }
+
+int orc_biifast(orc_sys *s, const int icnt[4], const int kidx[4], const int midx[4], int ibas,
+                int jbas, double *rr_re, double *rr_im) {
+  build_ls_tables(s);
+  bquartet_t q;
+  for (int k = 0; k < 4; ++k) {
+    int c = icnt[k] - 1;
+    q.icnt[k] = icnt[k];
+    q.kqn[k] = KAPA(s, c, kidx[k]);
+    q.lqn[k] = lval(q.kqn[k]);
+    q.mqn[k] = 2 * midx[k] - 1;
+    q.nbas[k] = NFNC(s, c, q.lqn[k]);
+    q.exl[k] = &BEXL(s, c, q.lqn[k], 0);
+    q.xyz[k] = s->xyz + 3 * c;
+  }
+  q.iabls = s->iadls[iadx(s, icnt[0], icnt[1], kidx[0], kidx[1], midx[0], midx[1])];
+  q.icdls = s->iadls[iadx(s, icnt[2], icnt[3], kidx[2], kidx[3], midx[2], midx[3])];
+  int lamab = q.lqn[0] + q.lqn[1] + 1, lamcd = q.lqn[2] + q.lqn[3] + 1;
+  brt_ws w;
+  brt_ws_alloc(&w, q.nbas[0] * q.nbas[1], q.nbas[2] * q.nbas[3], lamab, lamcd,
+               s->gaunt ? lamab + lamcd : lamab + lamcd + 2);
+  for (int k = 0; k < w.maxab; ++k) w.irij[k] = 1;
+  eilscd0(s, &w, q.icnt, q.lqn);
+  w.ieab = 1;
+  w.igab = 1;
+  w.ncd = 2 * (kidx[2] - 2 * q.lqn[2]) + (kidx[3] - 2 * q.lqn[3]);
+  zc *rr = calloc(16L * w.maxcd, sizeof(zc));
+  biifast(s, &w, &q, ibas, jbas, rr);
+  for (long k = 0; k < 16L * w.maxcd; ++k) { rr_re[k] = creal(rr[k]); rr_im[k] = cimag(rr[k]); }
+  int maxcd = w.maxcd;
+  free(rr);
+  brt_ws_free(&w);
+  return maxcd;
+}
+
+/* ------------------------------------------------------------------------ */
 /* set-up and accessors                                                     */
 /* ------------------------------------------------------------------------ */
 orc_sys *orc_create(int ncnt, const double *xyz, const int *lmax, const int *nfnc, int lstride,
@@ -1729,6 +2426,8 @@ void orc_destroy(orc_sys *s) {
   for (int c = 0; c < 4; ++c) { free(s->e0ll[c]); free(s->e0ss[c]); }
   free(s->iad0ll); free(s->iad0ss);
   free(s->gdsc);
+  for (int c = 0; c < 12; ++c) free(s->eils[c]);
+  free(s->iadls);
   free(s);
 }
 
@@ -1739,10 +2438,18 @@ void orc_labels(const orc_sys *s, int *icn, int *kqn, int *mqn) {
   for (int i = 0; i < s->nskp; ++i) { icn[i] = s->labicn[i]; kqn[i] = s->labkqn[i]; mqn[i] = s->labmqn[i]; }
 }
 
-long orc_etable(const orc_sys *s, int itt, int ica, int icb, int ka, int kb, int ma, int mb,
+long orc_etable(orc_sys *s, int itt, int ica, int icb, int ka, int kb, int ma, int mb,
                 double *out, long cap) {
   int la = lval(KAPA(s, ica - 1, ka)), lb = lval(KAPA(s, icb - 1, kb));
   long maxab = (long)NFNC(s, ica - 1, la) * NFNC(s, icb - 1, lb);
+  if (itt == 2) {
+    build_ls_tables(s);
+    long n = ntuv(la + lb + 1) * maxab;
+    if (cap < 12 * n) return -1;
+    long iad = s->iadls[iadx(s, ica, icb, ka, kb, ma, mb)];
+    for (int c = 0; c < 12; ++c) memcpy(out + c * n, s->eils[c] + iad, sizeof(double) * n);
+    return n;
+  }
   int lam = la + lb + (itt == 4 ? 2 : 0);
   long n = ntuv(lam) * maxab;
   if (cap < 4 * n) return -1;

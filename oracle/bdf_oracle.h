@@ -42,10 +42,11 @@ void orc_funfm(const double *t, int n, int lam, int itype, double *f);
 /* RMAKE (bertha-scf.f:31900): rc(n, ntuv(lam)), qp(n,3) = Q-P */
 void orc_rmake(const double *qp, const double *aph, int n, int lam, double *rc);
 
-/* E0LL (itt=1) or E0SS (itt=4) block; out[comp][tuv][ij]; returns ntuv*maxab
+/* E0LL (itt=1), E0SS (itt=4) or EILS (itt=2, 12 components: x, y, z
+   times Re/Im E11, Re/Im E21) block; out[comp][tuv][ij]; returns ntuv*maxab
    or -1 if out too small.  ka,kb are 1-based kappa indices, ma,mb 1-based |m|
    indices, ica,icb 1-based centres (as IAD0LL's arguments). */
-long orc_etable(const orc_sys *s, int itt, int ica, int icb, int ka, int kb,
+long orc_etable(orc_sys *s, int itt, int ica, int icb, int ka, int kb,
                 int ma, int mb, double *out, long cap);
 
 /* ERIFAST batch (bertha-scf.f:12269) after the CLMFAST set-up for a single
@@ -64,6 +65,20 @@ void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
    (digestion goes to thread-private matrices that are summed at the end) */
 void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,
                     double *gxch, int nthreads);
+
+/* BRTFAST (bertha-scf.f:17060): closed-shell Breit matrices BDIR (zero for
+   closed shells) and BXCH, LS blocks and their conjugate SL blocks */
+void orc_brtfast(orc_sys *s, int ilev, const double *dent, double *bdir,
+                 double *bxch);
+void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir,
+                    double *bxch, int nthreads);
+/* BIIFAST batch (bertha-scf.f:18337) for one (LS|LS) block quartet at a
+   fresh class; rr planes as orc_erifast */
+int orc_biifast(orc_sys *s, const int icnt[4], const int kidx[4],
+                const int midx[4], int ibas, int jbas, double *rr_re,
+                double *rr_im);
+/* GAUNT2 toggle: Gaunt interaction only (no gauge term) */
+void orc_set_gaunt(orc_sys *s, int on);
 
 /* CLMDIAG (bertha-scf.f:38126): GDSC, ndim x ndim real column-major */
 void orc_clmdiag(orc_sys *s, double *gdsc);

@@ -50,6 +50,10 @@ def _oracle_lib():
         lib.orc_pairbound.argtypes = [P, P]
         lib.orc_set_schwarz.argtypes = [P, C.c_int, C.c_double]
         lib.orc_schwarz_stats.argtypes = [P, P, P]
+        lib.orc_brtfast.argtypes = [P, C.c_int, P, P, P]
+        lib.orc_brtfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
+        lib.orc_set_gaunt.argtypes = [P, C.c_int]
+        lib.orc_biifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P]
         _olib = lib
     return _olib
 
@@ -103,12 +107,14 @@ class Oracle:
         return a
 
     def etable(self, itt, ica, icb, ka, kb, ma, mb):
-        cap = 4 * 400000
+        """E0LL (itt 1), E0SS (itt 4): [4][ntuv*maxab]; EILS (itt 2): [12][...]."""
+        nc = 12 if itt == 2 else 4
+        cap = nc * 400000
         out = np.zeros(cap)
         n = _oracle_lib().orc_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data, cap)
         if n < 0:
             raise ValueError("etable capacity")
-        return out[:4 * n].reshape(4, n)
+        return out[:nc * n].reshape(nc, n)
 
     def erifast(self, icnt, kidx, midx, ibas, jbas, itn, ilev=3):
         lc, ld = kidx[2] // 2, kidx[3] // 2
@@ -126,6 +132,32 @@ class Oracle:
         g = np.zeros((self.ndim, self.ndim), order="F")
         _oracle_lib().orc_clmdiag(self._h, g.ctypes.data)
         return g
+
+    def biifast(self, icnt, kidx, midx, ibas, jbas, gaunt=False):
+        """BIIFAST batch RR(16, maxcd) at a fresh class."""
+        maxcd = (self.system.nfnc_of(icnt[2] - 1, kidx[2] // 2) *
+                 self.system.nfnc_of(icnt[3] - 1, kidx[3] // 2))
+        re = np.zeros(16 * maxcd)
+        im = np.zeros(16 * maxcd)
+        a = [np.asarray(v, dtype=np.int32) for v in (icnt, kidx, midx)]
+        _oracle_lib().orc_set_gaunt(self._h, 1 if gaunt else 0)
+        _oracle_lib().orc_biifast(self._h, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
+                                  ibas, jbas, re.ctypes.data, im.ctypes.data)
+        return (re + 1j * im).reshape(16, maxcd)
+
+    def brtfast(self, dent, ilev=3, nthreads=1, gaunt=False):
+        """BRTFAST: (BDIR, BXCH) for total density dent."""
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        b = np.zeros((n, n), dtype=np.complex128, order="F")
+        x = np.zeros((n, n), dtype=np.complex128, order="F")
+        _oracle_lib().orc_set_gaunt(self._h, 1 if gaunt else 0)
+        if nthreads > 1:
+            _oracle_lib().orc_brtfast_mt(self._h, ilev, d.ctypes.data, b.ctypes.data,
+                                         x.ctypes.data, nthreads)
+        else:
+            _oracle_lib().orc_brtfast(self._h, ilev, d.ctypes.data, b.ctypes.data, x.ctypes.data)
+        return b, x
 
     def pairbound(self):
         """sqrt|(ab|ba)| table (ndim x ndim real)."""
@@ -205,14 +237,15 @@ class Reference:
         return a
 
     def etable(self, itt, ica, icb, ka, kb, ma, mb):
-        cap = 4 * 400000
+        nc = 12 if itt == 2 else 4
+        cap = nc * 400000
         out = np.zeros(cap)
         nt, mx = C.c_int(), C.c_int()
         _ref_lib().ref_etable(C.c_int(itt), C.c_int(ica), C.c_int(icb), C.c_int(ka), C.c_int(kb),
                               C.c_int(ma), C.c_int(mb), C.c_int(cap), C.c_void_p(out.ctypes.data),
                               C.byref(nt), C.byref(mx))
         n = nt.value * mx.value
-        return out[:4 * n].reshape(4, n)
+        return out[:nc * n].reshape(nc, n)
 
     def erifast(self, icnt, kidx, midx, ibas, jbas, itn, ilev=3):
         a = [np.asarray(v, dtype=np.int32) for v in (icnt, kidx, midx, itn)]
@@ -226,6 +259,30 @@ class Reference:
                                C.c_void_p(re.ctypes.data), C.c_void_p(im.ctypes.data),
                                C.byref(mo))
         return (re + 1j * im).reshape(16, maxcd)
+
+    def biifast(self, icnt, kidx, midx, ibas, jbas, gaunt=False):
+        a = [np.asarray(v, dtype=np.int32) for v in (icnt, kidx, midx)]
+        maxcd = (self.system.nfnc_of(icnt[2] - 1, kidx[2] // 2) *
+                 self.system.nfnc_of(icnt[3] - 1, kidx[3] // 2))
+        re = np.zeros(16 * maxcd)
+        im = np.zeros(16 * maxcd)
+        mo = C.c_int()
+        _ref_lib().ref_biifast(*[C.c_void_p(x.ctypes.data) for x in a], C.c_int(ibas),
+                               C.c_int(jbas), C.c_int(1 if gaunt else 0),
+                               C.c_void_p(re.ctypes.data), C.c_void_p(im.ctypes.data),
+                               C.byref(mo))
+        return (re + 1j * im).reshape(16, maxcd)
+
+    def brtfast(self, dent, ilev=3):
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        dre = np.asfortranarray(d.real)
+        dim = np.asfortranarray(d.imag)
+        out = [np.zeros((n, n), order="F") for _ in range(4)]
+        _ref_lib().ref_brtfast(C.c_int(ilev), C.c_void_p(dre.ctypes.data),
+                               C.c_void_p(dim.ctypes.data),
+                               *[C.c_void_p(o.ctypes.data) for o in out])
+        return out[0] + 1j * out[1], out[2] + 1j * out[3]
 
     def clmdiag(self):
         """CLMDIAG's GDSC (ndim x ndim real)."""

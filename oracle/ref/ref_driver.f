@@ -187,6 +187,49 @@ C     DENSITY IN: ROW-FASTEST (FORTRAN ORDER), LEADING DIMENSION NDIM
       END
 C
 C
+C     BRTFAST: CLOSED-SHELL BREIT MATRICES (BDIR, BXCH) FOR DENSITY DENT
+      SUBROUTINE REFBRTFAST(ILEVIN,DRE,DIM,BDRE,BDIM,BXRE,BXIM)
+     &                      BIND(C,NAME='ref_brtfast')
+      USE ISO_C_BINDING
+      IMPLICIT DOUBLE PRECISION (A-H,O-Z)
+      INCLUDE 'parameters.h'
+      INTEGER(C_INT), VALUE :: ILEVIN
+      REAL(C_DOUBLE) :: DRE(*),DIM(*),BDRE(*),BDIM(*),BXRE(*),BXIM(*)
+      COMPLEX*16 DENC(MDM,MDM),DENO(MDM,MDM),DENT(MDM,MDM)
+      COMPLEX*16 FOCK(MDM,MDM),OVLP(MDM,MDM),HNUC(MDM,MDM),
+     &           HKIN(MDM,MDM),GDIR(MDM,MDM),GXCH(MDM,MDM),
+     &           BDIR(MDM,MDM),BXCH(MDM,MDM),VANM(MDM,MDM),
+     &           VSLF(MDM,MDM),VUEH(MDM,MDM),VWKR(MDM,MDM),
+     &           VKSB(MDM,MDM),QDIR(MDM,MDM),QXCH(MDM,MDM),
+     &           WDIR(MDM,MDM),WXCH(MDM,MDM),CPLE(MDM,MDM)
+      COMMON/BDIM/NDIM,NSKP,NOCC,NVRT
+      COMMON/DENS/DENC,DENO,DENT
+      COMMON/LSHF/SHLEV(4),SHLV,ILEV
+      COMMON/MTRX/FOCK,OVLP,HNUC,HKIN,GDIR,GXCH,BDIR,BXCH,VANM,VSLF,
+     &            VUEH,VWKR,VKSB,QDIR,QXCH,WDIR,WXCH,CPLE
+      ILEV = ILEVIN
+      DO J=1,NDIM
+        DO I=1,NDIM
+          K = (J-1)*NDIM+I
+          DENT(I,J) = DCMPLX(DRE(K),DIM(K))
+          DENO(I,J) = DCMPLX(0.0D0,0.0D0)
+          DENC(I,J) = DENT(I,J)
+        ENDDO
+      ENDDO
+      CALL BRTFAST
+      DO J=1,NDIM
+        DO I=1,NDIM
+          K = (J-1)*NDIM+I
+          BDRE(K) = DREAL(BDIR(I,J))
+          BDIM(K) = DIMAG(BDIR(I,J))
+          BXRE(K) = DREAL(BXCH(I,J))
+          BXIM(K) = DIMAG(BXCH(I,J))
+        ENDDO
+      ENDDO
+      RETURN
+      END
+C
+C
 C     CLMDIAG (SCHWARZ DIAGONAL BOUNDS) -> GDSC, ROW-FASTEST, LD = NDIM.
 C     CLMDIAG SWITCHES RCFILE OFF FOR ITS OWN ERI CALLS; IT IS SWITCHED
 C     BACK ON SO LATER ERIFAST CALLS RUN AS IN THE SCF.
@@ -305,10 +348,86 @@ C
       END
 C
 C
+      SUBROUTINE REFBIIFAST(ICNTIN,KIN,MIN,IBASIN,JBASIN,IGNT,
+     &                      RRRE,RRIM,MAXCDO) BIND(C,NAME='ref_biifast')
+C     REPLAYS THE PER-CLASS AND PER-BLOCK SET-UP THAT BRTFAST DOES BEFORE
+C     ITS BIIFAST CALL (bertha-scf.f:17396-17410, 17420-17424, 17517-17521,
+C     17560-17590) FOR ONE (IBAS,JBAS) OF A FRESH CLASS.  IGNT = GAUNT2.
+      USE ISO_C_BINDING
+      IMPLICIT DOUBLE PRECISION (A-H,O-Z)
+      INCLUDE 'parameters.h'
+      INCLUDE 'scfoptions.h'
+      INTEGER(C_INT) :: ICNTIN(4),KIN(4),MIN(4),MAXCDO
+      INTEGER(C_INT), VALUE :: IBASIN,JBASIN,IGNT
+      REAL(C_DOUBLE) :: RRRE(*),RRIM(*)
+      DIMENSION EXL(MBS,4),XYZ(3,4),ICNT(4),LQN(4),KQN(4),MQN(4),ITN(2)
+      DIMENSION NBASL(4),MQNL(4)
+      COMPLEX*16 RR(MB2,16)
+      COMMON/BSET/BEXL(MBS,0:MEL,MCT),BXYZ(3,MCT),LRGE(MCT,MKP,MKP+1),
+     &            KAPA(MKP,MCT),NFNC(0:MEL,MCT),NKAP(MCT),IQNC(MCT),NCNT
+      COMMON/EILS/EILSFL(MFL,12),IADILS(MCT,MCT,MKP,MKP,MKP,MKP)
+      COMMON/I2EL/PAB1,PAB2,PCD1,PCD2,NA1,NB1,NC1,ND1,NA2,NB2,NC2,ND2,
+     &            IBAS,JBAS,MCNT,NADDAB,NADDCD,NBASL,MQNL,IQL,IQR
+      COMMON/IQTT/IABLL,ICDLL,IABSS,ICDSS,IABLS,ICDLS,IABSL,ICDSL
+      COMMON/IRCM/IEAB,IECD,NCD,IGAB,IRIJ(MBS,MBS)
+      COMMON/ISCR/IMTX(MB2,11),ISCR(MB2),IMAP(MB2),IBCH,ITOG,MAXN
+      COMMON/SWRZ/GDSC(MDM,MDM),BDSC(MDM,MDM)
+C
+      BGAUNT = IGNT.NE.0
+      DO N=1,4
+        ICNT(N) = ICNTIN(N)
+        KQN(N)  = KAPA(KIN(N),ICNT(N))
+        LQN(N)  = LVAL(KQN(N))
+        MQN(N)  = 2*MIN(N)-1
+        NBASL(N) = NFNC(LQN(N),ICNT(N))
+        MQNL(N)  = MQN(N)
+        DO IB=1,NBASL(N)
+          EXL(IB,N) = BEXL(IB,LQN(N),ICNT(N))
+        ENDDO
+        DO J=1,3
+          XYZ(J,N) = BXYZ(J,ICNT(N))
+        ENDDO
+      ENDDO
+      ITN(1) = 2
+      ITN(2) = 2
+      DO IB=1,NBASL(1)
+        DO JB=1,NBASL(2)
+          IRIJ(IB,JB) = 1
+        ENDDO
+      ENDDO
+      CALL EILSCD0(ICNT,LQN,NBASL)
+      IABLS = IADILS(ICNT(1),ICNT(2),KIN(1),KIN(2),MIN(1),MIN(2))
+      ICDLS = IADILS(ICNT(3),ICNT(4),KIN(3),KIN(4),MIN(3),MIN(4))
+      IEAB = 1
+      IECD = 1
+      IGAB = 1
+      NC = KIN(3)-2*LQN(3)
+      ND = KIN(4)-2*LQN(4)
+      NCD = 2*NC+ND+1
+      IBAS = IBASIN
+      JBAS = JBASIN
+      ITOG = 0
+      CALL SCHWARZ(BDSC,1.0D-12)
+C
+      CALL BIIFAST(RR,XYZ,ICNT,KQN,MQN,NBASL,EXL,IBAS,JBAS,ITN)
+C
+      MAXCDO = NBASL(3)*NBASL(4)
+      DO ITG=1,16
+        DO M=1,MAXCDO
+          K = (ITG-1)*MAXCDO+M
+          RRRE(K) = DREAL(RR(M,ITG))
+          RRIM(K) = DIMAG(RR(M,ITG))
+        ENDDO
+      ENDDO
+      RETURN
+      END
+C
+C
       SUBROUTINE REFETABLE(ITT,ICA,ICB,KA,KB,MA,MB,ENT,EOUT,NTUVO,
      &                     MAXABO) BIND(C,NAME='ref_etable')
-C     ITT = 1 (E0LL) OR 4 (E0SS); EOUT(4*NTUV*MAXAB) IN THE STORED
-C     ORDER [COMPONENT][TUV][IJ] WITH IJ FASTEST (bertha-scf.f:33493).
+C     ITT = 1 (E0LL), 4 (E0SS) OR 2 (EILS, 12 COMPONENTS);
+C     EOUT(NCOMP*NTUV*MAXAB) IN THE STORED ORDER [COMPONENT][TUV][IJ]
+C     WITH IJ FASTEST (bertha-scf.f:33493, 33757).
       USE ISO_C_BINDING
       IMPLICIT DOUBLE PRECISION (A-H,O-Z)
       INCLUDE 'parameters.h'
@@ -319,9 +438,26 @@ C     ORDER [COMPONENT][TUV][IJ] WITH IJ FASTEST (bertha-scf.f:33493).
      &            KAPA(MKP,MCT),NFNC(0:MEL,MCT),NKAP(MCT),IQNC(MCT),NCNT
       COMMON/E0LL/E0LLFL(MFL,4),IAD0LL(MCT,MCT,MKP,MKP,MKP,MKP)
       COMMON/E0SS/E0SSFL(MFL,4),IAD0SS(MCT,MCT,MKP,MKP,MKP,MKP)
+      COMMON/EILS/EILSFL(MFL,12),IADILS(MCT,MCT,MKP,MKP,MKP,MKP)
       LA = LVAL(KAPA(KA,ICA))
       LB = LVAL(KAPA(KB,ICB))
       MAXABO = NFNC(LA,ICA)*NFNC(LB,ICB)
+      IF(ITT.EQ.2) THEN
+        LAM = LA+LB+1
+        IAD = IADILS(ICA,ICB,KA,KB,MA,MB)
+        NTUVO = (LAM+1)*(LAM+2)*(LAM+3)/6
+        N = NTUVO*MAXABO
+        IF(ENT.LT.12*N) THEN
+          NTUVO = -1
+          RETURN
+        ENDIF
+        DO IC=1,12
+          DO K=1,N
+            EOUT((IC-1)*N+K) = EILSFL(IAD+K,IC)
+          ENDDO
+        ENDDO
+        RETURN
+      ENDIF
       IF(ITT.EQ.1) THEN
         LAM = LA+LB
         IAD = IAD0LL(ICA,ICB,KA,KB,MA,MB)

@@ -10,6 +10,8 @@ through oracle/ref/ref_driver.f), on small inputs:
     for an atom (ATOMIC), a diatomic on z (DIATOM) and an off-axis
     two-centre system (NOSYMM).
 The file holds inputs and outputs only.  Run:  python tests/golden/make_golden.py
+With --breit it writes tests/golden/breit_vectors.npz instead: EILS blocks,
+BIIFAST batches (Breit and Gaunt) and BRTFAST's BXCH (see breit()).
 """
 import json
 import os
@@ -42,10 +44,69 @@ def systems(pkg):
     }
 
 
+def _kq(k):
+    return k // 2 if k % 2 == 0 else -(k // 2) - 1
+
+
+def breit(pkg):
+    """tests/golden/breit_vectors.npz: EILS blocks, BIIFAST batches (Breit
+    and Gaunt) and BRTFAST's BXCH for DHFB versions of the golden systems
+    plus an f-shell pair."""
+    S = pkg.systems
+    ev = S.even_tempered
+    sy = systems(pkg)
+    sy["fsh"] = S.System("golden-f-shell", np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 1.9]]),
+                         [[ev(0.2, 3.0, 2), ev(0.3, 3.0, 2), ev(0.4, 3.0, 1), ev(0.5, 3.0, 1)],
+                          [ev(0.1, 3.0, 2), ev(0.3, 3.0, 1)]], shape="NOSYMM")
+    out, meta = {}, {}
+    rng = np.random.default_rng(2025)
+    for name, s0 in sy.items():
+        s = S.System(s0.name, s0.xyz, s0.shells, s0.shape, hmlt="DHFB")
+        R = pyoracle.Reference(s)
+        meta[name] = {"xyz": s.xyz.tolist(), "shells": [[e.tolist() for e in sh] for sh in s.shells],
+                      "shape": s.shape}
+        d = pkg.systems.random_density(R.ndim, seed=31, real=(s.shape != "NOSYMM"))
+        out[f"{name}_dent"] = d
+        R.clmfast(d, 1)  # fresh-process order
+        b, x = R.brtfast(d, 3)
+        out[f"{name}_bxch"] = x
+        blocks, batches = [], []
+        for _ in range(8):
+            a, bb = [int(v) for v in rng.integers(1, s.ncnt + 1, size=2)]
+            ka = int(rng.integers(1, 2 * s.lmax(a - 1) + 2))
+            kb = int(rng.integers(1, 2 * s.lmax(bb - 1) + 2))
+            ma = int(rng.integers(1, abs(_kq(ka)) + 1))
+            mb = int(rng.integers(1, abs(_kq(kb)) + 1))
+            key = (2, a, bb, ka, kb, ma, mb)
+            out[f"{name}_eils_{'_'.join(map(str, key))}"] = R.etable(*key)
+            blocks.append(key)
+        for k in range(12):
+            icnt = [int(v) for v in rng.integers(1, s.ncnt + 1, size=4)]
+            kidx, midx = [], []
+            for c in icnt:
+                kk = int(rng.integers(1, 2 * s.lmax(c - 1) + 2))
+                kidx.append(kk)
+                midx.append(int(rng.integers(1, abs(_kq(kk)) + 1)))
+            i = int(rng.integers(1, s.nfnc_of(icnt[0] - 1, kidx[0] // 2) + 1))
+            j = int(rng.integers(1, s.nfnc_of(icnt[1] - 1, kidx[1] // 2) + 1))
+            gaunt = k >= 8
+            out[f"{name}_bii{k}"] = R.biifast(icnt, kidx, midx, i, j, gaunt)
+            batches.append((icnt, kidx, midx, i, j, gaunt))
+        meta[name]["eils"] = blocks
+        meta[name]["biifast"] = batches
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    path = os.path.join(HERE, "breit_vectors.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
 def main():
     if not pyoracle.reference_available():
         sys.exit("oracle/_ref/libbertha_ref.so missing: run make -C oracle ref")
     pkg = load_package()
+    if "--breit" in sys.argv:
+        breit(pkg)
+        return
     out = {}
     meta = {}
     rng = np.random.default_rng(2024)

@@ -66,3 +66,33 @@ def test_clmfast_matches_reference_vectors(bdf, orc, gold, meta, name):
         g, x = O.clmfast(d, ilev)
         assert np.abs(g - gold[f"{name}_gdir{ilev}"]).max() < 1e-12, ilev
         assert np.abs(x - gold[f"{name}_gxch{ilev}"]).max() < 1e-12, ilev
+
+
+BREIT = os.path.join(HERE, "golden", "breit_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def bgold():
+    return np.load(BREIT, allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def bmeta(bgold):
+    return json.loads(bytes(bgold["meta"]).decode())
+
+
+@pytest.mark.parametrize("name", ["atom", "diat", "off", "fsh"])
+def test_breit_matches_reference_vectors(bdf, orc, bgold, bmeta, name):
+    m = bmeta[name]
+    sh = [[np.array(e) for e in s] for s in m["shells"]]
+    s = bdf.systems.System("golden", np.array(m["xyz"]), sh, shape=m["shape"], hmlt="DHFB")
+    O = orc.Oracle(s)
+    for key in m["eils"]:
+        ref = bgold[f"{name}_eils_{'_'.join(map(str, key))}"]
+        assert np.abs(O.etable(*key) - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1e-300)
+    for k, (icnt, kidx, midx, i, j, gaunt) in enumerate(m["biifast"]):
+        ref = bgold[f"{name}_bii{k}"]
+        rr = O.biifast(icnt, kidx, midx, i, j, gaunt)
+        assert np.abs(rr - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), k
+    b, x = O.brtfast(bgold[f"{name}_dent"], 3)
+    assert np.abs(x - bgold[f"{name}_bxch"]).max() < 1e-13
