@@ -28,6 +28,7 @@ BDF_ELIMIT = -4
 BDF_EUNSUPP = -5
 
 SHAPES = {"NOSYMM": 0, "LINEAR": 1, "DIATOM": 1, "ATOMIC": 2}
+CV_REFERENCE = 137.035999084  # speed of light of the reference (bertha-scf.f:974)
 HMLTS = {"DHFR": 1, "DHFB": 2}
 
 
@@ -92,6 +93,8 @@ def library() -> _C.CDLL:
                                       _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_schwarz_tables.argtypes = [P, P, P, _C.c_int]
         lib.bdf_breit.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
+        lib.bdf_one_electron.argtypes = [P, P, P, _C.c_double, P, P, P, _C.c_int]
+        lib.bdf_one_electron_device.argtypes = [P, P, P, _C.c_double, P, P, P]
         lib.bdf_breit_device.argtypes = [P, _C.c_int, P, P, P]
         lib.bdf_bii_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int]
         lib.bdf_breit_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
@@ -197,6 +200,28 @@ class Engine:
         _check(library().bdf_etable(self._h, itt, ica, icb, ka, kb, ma, mb, out.ctypes.data,
                                     cap, _C.byref(n)))
         return out[:nc * n.value].reshape(nc, n.value)
+
+    def one_electron(self, znuc=None, xi=None, cv: float = CV_REFERENCE):
+        """(OVLP, HNUC, HKIN) of OVRLP/ONEEL.  znuc defaults to the system's
+        charges; xi None: point nuclei, else per-centre Gaussian exponents."""
+        n = self.ndim
+        z = _np.ascontiguousarray(self.system.charges if znuc is None else znuc, dtype=_np.float64)
+        if z.size != self.system.ncnt:
+            raise ValueError("one nuclear charge per centre is needed")
+        x = None if xi is None else _np.ascontiguousarray(xi, dtype=_np.float64)
+        out = [_np.zeros((n, n), dtype=_np.complex128, order="F") for _ in range(3)]
+        _check(library().bdf_one_electron(self._h, z.ctypes.data,
+                                          None if x is None else x.ctypes.data, cv,
+                                          *[o.ctypes.data for o in out], n))
+        return tuple(out)
+
+    def one_electron_device(self, d_ovlp: int, d_hnuc: int, d_hkin: int, znuc=None, xi=None,
+                            cv: float = CV_REFERENCE) -> None:
+        z = _np.ascontiguousarray(self.system.charges if znuc is None else znuc, dtype=_np.float64)
+        x = None if xi is None else _np.ascontiguousarray(xi, dtype=_np.float64)
+        _check(library().bdf_one_electron_device(self._h, z.ctypes.data,
+                                                 None if x is None else x.ctypes.data, cv,
+                                                 d_ovlp, d_hnuc, d_hkin))
 
     def breit(self, dent: _np.ndarray, ilev: int = 3):
         """BDIR, BXCH (BRTFAST) for total density ``dent``."""

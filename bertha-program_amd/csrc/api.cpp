@@ -839,6 +839,111 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   return BDF_OK;
 }
 
+int bdf_one_electron_device(bdf_engine *e, const double *znuc, const double *xi, double cv,
+                            void *d_ovlp, void *d_hnuc, void *d_hkin) {
+  if (!e || !znuc || !d_ovlp || !d_hnuc || !d_hkin)
+    return fail(BDF_EINVAL, "bdf_one_electron_device: null argument");
+  if (!(cv > 0.0)) return fail(BDF_EINVAL, "bdf_one_electron_device: cv must be positive");
+  const Basis &B = e->B;
+  const Etabs &T = e->T;
+  std::vector<OneJob> jobs;
+  std::vector<int64_t> first;
+  int64_t acc = 0;
+  try {
+    for (int a = 0; a < B.ncnt; ++a)
+      for (int b = 0; b <= a; ++b)
+        for (int ka = 1; ka <= 2 * B.lmax[a] + 1; ++ka)
+          for (int kb = 1; kb <= 2 * B.lmax[b] + 1; ++kb) {
+            int kqa = B.kappa_of(a, ka), kqb = B.kappa_of(b, kb);
+            int la = kqa < 0 ? -kqa - 1 : kqa, lb = kqb < 0 ? -kqb - 1 : kqb;
+            for (int ma = 1; ma <= std::abs(kqa); ++ma)
+              for (int mb = 1; mb <= std::abs(kqb); ++mb) {
+                OneJob J;
+                J.il[0] = B.lrge_at(a, ka, 2 * ma - 1);
+                J.il[1] = B.lrge_at(a, ka, 2 * ma);
+                J.jl[0] = B.lrge_at(b, kb, 2 * mb - 1);
+                J.jl[1] = B.lrge_at(b, kb, 2 * mb);
+                if (J.il[0] < J.jl[0]) continue;  // written as the mirror of (B,A)
+                J.ell = T.pairs[T.pair_index.at(Etabs::key(a, b, ka, kb, ma, mb, kLL))].off;
+                J.ess = T.pairs[T.pair_index.at(Etabs::key(a, b, ka, kb, ma, mb, kSS))].off;
+                J.essba = T.pairs[T.pair_index.at(Etabs::key(b, a, kb, ka, mb, ma, kSS))].off;
+                J.nA = B.nfn(a, la);
+                J.nB = B.nfn(b, lb);
+                J.expA = B.expo(a, la);
+                J.expB = B.expo(b, lb);
+                J.la = la;
+                J.lb = lb;
+                int mq = (2 * ma - 1 - (2 * mb - 1)) / 2;
+                J.phs = ((mq % 2) ? -1.0 : 1.0) * ((kqa * kqb) > 0 ? 1.0 : -1.0);
+                J.ax = B.xyz[3 * a]; J.ay = B.xyz[3 * a + 1]; J.az = B.xyz[3 * a + 2];
+                J.bx = B.xyz[3 * b]; J.by = B.xyz[3 * b + 1]; J.bz = B.xyz[3 * b + 2];
+                first.push_back(acc);
+                acc += (int64_t)J.nA * J.nB;
+                jobs.push_back(J);
+              }
+          }
+  } catch (const std::out_of_range &) {
+    return fail(BDF_EINVAL, "bdf_one_electron: missing E-coefficient block");
+  }
+  HIPCHK(hipSetDevice(e->device));
+  int nd = B.ndim;
+  size_t nn = (size_t)nd * nd;
+  DevBuf<OneJob> djobs;
+  DevBuf<int64_t> dfirst;
+  DevBuf<double> dz, dxi, dxyz;
+  std::vector<double> z(znuc, znuc + B.ncnt), x(B.ncnt, 0.0);
+  if (xi) x.assign(xi, xi + B.ncnt);
+  HIPCHK(djobs.upload(jobs, e->stream));
+  HIPCHK(dfirst.upload(first, e->stream));
+  HIPCHK(dz.upload(z, e->stream));
+  HIPCHK(dxi.upload(x, e->stream));
+  HIPCHK(dxyz.upload(B.xyz, e->stream));
+  HIPCHK(launch_zero((double2 *)d_ovlp, (double2 *)d_hnuc, (int64_t)nn, nullptr, nullptr, e->stream));
+  HIPCHK(launch_zero((double2 *)d_hkin, nullptr, (int64_t)nn, nullptr, nullptr, e->stream));
+  OneArgs oa;
+  oa.jobs = djobs.p;
+  oa.njobs = (int)jobs.size();
+  oa.first = dfirst.p;
+  oa.nitems = acc;
+  oa.exps = e->exps.p;
+  oa.E = e->E.p;
+  oa.xyz = dxyz.p;
+  oa.znuc = dz.p;
+  oa.xi = dxi.p;
+  oa.ncnt = B.ncnt;
+  oa.cv = cv;
+  oa.ovlp = (double2 *)d_ovlp;
+  oa.hnuc = (double2 *)d_hnuc;
+  oa.hkin = (double2 *)d_hkin;
+  oa.ndim = nd;
+  oa.nskp = B.nskp;
+  HIPCHK(launch_oneel(oa, e->stream));
+  // the job tables are freed when this call returns
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_one_electron(bdf_engine *e, const double *znuc, const double *xi, double cv,
+                     double _Complex *ovlp, double _Complex *hnuc, double _Complex *hkin, int ld) {
+  if (!e || !ovlp || !hnuc || !hkin) return fail(BDF_EINVAL, "bdf_one_electron: null argument");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_one_electron: ld < ndim");
+  HIPCHK(hipSetDevice(e->device));
+  DevBuf<double2> o, h, k;
+  HIPCHK(o.alloc((size_t)nd * nd));
+  HIPCHK(h.alloc((size_t)nd * nd));
+  HIPCHK(k.alloc((size_t)nd * nd));
+  int rc = bdf_one_electron_device(e, znuc, xi, cv, o.p, h.p, k.p);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpy2D(ovlp, ld * sizeof(double2), o.p, nd * sizeof(double2), nd * sizeof(double2),
+                     nd, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy2D(hnuc, ld * sizeof(double2), h.p, nd * sizeof(double2), nd * sizeof(double2),
+                     nd, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy2D(hkin, ld * sizeof(double2), k.p, nd * sizeof(double2), nd * sizeof(double2),
+                     nd, hipMemcpyDeviceToHost));
+  return BDF_OK;
+}
+
 int bdf_breit_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items) {
   if (!e) return fail(BDF_EINVAL, "bdf_breit_plan_info: null engine");
   if (!e->breit) return fail(BDF_EINVAL, "bdf_breit_plan_info: engine built without DHFB");

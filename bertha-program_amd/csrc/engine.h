@@ -108,6 +108,17 @@ struct WorkItem {
   int ij;
 };
 
+// ---- one-electron matrices (OVRLP / ONEEL) ------------------------------
+// One (A, kappa_A, |m_A|; B, kappa_B, |m_B|) block pair with LRGE(A) >= LRGE(B).
+struct OneJob {
+  int64_t ell, ess, essba;  // E0LL(A,B), E0SS(A,B), E0SS(B,A) double4 offsets
+  int nA, nB, expA, expB;
+  int la, lb;
+  int il[2], jl[2];         // large-block offsets of A and B sign slots
+  double phs;               // PHS of the slot relations (ONEEL :7070)
+  double ax, ay, az, bx, by, bz;
+};
+
 // ---- Breit plan (BRTFAST, bertha-scf.f:17060) ------------------------------
 // One (kappa_A, kappa_B, |m_A|, |m_B|) combination: LS tables for sigma_x,y,z.
 struct BrAB {

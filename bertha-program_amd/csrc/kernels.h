@@ -88,6 +88,22 @@ struct BreitArgs {
   double2 *rr_out;  // test hook: RR[16][maxcd] of the first call instead of digesting
 };
 
+struct OneArgs {
+  const OneJob *jobs;
+  int njobs;
+  const int64_t *first;
+  int64_t nitems;
+  const double *exps;
+  const double4 *E;
+  const double *xyz;    // [ncnt][3]
+  const double *znuc;   // [ncnt]
+  const double *xi;     // [ncnt], <= 0: point nucleus
+  int ncnt;
+  double cv;
+  double2 *ovlp, *hnuc, *hkin;
+  int ndim, nskp;
+};
+
 struct CompleteArgs {
   double2 *gdir, *gxch;
   const int *icn, *kqn, *mqn;
@@ -105,5 +121,6 @@ hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
 hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_oneel(const OneArgs &a, hipStream_t s);
 
 }  // namespace bdf

@@ -1099,6 +1099,129 @@ __global__ void breit_complete_kernel(double2 *bdir, double2 *bxch, int ndim, in
   bxch[(int64_t)i * ndim + j] = make_double2(v.x, -v.y);
 }
 
+// ---------------------------------------------------------------------------
+// One-electron matrices (OVRLP :6683, ONEEL :6996)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+
+// slots 12 = -PHS conj(21), 22 = PHS conj(11)
+__device__ __forceinline__ void slots4(double2 (&v)[4], double phs) {
+  v[1] = make_double2(-phs * v[2].x, phs * v[2].y);
+  v[3] = make_double2(phs * v[0].x, -phs * v[0].y);
+}
+
+// a 2x2 slot block at rows r[s] + i, columns c[s] + j and its mirror, in the
+// reference's statement order
+__device__ void put4(double2 *M, int n, const int (&r)[2], const int (&c)[2], int i, int j,
+                     const double2 (&v)[4]) {
+  M[(int64_t)(c[0] + j) * n + r[0] + i] = v[0];
+  M[(int64_t)(c[1] + j) * n + r[0] + i] = v[1];
+  M[(int64_t)(c[0] + j) * n + r[1] + i] = v[2];
+  M[(int64_t)(c[1] + j) * n + r[1] + i] = v[3];
+  // mirrors read the element back, so where a block meets its own mirror
+  // the later statement wins exactly as in the reference
+  M[(int64_t)(r[0] + i) * n + c[0] + j] = conj2(M[(int64_t)(c[0] + j) * n + r[0] + i]);
+  M[(int64_t)(r[0] + i) * n + c[1] + j] = conj2(M[(int64_t)(c[1] + j) * n + r[0] + i]);
+  M[(int64_t)(r[1] + i) * n + c[0] + j] = conj2(M[(int64_t)(c[0] + j) * n + r[1] + i]);
+  M[(int64_t)(r[1] + i) * n + c[1] + j] = conj2(M[(int64_t)(c[1] + j) * n + r[1] + i]);
+}
+
+// -sum over nuclei of PNC E(tuv) R(tuv) for the 11 and 21 entries
+__device__ void vnuc1(const OneArgs &a, int64_t eoff, int maxab, int ij, int lam, double p,
+                      double px, double py, double pz, double2 &v11, double2 &v21) {
+  double f[kMaxLab + 1], r[ntuv(kMaxLab)];
+  int nt = ntuv(lam);
+  for (int iz = 0; iz < a.ncnt; ++iz) {
+    double xi = a.xi[iz], z = a.znuc[iz];
+    double alpha, pnc;
+    if (xi <= 0.0) {
+      alpha = p;
+      pnc = 2.0 * z * 3.1415926535897932 / p;
+    } else {
+      double esm = p + xi;
+      alpha = p * xi / esm;
+      pnc = 2.0 * 3.1415926535897932 * z * 1.0 * sqrt(xi / esm) / p;
+    }
+    double X = px - a.xyz[3 * iz], Y = py - a.xyz[3 * iz + 1], Z = pz - a.xyz[3 * iz + 2];
+    boys_ref<double>(alpha * (X * X + Y * Y + Z * Z), lam, f, 1);
+    rints<double>(r, f, 1, lam, alpha, X, Y, Z);
+    for (int t = 0; t < nt; ++t) {
+      double4 e = a.E[eoff + (int64_t)t * maxab + ij];
+      v11.x -= pnc * e.x * r[t];
+      v11.y -= pnc * e.y * r[t];
+      v21.x -= pnc * e.z * r[t];
+      v21.y -= pnc * e.w * r[t];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) oneel_kernel(OneArgs a) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nitems) return;
+  int lo = 0, hi = a.njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (a.first[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  const OneJob J = a.jobs[lo];
+  const int ij = (int)(g - a.first[lo]);
+  const int i = ij / J.nB, j = ij % J.nB;
+  const int maxab = J.nA * J.nB;
+  const bool diag = J.il[0] == J.jl[0];
+  const double ea = a.exps[J.expA + i], eb = a.exps[J.expB + j];
+  const double p = ea + eb;
+  const double rt = sqrt(3.1415926535897932 / p), eroot = rt * rt * rt;
+  const double px = (J.ax * ea + J.bx * eb) / p, py = (J.ay * ea + J.by * eb) / p,
+               pz = (J.az * ea + J.bz * eb) / p;
+  const int n = a.ndim, ns = a.nskp;
+  const int il[2] = {J.il[0], J.il[1]}, jl[2] = {J.jl[0], J.jl[1]};
+  const int is[2] = {J.il[0] + ns, J.il[1] + ns}, js[2] = {J.jl[0] + ns, J.jl[1] + ns};
+  double4 ell = a.E[J.ell + ij], ess = a.E[J.ess + ij];
+  double2 sll[4], sss[4], vll[4], vss[4], tls[4], tsl[4];
+  sll[0] = make_double2(eroot * ell.x, eroot * ell.y);
+  sll[2] = make_double2(eroot * ell.z, eroot * ell.w);
+  slots4(sll, J.phs);
+  sss[0] = make_double2(eroot * ess.x, eroot * ess.y);
+  sss[2] = make_double2(eroot * ess.z, eroot * ess.w);
+  slots4(sss, J.phs);
+  vll[0] = vll[2] = make_double2(0.0, 0.0);
+  vnuc1(a, J.ell, maxab, ij, J.la + J.lb, p, px, py, pz, vll[0], vll[2]);
+  slots4(vll, J.phs);
+  vss[0] = vss[2] = make_double2(0.0, 0.0);
+  vnuc1(a, J.ess, maxab, ij, J.la + J.lb + 2, p, px, py, pz, vss[0], vss[2]);
+  const double cv2 = 2.0 * a.cv * a.cv;  // 2 m c^2, m = 1
+  vss[0].x -= cv2 * sss[0].x; vss[0].y -= cv2 * sss[0].y;
+  vss[2].x -= cv2 * sss[2].x; vss[2].y -= cv2 * sss[2].y;
+  slots4(vss, J.phs);
+  // kinetic coupling (ONEEL :7820-7880): SL from the SS overlap of (A,B);
+  // LS from that of (B,A), conjugated with slots 12 and 21 exchanged
+  const double ejrt = a.cv * sqrt((double)(2 * J.lb + 3)) * sqrt(eb);
+  tsl[0] = make_double2(ejrt * eroot * ess.x, ejrt * eroot * ess.y);
+  tsl[2] = make_double2(ejrt * eroot * ess.z, ejrt * eroot * ess.w);
+  slots4(tsl, J.phs);
+  double4 eba = a.E[J.essba + j * J.nA + i];
+  const double eirt = a.cv * sqrt((double)(2 * J.la + 3)) * sqrt(ea);
+  double2 t[4];
+  t[0] = make_double2(eirt * eroot * eba.x, eirt * eroot * eba.y);
+  t[2] = make_double2(eirt * eroot * eba.z, eirt * eroot * eba.w);
+  slots4(t, J.phs);
+  tls[0] = conj2(t[0]);
+  tls[1] = conj2(t[2]);
+  tls[2] = conj2(t[1]);
+  tls[3] = conj2(t[3]);
+  if (!diag || i >= j) {
+    put4(a.ovlp, n, il, jl, i, j, sll);
+    put4(a.hnuc, n, il, jl, i, j, vll);
+    put4(a.ovlp, n, is, js, i, j, sss);
+    put4(a.hnuc, n, is, js, i, j, vss);
+  }
+  put4(a.hkin, n, il, js, i, j, tls);
+  if (!diag) put4(a.hkin, n, is, jl, i, j, tsl);
+}
+
 // largest |D| of the density (non-negative doubles order like their bit
 // patterns, so an integer max is exact)
 __global__ void dmax_kernel(const double2 *d, int64_t n, unsigned long long *out) {
@@ -1237,6 +1360,12 @@ hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(breit_complete_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s,
                        a.bdir, a.bxch, a.ndim, a.nskp);
   }
+  return hipGetLastError();
+}
+hipError_t launch_oneel(const OneArgs &a, hipStream_t s) {
+  if (a.nitems == 0) return hipSuccess;
+  int bs = 64;
+  hipLaunchKernelGGL(oneel_kernel, dim3((unsigned)((a.nitems + bs - 1) / bs)), dim3(bs), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s) {

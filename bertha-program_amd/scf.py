@@ -1,0 +1,291 @@
+"""Closed-shell Dirac-Hartree-Fock SCF driver on the engine.
+
+Restates the iteration of the reference's HFSCF (bertha-scf.f:5097-6680)
+for closed shells, with every matrix resident on the device:
+  - one-electron matrices once (OVRLP/ONEEL, :5317-5318);
+  - per iteration the two-electron matrices of the current stage (CLMFAST
+    :5401, BRTFAST :5441 for DHFB), Fock assembly F = HNUC + HKIN + GDIR -
+    GXCH + BDIR - BXCH (:5516-5523), energies from the previous density
+    (ENRGIES :3698), virtual level shift (LEVSHFT :3641), C-DIIS
+    (DIIS :3883, with the workload's 6-deep history instead of MSV = 20),
+    the generalised eigenproblem F C = S C e (ZHEGV :5557, here a Cholesky
+    reduction and a Hermitian eigensolver on the device) and the density of
+    the lowest positive-energy spinors (DENSTY :3580);
+  - the stage logic: (LL|LL) -> (LL|SS) -> (SS|SS) at weighted energy
+    changes below 1e-8 and 5e-11, convergence at 1e-12 with a density
+    change below 1e-10 (:5167-5170, :6037-6108); atoms start at stage 3
+    (CARDIN :527-529) and molecules at stage 2 (see run_scf).
+One-centre quartets are evaluated by McMurchie-Davidson like all others
+(the reference's RACAH1 = .FALSE. setting).
+
+Matrices are torch tensors in natural (row, column) order; the engine's
+column-major arrays are exchanged through transposed copies.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from .systems import System, nuclear_exponents, nuclear_repulsion
+
+# stage thresholds (HFSCF :5167-5170)
+ENRGLV1 = 1.0e-8
+ENRGLV2 = 5.0e-11
+ENRGTOL = 1.0e-12
+DSTYTOL = 1.0e-10
+ERRCOM = 1.0e-1  # DIIS extrapolates only below this error (DIIS :3947)
+
+
+@dataclasses.dataclass
+class Energies:
+    """ENRGIES (bertha-scf.f:3698-3880), closed shell."""
+    nuclear: float
+    kinetic: float
+    potential: float
+    coulomb_direct: float
+    coulomb_exchange: float
+    breit_direct: float = 0.0
+    breit_exchange: float = 0.0
+
+    @property
+    def total(self) -> float:
+        return (self.nuclear + self.kinetic + self.potential +
+                (self.coulomb_direct - self.coulomb_exchange) +
+                (self.breit_direct - self.breit_exchange))
+
+
+@dataclasses.dataclass
+class SCFResult:
+    energy: float
+    energies: Energies
+    iterations: int
+    converged: bool
+    history: List[float]
+    stages: List[int]
+    eigenvalues: torch.Tensor
+    density: torch.Tensor
+
+
+class EngineBackend:
+    """Two-electron and one-electron matrices from the HIP engine; inputs and
+    outputs stay on the device (bdf_*_device entry points)."""
+
+    def __init__(self, pkg, system: System, device: str = "cuda:0", **engine_kw):
+        self.system = system
+        self.eng = pkg.Engine(system, **engine_kw)
+        self.dev = torch.device(device)
+        self.n = self.eng.ndim
+        self.nskp = self.eng.nskp
+        n = self.n
+        self._d = torch.zeros((n, n), dtype=torch.complex128, device=self.dev)
+        self._a = torch.zeros_like(self._d)
+        self._b = torch.zeros_like(self._d)
+        self._c = torch.zeros_like(self._d)
+        self.stream = torch.cuda.ExternalStream(self.eng.stream(), device=self.dev)
+
+    def _sync(self):
+        # the engine runs on its own stream: order torch's work after it
+        torch.cuda.current_stream(self.dev).wait_stream(self.stream)
+
+    def one_electron(self, cv):
+        xi = nuclear_exponents(self.system)
+        torch.cuda.current_stream(self.dev).synchronize()
+        self.eng.one_electron_device(self._a.data_ptr(), self._b.data_ptr(), self._c.data_ptr(),
+                                     xi=xi, cv=cv)
+        return self._a.T.clone(), self._b.T.clone(), self._c.T.clone()
+
+    def coulomb(self, dens, ilev):
+        self._d.copy_(dens.T)
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        self.eng.clmfast_device(self._d.data_ptr(), self._a.data_ptr(), self._b.data_ptr(), ilev)
+        self._sync()
+        return self._a.T.clone(), self._b.T.clone()
+
+    def breit(self, dens, ilev):
+        self._d.copy_(dens.T)
+        self.stream.wait_stream(torch.cuda.current_stream(self.dev))
+        self.eng.breit_device(self._d.data_ptr(), self._a.data_ptr(), self._b.data_ptr(), ilev)
+        self._sync()
+        return self._a.T.clone(), self._b.T.clone()
+
+    def close(self):
+        self.eng.close()
+
+
+class DIIS:
+    """C-DIIS of the reference (DIIS, bertha-scf.f:3883-4233): error
+    F P S - S P F with P = conj(D); coefficients minimise |B c| subject to
+    sum(c) = 1 (ZGGLSE); the Fock matrix is replaced by the combination once
+    the error is below ERRCOM and more than two vectors are stored."""
+
+    def __init__(self, depth: int = 6):
+        self.msv = depth
+        self.err = [None] * depth
+        self.fck = [None] * depth
+        self.bsav = np.zeros((depth, depth), dtype=np.complex128)
+
+    def reset(self):
+        self.bsav[:] = 0.0
+
+    def __call__(self, nmlev: int, fock, dens, ovlp):
+        if nmlev == 1:
+            self.reset()
+        p = dens.conj()
+        evec = fock @ p @ ovlp - ovlp @ p @ fock
+        errmax = float(evec.abs().max())
+        klc = nmlev % self.msv
+        if klc == 0:
+            klc = self.msv
+        kss = min(nmlev, self.msv)
+        k = klc - 1
+        self.err[k] = evec
+        self.fck[k] = fock
+        for m in range(kss):
+            v = complex(torch.sum(self.err[m].conj() * evec))
+            self.bsav[m, k] = v
+            self.bsav[k, m] = np.conj(v)
+        if errmax > ERRCOM or kss <= 2:
+            return fock
+        b = self.bsav[:kss, :kss]
+        # min |B c|^2 subject to sum c = 1: KKT system of the normal equations
+        kkt = np.zeros((kss + 1, kss + 1), dtype=np.complex128)
+        kkt[:kss, :kss] = b.conj().T @ b
+        kkt[:kss, kss] = 1.0
+        kkt[kss, :kss] = 1.0
+        rhs = np.zeros(kss + 1, dtype=np.complex128)
+        rhs[kss] = 1.0
+        c = np.linalg.lstsq(kkt, rhs, rcond=None)[0][:kss]
+        out = torch.zeros_like(fock)
+        for m in range(kss):
+            out = out + complex(c[m]) * self.fck[m]
+        return out
+
+
+def _real_trace(a, b) -> float:
+    # sum_ij A_ij B_ij, the reference's energy contraction (ENRGIES :3745)
+    return float(torch.sum(a * b).real)
+
+
+def energies(system: System, dens, hkin, hnuc, gdir, gxch, bdir=None, bxch=None) -> Energies:
+    e = Energies(nuclear=nuclear_repulsion(system),
+                 kinetic=_real_trace(dens, hkin),
+                 potential=_real_trace(dens, hnuc),
+                 coulomb_direct=0.5 * _real_trace(dens, gdir),
+                 coulomb_exchange=0.5 * _real_trace(dens, gxch))
+    if bdir is not None:
+        e.breit_direct = 0.5 * _real_trace(dens, bdir)
+        e.breit_exchange = 0.5 * _real_trace(dens, bxch)
+    return e
+
+
+def solve_fock(fock, ovlp):
+    """ZHEGV(1,'V','L'): F C = S C e from the lower triangles."""
+    f = torch.tril(fock) + torch.tril(fock, -1).conj().T
+    s = torch.tril(ovlp) + torch.tril(ovlp, -1).conj().T
+    L = torch.linalg.cholesky(s)
+    li = torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype, device=L.device),
+                                       upper=False)
+    fp = li @ f @ li.conj().T
+    if fp.is_cuda:
+        # the default device eigensolver returns non-finite eigenvectors for
+        # these Kramers-degenerate complex matrices; MAGMA's does not
+        prev = torch.backends.cuda.preferred_linalg_library()
+        torch.backends.cuda.preferred_linalg_library("magma")
+        try:
+            w, v = torch.linalg.eigh(fp)
+        finally:
+            torch.backends.cuda.preferred_linalg_library(prev)
+    else:
+        w, v = torch.linalg.eigh(fp)
+    if not bool(torch.isfinite(v).all()):
+        raise RuntimeError("eigensolver returned non-finite eigenvectors")
+    return w, li.conj().T @ v
+
+
+def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev: int = 1,
+            diis_depth: int = 6, max_iter: int = 100, cv: float = 137.035999084,
+            verbose: bool = False) -> SCFResult:
+    """HFSCF for a closed shell of nocc electrons (NCLS = NOCC, NOPN = 0)."""
+    n, nskp = backend.n, backend.nskp
+    dev = backend.dev
+    breit = system.hmlt.upper() == "DHFB"
+    if system.ncnt == 1:
+        ilev = 3
+    elif ilev == 1:
+        # With one-centre quartets by McMurchie-Davidson, the reference's
+        # stage 1 builds every one-centre G from SS column flags that
+        # E0SSCD0 has not yet set (CLMFAST :11742, G reused across IT2 at
+        # :12678): all one-centre repulsion vanishes and the SCF collapses.
+        # The reference avoids this only through RACAH1 (COULOMB1), so
+        # molecules start at stage 2.
+        ilev = 2
+    shlv = shifts[ilev - 1]
+    ovlp, hnuc, hkin = backend.one_electron(cv)
+    h1 = hnuc + hkin
+    dens = torch.zeros((n, n), dtype=torch.complex128, device=dev)
+    dtmp = torch.zeros_like(dens)
+    coef = None
+    diis = DIIS(diis_depth)
+    nmlev = [0, 0, 0, 0]
+    esav = [0.0]
+    wedn = [0.0]
+    history, stages = [], []
+    enrglv2 = ENRGLV2
+    e = None
+    eig = None
+    converged = False
+    it = 0
+    for it in range(1, max_iter + 1):
+        if nocc > 1:
+            gdir, gxch = backend.coulomb(dens, ilev)
+        else:
+            gdir = gxch = torch.zeros_like(dens)
+        bdir = bxch = None
+        fock = h1 + gdir - gxch
+        if breit and nocc > 1:
+            bdir, bxch = backend.breit(dens, ilev)
+            fock = fock + bdir - bxch
+        e = energies(system, dens, hkin, hnuc, gdir, gxch, bdir, bxch)
+        etot = e.total
+        if it != 1 and coef is not None:
+            a = ovlp @ coef[:, nskp + nocc:]
+            fock = fock + shlv * (a @ a.conj().T)
+        nmlev[ilev] += 1
+        fock = diis(nmlev[ilev], fock, dens, ovlp)
+        eig, coef = solve_fock(fock, ovlp)
+        occ = coef[:, nskp:nskp + nocc]
+        dens = occ.conj() @ occ.T  # D_ij = sum_o conj(C_io) C_jo
+        wedn.append(abs(esav[-1] - etot) / (abs(etot) + 1.0))
+        esav.append(etot)
+        dnrm = float(torch.sqrt(torch.sum((dens - dtmp).abs() ** 2))) / float(n * n)
+        dtmp = dens.clone()
+        if dnrm <= 1.0e-9:
+            enrglv2 = 1.0e2 * dnrm
+        history.append(etot)
+        stages.append(ilev)
+        if verbose:
+            print(f"iter {it:3d} stage {ilev} E = {etot:.12f} dE/E {wedn[-1]:.2e} dD {dnrm:.2e}",
+                  flush=True)
+        if nmlev[ilev] > 15 and len(wedn) > 6:
+            w = wedn[-6:]
+            if all(w[k + 1] > w[k] for k in range(5)):
+                raise RuntimeError("SCF diverging: six rising energy changes (HFSCF :6005-6026)")
+        if nocc <= 1:
+            converged = True
+            break
+        if ilev == 1:
+            if wedn[-1] < ENRGLV1:
+                ilev = 2
+                shlv = shifts[ilev - 1]
+        elif ilev == 2:
+            if wedn[-1] < enrglv2:
+                ilev = 3
+                shlv = shifts[ilev - 1]
+        elif wedn[-1] < ENRGTOL and dnrm < DSTYTOL:
+            converged = True
+            break
+    return SCFResult(energy=e.total, energies=e, iterations=it, converged=converged,
+                     history=history, stages=stages, eigenvalues=eig, density=dens)

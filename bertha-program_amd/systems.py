@@ -21,6 +21,8 @@ class System:
     shape: str = "NOSYMM"           # ATOMIC | LINEAR | DIATOM | NOSYMM
     hmlt: str = "DHFR"              # DHFR (Dirac-Coulomb) | DHFB (+ Breit)
     charges: Sequence[float] = ()
+    nucleus: str = "POINT"          # POINT | GAUSS (NMDL, bertha-scf.f:1182-1204)
+    masses: Sequence[float] = ()    # mass numbers A (Gaussian nuclei)
 
     @property
     def ncnt(self) -> int:
@@ -47,6 +49,39 @@ class System:
                 nfnc[c, l] = len(e)
                 exps[c, l, :len(e)] = e
         return xyz, lmax, nfnc, exps, lstride, mbs
+
+
+# conversions and constants of PHYSPRM (bertha-scf.f:954-993)
+BOHR_FM = 5.29177211e4   # CFM: femtometres per bohr
+PROTON_RADIUS_FM = 0.842  # PRAD
+
+
+def nuclear_exponents(system: "System"):
+    """Gaussian-nucleus exponents xi = 3/(2 R_rms^2) per centre with the
+    reference's RMS-radius rule (bertha-scf.f:1182-1204); None for point
+    nuclei."""
+    if system.nucleus.upper() == "POINT":
+        return None
+    xi = []
+    for z, a in zip(system.charges, system.masses):
+        if int(z) == 1:
+            rfm = PROTON_RADIUS_FM
+        elif z < 90.0:
+            rfm = 0.836 * a ** (1.0 / 3.0) + 0.57
+        else:
+            rfm = 0.770 * a ** (1.0 / 3.0) + 0.98
+        r = rfm / BOHR_FM
+        xi.append(1.5 / (r * r))
+    return np.array(xi)
+
+
+def nuclear_repulsion(system: "System") -> float:
+    """ENUC of point charges (bertha-scf.f:2549-2574)."""
+    e = 0.0
+    for i in range(system.ncnt):
+        for j in range(i):
+            e += system.charges[i] * system.charges[j] / np.linalg.norm(system.xyz[i] - system.xyz[j])
+    return e
 
 
 def even_tempered(alpha0: float, beta: float, n: int) -> np.ndarray:
@@ -83,8 +118,10 @@ def tlf(seed: int = 0, hmlt: str = "DHFB") -> System:
     tl = [even_tempered(0.02, 2.1, 28), even_tempered(0.02, 2.1, 24),
           even_tempered(0.04, 2.2, 18), even_tempered(0.08, 2.3, 12)]
     f = [even_tempered(0.06, 2.5, 14), even_tempered(0.06, 2.5, 9), even_tempered(0.2, 2.5, 4)]
+    # Gaussian nuclei with A = 1.5 Z, as the workload states
     return System(f"tlf-s{seed}", np.array([[0.0, 0.0, 0.0], [0.0, 0.0, r]]), [tl, f],
-                  shape="DIATOM", hmlt=hmlt, charges=[81.0, 9.0])
+                  shape="DIATOM", hmlt=hmlt, charges=[81.0, 9.0], nucleus="GAUSS",
+                  masses=[1.5 * 81.0, 1.5 * 9.0])
 
 
 def ring(seed: int = 0) -> System:

@@ -144,6 +144,22 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
                   const int midx[4], int ibas, int jbas, double _Complex *rr,
                   int maxcd);
 
+/* Replaces CALL OVRLP / CALL ONEEL (bertha-scf.f:5317-5318; routines at
+   :6683 and :6996): overlap OVLP, nuclear attraction HNUC (LL, and SS
+   including -2 m c^2 S_SS) and kinetic coupling HKIN (LS/SL blocks).
+   znuc[ncnt] nuclear charges; xi[ncnt] Gaussian-nucleus exponents
+   (xi[c] <= 0 or xi == NULL: point nucleus; the reference's Gaussian model
+   uses xi = 3/(2 R_rms^2), :1199-1204); cv the speed of light
+   (reference 137.035999084, :974).  Host arrays, ld >= ndim. */
+int bdf_one_electron(bdf_engine *e, const double *znuc, const double *xi, double cv,
+                     double _Complex *ovlp, double _Complex *hnuc, double _Complex *hkin,
+                     int ld);
+
+/* Same into device arrays (ld = ndim); synchronises the engine's stream
+   before returning. */
+int bdf_one_electron_device(bdf_engine *e, const double *znuc, const double *xi, double cv,
+                            void *d_ovlp, void *d_hnuc, void *d_hkin);
+
 /* Size of the stage-ilev Breit build (as bdf_plan_info). */
 int bdf_breit_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets,
                         long *items);

@@ -104,6 +104,24 @@ module bertha_amd
       integer(c_long), intent(out) :: items
     end function
 
+    integer(c_int) function bdf_one_electron(eng, znuc, xi, cv, ovlp, hnuc, hkin, ld) &
+        bind(c, name='bdf_one_electron')
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: eng
+      real(c_double), intent(in) :: znuc(*), xi(*)
+      real(c_double), value :: cv
+      integer(c_int), value :: ld
+      complex(c_double_complex), intent(out) :: ovlp(ld, *), hnuc(ld, *), hkin(ld, *)
+    end function
+
+    integer(c_int) function bdf_one_electron_device(eng, znuc, xi, cv, d_ovlp, d_hnuc, d_hkin) &
+        bind(c, name='bdf_one_electron_device')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: eng, d_ovlp, d_hnuc, d_hkin
+      real(c_double), intent(in) :: znuc(*), xi(*)
+      real(c_double), value :: cv
+    end function
+
     integer(c_int) function bdf_schwarz_tables(eng, gdsc, qbnd, ld) &
         bind(c, name='bdf_schwarz_tables')
       import :: c_int, c_ptr

@@ -2374,6 +2374,186 @@ int orc_biifast(orc_sys *s, const int icnt[4], const int kidx[4], const int midx
 }
 
 /* ------------------------------------------------------------------------ */
+/* One-electron matrices: OVRLP 6683, ONEEL 6996-8154                       */
+/* ------------------------------------------------------------------------ */
+/* Block of four sign-slot values (11, 12, 21, 22) from the computed 11 and
+   21 entries: 12 = -PHS conj(21), 22 = PHS conj(11) */
+static void slots(zc v[4], double phs) {
+  v[1] = -phs * conj(v[2]);
+  v[3] = phs * conj(v[0]);
+}
+
+/* write a 2x2 slot block at (r0/r1 + i, c0/c1 + j) and its mirror, in the
+   reference's statement order (6920-6935) */
+static void put4(zc *M, int n, int r0, int r1, int c0, int c1, int i, int j, const zc v[4]) {
+#define AT(r, c) M[(long)(c) * n + (r)]
+  AT(r0 + i, c0 + j) = v[0];
+  AT(r0 + i, c1 + j) = v[1];
+  AT(r1 + i, c0 + j) = v[2];
+  AT(r1 + i, c1 + j) = v[3];
+  AT(c0 + j, r0 + i) = conj(AT(r0 + i, c0 + j));
+  AT(c1 + j, r0 + i) = conj(AT(r0 + i, c1 + j));
+  AT(c0 + j, r1 + i) = conj(AT(r1 + i, c0 + j));
+  AT(c1 + j, r1 + i) = conj(AT(r1 + i, c1 + j));
+#undef AT
+}
+
+/* nuclear attraction of one block pair, summed over the nuclei: point
+   nucleus (PNC = 2 pi Z / p, exponent p) or Gaussian nucleus of exponent xi
+   (exponent p xi/(p+xi), PNC = 2 pi Z sqrt(xi/(p+xi)) / p), ONEEL 7120-7190 */
+static void vnuc(const orc_sys *s, const double *znuc, const double *xi, const double *xa,
+                 const double *xb, const double *ea, int na, const double *eb, int nb,
+                 double *const *pl, long iad, int lam, double phs, zc *v /* [maxab][4] */) {
+  int maxab = na * nb;
+  long nt = ntuv(lam);
+  double *cp = malloc(sizeof(double) * 3 * maxab), *aph = malloc(sizeof(double) * maxab);
+  double *pnc = malloc(sizeof(double) * maxab);
+  double *rc = malloc(sizeof(double) * nt * maxab);
+  for (int iz = 0; iz < s->ncnt; ++iz) {
+    const double *c = s->xyz + 3 * iz;
+    for (int i = 0, m = 0; i < na; ++i)
+      for (int j = 0; j < nb; ++j, ++m) {
+        double eij = ea[i] + eb[j];
+        double px = (xa[0] * ea[i] + xb[0] * eb[j]) / eij;
+        double py = (xa[1] * ea[i] + xb[1] * eb[j]) / eij;
+        double pz = (xa[2] * ea[i] + xb[2] * eb[j]) / eij;
+        cp[m] = c[0] - px; cp[maxab + m] = c[1] - py; cp[2 * maxab + m] = c[2] - pz;
+        if (xi[iz] <= 0.0) {
+          aph[m] = eij;
+          pnc[m] = 2.0 * znuc[iz] * PI / eij;
+        } else {
+          double esm = eij + xi[iz];
+          aph[m] = eij * xi[iz] / esm;
+          pnc[m] = 2.0 * PI * znuc[iz] * 1.0 * sqrt(xi[iz] / esm) / eij;
+        }
+      }
+    orc_rmake(cp, aph, maxab, lam, rc);
+    for (int m = 0; m < maxab; ++m) {
+      for (long t = 0; t < nt; ++t) {
+        zc e11 = pl[0][iad + t * maxab + m] + I * pl[1][iad + t * maxab + m];
+        zc e21 = pl[2][iad + t * maxab + m] + I * pl[3][iad + t * maxab + m];
+        v[4 * m + 0] = v[4 * m + 0] - (pnc[m] * e11) * rc[t * maxab + m];
+        v[4 * m + 2] = v[4 * m + 2] - (pnc[m] * e21) * rc[t * maxab + m];
+      }
+      slots(v + 4 * m, phs);
+    }
+  }
+  free(cp); free(aph); free(pnc); free(rc);
+}
+
+void orc_oneel(orc_sys *s, const double *znuc, const double *xi, double cv, double *ovlp_,
+               double *hnuc_, double *hkin_) {
+  int n = s->ndim, nskp = s->nskp;
+  zc *ovlp = (zc *)ovlp_, *hnuc = (zc *)hnuc_, *hkin = (zc *)hkin_;
+  memset(ovlp, 0, sizeof(zc) * n * n);
+  memset(hnuc, 0, sizeof(zc) * n * n);
+  memset(hkin, 0, sizeof(zc) * n * n);
+  const double emss = 1.0;
+  for (int a = 0; a < s->ncnt; ++a)
+    for (int b = 0; b <= a; ++b)
+      for (int ka = 1; ka <= s->nkap[a]; ++ka)
+        for (int kb = 1; kb <= s->nkap[b]; ++kb) {
+          int kqa = KAPA(s, a, ka), kqb = KAPA(s, b, kb);
+          int la = lval(kqa), lb = lval(kqb);
+          int na = NFNC(s, a, la), nb = NFNC(s, b, lb), maxab = na * nb;
+          const double *ea = &BEXL(s, a, la, 0), *eb = &BEXL(s, b, lb, 0);
+          zc *sll = calloc(4L * maxab, sizeof(zc)), *sss = calloc(4L * maxab, sizeof(zc));
+          zc *vll = calloc(4L * maxab, sizeof(zc)), *vss = calloc(4L * maxab, sizeof(zc));
+          zc *tls = calloc(4L * maxab, sizeof(zc)), *tsl = calloc(4L * maxab, sizeof(zc));
+          for (int ma = 1; ma <= abs(kqa); ++ma)
+            for (int mb = 1; mb <= abs(kqb); ++mb) {
+              int mqa = 2 * ma - 1, mqb = 2 * mb - 1;
+              double phs = m1pow((mqa - mqb) / 2) * (double)((kqa * kqb) / abs(kqa * kqb));
+              memset(vll, 0, sizeof(zc) * 4 * maxab);
+              memset(vss, 0, sizeof(zc) * 4 * maxab);
+              long ill = s->iad0ll[iadx(s, a + 1, b + 1, ka, kb, ma, mb)];
+              long iss = s->iad0ss[iadx(s, a + 1, b + 1, ka, kb, ma, mb)];
+              long iba = s->iad0ss[iadx(s, b + 1, a + 1, kb, ka, mb, ma)];
+              for (int i = 0, m = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j, ++m) {
+                  double eij = ea[i] + eb[j];
+                  double r = sqrt(PI / eij), eroot = r * r * r;
+                  sll[4 * m + 0] = eroot * (s->e0ll[0][ill + m] + I * s->e0ll[1][ill + m]);
+                  sll[4 * m + 2] = eroot * (s->e0ll[2][ill + m] + I * s->e0ll[3][ill + m]);
+                  slots(sll + 4 * m, phs);
+                  sss[4 * m + 0] = eroot * (s->e0ss[0][iss + m] + I * s->e0ss[1][iss + m]);
+                  sss[4 * m + 2] = eroot * (s->e0ss[2][iss + m] + I * s->e0ss[3][iss + m]);
+                  slots(sss + 4 * m, phs);
+                }
+              vnuc(s, znuc, xi, s->xyz + 3 * a, s->xyz + 3 * b, ea, na, eb, nb, s->e0ll, ill,
+                   la + lb, phs, vll);
+              vnuc(s, znuc, xi, s->xyz + 3 * a, s->xyz + 3 * b, ea, na, eb, nb, s->e0ss, iss,
+                   la + lb + 2, phs, vss);
+              double cv2 = 2.0 * emss * cv * cv;
+              for (int m = 0; m < maxab; ++m) {
+                vss[4 * m + 0] = vss[4 * m + 0] - cv2 * sss[4 * m + 0];
+                vss[4 * m + 2] = vss[4 * m + 2] - cv2 * sss[4 * m + 2];
+                slots(vss + 4 * m, phs);
+              }
+              /* kinetic coupling (7820-7880): SL from the SS overlap of (A,B),
+                 LS from that of (B,A), conjugated and slot-swapped */
+              double fsl = cv * sqrt((double)(2 * lb + 3)), fls = cv * sqrt((double)(2 * la + 3));
+              for (int i = 0, m = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j, ++m) {
+                  double eij = ea[i] + eb[j];
+                  double r = sqrt(PI / eij), eroot = r * r * r;
+                  double ejrt = fsl * sqrt(eb[j]);
+                  tsl[4 * m + 0] = ejrt * eroot * (s->e0ss[0][iss + m] + I * s->e0ss[1][iss + m]);
+                  tsl[4 * m + 2] = ejrt * eroot * (s->e0ss[2][iss + m] + I * s->e0ss[3][iss + m]);
+                  slots(tsl + 4 * m, phs);
+                  long mba = (long)j * na + i;
+                  double eirt = fls * sqrt(ea[i]);
+                  zc t[4];
+                  t[0] = eirt * eroot * (s->e0ss[0][iba + mba] + I * s->e0ss[1][iba + mba]);
+                  t[2] = eirt * eroot * (s->e0ss[2][iba + mba] + I * s->e0ss[3][iba + mba]);
+                  slots(t, phs);
+                  tls[4 * m + 0] = conj(t[0]);
+                  tls[4 * m + 1] = conj(t[2]);
+                  tls[4 * m + 2] = conj(t[1]);
+                  tls[4 * m + 3] = conj(t[3]);
+                }
+              int il1 = LRGE(s, a, ka, mqa), il2 = LRGE(s, a, ka, mqa + 1);
+              int jl1 = LRGE(s, b, kb, mqb), jl2 = LRGE(s, b, kb, mqb + 1);
+              int is1 = il1 + nskp, is2 = il2 + nskp, js1 = jl1 + nskp, js2 = jl2 + nskp;
+              if (il1 >= jl1) {
+                for (int j = 0; j < nb; ++j)
+                  for (int i = (il1 == jl1 ? j : 0); i < na; ++i) {
+                    int m = i * nb + j;
+                    put4(ovlp, n, il1, il2, jl1, jl2, i, j, sll + 4 * m);
+                  }
+                for (int j = 0; j < nb; ++j)
+                  for (int i = (il1 == jl1 ? j : 0); i < na; ++i) {
+                    int m = i * nb + j;
+                    put4(hnuc, n, il1, il2, jl1, jl2, i, j, vll + 4 * m);
+                  }
+                for (int j = 0; j < nb; ++j)
+                  for (int i = (il1 == jl1 ? j : 0); i < na; ++i) {
+                    int m = i * nb + j;
+                    put4(ovlp, n, is1, is2, js1, js2, i, j, sss + 4 * m);
+                  }
+                for (int j = 0; j < nb; ++j)
+                  for (int i = (il1 == jl1 ? j : 0); i < na; ++i) {
+                    int m = i * nb + j;
+                    put4(hnuc, n, is1, is2, js1, js2, i, j, vss + 4 * m);
+                  }
+                for (int j = 0; j < nb; ++j)
+                  for (int i = 0; i < na; ++i) {
+                    int m = i * nb + j;
+                    put4(hkin, n, il1, il2, js1, js2, i, j, tls + 4 * m);
+                  }
+                if (il1 > jl1)
+                  for (int j = 0; j < nb; ++j)
+                    for (int i = 0; i < na; ++i) {
+                      int m = i * nb + j;
+                      put4(hkin, n, is1, is2, jl1, jl2, i, j, tsl + 4 * m);
+                    }
+              }
+            }
+          free(sll); free(sss); free(vll); free(vss); free(tls); free(tsl);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
 /* set-up and accessors                                                     */
 /* ------------------------------------------------------------------------ */
 orc_sys *orc_create(int ncnt, const double *xyz, const int *lmax, const int *nfnc, int lstride,

@@ -80,6 +80,13 @@ int orc_biifast(orc_sys *s, const int icnt[4], const int kidx[4],
 /* GAUNT2 toggle: Gaunt interaction only (no gauge term) */
 void orc_set_gaunt(orc_sys *s, int on);
 
+/* OVRLP (bertha-scf.f:6683) and ONEEL (:6996): overlap, nuclear attraction
+   (LL, and SS including -2 c^2 S_SS) and kinetic LS/SL coupling matrices,
+   ndim x ndim complex column-major.  znuc[c] nuclear charges; xi[c] the
+   Gaussian-nucleus exponent (<= 0: point nucleus); cv the speed of light. */
+void orc_oneel(orc_sys *s, const double *znuc, const double *xi, double cv,
+               double *ovlp, double *hnuc, double *hkin);
+
 /* CLMDIAG (bertha-scf.f:38126): GDSC, ndim x ndim real column-major */
 void orc_clmdiag(orc_sys *s, double *gdsc);
 

@@ -54,6 +54,7 @@ def _oracle_lib():
         lib.orc_brtfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
         lib.orc_set_gaunt.argtypes = [P, C.c_int]
         lib.orc_biifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P]
+        lib.orc_oneel.argtypes = [P, P, P, C.c_double, P, P, P]
         _olib = lib
     return _olib
 
@@ -132,6 +133,16 @@ class Oracle:
         g = np.zeros((self.ndim, self.ndim), order="F")
         _oracle_lib().orc_clmdiag(self._h, g.ctypes.data)
         return g
+
+    def oneel(self, znuc, xi, cv=137.035999084):
+        """OVRLP/ONEEL: (OVLP, HNUC, HKIN); xi[c] <= 0 is a point nucleus."""
+        n = self.ndim
+        z = np.ascontiguousarray(znuc, dtype=np.float64)
+        x = np.ascontiguousarray(xi, dtype=np.float64)
+        out = [np.zeros((n, n), dtype=np.complex128, order="F") for _ in range(3)]
+        _oracle_lib().orc_oneel(self._h, z.ctypes.data, x.ctypes.data, cv,
+                                *[o.ctypes.data for o in out])
+        return tuple(out)
 
     def biifast(self, icnt, kidx, midx, ibas, jbas, gaunt=False):
         """BIIFAST batch RR(16, maxcd) at a fresh class."""
@@ -283,6 +294,17 @@ class Reference:
                                C.c_void_p(dim.ctypes.data),
                                *[C.c_void_p(o.ctypes.data) for o in out])
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+    def oneel(self, znuc, anuc, gauss=False):
+        """ONEEL: (OVLP, HNUC, HKIN) for charges znuc, mass numbers anuc."""
+        n = self.ndim
+        z = np.ascontiguousarray(znuc, dtype=np.float64)
+        a = np.ascontiguousarray(anuc, dtype=np.float64)
+        out = [np.zeros(n * n) for _ in range(6)]
+        _ref_lib().ref_oneel(C.c_void_p(z.ctypes.data), C.c_void_p(a.ctypes.data),
+                             C.c_int(1 if gauss else 0), *[C.c_void_p(o.ctypes.data) for o in out])
+        m = [o.reshape(n, n, order="F") for o in out]
+        return m[0] + 1j * m[1], m[2] + 1j * m[3], m[4] + 1j * m[5]
 
     def clmdiag(self):
         """CLMDIAG's GDSC (ndim x ndim real)."""

@@ -230,6 +230,71 @@ C     BRTFAST: CLOSED-SHELL BREIT MATRICES (BDIR, BXCH) FOR DENSITY DENT
       END
 C
 C
+C     OVRLP/ONEEL: OVLP, HNUC, HKIN FOR NUCLEAR CHARGES ZIN (MASS NUMBERS
+C     AIN) WITH POINT (IMDL=0) OR GAUSSIAN (IMDL=1) NUCLEI; THE GAUSSIAN
+C     WIDTH FOLLOWS THE REFERENCE'S RMS-RADIUS FORMULA (bertha-scf.f:
+C     1182-1204) WITH NNUC = 0.
+      SUBROUTINE REFONEEL(ZIN,AIN,IMDL,OVRE,OVIM,HNRE,HNIM,HKRE,HKIM)
+     &                    BIND(C,NAME='ref_oneel')
+      USE ISO_C_BINDING
+      IMPLICIT DOUBLE PRECISION (A-H,O-Z)
+      INCLUDE 'parameters.h'
+      INCLUDE 'scfoptions.h'
+      REAL(C_DOUBLE) :: ZIN(*),AIN(*),OVRE(*),OVIM(*),HNRE(*),HNIM(*),
+     &                  HKRE(*),HKIM(*)
+      INTEGER(C_INT), VALUE :: IMDL
+      CHARACTER*5 NMDL
+      COMPLEX*16 FOCK(MDM,MDM),OVLP(MDM,MDM),HNUC(MDM,MDM),
+     &           HKIN(MDM,MDM),GDIR(MDM,MDM),GXCH(MDM,MDM),
+     &           BDIR(MDM,MDM),BXCH(MDM,MDM),VANM(MDM,MDM),
+     &           VSLF(MDM,MDM),VUEH(MDM,MDM),VWKR(MDM,MDM),
+     &           VKSB(MDM,MDM),QDIR(MDM,MDM),QXCH(MDM,MDM),
+     &           WDIR(MDM,MDM),WXCH(MDM,MDM),CPLE(MDM,MDM)
+      COMMON/BDIM/NDIM,NSKP,NOCC,NVRT
+      COMMON/BNUC/ZNUC(MCT),ANUC(MCT),TFMI(MCT),AFMI(MCT),RNUC(MCT),
+     &            FNUC(MCT,0:MFT),XNUC(MCT,0:MFT),NNUC(MCT),NMDL(MCT)
+      COMMON/BSET/BEXL(MBS,0:MEL,MCT),BXYZ(3,MCT),LRGE(MCT,MKP,MKP+1),
+     &            KAPA(MKP,MCT),NFNC(0:MEL,MCT),NKAP(MCT),IQNC(MCT),NCNT
+      COMMON/CONV/CHZ,CEV,CCM,CFM,CNG,CDB
+      COMMON/PHYS/CV,EMSS,UMSS,TMSS,PMSS,PRAD,CMPW,GFREE,GFRMI,WEIN
+      COMMON/MTRX/FOCK,OVLP,HNUC,HKIN,GDIR,GXCH,BDIR,BXCH,VANM,VSLF,
+     &            VUEH,VWKR,VKSB,QDIR,QXCH,WDIR,WXCH,CPLE
+      DO IZ=1,NCNT
+        ZNUC(IZ) = ZIN(IZ)
+        ANUC(IZ) = AIN(IZ)
+        IF(IMDL.EQ.0) THEN
+          NMDL(IZ) = 'POINT'
+        ELSE
+          NMDL(IZ) = 'GAUSS'
+        ENDIF
+        IF(INT(ZNUC(IZ)).EQ.1) THEN
+          RFM = PRAD
+        ELSEIF(ZNUC(IZ).GT.1.0D0.AND.ZNUC(IZ).LT.90.0D0) THEN
+          RFM = 0.836D0*(ANUC(IZ)**(1.0D0/3.0D0)) + 0.57D0
+        ELSE
+          RFM = 0.770D0*(ANUC(IZ)**(1.0D0/3.0D0)) + 0.98D0
+        ENDIF
+        RNUC(IZ) = RFM/CFM
+        NNUC(IZ)   = 0
+        FNUC(IZ,0) = 1.0D0
+        XNUC(IZ,0) = 1.5D0/(RNUC(IZ)*RNUC(IZ))
+      ENDDO
+      CALL ONEEL
+      DO J=1,NDIM
+        DO I=1,NDIM
+          K = (J-1)*NDIM+I
+          OVRE(K) = DREAL(OVLP(I,J))
+          OVIM(K) = DIMAG(OVLP(I,J))
+          HNRE(K) = DREAL(HNUC(I,J))
+          HNIM(K) = DIMAG(HNUC(I,J))
+          HKRE(K) = DREAL(HKIN(I,J))
+          HKIM(K) = DIMAG(HKIN(I,J))
+        ENDDO
+      ENDDO
+      RETURN
+      END
+C
+C
 C     CLMDIAG (SCHWARZ DIAGONAL BOUNDS) -> GDSC, ROW-FASTEST, LD = NDIM.
 C     CLMDIAG SWITCHES RCFILE OFF FOR ITS OWN ERI CALLS; IT IS SWITCHED
 C     BACK ON SO LATER ERIFAST CALLS RUN AS IN THE SCF.

@@ -1,0 +1,29 @@
+"""Oracle-backed matrices for the SCF driver (test infrastructure): the same
+HFSCF iteration run on the oracle's CPU restatement, to compare against the
+engine-backed run."""
+import numpy as np
+import torch
+
+
+class OracleBackend:
+    def __init__(self, pkg, orc, system, nthreads: int = 8):
+        self.pkg = pkg
+        self.system = system
+        self.o = orc.Oracle(system)
+        self.n, self.nskp = self.o.ndim, self.o.nskp
+        self.dev = torch.device("cpu")
+        self.nthreads = nthreads
+
+    def one_electron(self, cv):
+        xi = self.pkg.systems.nuclear_exponents(self.system)
+        xi = np.zeros(self.system.ncnt) if xi is None else xi
+        mats = self.o.oneel(np.array(self.system.charges, float), xi, cv)
+        return [torch.from_numpy(np.ascontiguousarray(x)) for x in mats]
+
+    def coulomb(self, dens, ilev):
+        g, x = self.o.clmfast(dens.numpy(), ilev, nthreads=self.nthreads)
+        return torch.from_numpy(np.ascontiguousarray(g)), torch.from_numpy(np.ascontiguousarray(x))
+
+    def breit(self, dens, ilev):
+        b, x = self.o.brtfast(dens.numpy(), ilev, nthreads=self.nthreads)
+        return torch.from_numpy(np.ascontiguousarray(b)), torch.from_numpy(np.ascontiguousarray(x))

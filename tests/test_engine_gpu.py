@@ -169,3 +169,17 @@ def test_mixed_precision_split(bdf, orc, systems, name):
     with bdf.Engine(s, precision="mixed", split_threshold=1e-5) as eng:
         g, x = eng.clmfast(d, 3)
     assert abs(_e2(d, g, x) - e64) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["ne", "off", "tri"])
+def test_one_electron_matches_oracle(bdf, orc, systems, name):
+    s = systems[name]
+    print(f"[one-electron] {name}", flush=True)
+    z = np.array([10.0, 3.0, 2.0][:s.ncnt])
+    O = orc.Oracle(s)
+    for xi in (np.zeros(s.ncnt), np.array([2.0e9, 5.0e8, 1.0e9][:s.ncnt])):
+        ref = O.oneel(z, xi)
+        with bdf.Engine(s) as eng:
+            got = eng.one_electron(z, xi)
+        for r, g in zip(ref, got):
+            assert np.abs(r - g).max() <= 1e-13 * np.abs(r).max()
