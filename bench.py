@@ -43,6 +43,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--split", type=float, default=0.1,
                     help="Schwarz split threshold of the mixed-precision leg (0: skip the leg)")
+    ap.add_argument("--serve", type=int, default=32,
+                    help="requests of the served-SCF leg (0: skip the leg)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU design (partial J/K "
@@ -123,7 +127,7 @@ def main() -> int:
         ms_m = ev0.elapsed_time(ev1) / args.steps
         gm = gm_t.cpu().numpy().T
         xm = xm_t.cpu().numpy().T
-        de_m = 0.5 * np.real(np.sum(dens.T * ((gm - xm) - (g - x))))
+        de_m = 0.5 * np.real(np.sum(dens * ((gm - xm) - (g - x))))
         mixed = {"value": 1000.0 / ms_m, "ms_per_step": ms_m, "split_threshold": args.split,
                  "abs_dE2e_vs_fp64_Eh": float(abs(de_m)),
                  "max_abs_dG_vs_fp64": float(max(np.abs(gm - g).max(), np.abs(xm - x).max())),
@@ -144,10 +148,30 @@ def main() -> int:
                "sample": f"one full stage-3 Dirac-Coulomb build of the {system.name} workload "
                          f"(NDIM {n}), same density, oracle/bdf_oracle.c with OpenMP over centre "
                          f"quadruples: {tc:.2f} s"}
-        de = 0.5 * np.real(np.sum(dens.T * ((g - x) - (go - xo))))
+        de = 0.5 * np.real(np.sum(dens * ((g - x) - (go - xo))))
         parity = {"max_abs_dGDIR": float(np.abs(g - go).max()),
                   "max_abs_dGXCH": float(np.abs(x - xo).max()),
                   "abs_dE2e_Eh": float(abs(de))}
+
+    # served-SCF leg: DHF solves of atoms and diatomics arriving over time
+    serving = None
+    if args.serve > 0:
+        import importlib
+        sv = importlib.import_module(pkg.__name__ + ".serve")
+        reqs = sv.request_mix(args.serve, seed=args.seed, rate=args.serve_rate)
+        server = sv.Server(pkg, slots=args.serve_slots, workgroups=256)
+        t0 = time.perf_counter()
+        res = server.run(reqs)
+        span = time.perf_counter() - t0
+        p50, p99 = sv.percentiles(res)
+        serving = {"requests": len(res), "p50_latency_s": p50, "p99_latency_s": p99,
+                   "solves_per_s": len(res) / span, "wall_s": span,
+                   "converged": sum(1 for r in res if r.converged),
+                   "arrival_rate_per_s": args.serve_rate, "slots": args.serve_slots,
+                   "peak_concurrent": server.peak_active,
+                   "ndim_range": [min(r.ndim for r in res), max(r.ndim for r in res)],
+                   "mix": "60% atoms Z in {2,4,10,12,18,20,36,54}, 40% diatomics of the "
+                          "same elements, R ~ U(2.5, 8.0) bohr, Poisson arrivals"}
 
     out = {
         "metric": METRIC,
@@ -172,6 +196,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "mixed": mixed,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -64,6 +64,7 @@ class _Options(_C.Structure):
         ("schwarz", _C.c_int),
         ("sens", _C.c_double),
         ("gaunt", _C.c_int),
+        ("workgroups", _C.c_int),
     ]
 
 
@@ -99,6 +100,7 @@ def library() -> _C.CDLL:
         lib.bdf_bii_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int]
         lib.bdf_breit_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
                                             _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
+        lib.bdf_memory_info.argtypes = [P, _C.POINTER(_C.c_size_t)]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -122,7 +124,8 @@ class Engine:
 
     def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
                  device: int = 0, schwarz: bool = False, sens: float = 0.0,
-                 precision: str = "fp64", split_threshold: float = 0.0, gaunt: bool = False):
+                 precision: str = "fp64", split_threshold: float = 0.0, gaunt: bool = False,
+                 workgroups: int = 0):
         lib = library()
         self.system = system
         xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
@@ -134,7 +137,7 @@ class Engine:
                     SHAPES[system.shape], HMLTS[system.hmlt])
         prec = {"fp64": 0, "mixed": 1}[precision]
         o = _Options(1 if intsym else 0, 1 if racah1 else 0, prec, split_threshold, device,
-                     1 if schwarz else 0, sens, 1 if gaunt else 0)
+                     1 if schwarz else 0, sens, 1 if gaunt else 0, workgroups)
         h = _C.c_void_p()
         nd = _C.c_int()
         ns = _C.c_int()
@@ -271,6 +274,12 @@ class Engine:
         n = _C.c_long()
         _check(library().bdf_plan_info(self._h, ilev, _C.byref(f), _C.byref(q), _C.byref(n)))
         return f.value, q.value, n.value
+
+    def device_bytes(self) -> int:
+        """Device memory held by the engine."""
+        b = _C.c_size_t()
+        _check(library().bdf_memory_info(self._h, _C.byref(b)))
+        return b.value
 
     def stream(self) -> int:
         return int(library().bdf_stream(self._h) or 0)

@@ -400,7 +400,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   if (sys->hmlt != BDF_HMLT_DHFR && sys->hmlt != BDF_HMLT_DHFB)
     return fail(BDF_EINVAL, "bdf_create: unknown hamiltonian");
   if (sys->shape < 0 || sys->shape > 2) return fail(BDF_EINVAL, "bdf_create: unknown shape");
-  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0, 0, 0.0, 0};
+  bdf_options o = {1, 0, BDF_PREC_FP64, 0.0, 0, 0, 0.0, 0, 0};
   if (opt) o = *opt;
   if (o.precision != BDF_PREC_FP64 && o.precision != BDF_PREC_MIXED)
     return fail(BDF_EINVAL, "bdf_create: unknown precision mode");
@@ -477,7 +477,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(e->dmax.alloc(1));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
-    e->nblocks = prop.multiProcessorCount * 8;
+    e->nblocks = o.workgroups > 0 ? o.workgroups : prop.multiProcessorCount * 8;
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->mixed) {
       HIPCHK(e->E32.alloc((size_t)e->T.total));
@@ -955,6 +955,29 @@ int bdf_breit_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets
   if (flops) *flops = P.flops;
   if (quartets) *quartets = P.quartets;
   if (items) *items = (long)P.items.size();
+  return BDF_OK;
+}
+
+int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
+  if (!e || !device_bytes) return fail(BDF_EINVAL, "bdf_memory_info: null argument");
+  size_t b = 0;
+  auto add = [&](size_t n, size_t w) { b += n * w; };
+  add(e->E.n, sizeof(double4));
+  add(e->E32.n, sizeof(float4));
+  add(e->flags.n, 1);
+  add(e->slab.n, sizeof(double));
+  add(e->gdsc.n + e->qbnd.n, sizeof(double));
+  add(e->dent.n + e->gdir.n + e->gxch.n, sizeof(double2));
+  add(e->pairs.n, sizeof(EPair));
+  add(e->terms.n, sizeof(ETerm));
+  add(e->monos.n, sizeof(Mono));
+  for (const DevPlan &p : e->plans)
+    b += p.classes.n * sizeof(ClassDesc) + p.abs.n * sizeof(ABCombo) + p.cds.n * sizeof(CDCall) +
+         p.masks.n + p.items.n * sizeof(WorkItem) + p.qitem.n * sizeof(double);
+  for (const DevBreitPlan &p : e->bplans)
+    b += p.classes.n * sizeof(ClassDesc) + p.abs.n * sizeof(BrAB) + p.cds.n * sizeof(BrCD) +
+         p.masks.n + p.items.n * sizeof(WorkItem) + p.stale.n;
+  *device_bytes = b;
   return BDF_OK;
 }
 

@@ -74,6 +74,8 @@ class EngineBackend:
     outputs stay on the device (bdf_*_device entry points)."""
 
     def __init__(self, pkg, system: System, device: str = "cuda:0", **engine_kw):
+        # the engine's HIP stream orders its kernels; torch's work runs on
+        # the calling thread's current stream and is joined by events
         self.system = system
         self.eng = pkg.Engine(system, **engine_kw)
         self.dev = torch.device(device)

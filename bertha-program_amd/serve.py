@@ -1,0 +1,220 @@
+"""Serving loop: Dirac-Hartree-Fock solves of uneven size on one GPU.
+
+Requests (an atom or a diatomic, with its basis and electron count) arrive
+over time.  A request is admitted as soon as a slot is free and its device
+footprint fits the HBM budget; each admitted solve runs the SCF driver on
+its own engine, HIP stream and torch stream, so solves that are in flight
+together overlap on the device (continuous batching).  Each solve's
+density, Fock matrices and DIIS history stay on the device for its whole
+life.  Latency is measured from arrival to the converged energy.
+No counterpart in the reference, which runs one SCF per process.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import threading
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import scf as _scf
+from .systems import System, even_tempered
+
+# device bytes of the SCF state per basis function pair: S, HNUC, HKIN, F,
+# D, G, X, B, temporaries and a 6-deep DIIS history of errors and Fock
+# matrices, complex128
+_SCF_MATRICES = 40
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    system: System
+    nocc: int
+    arrival: float = 0.0  # seconds after the start of the run
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    name: str
+    ndim: int
+    energy: float
+    iterations: int
+    converged: bool
+    arrival: float
+    start: float
+    finish: float
+    error: str = ""
+
+    @property
+    def latency(self) -> float:
+        return self.finish - self.arrival
+
+
+def estimate_bytes(system: System, workgroups: int) -> int:
+    """Device footprint of one solve before its engine exists: E tables,
+    per-workgroup scratch and the SCF matrices."""
+    words = 0
+    lmax_all = 0
+    maxn = 1
+    for a in range(system.ncnt):
+        for b in range(system.ncnt):
+            for la in range(system.lmax(a) + 1):
+                for lb in range(system.lmax(b) + 1):
+                    na, nb = system.nfnc_of(a, la), system.nfnc_of(b, lb)
+                    # kappa/|m| combinations of two l shells: (2l+1)(2l'+1)
+                    nc = (2 * la + 1) * (2 * lb + 1)
+                    for extra in ((0, 2, 1, 1, 1) if system.hmlt == "DHFB" else (0, 2)):
+                        L = la + lb + extra
+                        words += nc * na * nb * (L + 1) * (L + 2) * (L + 3) // 6
+            lmax_all = max(lmax_all, system.lmax(a))
+            maxn = max([maxn] + [system.nfnc_of(a, l) for l in range(system.lmax(a) + 1)])
+    ld = ((maxn * maxn + 63) // 64) * 64
+    rows = 2426 if system.hmlt == "DHFB" else 1646
+    n = system.ndim
+    return 32 * words + 8 * rows * ld * workgroups + _SCF_MATRICES * 16 * n * n
+
+
+class Server:
+    """Continuous-batching DHF server on one device."""
+
+    def __init__(self, pkg, slots: int = 4, hbm_bytes: Optional[int] = None,
+                 workgroups: int = 256, device: str = "cuda:0", max_iter: int = 100):
+        self.pkg = pkg
+        self.slots = slots
+        self.workgroups = workgroups
+        self.device = torch.device(device)
+        if hbm_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_bytes = int(0.8 * free)
+        self.budget = hbm_bytes
+        self.max_iter = max_iter
+        self._lock = threading.Condition()
+        self._pending: collections.deque = collections.deque()
+        self._used = 0
+        self._active = 0
+        self._closed = False
+        self.results: List[Result] = []
+        self.peak_active = 0
+
+    def _take(self):
+        """Next pending request that fits (arrival order, first fit)."""
+        with self._lock:
+            while True:
+                for k, (req, need) in enumerate(self._pending):
+                    if self._used + need <= self.budget or self._active == 0:
+                        del self._pending[k]
+                        self._used += need
+                        self._active += 1
+                        self.peak_active = max(self.peak_active, self._active)
+                        return req, need
+                if self._closed and not self._pending:
+                    return None, 0
+                self._lock.wait(0.01)
+
+    def _release(self, need: int):
+        with self._lock:
+            self._used -= need
+            self._active -= 1
+            self._lock.notify_all()
+
+    def _worker(self, t0: float):
+        stream = torch.cuda.Stream(device=self.device)
+        while True:
+            req, need = self._take()
+            if req is None:
+                return
+            start = time.perf_counter() - t0
+            energy, iters, ok, err = float("nan"), 0, False, ""
+            try:
+                with torch.cuda.stream(stream):
+                    be = _scf.EngineBackend(self.pkg, req.system, device=str(self.device),
+                                            workgroups=self.workgroups)
+                    try:
+                        r = _scf.run_scf(req.system, req.nocc, be, max_iter=self.max_iter)
+                        energy, iters, ok = r.energy, r.iterations, r.converged
+                    finally:
+                        be.close()
+            except RuntimeError as x:  # a solve that diverges is reported, not fatal
+                err = str(x)
+            finish = time.perf_counter() - t0
+            res = Result(req.rid, req.system.name, req.system.ndim, energy, iters, ok,
+                         req.arrival, start, finish, err)
+            with self._lock:
+                self.results.append(res)
+            self._release(need)
+
+    def run(self, requests: Sequence[Request]) -> List[Result]:
+        """Serve requests at their arrival times; returns results by id."""
+        self.results = []
+        t0 = time.perf_counter()
+        workers = [threading.Thread(target=self._worker, args=(t0,), daemon=True)
+                   for _ in range(self.slots)]
+        for w in workers:
+            w.start()
+        for req in sorted(requests, key=lambda r: r.arrival):
+            delay = req.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            need = estimate_bytes(req.system, self.workgroups)
+            with self._lock:
+                self._pending.append((req, need))
+                self._lock.notify_all()
+        with self._lock:
+            self._closed = True
+            self._lock.notify_all()
+        for w in workers:
+            w.join()
+        return sorted(self.results, key=lambda r: r.rid)
+
+
+# ---- the serving workload -------------------------------------------------
+ATOMS = (2, 4, 10, 12, 18, 20, 36, 54)
+
+
+def atom_basis(z: int):
+    """Even-tempered uncontracted basis sized by the nuclear charge (the
+    workload leaves the recipe open): more and tighter functions for heavier
+    atoms, d functions from Ar on."""
+    ns = 10 + z // 4
+    np_ = 0 if z <= 2 else 4 + z // 4
+    nd = 0 if z < 18 else 2 + z // 12
+    sh = [even_tempered(0.05, 2.5, ns)]
+    if np_:
+        sh.append(even_tempered(0.08, 2.5, np_))
+    if nd:
+        sh.append(even_tempered(0.15, 2.6, nd))
+    return sh
+
+
+def request_mix(n: int, seed: int = 0, rate: float = 20.0, atoms: Sequence[int] = ATOMS):
+    """n requests, Poisson arrivals at `rate` per second; 60 % atoms, 40 %
+    diatomics of the same elements with R ~ U(2.5, 8.0) bohr."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        if rng.uniform() < 0.6:
+            z = int(rng.choice(atoms))
+            s = System(f"Z{z}", np.zeros((1, 3)), [atom_basis(z)], shape="ATOMIC",
+                       charges=[float(z)])
+            nocc = z
+        else:
+            z1, z2 = (int(x) for x in rng.choice(atoms, size=2))
+            r = float(rng.uniform(2.5, 8.0))
+            s = System(f"Z{z1}-Z{z2}", np.array([[0.0, 0.0, 0.0], [0.0, 0.0, r]]),
+                       [atom_basis(z1), atom_basis(z2)], shape="DIATOM",
+                       charges=[float(z1), float(z2)])
+            nocc = z1 + z2
+        out.append(Request(rid, s, nocc, t))
+    return out
+
+
+def percentiles(results: Sequence[Result]):
+    lat = np.array([r.latency for r in results])
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))

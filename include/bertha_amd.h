@@ -82,6 +82,9 @@ typedef struct {
                           (bertha-scf.f:11567) */
   int gaunt;           /* GAUNT2: Gaunt interaction instead of Breit in
                           bdf_breit (reference default 0, bertha-scf.f:5246) */
+  int workgroups;      /* persistent workgroups of the quartet kernels (0: 8
+                          per compute unit); fewer shrink the per-workgroup
+                          scratch when several engines share the device */
 } bdf_options;
 
 typedef struct bdf_engine bdf_engine;
@@ -182,6 +185,10 @@ int bdf_etable(bdf_engine *e, int itt, int ica, int icb, int ka, int kb,
            (|(ab|cd)| <= qbnd(a,b) qbnd(c,d)); the engine's FP32/FP64 split
            uses it.  No counterpart in the reference. */
 int bdf_schwarz_tables(bdf_engine *e, double *gdsc, double *qbnd, int ld);
+
+/* Device memory held by the engine (tables, plans, scratch), in bytes.
+   No counterpart in the reference. */
+int bdf_memory_info(bdf_engine *e, size_t *device_bytes);
 
 /* Engine's HIP stream (hipStream_t) for callers that order their own work
    after bdf_coulomb_device. */

@@ -21,7 +21,7 @@ module bertha_amd
     real(c_double) :: split_threshold
     integer(c_int) :: device, schwarz
     real(c_double) :: sens
-    integer(c_int) :: gaunt
+    integer(c_int) :: gaunt, workgroups
   end type bdf_options
 
   interface
@@ -127,6 +127,12 @@ module bertha_amd
       import :: c_int, c_ptr
       type(c_ptr), value :: eng, gdsc, qbnd
       integer(c_int), value :: ld
+    end function
+
+    integer(c_int) function bdf_memory_info(eng, device_bytes) bind(c, name='bdf_memory_info')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: eng
+      integer(c_size_t), intent(out) :: device_bytes
     end function
 
     type(c_ptr) function bdf_stream(eng) bind(c, name='bdf_stream')
