@@ -110,6 +110,8 @@ struct bdf_engine {
   int gaunt = 0;  // GAUNT2
   DevBreitPlan bplans[3];
   DevBuf<float4> E32;
+  DevBuf<double4> Ecd;  // masked, signed E(CD) of the second contraction
+  DevBuf<float4> Ecd32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_ms = 0.0, last_quartets = 0.0;
@@ -258,7 +260,7 @@ static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double>
     const ClassDesc &cl = P.classes[it.cls];
     int ib = it.ij / cl.nbas[1], jb = it.ij % cl.nbas[1];
     double m = 0.0;
-    for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+    for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
       const ABCombo &ab = P.abs[abi];
       for (int s = 0; s < 4; ++s) m = std::max(m, at(ab.fa[s >> 1] + ib, ab.fb[s & 1] + jb));
     }
@@ -305,6 +307,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.qitem = mixed ? dp.qitem.p : nullptr;
   a.dmax_bits = e->dmax.p;
   a.E32 = e->E32.p;
+  a.Ecd = e->Ecd.p;
+  a.Ecd32 = e->Ecd32.p;
   int nb = std::min(e->nblocks, std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream));
@@ -466,6 +470,10 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     fa.flag_off = e->flag_off.p;
     fa.E = e->E.p;
     fa.flags = e->flags.p;
+    HIPCHK(e->Ecd.alloc((size_t)e->T.total));
+    if (e->mixed) HIPCHK(e->Ecd32.alloc((size_t)e->T.total));
+    fa.Ecd = e->Ecd.p;
+    fa.Ecd32 = e->mixed ? e->Ecd32.p : nullptr;
     HIPCHK(launch_eflag(fa, e->stream));
     e->eflags.resize(e->T.nflag);
     HIPCHK(hipMemcpyAsync(e->eflags.data(), e->flags.p, e->T.nflag, hipMemcpyDeviceToHost,
@@ -473,7 +481,7 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(e->icn.upload(e->B.labicn, e->stream));
     HIPCHK(e->kqn.upload(e->B.labkqn, e->stream));
     HIPCHK(e->mqn.upload(e->B.labmqn, e->stream));
-    HIPCHK(e->counter.alloc(1));
+    HIPCHK(e->counter.alloc(2));
     HIPCHK(e->dmax.alloc(1));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
@@ -615,7 +623,7 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   dp.P.classes.push_back(cl);
   dp.P.abs.push_back(ab);
   dp.P.cds.push_back(cd);
-  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1)});
+  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1), cl.ab_begin, cl.ab_end});
   dp.P.max_maxcd = mcd;
   HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
   HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
@@ -818,7 +826,7 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
     return fail(BDF_EINVAL, "bdf_bii_batch: missing E-coefficient block");
   }
   dp.P.classes.push_back(cl);
-  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1)});
+  dp.P.items.push_back({0, (ibas - 1) * cl.nbas[1] + (jbas - 1), cl.ab_begin, cl.ab_end});
   dp.P.nflag = ntuv(cl.lamfl);
   dp.P.max_maxcd = mcd;
   dp.P.max_ntg = ntcd;
@@ -964,6 +972,8 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   auto add = [&](size_t n, size_t w) { b += n * w; };
   add(e->E.n, sizeof(double4));
   add(e->E32.n, sizeof(float4));
+  add(e->Ecd.n, sizeof(double4));
+  add(e->Ecd32.n, sizeof(float4));
   add(e->flags.n, 1);
   add(e->slab.n, sizeof(double));
   add(e->gdsc.n + e->qbnd.n, sizeof(double));

@@ -106,6 +106,7 @@ struct DiagJob {
 struct WorkItem {
   int cls;
   int ij;
+  int ab_lo, ab_hi;  // the class's AB combinations [ab_lo, ab_hi) this item covers
 };
 
 // ---- one-electron matrices (OVRLP / ONEEL) ------------------------------

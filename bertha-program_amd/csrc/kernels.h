@@ -23,6 +23,8 @@ struct EflagArgs {
   const int64_t *flag_off;
   const double4 *E;
   uint8_t *flags;
+  double4 *Ecd;   // if set: masked, signed E(CD) copy (and its FP32 copy Ecd32)
+  float4 *Ecd32;
 };
 
 struct FockArgs {
@@ -53,6 +55,9 @@ struct FockArgs {
   const double *qitem;
   const unsigned long long *dmax_bits;
   const float4 *E32;
+  // E(CD) with the column masks and ERIFAST's Z sign applied (FP64, FP32)
+  const double4 *Ecd;
+  const float4 *Ecd32;
 };
 
 struct DiagArgs {

@@ -16,7 +16,7 @@
 
 namespace bdf {
 
-__constant__ uint16_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits
+__constant__ uint32_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits (32-bit words: scalar loads)
 __constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
 __constant__ float c_rcpf[2 * kMaxLfl + 2 * 60 + 4];
 
@@ -25,7 +25,7 @@ __device__ __forceinline__ int hidx(int t, int u, int v) {
   return L * (L + 1) * (L + 2) / 6 + t * (L + 1) - t * (t - 1) / 2 + u;
 }
 __device__ __forceinline__ void dec(int n, int &t, int &u, int &v) {
-  uint16_t x = c_tuv[n];
+  uint32_t x = c_tuv[n];
   t = x >> 10;
   u = (x >> 5) & 31;
   v = x & 31;
@@ -155,6 +155,29 @@ __global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
     }
     uint8_t *f = a.flags + a.flag_off[p] + 4 * n;
     f[0] = s0 > 1.0e-10; f[1] = s1 > 1.0e-10; f[2] = s2 > 1.0e-10; f[3] = s3 > 1.0e-10;
+  }
+}
+
+// E(CD) copy for ERIFAST's second contraction (:12809-13119): the column
+// masks and the sign Z = (-1)^(t'+u'+v') applied once per table entry
+// instead of once per quartet; an FP32 copy of it for the mixed build.
+__global__ void __launch_bounds__(256) fold_kernel(EflagArgs a) {
+  const int p = blockIdx.x;
+  const EPair P = a.pairs[p];
+  const int maxab = P.nA * P.nB;
+  const int64_t nw = (int64_t)ntuv(P.lam) * maxab;
+  for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) {
+    const int n = (int)(w / maxab);
+    int t, u, v;
+    dec(n, t, u, v);
+    const double z = ((t + u + v) & 1) ? -1.0 : 1.0;
+    const uint8_t *f = a.flags + a.flag_off[p] + 4 * n;
+    const double4 e = a.E[P.off + w];
+    const double4 o = make_double4(f[0] ? z * e.x : 0.0, f[1] ? z * e.y : 0.0,
+                                   f[2] ? z * e.z : 0.0, f[3] ? z * e.w : 0.0);
+    a.Ecd[P.off + w] = o;
+    if (a.Ecd32)
+      a.Ecd32[P.off + w] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
   }
 }
 
@@ -361,11 +384,54 @@ constexpr Batch kBatchTab[11] = {
     {1, 2, 1, 0, 0, 0, 3, 0, 0, 0, 0}, {1, 2, 0, 0, 1, 1, 3, 1, 0, 1, 0},
     {1, 3, 1, 1, 0, 0, 2, 0, 1, 0, 1}};
 
+// The four raw sums of the second contraction for one kl: a_s = sum Z E_CD G,
+// b_s = sum Z E_CD conj(G) for the pairings s = (E11cd,G11), (E21cd,G11),
+// (E11cd,G21), (E21cd,G21), with the phases PAB1 and PAB1*PCD1 of the call.
+struct Raw {
+  double ar[4], ai[4], br[4], bi[4];
+  double pab, pabcd;
+};
+
+// RR(n), n = 1 + 8 sA + 4 sB + 2 sC + sD (s = 1 for +|m|), from the raw sums
+// (ERIFAST :12822-13119); the real path (ISYM 1, 2) keeps real parts of the
+// sign-diagonal combinations only.  n is a compile-time constant after
+// unrolling, so only the needed products are formed.
+template <bool REAL>
+__device__ __forceinline__ void rrv(const Raw &w, int n, double &re, double &im) {
+  const double p = w.pab, q = w.pabcd;
+  re = 0.0;
+  im = 0.0;
+  switch (n) {
+    case 0: re = w.ar[0]; im = w.ai[0]; break;
+    case 12: re = p * w.br[0]; im = p * w.bi[0]; break;
+    case 15: re = q * w.ar[0]; im = -q * w.ai[0]; break;
+    case 3: re = q * (p * w.br[0]); im = -q * (p * w.bi[0]); break;
+    case 10: re = w.ar[3]; im = w.ai[3]; break;
+    case 6: re = -p * w.br[3]; im = -p * w.bi[3]; break;
+    case 5: re = q * w.ar[3]; im = -q * w.ai[3]; break;
+    case 9: re = q * (-p * w.br[3]); im = -q * (-p * w.bi[3]); break;
+    case 2: re = w.ar[1]; im = w.ai[1]; break;
+    case 14: re = p * w.br[1]; im = p * w.bi[1]; break;
+    case 13: re = -q * w.ar[1]; im = q * w.ai[1]; break;
+    case 1: re = -q * (p * w.br[1]); im = q * (p * w.bi[1]); break;
+    case 8: re = w.ar[2]; im = w.ai[2]; break;
+    case 4: re = -p * w.br[2]; im = -p * w.bi[2]; break;
+    case 7: re = -q * w.ar[2]; im = q * w.ai[2]; break;
+    default: re = -q * (-p * w.br[2]); im = q * (-p * w.bi[2]); break;  // 11
+  }
+  if (REAL) {
+    const bool diag = n == 0 || n == 12 || n == 15 || n == 3 || n == 10 || n == 6 || n == 5 ||
+                      n == 9;
+    if (!diag) re = 0.0;
+    im = 0.0;
+  }
+}
+
 // One digestion batch for one (ij,kl): out[2*rowslot+colslot] accumulates
 // the four output sub-blocks.  Everything indexed by the batch table folds to
 // constants after unrolling, so the 16 terms stay in registers.
 template <int B, bool REAL>
-__device__ __forceinline__ void batch_terms(const double (&rr)[16], const double (&ri)[16],
+__device__ __forceinline__ void batch_terms(const Raw &raw,
                                             const int (&off)[4][2], const int (&idx)[4],
                                             const double2 *__restrict__ Dm, int nd,
                                             double pab1, double pab2, double pcd1, double pcd2,
@@ -391,11 +457,13 @@ __device__ __forceinline__ void batch_terms(const double (&rr)[16], const double
     if (bt.pab) ph *= (sl[0] != sl[1]) ? pab2 : pab1;
     if (bt.pcd) ph *= (sl[2] != sl[3]) ? pcd2 : pcd1;
     const double2 dv = d[2 * drs + dcs];
+    double rr, ri;
+    rrv<REAL>(raw, n, rr, ri);
     if (REAL) {
-      out[2 * ors + ocs][0] += ph * rr[n] * dv.x;
+      out[2 * ors + ocs][0] += ph * rr * dv.x;
     } else {
-      out[2 * ors + ocs][0] += ph * (rr[n] * dv.x - ri[n] * dv.y);
-      out[2 * ors + ocs][1] += ph * (rr[n] * dv.y + ri[n] * dv.x);
+      out[2 * ors + ocs][0] += ph * (rr * dv.x - ri * dv.y);
+      out[2 * ors + ocs][1] += ph * (rr * dv.y + ri * dv.x);
     }
   }
 }
@@ -422,19 +490,19 @@ __device__ __forceinline__ void lds_add4(double (&dst)[4][kMaxPrim][2], int pos,
 // :13196-13514 when REAL): batches 1-2 into registers (G(A,B) is fixed by
 // ij), 3-4 straight to global memory, exchange batches into LDS
 template <bool REAL>
-__device__ __forceinline__ void digest(const double (&rr)[16], const double (&ri)[16],
+__device__ __forceinline__ void digest(const Raw &raw,
                                        const int (&off)[4][2], const int (&idx)[4],
                                        const double2 *__restrict__ Dm, int nd, int iflg,
                                        int intsym, double pab1, double pab2, double pcd1,
                                        double pcd2, double (&accAB)[4][2], XAcc &xa,
                                        double2 *gdir) {
-  if (iflg & 1) batch_terms<0, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
+  if (iflg & 1) batch_terms<0, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
   if (!intsym) iflg &= 0x11;
-  if (iflg & 2) batch_terms<1, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
+  if (iflg & 2) batch_terms<1, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
   if (iflg & 12) {
     double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    if (iflg & 4) batch_terms<2, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
-    if (iflg & 8) batch_terms<3, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
+    if (iflg & 4) batch_terms<2, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
+    if (iflg & 8) batch_terms<3, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int rs = s >> 1, cs = s & 1;
@@ -446,7 +514,7 @@ __device__ __forceinline__ void digest(const double (&rr)[16], const double (&ri
 #define XB(B, ARR, POS)                                                                     \
   if (iflg & (1 << B)) {                                                                    \
     double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};                                      \
-    batch_terms<B, REAL>(rr, ri, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);              \
+    batch_terms<B, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);              \
     lds_add4<REAL>(ARR, POS, o);                                                            \
   }
   XB(4, xa.l[0], idx[3])
@@ -519,6 +587,17 @@ __device__ __forceinline__ void lde(const FockArgs &a, int64_t i, float &x, floa
   float4 e = a.E32[i];
   x = e.x; y = e.y; z = e.z; w = e.w;
 }
+// the same for the masked, signed E(CD) table of the second contraction
+__device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, double &x, double &y, double &z,
+                                     double &w) {
+  double4 e = a.Ecd[i];
+  x = e.x; y = e.y; z = e.z; w = e.w;
+}
+__device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, float &x, float &y, float &z,
+                                     float &w) {
+  float4 e = a.Ecd32[i];
+  x = e.x; y = e.y; z = e.z; w = e.w;
+}
 
 // Flush the exchange accumulators of one call into the exchange matrix (one
 // atomic per output element instead of one per quartet), then clear them.
@@ -550,14 +629,17 @@ __device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib
   __syncthreads();
 }
 
-// Shared-memory state of one workgroup
+// Shared-memory state of one workgroup (kept under 20 KB so that eight
+// workgroups fit a CU's 160 KB LDS)
 struct FockShared {
-  double eab[ntuv(kMaxLab) * 4];
-  uint8_t cflag[ntuv(kMaxLfl)];
-  uint8_t gm[ntuv(kMaxLab) * 4];
-  uint8_t km[ntuv(kMaxLab) * 4];
+  union {
+    uint8_t cflag[ntuv(kMaxLfl)];  // R column flags (until the prefactor pass)
+    double eab[ntuv(kMaxLab) * 4];  // E(AB) row of the current AB combination
+  };
+  uint8_t gm[ntuv(kMaxLab) * 4];    // its G masks
   XAcc x;
 };
+static_assert(sizeof(FockShared) + 16 <= 160 * 1024 / 8, "FockShared exceeds 1/8 of the LDS");
 
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
 // E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
@@ -623,7 +705,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   }
   __syncthreads();
 
-  for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+  for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
     const ABCombo ab = a.abs[abi];
     for (int t = lane; t < ab.ntab; t += 64) {
       T ex, ey, ez, ew;
@@ -645,6 +727,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         if (sh.gm[4 * tp] | sh.gm[4 * tp + 1] | sh.gm[4 * tp + 2] | sh.gm[4 * tp + 3]) {
           int t2, u2, v2;
           dec(tp, t2, u2, v2);
+#pragma unroll 4
           for (int t = 0; t < ab.ntab; ++t) {
             int t1, u1, v1;
             dec(t, t1, u1, v1);
@@ -665,9 +748,6 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
     double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
     for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
       const CDCall cd = a.cds[ci];
-      __syncthreads();
-      for (int t = lane; t < 4 * cd.ntcd; t += 64) sh.km[t] = a.masks[cd.kmask + t];
-      __syncthreads();
       const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
       const double pabcd = pab1 * pcd1;
       for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
@@ -677,16 +757,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
         T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
         T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+#pragma unroll 4
         for (int tp = 0; tp < cd.ntcd; ++tp) {
-          T ex, ey, ez, ew;
-          lde(a, cd.eoff + (int64_t)tp * maxcd + kl, ex, ey, ez, ew);
-          int t2, u2, v2;
-          dec(tp, t2, u2, v2);
-          T z = ((t2 + u2 + v2) & 1) ? (T)-1 : (T)1;
-          T e11r = sh.km[4 * tp + 0] ? z * ex : (T)0;
-          T e11i = sh.km[4 * tp + 1] ? z * ey : (T)0;
-          T e21r = sh.km[4 * tp + 2] ? z * ez : (T)0;
-          T e21i = sh.km[4 * tp + 3] ? z * ew : (T)0;
+          // masked E(CD) times Z (fold_kernel)
+          T e11r, e11i, e21r, e21i;
+          ldcd(a, cd.eoff + (int64_t)tp * maxcd + kl, e11r, e11i, e21r, e21i);
           const T *gp = G + (int64_t)(4 * tp) * ld + kl;
           T g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
           // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
@@ -700,39 +775,20 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
           br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
         }
-        // RR(n), n = 1 + 8 sA + 4 sB + 2 sC + sD (s = 1 for +|m|), in FP64
-        double rr_r[16], rr_i[16];
-        if (a.isym != 0) {
-          for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
-          rr_r[0] = ar[0];
-          rr_r[12] = pab1 * br[0];
-          rr_r[15] = pabcd * rr_r[0];
-          rr_r[3] = pabcd * rr_r[12];
-          rr_r[10] = ar[3];
-          rr_r[6] = -pab1 * br[3];
-          rr_r[5] = pabcd * rr_r[10];
-          rr_r[9] = pabcd * rr_r[6];
-        } else {
-          rr_r[0] = ar[0]; rr_i[0] = ai[0];
-          rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
-          rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
-          rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
-          rr_r[2] = ar[1]; rr_i[2] = ai[1];
-          rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
-          rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
-          rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
-          rr_r[8] = ar[2]; rr_i[8] = ai[2];
-          rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
-          rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
-          rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
-          rr_r[10] = ar[3]; rr_i[10] = ai[3];
-          rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
-          rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
-          rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
+        // raw sums in FP64; RR(n) is formed where the digestion needs it
+        Raw raw;
+        for (int k = 0; k < 4; ++k) {
+          raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
         }
+        raw.pab = pab1;
+        raw.pabcd = pabcd;
         if (a.rr_out) {
-          for (int n = 0; n < 16; ++n)
-            a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
+#pragma unroll
+          for (int n = 0; n < 16; ++n) {
+            double re, im;
+            if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
+            a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+          }
           continue;
         }
         // digestion (CLMMTG / CLMMTZ)
@@ -744,10 +800,10 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
           flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
         if (a.isym != 0)
-          digest<true>(rr_r, rr_i, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                        cd.pcd[1], accAB, sh.x, a.gdir);
         else
-          digest<false>(rr_r, rr_i, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                         cd.pcd[1], accAB, sh.x, a.gdir);
       }
       if (!a.rr_out) {
@@ -775,12 +831,17 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 // Persistent work-queue kernel: each 64-lane workgroup takes (class, ij)
 // items from a global counter until the list is exhausted.  With the Schwarz
 // split on, an item whose largest possible quartet contribution
-// qitem * max|D| is below the split threshold runs in FP32.
-__global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
+// qitem * max|D| is below the split threshold belongs to the FP32 instance
+// of the kernel, every other item to the FP64 instance; the two instances
+// keep their own counters and register allocations.
+template <typename T>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) fock_kernel(FockArgs a) {
   __shared__ FockShared sh;
   __shared__ int s_item;
+  constexpr bool kLow = sizeof(T) == sizeof(float);
   const int lane = threadIdx.x;
   double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
+  int *counter = a.counter + (kLow ? 1 : 0);
   {
     double *z = &sh.x.l[0][0][0][0];
     for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
@@ -788,16 +849,14 @@ __global__ void __launch_bounds__(64) fock_kernel(FockArgs a) {
   const double dmax = a.mixed ? __longlong_as_double(*a.dmax_bits) : 0.0;
   __syncthreads();
   for (;;) {
-    if (lane == 0) s_item = atomicAdd(a.counter, 1);
+    if (lane == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
     int idx = s_item;
     __syncthreads();
     if (idx >= a.nitems) break;
-    const WorkItem it = a.items[idx];
-    if (a.mixed && a.qitem[idx] * dmax < a.split)
-      fock_item<float>(a, it, slab, sh, lane);
-    else
-      fock_item<double>(a, it, slab, sh, lane);
+    const bool low = a.mixed && a.qitem[idx] * dmax < a.split;
+    if (low != kLow) continue;
+    fock_item<T>(a, a.items[idx], slab, sh, lane);
   }
 }
 
@@ -909,7 +968,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   }
   __syncthreads();
 
-  for (int abi = cl.ab_begin; abi < cl.ab_end; ++abi) {
+  for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
     const BrAB ab = a.abs[abi];
     for (int w = lane; w < 3 * ab.ntab; w += 64) {
       int ix = w / ab.ntab, t = w % ab.ntab;
@@ -1010,39 +1069,21 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
             ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
             br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
           }
-        double rr_r[16], rr_i[16];
-        if (a.isym != 0) {
-          for (int n = 0; n < 16; ++n) rr_r[n] = rr_i[n] = 0.0;
-          rr_r[0] = ar[0];
-          rr_r[12] = pab1 * br[0];
-          rr_r[15] = pabcd * rr_r[0];
-          rr_r[3] = pabcd * rr_r[12];
-          rr_r[10] = ar[3];
-          rr_r[6] = -pab1 * br[3];
-          rr_r[5] = pabcd * rr_r[10];
-          rr_r[9] = pabcd * rr_r[6];
-        } else {
-          rr_r[0] = ar[0]; rr_i[0] = ai[0];
-          rr_r[12] = pab1 * br[0]; rr_i[12] = pab1 * bi[0];
-          rr_r[15] = pabcd * rr_r[0]; rr_i[15] = -pabcd * rr_i[0];
-          rr_r[3] = pabcd * rr_r[12]; rr_i[3] = -pabcd * rr_i[12];
-          rr_r[2] = ar[1]; rr_i[2] = ai[1];
-          rr_r[14] = pab1 * br[1]; rr_i[14] = pab1 * bi[1];
-          rr_r[13] = -pabcd * rr_r[2]; rr_i[13] = pabcd * rr_i[2];
-          rr_r[1] = -pabcd * rr_r[14]; rr_i[1] = pabcd * rr_i[14];
-          rr_r[8] = ar[2]; rr_i[8] = ai[2];
-          rr_r[4] = -pab1 * br[2]; rr_i[4] = -pab1 * bi[2];
-          rr_r[7] = -pabcd * rr_r[8]; rr_i[7] = pabcd * rr_i[8];
-          rr_r[11] = -pabcd * rr_r[4]; rr_i[11] = pabcd * rr_i[4];
-          rr_r[10] = ar[3]; rr_i[10] = ai[3];
-          rr_r[6] = -pab1 * br[3]; rr_i[6] = -pab1 * bi[3];
-          rr_r[5] = pabcd * rr_r[10]; rr_i[5] = -pabcd * rr_i[10];
-          rr_r[9] = pabcd * rr_r[6]; rr_i[9] = -pabcd * rr_i[6];
+        Raw raw;
+        for (int k = 0; k < 4; ++k) {
+          raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
         }
+        raw.pab = pab1;
+        raw.pabcd = pabcd;
         if (a.rr_out) {
-          if (ci == 0)
-            for (int n = 0; n < 16; ++n)
-              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(rr_r[n], rr_i[n]);
+          if (ci == 0) {
+#pragma unroll
+            for (int n = 0; n < 16; ++n) {
+              double re, im;
+              if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
+              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+            }
+          }
           continue;
         }
         const int idx[4] = {ib, jb, kl / nD, kl % nD};
@@ -1050,10 +1091,10 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
         double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
         if (a.isym != 0)
-          digest<true>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
                        cd.pcd[1], acc_unused, sh.x, a.bdir);
         else
-          digest<false>(rr_r, rr_i, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<false>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
                         cd.pcd[1], acc_unused, sh.x, a.bdir);
       }
       if (!a.rr_out) {
@@ -1268,7 +1309,10 @@ __global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter,
     a[g] = make_double2(0.0, 0.0);
     if (b) b[g] = make_double2(0.0, 0.0);
   }
-  if (g == 0 && counter) *counter = 0;
+  if (g == 0 && counter) {
+    counter[0] = 0;  // FP64 (and Breit) work queue
+    counter[1] = 0;  // FP32 work queue
+  }
   if (g == 0 && dmax) *dmax = 0ull;
 }
 
@@ -1276,14 +1320,14 @@ __global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter,
 // host launchers
 // ---------------------------------------------------------------------------
 hipError_t upload_constants() {
-  uint16_t tuv[ntuv(kMaxLfl)];
+  uint32_t tuv[ntuv(kMaxLfl)];
   // Hermite order: by L, then t ascending, then u ascending (CARTIND :2678-2700)
   int n = 0;
   for (int L = 0; L <= kMaxLfl; ++L)
     for (int t = 0; t <= L; ++t)
       for (int u = 0; u <= L - t; ++u) {
         int v = L - t - u;
-        tuv[n++] = (uint16_t)((t << 10) | (u << 5) | v);
+        tuv[n++] = (uint32_t)((t << 10) | (u << 5) | v);
       }
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tuv), tuv, sizeof(tuv));
   if (e != hipSuccess) return e;
@@ -1308,6 +1352,9 @@ hipError_t launch_etab(const EtabArgs &a, hipStream_t s) {
 hipError_t launch_eflag(const EflagArgs &a, hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;
   hipLaunchKernelGGL(eflag_kernel, dim3(a.npairs), dim3(64), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !a.Ecd) return e;
+  hipLaunchKernelGGL(fold_kernel, dim3(a.npairs), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
@@ -1337,7 +1384,8 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
   return hipGetLastError();
 }
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(fock_kernel, dim3(nblocks), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), 0, s, a);
+  if (a.mixed) hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {

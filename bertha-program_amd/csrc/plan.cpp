@@ -393,6 +393,40 @@ static int ncntrs(int a, int b, int c, int d) {
   return n;
 }
 
+// Items of a plan: every (class, ij), its AB combinations cut into
+// consecutive runs of cost at most max(total / 8192, one combination), then
+// ordered by cost, largest first.
+static void make_items(const std::vector<ClassDesc> &classes, const std::vector<double> &abcost,
+                       const std::vector<double> &rcost, std::vector<WorkItem> &items) {
+  double total = 0.0;
+  for (size_t c = 0; c < classes.size(); ++c) {
+    double w = rcost[c];
+    for (int a = classes[c].ab_begin; a < classes[c].ab_end; ++a) w += abcost[a];
+    total += w * classes[c].nbas[0] * classes[c].nbas[1];
+  }
+  const double target = total / 8192.0;
+  std::vector<std::pair<double, WorkItem>> runs;
+  for (size_t c = 0; c < classes.size(); ++c) {
+    const ClassDesc &cl = classes[c];
+    int lo = cl.ab_begin;
+    while (lo < cl.ab_end) {
+      double w = rcost[c] + abcost[lo];
+      int hi = lo + 1;
+      while (hi < cl.ab_end && w + abcost[hi] <= target) w += abcost[hi++];
+      for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij)
+        runs.push_back({w, WorkItem{(int)c, ij, lo, hi}});
+      lo = hi;
+    }
+  }
+  std::stable_sort(runs.begin(), runs.end(),
+                   [](const std::pair<double, WorkItem> &x, const std::pair<double, WorkItem> &y) {
+                     return x.first > y.first;
+                   });
+  items.clear();
+  items.reserve(runs.size());
+  for (const auto &r : runs) items.push_back(r.second);
+}
+
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                        int isym, int intsym, int racah1, int ilev, Plan &P) {
   P = Plan();
@@ -583,11 +617,9 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
             }
             cl.ab_end = (int)P.abs.size();
             if (cl.ab_end == cl.ab_begin) continue;
-            int cidx = (int)P.classes.size();
             P.classes.push_back(cl);
             P.max_lfl = std::max(P.max_lfl, cl.lamfl);
             P.max_maxcd = std::max(P.max_maxcd, cl.maxcd);
-            for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij) P.items.push_back({cidx, ij});
           }
         }
   }
@@ -612,20 +644,23 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
       }
     }
   }
-  // largest work first, so the persistent kernel's queue drains evenly
-  std::vector<double> cost(P.classes.size(), 0.0);
+  // Work items: (class, ij) split into runs of AB combinations so that no
+  // item is much longer than the average share of a workgroup (each item
+  // recomputes its R batch, which costs far less than one combination's
+  // contractions); largest first, so the persistent kernel's queue drains
+  // evenly.
+  std::vector<double> abcost(P.abs.size(), 0.0), rcost(P.classes.size(), 0.0);
   for (size_t c = 0; c < P.classes.size(); ++c) {
     const ClassDesc &cl = P.classes[c];
-    double w = (double)ntuv(cl.lamfl) * cl.maxcd;
+    rcost[c] = (double)ntuv(cl.lamfl) * cl.maxcd;
     for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
       const ABCombo &ab = P.abs[a];
-      w += (double)ab.ntab * ab.ntg * cl.maxcd;
+      double w = (double)ab.ntab * ab.ntg * cl.maxcd;
       for (int k = ab.cd_begin; k < ab.cd_end; ++k) w += 8.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
+      abcost[a] = w;
     }
-    cost[c] = w;
   }
-  std::stable_sort(P.items.begin(), P.items.end(),
-                   [&](const WorkItem &x, const WorkItem &y) { return cost[x.cls] > cost[y.cls]; });
+  make_items(P.classes, abcost, rcost, P.items);
   return "";
 }
 
@@ -794,12 +829,10 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
             }
             cl.ab_end = (int)P.abs.size();
             if (cl.ab_end == cl.ab_begin) continue;
-            int cidx = (int)P.classes.size();
             P.classes.push_back(cl);
             P.nflag += ntuv(cl.lamfl);
             P.max_lfl = std::max(P.max_lfl, cl.lamfl);
             P.max_maxcd = std::max(P.max_maxcd, cl.maxcd);
-            for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij) P.items.push_back({cidx, ij});
           }
         }
   // Algorithmic FP64 work, counted as for the Coulomb build: R batch per
@@ -822,19 +855,19 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
       }
     }
   }
-  std::vector<double> cost(P.classes.size(), 0.0);
+  // work items as for the Coulomb plan
+  std::vector<double> abcost(P.abs.size(), 0.0), rcost(P.classes.size(), 0.0);
   for (size_t c = 0; c < P.classes.size(); ++c) {
     const ClassDesc &cl = P.classes[c];
-    double w = (double)ntuv(cl.lamfl) * cl.maxcd;
+    rcost[c] = (double)ntuv(cl.lamfl) * cl.maxcd;
     for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
       const BrAB &ab = P.abs[a];
-      w += 30.0 * ab.ntab * ab.ntg * cl.maxcd;
+      double w = 30.0 * ab.ntab * ab.ntg * cl.maxcd;
       for (int k = ab.cd_begin; k < ab.cd_end; ++k) w += 24.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
+      abcost[a] = w;
     }
-    cost[c] = w;
   }
-  std::stable_sort(P.items.begin(), P.items.end(),
-                   [&](const WorkItem &x, const WorkItem &y) { return cost[x.cls] > cost[y.cls]; });
+  make_items(P.classes, abcost, rcost, P.items);
   return "";
 }
 
