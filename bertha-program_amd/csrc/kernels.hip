@@ -17,6 +17,10 @@
 namespace bdf {
 
 __constant__ uint32_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits (32-bit words: scalar loads)
+// R row of a sum of Hermite indices: c_hsum[tp * kHsRow + t] = hidx(tuv(t) + tuv(tp))
+// for t, tp < N(kMaxLab); rows padded to a multiple of 4 for 16-byte scalar loads
+constexpr int kHsRow = (ntuv(kMaxLab) + 3) / 4 * 4;
+__constant__ int c_hsum[ntuv(kMaxLab) * kHsRow];
 __constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
 __constant__ float c_rcpf[2 * kMaxLfl + 2 * 60 + 4];
 
@@ -725,13 +729,22 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       for (int tp = 0; tp < ab.ntg; ++tp) {
         T g0 = 0, g1 = 0, g2 = 0, g3 = 0;
         if (sh.gm[4 * tp] | sh.gm[4 * tp + 1] | sh.gm[4 * tp + 2] | sh.gm[4 * tp + 3]) {
-          int t2, u2, v2;
-          dec(tp, t2, u2, v2);
-#pragma unroll 4
-          for (int t = 0; t < ab.ntab; ++t) {
-            int t1, u1, v1;
-            dec(t, t1, u1, v1);
-            T r = R[(int64_t)hidx(t1 + t2, u1 + u2, v1 + v2) * ld + kl];
+          // R rows from the sum table, four independent loads in flight
+          const int *hs = c_hsum + tp * kHsRow;
+          int t = 0;
+          for (; t + 4 <= ab.ntab; t += 4) {
+            const T r0 = R[(int64_t)hs[t] * ld + kl];
+            const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
+            const T r2 = R[(int64_t)hs[t + 2] * ld + kl];
+            const T r3 = R[(int64_t)hs[t + 3] * ld + kl];
+            const T *e = s_eab + 4 * t;
+            g0 += e[0] * r0; g1 += e[1] * r0; g2 += e[2] * r0; g3 += e[3] * r0;
+            g0 += e[4] * r1; g1 += e[5] * r1; g2 += e[6] * r1; g3 += e[7] * r1;
+            g0 += e[8] * r2; g1 += e[9] * r2; g2 += e[10] * r2; g3 += e[11] * r2;
+            g0 += e[12] * r3; g1 += e[13] * r3; g2 += e[14] * r3; g3 += e[15] * r3;
+          }
+          for (; t < ab.ntab; ++t) {
+            const T r = R[(int64_t)hs[t] * ld + kl];
             g0 += s_eab[4 * t + 0] * r;
             g1 += s_eab[4 * t + 1] * r;
             g2 += s_eab[4 * t + 2] * r;
@@ -1330,6 +1343,20 @@ hipError_t upload_constants() {
         tuv[n++] = (uint32_t)((t << 10) | (u << 5) | v);
       }
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tuv), tuv, sizeof(tuv));
+  if (e != hipSuccess) return e;
+  static int hs[ntuv(kMaxLab) * kHsRow];
+  for (int tp = 0; tp < ntuv(kMaxLab); ++tp)
+    for (int t = 0; t < kHsRow; ++t) {
+      int v = 0;
+      if (t < ntuv(kMaxLab)) {
+        const uint32_t a = tuv[t], b = tuv[tp];
+        const int T = (int)((a >> 10) + (b >> 10)), U = (int)(((a >> 5) & 31) + ((b >> 5) & 31));
+        const int V = (int)((a & 31) + (b & 31)), L = T + U + V;
+        v = L * (L + 1) * (L + 2) / 6 + T * (L + 1) - T * (T - 1) / 2 + U;
+      }
+      hs[tp * kHsRow + t] = v;
+    }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_hsum), hs, sizeof(hs));
   if (e != hipSuccess) return e;
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
   float rcpf[2 * kMaxLfl + 2 * 60 + 4];
