@@ -114,6 +114,8 @@ struct bdf_engine {
   DevBuf<float4> Ecd32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipStream_t stream32 = nullptr;  // FP32 instance of the mixed build
+  hipEvent_t fork = nullptr, join = nullptr;
   double last_ms = 0.0, last_quartets = 0.0;
 };
 
@@ -128,7 +130,8 @@ static int ensure_slab(bdf_engine *e, int maxcd, int g_rows) {
     e->slab_ld = std::max(ld, e->slab_ld);
     e->slab_rows = std::max(rows, e->slab_rows);
     e->slab_stride = (int64_t)e->slab_rows * e->slab_ld;
-    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks));
+    // the mixed build's FP32 instance runs concurrently on a second set of blocks
+    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks * (e->mixed ? 2 : 1)));
   }
   return BDF_OK;
 }
@@ -311,7 +314,7 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.Ecd32 = e->Ecd32.p;
   int nb = std::min(e->nblocks, std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
-  if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream));
+  if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));
   HIPCHK(hipEventRecord(e->ev2, e->stream));
   if (!rr_out) {
     CompleteArgs c;
@@ -434,6 +437,11 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
     HIPCHK(hipEventCreate(&e->ev2));
+    if (e->mixed) {
+      HIPCHK(hipStreamCreateWithFlags(&e->stream32, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&e->join, hipEventDisableTiming));
+    }
     HIPCHK(upload_constants());
     build_etables(e->B, e->T, e->breit);
     HIPCHK(e->exps.upload(e->B.exps, e->stream));
@@ -1024,6 +1032,10 @@ void bdf_destroy(bdf_engine *e) {
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
   if (e->ev2) (void)hipEventDestroy(e->ev2);
+  if (e->stream32) (void)hipStreamSynchronize(e->stream32);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->join) (void)hipEventDestroy(e->join);
+  if (e->stream32) (void)hipStreamDestroy(e->stream32);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

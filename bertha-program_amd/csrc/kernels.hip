@@ -431,17 +431,11 @@ __device__ __forceinline__ void rrv(const Raw &w, int n, double &re, double &im)
   }
 }
 
-// One digestion batch for one (ij,kl): out[2*rowslot+colslot] accumulates
-// the four output sub-blocks.  Everything indexed by the batch table folds to
-// constants after unrolling, so the 16 terms stay in registers.
+// The density elements one digestion batch reads, d[2*rowslot+colslot]
 template <int B, bool REAL>
-__device__ __forceinline__ void batch_terms(const Raw &raw,
-                                            const int (&off)[4][2], const int (&idx)[4],
-                                            const double2 *__restrict__ Dm, int nd,
-                                            double pab1, double pab2, double pcd1, double pcd2,
-                                            double (&out)[4][2]) {
+__device__ __forceinline__ void load_d(const int (&off)[4][2], const int (&idx)[4],
+                                       const double2 *__restrict__ Dm, int nd, double2 (&d)[4]) {
   constexpr Batch bt = kBatchTab[B];
-  double2 d[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int rs = s >> 1, cs = s & 1;
@@ -451,6 +445,16 @@ __device__ __forceinline__ void batch_terms(const Raw &raw,
     }
     d[s] = Dm[(int64_t)(off[bt.dcol][cs] + idx[bt.dcol]) * nd + off[bt.drow][rs] + idx[bt.drow]];
   }
+}
+
+// One digestion batch for one (ij,kl): out[2*rowslot+colslot] accumulates
+// the four output sub-blocks.  Everything indexed by the batch table folds to
+// constants after unrolling, so the 16 terms stay in registers.
+template <int B, bool REAL>
+__device__ __forceinline__ void batch_terms(const Raw &raw, const double2 (&d)[4],
+                                            double pab1, double pab2, double pcd1, double pcd2,
+                                            double (&out)[4][2]) {
+  constexpr Batch bt = kBatchTab[B];
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const int sl[4] = {(n >> 3) & 1, (n >> 2) & 1, (n >> 1) & 1, n & 1};
@@ -492,42 +496,69 @@ __device__ __forceinline__ void lds_add4(double (&dst)[4][kMaxPrim][2], int pos,
 
 // digestion of one RR batch element (CLMMTG :13902-14392, CLMMTZ
 // :13196-13514 when REAL): batches 1-2 into registers (G(A,B) is fixed by
-// ij), 3-4 straight to global memory, exchange batches into LDS
+// ij), 3-4 straight to global memory, exchange batches into LDS.  The density
+// loads of a pair of batches are issued together so their latencies
+// overlap; batches 3-4 read D(A,B)(i,j), which is the same for every kl and
+// comes in preloaded (dab).
 template <bool REAL>
 __device__ __forceinline__ void digest(const Raw &raw,
                                        const int (&off)[4][2], const int (&idx)[4],
                                        const double2 *__restrict__ Dm, int nd, int iflg,
                                        int intsym, double pab1, double pab2, double pcd1,
-                                       double pcd2, double (&accAB)[4][2], XAcc &xa,
-                                       double2 *gdir) {
-  if (iflg & 1) batch_terms<0, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
+                                       double pcd2, const double2 (&dab)[2][4],
+                                       double (&accAB)[4][2], XAcc &xa, double2 *gdir) {
   if (!intsym) iflg &= 0x11;
-  if (iflg & 2) batch_terms<1, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, accAB);
-  if (iflg & 12) {
-    double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    if (iflg & 4) batch_terms<2, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
-    if (iflg & 8) batch_terms<3, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);
+  {
+    double2 d0[4], d1[4];
+    if (iflg & 1) load_d<0, REAL>(off, idx, Dm, nd, d0);
+    if (iflg & 2) load_d<1, REAL>(off, idx, Dm, nd, d1);
+    if (iflg & 12) {
+      double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+      if (iflg & 4) batch_terms<2, REAL>(raw, dab[0], pab1, pab2, pcd1, pcd2, o);
+      if (iflg & 8) batch_terms<3, REAL>(raw, dab[1], pab1, pab2, pcd1, pcd2, o);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int rs = s >> 1, cs = s & 1;
-      if (REAL && rs != cs) continue;
-      atomic_add2(&gdir[(int64_t)(off[3][cs] + idx[3]) * nd + off[2][rs] + idx[2]], o[s][0],
-                  o[s][1]);
+      for (int s = 0; s < 4; ++s) {
+        const int rs = s >> 1, cs = s & 1;
+        if (REAL && rs != cs) continue;
+        atomic_add2(&gdir[(int64_t)(off[3][cs] + idx[3]) * nd + off[2][rs] + idx[2]], o[s][0],
+                    o[s][1]);
+      }
     }
+    if (iflg & 1) batch_terms<0, REAL>(raw, d0, pab1, pab2, pcd1, pcd2, accAB);
+    if (iflg & 2) batch_terms<1, REAL>(raw, d1, pab1, pab2, pcd1, pcd2, accAB);
   }
-#define XB(B, ARR, POS)                                                                     \
-  if (iflg & (1 << B)) {                                                                    \
-    double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};                                      \
-    batch_terms<B, REAL>(raw, off, idx, Dm, nd, pab1, pab2, pcd1, pcd2, o);              \
-    lds_add4<REAL>(ARR, POS, o);                                                            \
+#define XL(B, D) \
+  if (iflg & (1 << B)) load_d<B, REAL>(off, idx, Dm, nd, D);
+#define XB(B, D, ARR, POS)                                               \
+  if (iflg & (1 << B)) {                                                 \
+    double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};                   \
+    batch_terms<B, REAL>(raw, D, pab1, pab2, pcd1, pcd2, o);             \
+    lds_add4<REAL>(ARR, POS, o);                                         \
   }
-  XB(4, xa.l[0], idx[3])
-  XB(5, xa.k[0], idx[2])
-  XB(6, xa.l[1], idx[3])
-  XB(7, xa.k[1], idx[2])
-  XB(8, xa.k[2], idx[2])
-  XB(9, xa.k[3], idx[2])
-  XB(10, xa.l[2], idx[3])
+  {
+    double2 d4[4], d5[4];
+    XL(4, d4) XL(5, d5)
+    XB(4, d4, xa.l[0], idx[3])
+    XB(5, d5, xa.k[0], idx[2])
+  }
+  {
+    double2 d6[4], d7[4];
+    XL(6, d6) XL(7, d7)
+    XB(6, d6, xa.l[1], idx[3])
+    XB(7, d7, xa.k[1], idx[2])
+  }
+  {
+    double2 d8[4], d9[4];
+    XL(8, d8) XL(9, d9)
+    XB(8, d8, xa.k[2], idx[2])
+    XB(9, d9, xa.k[3], idx[2])
+  }
+  {
+    double2 d10[4];
+    XL(10, d10)
+    XB(10, d10, xa.l[2], idx[3])
+  }
+#undef XL
 #undef XB
 }
 
@@ -641,6 +672,7 @@ struct FockShared {
     double eab[ntuv(kMaxLab) * 4];  // E(AB) row of the current AB combination
   };
   uint8_t gm[ntuv(kMaxLab) * 4];    // its G masks
+  double2 dab[2][4];                 // its D(A,B)(i,j), D(B,A)(j,i) slots
   XAcc x;
 };
 static_assert(sizeof(FockShared) + 16 <= 160 * 1024 / 8, "FockShared exceeds 1/8 of the LDS");
@@ -721,6 +753,15 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       s_eab[4 * t + 3] = fabs(ew) > (T)1.0e-16 ? ew : (T)0;
     }
     for (int t = lane; t < 4 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
+    // D(A,B)(i,j) and D(B,A)(j,i) of batches 3 and 4: fixed for the combination
+    // (no density on the RR test path)
+    if (!a.rr_out && lane < 8) {
+      const int b = lane >> 2, sl = lane & 3, rs = sl >> 1, cs = sl & 1;
+      const bool off_diag = a.isym != 0 && rs != cs;
+      const int row = b == 0 ? ab.fa[rs] + ib : ab.fb[rs] + jb;
+      const int col = b == 0 ? ab.fb[cs] + jb : ab.fa[cs] + ib;
+      sh.dab[b][sl] = off_diag ? make_double2(0.0, 0.0) : Dm[(int64_t)col * nd + row];
+    }
     __syncthreads();
     // first contraction G(kl,tuv') = sum_tuv E_AB(tuv) R(tuv+tuv')
     for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
@@ -759,6 +800,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       }
     }
     double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
+    const double2 (&dab)[2][4] = sh.dab;
     for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
       const CDCall cd = a.cds[ci];
       const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
@@ -814,10 +856,10 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
         if (a.isym != 0)
           digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], accAB, sh.x, a.gdir);
+                       cd.pcd[1], dab, accAB, sh.x, a.gdir);
         else
           digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], accAB, sh.x, a.gdir);
+                        cd.pcd[1], dab, accAB, sh.x, a.gdir);
       }
       if (!a.rr_out) {
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
@@ -1102,13 +1144,15 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
         const int idx[4] = {ib, jb, kl / nD, kl % nD};
         const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+        // exchange batches only: the direct batches' accumulators and D(A,B) stay unused
         double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        const double2 dab_unused[2][4] = {};
         if (a.isym != 0)
           digest<true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], acc_unused, sh.x, a.bdir);
+                       cd.pcd[1], dab_unused, acc_unused, sh.x, a.bdir);
         else
           digest<false>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], acc_unused, sh.x, a.bdir);
+                        cd.pcd[1], dab_unused, acc_unused, sh.x, a.bdir);
       }
       if (!a.rr_out) {
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
@@ -1410,10 +1454,25 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
   hipLaunchKernelGGL(dmax_kernel, dim3(256), dim3(256), 0, s, d, n, out);
   return hipGetLastError();
 }
-hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s) {
+hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
+                       hipEvent_t fork, hipEvent_t join) {
+  if (!a.mixed) {
+    hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+  }
+  // the FP32 instance runs beside the FP64 one on its own stream, with the
+  // second half of the slab; s waits for both
+  hipError_t e = hipEventRecord(fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s32, fork, 0);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), 0, s, a);
-  if (a.mixed) hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), 0, s, a);
-  return hipGetLastError();
+  FockArgs b = a;
+  b.slab = a.slab + (int64_t)nblocks * a.slab_stride;
+  hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), 0, s32, b);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(join, s32);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+  return e;
 }
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
   if (a.nitems == 0) return hipSuccess;
