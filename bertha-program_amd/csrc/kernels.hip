@@ -639,12 +639,23 @@ __device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, float &x, flo
 __device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib, int jb, int nC,
                                int nD, double2 *out, int nd, int lane) {
   __syncthreads();
-  for (int w = lane; w < 7 * 4 * kMaxPrim; w += 64) {
-    const int bsel = w / (4 * kMaxPrim), s = (w / kMaxPrim) & 3, pos = w % kMaxPrim;
-    const int b = 4 + bsel;  // batch index 4..10 (5..11)
+  // entries in use: three l-indexed batches x 4 slots x nD, then four
+  // k-indexed batches x 4 slots x nC
+  const int nl = 12 * nD, ntot = nl + 16 * nC;
+  for (int w = lane; w < ntot; w += 64) {
+    int b, s, pos;
+    if (w < nl) {
+      const int li = w / (4 * nD), r = w - li * 4 * nD;
+      s = r / nD;
+      pos = r - s * nD;
+      b = li == 0 ? 4 : (li == 1 ? 6 : 10);
+    } else {
+      const int ki = (w - nl) / (4 * nC), r = (w - nl) - ki * 4 * nC;
+      s = r / nC;
+      pos = r - s * nC;
+      b = ki == 0 ? 5 : ki + 6;
+    }
     if (!((flg >> b) & 1)) continue;
-    const bool isl = (b == 4 || b == 6 || b == 10);
-    if (pos >= (isl ? nD : nC)) continue;
     double *v;
     int row, col;
     const int rs = s >> 1, cs = s & 1;
