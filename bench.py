@@ -32,6 +32,21 @@ METRIC = "Fock builds/s, mixed vs FP64 at |ΔE|<1µEh; served SCF p50/p99 latenc
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
 
+def profiled_traffic(workload: str):
+    """HBM bytes per fock_kernel launch for this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against kernels of
+    known traffic); None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"fock_kernel_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +62,9 @@ def main() -> int:
                     help="requests of the served-SCF leg (0: skip the leg)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a kernel of known traffic that "
+                         "calibrates FETCH_SIZE/WRITE_SIZE counters in a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU design (partial J/K "
@@ -67,6 +85,17 @@ def main() -> int:
     dens = S.random_density(n, seed=11, real=(system.shape != "NOSYMM"))
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        # a fill kernel (writes nbytes) and an elementwise scale (reads and
+        # writes nbytes each): their counter readings give the bytes-per-count
+        # factors of the profiled run
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        print(f"copy calibration: fill and scale of {nbytes} bytes", file=sys.stderr, flush=True)
+        del src, dst
     # column-major matrices: a row-major torch tensor of D^T has D's layout
     d_t = torch.from_numpy(np.ascontiguousarray(dens.T)).to(dev)
     g_t = torch.zeros((n, n), dtype=torch.complex128, device=dev)
@@ -190,7 +219,8 @@ def main() -> int:
                    "quartets_per_build": quartets, "work_items": items,
                    "host_wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": profiled_traffic(args.workload),
                      "kernel": "fock_kernel", "kernel_ms": kernel_ms,
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,

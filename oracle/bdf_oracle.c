@@ -36,6 +36,25 @@
  * built for the first CD call's compacted kl list (IMAP) in later calls whose
  * IMAP differs (IGAB, 12678); here every call sees its own kl's integrals, as
  * the screening intends.  Open-shell matrices are not restated.
+ *   GDSC is pinned to the reference's ERIFAST self-pair batches (the
+ *   reference's own CLMDIAG returns zeros in a fresh process: ERI's MAXN is
+ *   unset before the first SCHWARZ call); the (ab|ba) bound table by the
+ *   Cauchy-Schwarz inequality on random batches.  The SCHWARZ masks have no
+ *   runnable reference counterpart: parity unpinned (monotonicity in SENS
+ *   only).
+ *
+ * Breit (BRTFAST 17060-17670, EILSGN 33644 / EQLS 34937, EILSCD0 17997,
+ * BIIFAST 18337-19169, BRTMTZ 19172 / BRTMTG 19815), including the reference's
+ * per-class reuse of the R-column flags of the class's last primitive pair:
+ * pinned to oracle/_ref (EILS blocks, single BIIFAST batches for Breit and
+ * Gaunt, whole builds) and golden vectors (tests/golden/breit_vectors.npz).
+ *
+ * One-electron matrices (OVRLP 6683, ONEEL 6996-8154; point and Gaussian
+ * nuclei): pinned to oracle/_ref.  The SCF driver above the engine is not
+ * restated here; its converged energy is pinned only against the same driver
+ * run on this oracle's matrices and the Ne literature values (parity
+ * unpinned against the reference's HFSCF, whose ZHEGV/ZGGLSE come from an
+ * unpinned LAPACK).
  */
 #include "bdf_oracle.h"
 
