@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -257,6 +259,24 @@ static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double>
       }
     qcd[c] = m;
   }
+  // FP32 range guard: before the prefactor an item's R batch grows like
+  // (2 alpha max(1, |PQ|))^L, alpha = pq/(p+q), and tight pairs also carry
+  // large Hermite expansions; an item whose growth exceeds 1e20 stays in FP64
+  // whatever its Schwarz factor (FP32 overflows at 3.4e38).
+  std::vector<double> qmax(P.classes.size(), 0.0), dmax(P.classes.size(), 1.0);
+  const std::vector<double> &X = e->B.exps;
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    for (int k = 0; k < cl.nbas[2]; ++k)
+      for (int l = 0; l < cl.nbas[3]; ++l)
+        qmax[c] = std::max(qmax[c], X[cl.exp[2] + k] + X[cl.exp[3] + l]);
+    for (int u = 0; u < 4; ++u)
+      for (int v = 0; v < 4; ++v) {
+        double dx = cl.xyz[u][0] - cl.xyz[v][0], dy = cl.xyz[u][1] - cl.xyz[v][1];
+        double dz = cl.xyz[u][2] - cl.xyz[v][2];
+        dmax[c] = std::max(dmax[c], std::sqrt(dx * dx + dy * dy + dz * dz));
+      }
+  }
   q.resize(P.items.size());
   for (size_t i = 0; i < P.items.size(); ++i) {
     const WorkItem &it = P.items[i];
@@ -268,6 +288,10 @@ static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double>
       for (int s = 0; s < 4; ++s) m = std::max(m, at(ab.fa[s >> 1] + ib, ab.fb[s & 1] + jb));
     }
     q[i] = m * qcd[it.cls];
+    const double p = X[cl.exp[0] + ib] + X[cl.exp[1] + jb];
+    const double alpha = p * qmax[it.cls] / (p + qmax[it.cls]);
+    const double growth = cl.lamfl * std::log10(std::max(1.0, 2.0 * alpha) * dmax[it.cls]);
+    if (growth > 20.0) q[i] = std::numeric_limits<double>::infinity();
   }
 }
 

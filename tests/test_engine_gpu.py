@@ -171,6 +171,30 @@ def test_mixed_precision_split(bdf, orc, systems, name):
     assert abs(_e2(d, g, x) - e64) < 1e-6
 
 
+def test_mixed_precision_tight_exponents(bdf, orc):
+    """Exponents up to 5e6 (s) and 5e3 (d): R batches of tight quartets
+    exceed the FP32 range before the prefactor, so those items must stay in
+    FP64 whatever their Schwarz bound; every other item may run in FP32."""
+    S = bdf.systems
+    ev = S.even_tempered
+    s = S.System("tight-pair", np.array([[0, 0, 0.0], [0.0, 0.4, 2.3]]),
+                 [[ev(0.5, 6.0, 9), ev(0.5, 5.0, 4), ev(50.0, 10.0, 3)],
+                  [ev(0.3, 4.0, 4), ev(0.5, 3.0, 2)]], shape="NOSYMM")
+    print("[mixed] tight-pair", flush=True)
+    d = S.random_density(s.ndim, seed=8)
+    go, xo = orc.Oracle(s).clmfast(d, 3, nthreads=8)
+    e64 = _e2(d, go, xo)
+    scale = np.abs(go).max()
+    with bdf.Engine(s, precision="mixed", split_threshold=1e300) as eng:
+        g, x = eng.clmfast(d, 3)
+    assert np.isfinite(g).all() and np.isfinite(x).all()
+    assert np.abs(g - go).max() < 1e-4 * scale
+    assert np.abs(x - xo).max() < 1e-4 * scale
+    with bdf.Engine(s, precision="mixed", split_threshold=1e-5) as eng:
+        g, x = eng.clmfast(d, 3)
+    assert abs(_e2(d, g, x) - e64) < 1e-6
+
+
 @pytest.mark.parametrize("name", ["ne", "off", "tri"])
 def test_one_electron_matches_oracle(bdf, orc, systems, name):
     s = systems[name]
