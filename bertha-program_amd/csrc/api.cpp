@@ -336,6 +336,9 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.E32 = e->E32.p;
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
+  a.xacc_n = 0;
+  for (const ClassDesc &cl : dp.P.classes)
+    a.xacc_n = std::max(a.xacc_n, 2 * (12 * cl.nbas[3] + 16 * cl.nbas[2]));
   int nb = std::min(e->nblocks, std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));

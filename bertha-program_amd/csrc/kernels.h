@@ -58,6 +58,9 @@ struct FockArgs {
   // E(CD) with the column masks and ERIFAST's Z sign applied (FP64, FP32)
   const double4 *Ecd;
   const float4 *Ecd32;
+  // exchange accumulators in dynamic LDS: 2 (12 nD + 16 nC) doubles for the
+  // plan's largest class
+  int xacc_n;
 };
 
 struct DiagArgs {

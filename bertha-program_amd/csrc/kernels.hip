@@ -476,21 +476,27 @@ __device__ __forceinline__ void batch_terms(const Raw &raw, const double2 (&d)[4
   }
 }
 
-// LDS accumulators of exchange batches whose output varies only with l
-// (batches 5, 7, 11) or only with k (6, 8, 9, 10) inside one call
-struct XAcc {
-  double l[3][4][kMaxPrim][2];
-  double k[4][4][kMaxPrim][2];
+// LDS accumulators of the exchange batches of one call, complex, packed for
+// the class at hand: the batches whose output varies only with l (5, 7, 11)
+// as [3][4 slots][nD], then those varying only with k (6, 8, 9, 10) as
+// [4][4 slots][nC]; 2 * (12 nD + 16 nC) doubles.
+struct XView {
+  double *p;
+  int nC, nD;
 };
+constexpr int kXAccMax = 2 * (12 * kMaxPrim + 16 * kMaxPrim);  // doubles, any class
+__device__ __forceinline__ int xacc_doubles(int nC, int nD) { return 2 * (12 * nD + 16 * nC); }
 
 template <bool REAL>
-__device__ __forceinline__ void lds_add4(double (&dst)[4][kMaxPrim][2], int pos,
+__device__ __forceinline__ void lds_add4(const XView &x, bool kbatch, int which, int pos,
                                          const double (&o)[4][2]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (REAL && (s == 1 || s == 2)) continue;
-    if (o[s][0] != 0.0) unsafeAtomicAdd(&dst[s][pos][0], o[s][0]);
-    if (!REAL && o[s][1] != 0.0) unsafeAtomicAdd(&dst[s][pos][1], o[s][1]);
+    double *d = kbatch ? x.p + 2 * (12 * x.nD + (which * 4 + s) * x.nC + pos)
+                       : x.p + 2 * ((which * 4 + s) * x.nD + pos);
+    if (o[s][0] != 0.0) unsafeAtomicAdd(d, o[s][0]);
+    if (!REAL && o[s][1] != 0.0) unsafeAtomicAdd(d + 1, o[s][1]);
   }
 }
 
@@ -506,7 +512,7 @@ __device__ __forceinline__ void digest(const Raw &raw,
                                        const double2 *__restrict__ Dm, int nd, int iflg,
                                        int intsym, double pab1, double pab2, double pcd1,
                                        double pcd2, const double2 (&dab)[2][4],
-                                       double (&accAB)[4][2], XAcc &xa, double2 *gdir) {
+                                       double (&accAB)[4][2], const XView &xa, double2 *gdir) {
   if (!intsym) iflg &= 0x11;
   {
     double2 d0[4], d1[4];
@@ -529,34 +535,34 @@ __device__ __forceinline__ void digest(const Raw &raw,
   }
 #define XL(B, D) \
   if (iflg & (1 << B)) load_d<B, REAL>(off, idx, Dm, nd, D);
-#define XB(B, D, ARR, POS)                                               \
+#define XB(B, D, KB, W, POS)                                             \
   if (iflg & (1 << B)) {                                                 \
     double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};                   \
     batch_terms<B, REAL>(raw, D, pab1, pab2, pcd1, pcd2, o);             \
-    lds_add4<REAL>(ARR, POS, o);                                         \
+    lds_add4<REAL>(xa, KB, W, POS, o);                                   \
   }
   {
     double2 d4[4], d5[4];
     XL(4, d4) XL(5, d5)
-    XB(4, d4, xa.l[0], idx[3])
-    XB(5, d5, xa.k[0], idx[2])
+    XB(4, d4, false, 0, idx[3])
+    XB(5, d5, true, 0, idx[2])
   }
   {
     double2 d6[4], d7[4];
     XL(6, d6) XL(7, d7)
-    XB(6, d6, xa.l[1], idx[3])
-    XB(7, d7, xa.k[1], idx[2])
+    XB(6, d6, false, 1, idx[3])
+    XB(7, d7, true, 1, idx[2])
   }
   {
     double2 d8[4], d9[4];
     XL(8, d8) XL(9, d9)
-    XB(8, d8, xa.k[2], idx[2])
-    XB(9, d9, xa.k[3], idx[2])
+    XB(8, d8, true, 2, idx[2])
+    XB(9, d9, true, 3, idx[2])
   }
   {
     double2 d10[4];
     XL(10, d10)
-    XB(10, d10, xa.l[2], idx[3])
+    XB(10, d10, false, 2, idx[3])
   }
 #undef XL
 #undef XB
@@ -636,11 +642,10 @@ __device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, float &x, flo
 
 // Flush the exchange accumulators of one call into the exchange matrix (one
 // atomic per output element instead of one per quartet), then clear them.
-__device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib, int jb, int nC,
-                               int nD, double2 *out, int nd, int lane) {
+__device__ void flush_exchange(const XView &x, int flg, const int (&offs)[4][2], int ib, int jb,
+                               double2 *out, int nd, int lane) {
   __syncthreads();
-  // entries in use: three l-indexed batches x 4 slots x nD, then four
-  // k-indexed batches x 4 slots x nC
+  const int nC = x.nC, nD = x.nD;
   const int nl = 12 * nD, ntot = nl + 16 * nC;
   for (int w = lane; w < ntot; w += 64) {
     int b, s, pos;
@@ -656,17 +661,17 @@ __device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib
       b = ki == 0 ? 5 : ki + 6;
     }
     if (!((flg >> b) & 1)) continue;
-    double *v;
+    double *v = x.p + 2 * w;
     int row, col;
     const int rs = s >> 1, cs = s & 1;
     switch (b) {
-      case 4: v = x.l[0][s][pos]; row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
-      case 5: v = x.k[0][s][pos]; row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
-      case 6: v = x.l[1][s][pos]; row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
-      case 7: v = x.k[1][s][pos]; row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
-      case 8: v = x.k[2][s][pos]; row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
-      case 9: v = x.k[3][s][pos]; row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
-      default: v = x.l[2][s][pos]; row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
+      case 4: row = offs[0][rs] + ib; col = offs[3][cs] + pos; break;
+      case 5: row = offs[0][rs] + ib; col = offs[2][cs] + pos; break;
+      case 6: row = offs[1][rs] + jb; col = offs[3][cs] + pos; break;
+      case 7: row = offs[1][rs] + jb; col = offs[2][cs] + pos; break;
+      case 8: row = offs[2][rs] + pos; col = offs[1][cs] + jb; break;
+      case 9: row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
+      default: row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
     }
     if (v[0] != 0.0 || v[1] != 0.0) atomic_add2(&out[(int64_t)col * nd + row], v[0], v[1]);
     v[0] = 0.0;
@@ -675,8 +680,7 @@ __device__ void flush_exchange(XAcc &x, int flg, const int (&offs)[4][2], int ib
   __syncthreads();
 }
 
-// Shared-memory state of one workgroup (kept under 20 KB so that eight
-// workgroups fit a CU's 160 KB LDS)
+// Shared-memory state of one workgroup
 struct FockShared {
   union {
     uint8_t cflag[ntuv(kMaxLfl)];  // R column flags (until the prefactor pass)
@@ -684,26 +688,31 @@ struct FockShared {
   };
   uint8_t gm[ntuv(kMaxLab) * 4];    // its G masks
   double2 dab[2][4];                 // its D(A,B)(i,j), D(B,A)(j,i) slots
-  XAcc x;
 };
-static_assert(sizeof(FockShared) + 16 <= 160 * 1024 / 8, "FockShared exceeds 1/8 of the LDS");
+// the exchange accumulators follow in dynamic LDS, sized for the plan's
+// largest class: eight workgroups per CU fit even at 32 primitives per shell
+static_assert(sizeof(FockShared) + 16 + kXAccMax * sizeof(double) <= 160 * 1024 / 8,
+              "fock_kernel's LDS exceeds 1/8 of a CU's");
 
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
 // E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
 // accumulates in FP64.
 template <typename T>
 __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, FockShared &sh,
-                          int lane) {
-  const int ld = a.ld;
+                          double *xacc, int lane) {
+  const ClassDesc &cl = a.classes[it.cls];
+  // slab rows packed at the class's own kl count, so the rows a workgroup
+  // touches occupy as few cache lines as possible
+  const int ld = cl.maxcd;
   T *R = reinterpret_cast<T *>(slab);            // [rows][ld]
   T *F = R + (int64_t)ntuv(kMaxLfl) * ld;         // [kMaxLfl+1][ld]
   T *G = F + (int64_t)(kMaxLfl + 1) * ld;         // [ntg*4][ld]
   T *s_eab = reinterpret_cast<T *>(sh.eab);
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
-  const ClassDesc &cl = a.classes[it.cls];
   const int nB = cl.nbas[1], nD = cl.nbas[3];
   const int ib = it.ij / nB, jb = it.ij % nB;
+  const XView xv{xacc, cl.nbas[2], nD};
   const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
   const double p = ea + eb;
   const double Px = (cl.xyz[0][0] * ea + cl.xyz[1][0] * eb) / p;
@@ -867,16 +876,16 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
         if (a.isym != 0)
           digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], dab, accAB, sh.x, a.gdir);
+                       cd.pcd[1], dab, accAB, xv, a.gdir);
         else
           digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], dab, accAB, sh.x, a.gdir);
+                        cd.pcd[1], dab, accAB, xv, a.gdir);
       }
       if (!a.rr_out) {
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        flush_exchange(sh.x, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, cl.nbas[2],
-                       nD, a.gxch, nd, lane);
+        flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.gxch, nd,
+                       lane);
       }
     }
     if (!a.rr_out) {
@@ -904,14 +913,12 @@ template <typename T>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) fock_kernel(FockArgs a) {
   __shared__ FockShared sh;
   __shared__ int s_item;
+  extern __shared__ double xacc[];  // a.xacc_n doubles (launch_fock)
   constexpr bool kLow = sizeof(T) == sizeof(float);
   const int lane = threadIdx.x;
   double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
   int *counter = a.counter + (kLow ? 1 : 0);
-  {
-    double *z = &sh.x.l[0][0][0][0];
-    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
-  }
+  for (int w = lane; w < a.xacc_n; w += 64) xacc[w] = 0.0;
   const double dmax = a.mixed ? __longlong_as_double(*a.dmax_bits) : 0.0;
   __syncthreads();
   for (;;) {
@@ -922,7 +929,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     if (idx >= a.nitems) break;
     const bool low = a.mixed && a.qitem[idx] * dmax < a.split;
     if (low != kLow) continue;
-    fock_item<T>(a, a.items[idx], slab, sh, lane);
+    fock_item<T>(a, a.items[idx], slab, sh, xacc, lane);
   }
 }
 
@@ -970,11 +977,11 @@ __device__ void r_flags(const double *R, int ld, int nt, int maxcd, uint8_t *fla
 // the class's last pair ij = MAXAB: computed here, one class per pass.
 __global__ void __launch_bounds__(64) breit_flags_kernel(BreitArgs a) {
   const int lane = threadIdx.x;
-  const int ld = a.ld;
   double *R = a.slab + (int64_t)blockIdx.x * a.slab_stride;
-  double *F = R + (int64_t)ntuv(kMaxLfl) * ld;
   for (int c = blockIdx.x; c < a.nclasses; c += gridDim.x) {
     const ClassDesc cl = a.classes[c];
+    const int ld = cl.maxcd;
+    double *F = R + (int64_t)ntuv(kMaxLfl) * ld;
     r_columns(cl, a.exps, cl.nbas[0] * cl.nbas[1] - 1, R, F, ld, cl.lamfl, lane);
     __syncthreads();
     r_flags(R, ld, ntuv(cl.lamfl), cl.maxcd, a.stale + cl.flagoff, lane);
@@ -988,7 +995,7 @@ struct BreitShared {
   uint8_t gflag[ntuv(kMaxLfl)];
   uint8_t gm[ntuv(2 * kMaxL + 1) * 12];
   uint8_t km[ntuv(2 * kMaxL + 1) * 12];
-  XAcc x;
+  double xacc[kXAccMax];  // exchange accumulators (XView)
 };
 
 __device__ __forceinline__ int any4(const double *e) {
@@ -1002,14 +1009,15 @@ __device__ __forceinline__ int any4(const double *e) {
 // (:18830-19150) and the exchange digestion into BXCH.
 __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                            BreitShared &sh, int lane) {
-  const int ld = a.ld;
+  const ClassDesc &cl = a.classes[it.cls];
+  const int ld = cl.maxcd;  // slab rows packed at the class's kl count
   double *R = slab;
   double *F = R + (int64_t)ntuv(kMaxLfl) * ld;
   double *G = F + (int64_t)(kMaxLfl + 1) * ld;
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
-  const ClassDesc &cl = a.classes[it.cls];
   const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const XView xv{sh.xacc, cl.nbas[2], nD};
   const int ib = it.ij / nB, jb = it.ij % nB;
   const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
   const double p = ea + eb;
@@ -1160,16 +1168,15 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
         const double2 dab_unused[2][4] = {};
         if (a.isym != 0)
           digest<true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], dab_unused, acc_unused, sh.x, a.bdir);
+                       cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
         else
           digest<false>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], dab_unused, acc_unused, sh.x, a.bdir);
+                        cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
       }
       if (!a.rr_out) {
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        flush_exchange(sh.x, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, cl.nbas[2], nD,
-                       a.bxch, nd, lane);
+        flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.bxch, nd, lane);
       }
     }
     __syncthreads();
@@ -1181,10 +1188,7 @@ __global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
   __shared__ int s_item;
   const int lane = threadIdx.x;
   double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
-  {
-    double *z = &sh.x.l[0][0][0][0];
-    for (int w = lane; w < (int)(sizeof(XAcc) / sizeof(double)); w += 64) z[w] = 0.0;
-  }
+  for (int w = lane; w < kXAccMax; w += 64) sh.xacc[w] = 0.0;
   __syncthreads();
   for (;;) {
     if (lane == 0) s_item = atomicAdd(a.counter, 1);
@@ -1468,7 +1472,8 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
                        hipEvent_t fork, hipEvent_t join) {
   if (!a.mixed) {
-    hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s,
+                       a);
     return hipGetLastError();
   }
   // the FP32 instance runs beside the FP64 one on its own stream, with the
@@ -1476,10 +1481,12 @@ hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_
   hipError_t e = hipEventRecord(fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(s32, fork, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s,
+                     a);
   FockArgs b = a;
   b.slab = a.slab + (int64_t)nblocks * a.slab_stride;
-  hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), 0, s32, b);
+  hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s32,
+                     b);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipEventRecord(join, s32);
   if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
