@@ -8,7 +8,9 @@ NDIM 380, seeded geometry).  Prints one JSON line.
 
 Legs:
   timed     K builds back to back on the engine's stream (HIP events and a
-            device synchronisation on both sides)
+            device synchronisation on both sides), FP64 and then mixed
+            precision; the headline value is the mixed build when its
+            two-electron energy is within 1 microhartree of the FP64 build's
   kernel    average duration of the quartet kernel (HIP events bracketing it
             on the engine's stream) -> roofline
   cpu       the oracle (C restatement of the reference path) timed on the
@@ -202,26 +204,31 @@ def main() -> int:
                    "mix": "60% atoms Z in {2,4,10,12,18,20,36,54}, 40% diatomics of the "
                           "same elements, R ~ U(2.5, 8.0) bohr, Poisson arrivals"}
 
+    # headline: the mixed build when it meets the metric's |dE| < 1 microhartree
+    # against the FP64 build, else the FP64 build
+    headline = mixed if (mixed is not None and mixed["abs_dE2e_vs_fp64_Eh"] < 1e-6) else None
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": headline["value"] if headline else value,
         "unit": "Fock builds/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": headline["ms_per_step"] if headline else ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32 (Schwarz split)" if headline else "f64",
         "data": "synthetic Hermitian density; even-tempered basis and seeded geometry of the workload",
-        "config": {"workload": system.name, "ndim": n, "precision": "fp64", "stage": 3,
-                   "quartets_per_build": quartets, "work_items": items,
+        "config": {"workload": system.name, "ndim": n,
+                   "precision": (f"mixed, FP32 below split {args.split}" if headline else "fp64"),
+                   "stage": 3, "quartets_per_build": quartets, "work_items": items,
                    "host_wall_s": wall},
+        "fp64": {"value": value, "ms_per_step": ms_per_step},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic(args.workload),
-                     "kernel": "fock_kernel", "kernel_ms": kernel_ms,
+                     "kernel": "fock_kernel<double> (FP64 build)", "kernel_ms": kernel_ms,
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
         "parity": parity,

@@ -128,6 +128,7 @@ class DIIS:
         self.err = [None] * depth
         self.fck = [None] * depth
         self.bsav = np.zeros((depth, depth), dtype=np.complex128)
+        self.last_errmax = float("inf")
 
     def reset(self):
         self.bsav[:] = 0.0
@@ -137,7 +138,6 @@ class DIIS:
             self.reset()
         p = dens.conj()
         evec = fock @ p @ ovlp - ovlp @ p @ fock
-        errmax = float(evec.abs().max())
         klc = nmlev % self.msv
         if klc == 0:
             klc = self.msv
@@ -145,8 +145,13 @@ class DIIS:
         k = klc - 1
         self.err[k] = evec
         self.fck[k] = fock
+        # max|error| and the new row of B in one device-to-host transfer
+        row = torch.stack([torch.sum(self.err[m].conj() * evec) for m in range(kss)])
+        vals = torch.cat([evec.abs().max().reshape(1).to(row.dtype), row]).cpu().numpy()
+        errmax = float(vals[0].real)
+        self.last_errmax = errmax
         for m in range(kss):
-            v = complex(torch.sum(self.err[m].conj() * evec))
+            v = complex(vals[1 + m])
             self.bsav[m, k] = v
             self.bsav[k, m] = np.conj(v)
         if errmax > ERRCOM or kss <= 2:
@@ -166,30 +171,33 @@ class DIIS:
         return out
 
 
-def _real_trace(a, b) -> float:
-    # sum_ij A_ij B_ij, the reference's energy contraction (ENRGIES :3745)
-    return float(torch.sum(a * b).real)
-
-
 def energies(system: System, dens, hkin, hnuc, gdir, gxch, bdir=None, bxch=None) -> Energies:
-    e = Energies(nuclear=nuclear_repulsion(system),
-                 kinetic=_real_trace(dens, hkin),
-                 potential=_real_trace(dens, hnuc),
-                 coulomb_direct=0.5 * _real_trace(dens, gdir),
-                 coulomb_exchange=0.5 * _real_trace(dens, gxch))
+    # the reference's contraction sum_ij D_ij M_ij (ENRGIES :3745) of every
+    # matrix, fetched from the device in one transfer
+    mats = [hkin, hnuc, gdir, gxch] + ([bdir, bxch] if bdir is not None else [])
+    t = torch.stack([torch.sum(dens * m).real for m in mats]).cpu().tolist()
+    e = Energies(nuclear=nuclear_repulsion(system), kinetic=t[0], potential=t[1],
+                 coulomb_direct=0.5 * t[2], coulomb_exchange=0.5 * t[3])
     if bdir is not None:
-        e.breit_direct = 0.5 * _real_trace(dens, bdir)
-        e.breit_exchange = 0.5 * _real_trace(dens, bxch)
+        e.breit_direct = 0.5 * t[4]
+        e.breit_exchange = 0.5 * t[5]
     return e
 
 
-def solve_fock(fock, ovlp):
-    """ZHEGV(1,'V','L'): F C = S C e from the lower triangles."""
-    f = torch.tril(fock) + torch.tril(fock, -1).conj().T
+def orthogonaliser(ovlp):
+    """L^-1 of ZHEGV's Cholesky factor S = L L^H, from the lower triangle of S
+    (once per geometry: S does not change between iterations)."""
     s = torch.tril(ovlp) + torch.tril(ovlp, -1).conj().T
     L = torch.linalg.cholesky(s)
-    li = torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype, device=L.device),
-                                       upper=False)
+    return torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype, device=L.device),
+                                         upper=False)
+
+
+def solve_fock(fock, li):
+    """ZHEGV(1,'V','L'): F C = S C e from the lower triangle of F, with
+    li = orthogonaliser(S).  Returns (e, C, finite) where finite is a device
+    flag telling whether every eigenvector is finite."""
+    f = torch.tril(fock) + torch.tril(fock, -1).conj().T
     fp = li @ f @ li.conj().T
     if fp.is_cuda:
         # the default device eigensolver returns non-finite eigenvectors for
@@ -202,15 +210,18 @@ def solve_fock(fock, ovlp):
             torch.backends.cuda.preferred_linalg_library(prev)
     else:
         w, v = torch.linalg.eigh(fp)
-    if not bool(torch.isfinite(v).all()):
-        raise RuntimeError("eigensolver returned non-finite eigenvectors")
-    return w, li.conj().T @ v
+    return w, li.conj().T @ v, torch.isfinite(v).all()
 
 
 def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev: int = 1,
             diis_depth: int = 6, max_iter: int = 100, cv: float = 137.035999084,
-            verbose: bool = False) -> SCFResult:
-    """HFSCF for a closed shell of nocc electrons (NCLS = NOCC, NOPN = 0)."""
+            damping: float = 0.0, verbose: bool = False) -> SCFResult:
+    """HFSCF for a closed shell of nocc electrons (NCLS = NOCC, NOPN = 0).
+
+    damping > 0 (not in the reference) mixes that fraction of the previous
+    density into the new one until the DIIS error first falls to ERRCOM, and
+    never after: it stops the two-state oscillations a bare-nucleus start can
+    fall into (3d below 4s in Ca, heavy diatomics at stage 2)."""
     n, nskp = backend.n, backend.nskp
     dev = backend.dev
     breit = system.hmlt.upper() == "DHFB"
@@ -227,6 +238,7 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
     shlv = shifts[ilev - 1]
     ovlp, hnuc, hkin = backend.one_electron(cv)
     h1 = hnuc + hkin
+    li = orthogonaliser(ovlp)
     dens = torch.zeros((n, n), dtype=torch.complex128, device=dev)
     dtmp = torch.zeros_like(dens)
     coef = None
@@ -257,12 +269,21 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
             fock = fock + shlv * (a @ a.conj().T)
         nmlev[ilev] += 1
         fock = diis(nmlev[ilev], fock, dens, ovlp)
-        eig, coef = solve_fock(fock, ovlp)
+        eig, coef, finite = solve_fock(fock, li)
         occ = coef[:, nskp:nskp + nocc]
+        dprev = dens
         dens = occ.conj() @ occ.T  # D_ij = sum_o conj(C_io) C_jo
+        if it > 1 and diis.last_errmax <= ERRCOM:
+            damping = 0.0  # DIIS has engaged: no more damping in this solve
+        if damping > 0.0 and it > 1:
+            dens = (1.0 - damping) * dens + damping * dprev
         wedn.append(abs(esav[-1] - etot) / (abs(etot) + 1.0))
         esav.append(etot)
-        dnrm = float(torch.sqrt(torch.sum((dens - dtmp).abs() ** 2))) / float(n * n)
+        chk = torch.stack([torch.sqrt(torch.sum((dens - dtmp).abs() ** 2)),
+                           finite.to(torch.float64)]).cpu().tolist()
+        if chk[1] == 0.0:
+            raise RuntimeError("eigensolver returned non-finite eigenvectors")
+        dnrm = chk[0] / float(n * n)
         dtmp = dens.clone()
         if dnrm <= 1.0e-9:
             enrglv2 = 1.0e2 * dnrm

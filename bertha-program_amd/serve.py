@@ -83,7 +83,8 @@ class Server:
     """Continuous-batching DHF server on one device."""
 
     def __init__(self, pkg, slots: int = 4, hbm_bytes: Optional[int] = None,
-                 workgroups: int = 256, device: str = "cuda:0", max_iter: int = 100):
+                 workgroups: int = 256, device: str = "cuda:0", max_iter: int = 100,
+                 damping: float = 0.5):
         self.pkg = pkg
         self.slots = slots
         self.workgroups = workgroups
@@ -93,6 +94,9 @@ class Server:
             hbm_bytes = int(0.8 * free)
         self.budget = hbm_bytes
         self.max_iter = max_iter
+        # density damping before DIIS engages (see scf.run_scf): bare-nucleus
+        # starts of Ca- and Kr-containing requests oscillate without it
+        self.damping = damping
         self._lock = threading.Condition()
         self._pending: collections.deque = collections.deque()
         self._used = 0
@@ -135,7 +139,8 @@ class Server:
                     be = _scf.EngineBackend(self.pkg, req.system, device=str(self.device),
                                             workgroups=self.workgroups)
                     try:
-                        r = _scf.run_scf(req.system, req.nocc, be, max_iter=self.max_iter)
+                        r = _scf.run_scf(req.system, req.nocc, be, max_iter=self.max_iter,
+                                         damping=self.damping)
                         energy, iters, ok = r.energy, r.iterations, r.converged
                     finally:
                         be.close()
