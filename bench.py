@@ -63,7 +63,7 @@ def main() -> int:
     ap.add_argument("--serve", type=int, default=32,
                     help="requests of the served-SCF leg (0: skip the leg)")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a kernel of known traffic that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters in a profiled run)")
@@ -190,7 +190,7 @@ def main() -> int:
         import importlib
         sv = importlib.import_module(pkg.__name__ + ".serve")
         reqs = sv.request_mix(args.serve, seed=args.seed, rate=args.serve_rate)
-        server = sv.Server(pkg, slots=args.serve_slots, workgroups=256)
+        server = sv.Server(pkg, slots=args.serve_slots)
         t0 = time.perf_counter()
         res = server.run(reqs)
         span = time.perf_counter() - t0

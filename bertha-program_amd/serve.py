@@ -83,11 +83,16 @@ class Server:
     """Continuous-batching DHF server on one device."""
 
     def __init__(self, pkg, slots: int = 4, hbm_bytes: Optional[int] = None,
-                 workgroups: int = 256, device: str = "cuda:0", max_iter: int = 100,
+                 workgroups: int = 0, device: str = "cuda:0", max_iter: int = 100,
                  damping: float = 0.5):
         self.pkg = pkg
         self.slots = slots
+        # persistent workgroups per solve's engine (0: the engine's default,
+        # eight per CU): every solve may use the whole device, and solves in
+        # flight together share it through the hardware queues
         self.workgroups = workgroups
+        self._wg_count = workgroups or 8 * torch.cuda.get_device_properties(
+            torch.device(device)).multi_processor_count
         self.device = torch.device(device)
         if hbm_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -165,7 +170,7 @@ class Server:
             delay = req.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            need = estimate_bytes(req.system, self.workgroups)
+            need = estimate_bytes(req.system, self._wg_count)
             with self._lock:
                 self._pending.append((req, need))
                 self._lock.notify_all()
