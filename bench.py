@@ -56,6 +56,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="water", choices=["water", "ne", "tlf", "ring"])
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--stage", type=int, default=3, choices=[1, 2, 3],
+                    help="integral-inclusion stage of every leg (ILEV; 3 = full Dirac-Coulomb)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--split", type=float, default=0.1,
@@ -83,7 +85,7 @@ def main() -> int:
     system = S.config_system(args.workload, **kw)
     eng = pkg.Engine(system)
     n = eng.ndim
-    flops, quartets, items = eng.plan_info(3)
+    flops, quartets, items = eng.plan_info(args.stage)
     dens = S.random_density(n, seed=11, real=(system.shape != "NOSYMM"))
 
     dev = torch.device("cuda:0")
@@ -106,7 +108,7 @@ def main() -> int:
     torch.cuda.synchronize()
 
     def build():
-        eng.clmfast_device(d_t.data_ptr(), g_t.data_ptr(), x_t.data_ptr(), 3)
+        eng.clmfast_device(d_t.data_ptr(), g_t.data_ptr(), x_t.data_ptr(), args.stage)
 
     for _ in range(args.warmup):
         build()
@@ -145,7 +147,7 @@ def main() -> int:
         xm_t = torch.zeros_like(x_t)
 
         def build_m():
-            eng_m.clmfast_device(d_t.data_ptr(), gm_t.data_ptr(), xm_t.data_ptr(), 3)
+            eng_m.clmfast_device(d_t.data_ptr(), gm_t.data_ptr(), xm_t.data_ptr(), args.stage)
 
         for _ in range(args.warmup):
             build_m()
@@ -173,10 +175,10 @@ def main() -> int:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         orc = pyoracle.Oracle(system)
         t0 = time.perf_counter()
-        go, xo = orc.clmfast(dens, 3, nthreads=threads)
+        go, xo = orc.clmfast(dens, args.stage, nthreads=threads)
         tc = time.perf_counter() - t0
         cpu = {"value": 1.0 / tc, "unit": "Fock builds/s", "cores": threads, "kind": "port",
-               "sample": f"one full stage-3 Dirac-Coulomb build of the {system.name} workload "
+               "sample": f"one full stage-{args.stage} Dirac-Coulomb build of the {system.name} workload "
                          f"(NDIM {n}), same density, oracle/bdf_oracle.c with OpenMP over centre "
                          f"quadruples: {tc:.2f} s"}
         de = 0.5 * np.real(np.sum(dens * ((g - x) - (go - xo))))
@@ -222,7 +224,7 @@ def main() -> int:
         "data": "synthetic Hermitian density; even-tempered basis and seeded geometry of the workload",
         "config": {"workload": system.name, "ndim": n,
                    "precision": (f"mixed, FP32 below split {args.split}" if headline else "fp64"),
-                   "stage": 3, "quartets_per_build": quartets, "work_items": items,
+                   "stage": args.stage, "quartets_per_build": quartets, "work_items": items,
                    "host_wall_s": wall},
         "fp64": {"value": value, "ms_per_step": ms_per_step},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,

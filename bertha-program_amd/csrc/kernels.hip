@@ -680,18 +680,23 @@ __device__ void flush_exchange(const XView &x, int flg, const int (&offs)[4][2],
   __syncthreads();
 }
 
+// AB combinations whose first contraction shares the R loads
+constexpr int kGroup = 4;
+
 // Shared-memory state of one workgroup
 struct FockShared {
   union {
     uint8_t cflag[ntuv(kMaxLfl)];  // R column flags (until the prefactor pass)
-    double eab[ntuv(kMaxLab) * 4];  // E(AB) row of the current AB combination
+    double eab[ntuv(kMaxLab) * 4];  // E(AB) rows of the current AB combinations
   };
-  uint8_t gm[ntuv(kMaxLab) * 4];    // its G masks
-  double2 dab[2][4];                 // its D(A,B)(i,j), D(B,A)(j,i) slots
+  uint8_t gm[ntuv(kMaxLab) * 4];    // their G masks
+  double2 dab[kGroup][2][4];         // their D(A,B)(i,j), D(B,A)(j,i) slots
 };
 // the exchange accumulators follow in dynamic LDS, sized for the plan's
-// largest class: eight workgroups per CU fit even at 32 primitives per shell
-static_assert(sizeof(FockShared) + 16 + kXAccMax * sizeof(double) <= 160 * 1024 / 8,
+// largest class: eight workgroups per CU fit up to 28 primitives per shell
+// (seven at the 32-primitive limit)
+static_assert(sizeof(FockShared) + 16 + 2 * (12 * 28 + 16 * 28) * sizeof(double) <=
+                  160 * 1024 / 8,
               "fock_kernel's LDS exceeds 1/8 of a CU's");
 
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
@@ -761,145 +766,189 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   }
   __syncthreads();
 
-  for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
-    const ABCombo ab = a.abs[abi];
-    for (int t = lane; t < ab.ntab; t += 64) {
+  for (int a0 = it.ab_lo; a0 < it.ab_hi;) {
+    // a group of up to kGroup consecutive combinations of one shape (the plan
+    // orders them so) shares every R load of the first contraction
+    const ABCombo ab0 = a.abs[a0];
+    const int ntab = ab0.ntab, ntg = ab0.ntg;
+    int na = 1;
+    while (na < kGroup && a0 + na < it.ab_hi && (na + 1) * ntab <= ntuv(kMaxLab) &&
+           (na + 1) * ntg <= ntuv(kMaxLab) && a.abs[a0 + na].ntab == ntab &&
+           a.abs[a0 + na].ntg == ntg)
+      ++na;
+    for (int w = lane; w < na * ntab; w += 64) {
+      const int g = w / ntab, t = w - g * ntab;
       T ex, ey, ez, ew;
-      lde(a, ab.eoff + (int64_t)t * maxab + it.ij, ex, ey, ez, ew);
+      lde(a, a.abs[a0 + g].eoff + (int64_t)t * maxab + it.ij, ex, ey, ez, ew);
       // E(AB) entries of magnitude <= 1e-16 are skipped by ERIFAST (:12445)
-      s_eab[4 * t + 0] = fabs(ex) > (T)1.0e-16 ? ex : (T)0;
-      s_eab[4 * t + 1] = fabs(ey) > (T)1.0e-16 ? ey : (T)0;
-      s_eab[4 * t + 2] = fabs(ez) > (T)1.0e-16 ? ez : (T)0;
-      s_eab[4 * t + 3] = fabs(ew) > (T)1.0e-16 ? ew : (T)0;
+      s_eab[4 * w + 0] = fabs(ex) > (T)1.0e-16 ? ex : (T)0;
+      s_eab[4 * w + 1] = fabs(ey) > (T)1.0e-16 ? ey : (T)0;
+      s_eab[4 * w + 2] = fabs(ez) > (T)1.0e-16 ? ez : (T)0;
+      s_eab[4 * w + 3] = fabs(ew) > (T)1.0e-16 ? ew : (T)0;
     }
-    for (int t = lane; t < 4 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
-    // D(A,B)(i,j) and D(B,A)(j,i) of batches 3 and 4: fixed for the combination
-    // (no density on the RR test path)
-    if (!a.rr_out && lane < 8) {
-      const int b = lane >> 2, sl = lane & 3, rs = sl >> 1, cs = sl & 1;
+    for (int w = lane; w < na * 4 * ntg; w += 64) {
+      const int g = w / (4 * ntg);
+      sh.gm[w] = a.masks[a.abs[a0 + g].gmask + (w - g * 4 * ntg)];
+    }
+    // D(A,B)(i,j) and D(B,A)(j,i) of batches 3 and 4: fixed for each
+    // combination (no density on the RR test path)
+    if (!a.rr_out && lane < 8 * na) {
+      const ABCombo &abg = a.abs[a0 + (lane >> 3)];
+      const int b = (lane >> 2) & 1, sl = lane & 3, rs = sl >> 1, cs = sl & 1;
       const bool off_diag = a.isym != 0 && rs != cs;
-      const int row = b == 0 ? ab.fa[rs] + ib : ab.fb[rs] + jb;
-      const int col = b == 0 ? ab.fb[cs] + jb : ab.fa[cs] + ib;
-      sh.dab[b][sl] = off_diag ? make_double2(0.0, 0.0) : Dm[(int64_t)col * nd + row];
+      const int row = b == 0 ? abg.fa[rs] + ib : abg.fb[rs] + jb;
+      const int col = b == 0 ? abg.fb[cs] + jb : abg.fa[cs] + ib;
+      sh.dab[lane >> 3][b][sl] = off_diag ? make_double2(0.0, 0.0) : Dm[(int64_t)col * nd + row];
     }
     __syncthreads();
-    // first contraction G(kl,tuv') = sum_tuv E_AB(tuv) R(tuv+tuv')
+    // first contraction G_g(kl,tuv') = sum_tuv E_AB,g(tuv) R(tuv+tuv') for
+    // the group's combinations g, rows [g][tuv'][4] of the slab's G region
     for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
       int kl = kl0 + lane;
       if (kl >= maxcd) continue;
-      for (int tp = 0; tp < ab.ntg; ++tp) {
-        T g0 = 0, g1 = 0, g2 = 0, g3 = 0;
-        if (sh.gm[4 * tp] | sh.gm[4 * tp + 1] | sh.gm[4 * tp + 2] | sh.gm[4 * tp + 3]) {
+      for (int tp = 0; tp < ntg; ++tp) {
+        T acc[kGroup][4];
+        int any = 0;
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+          acc[g][0] = acc[g][1] = acc[g][2] = acc[g][3] = 0;
+          if (g < na) {
+            const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
+            any |= m[0] | m[1] | m[2] | m[3];
+          }
+        }
+        if (any) {
           // R rows from the sum table, four independent loads in flight
           const int *hs = c_hsum + tp * kHsRow;
           int t = 0;
-          for (; t + 4 <= ab.ntab; t += 4) {
+          for (; t + 4 <= ntab; t += 4) {
             const T r0 = R[(int64_t)hs[t] * ld + kl];
             const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
             const T r2 = R[(int64_t)hs[t + 2] * ld + kl];
             const T r3 = R[(int64_t)hs[t + 3] * ld + kl];
-            const T *e = s_eab + 4 * t;
-            g0 += e[0] * r0; g1 += e[1] * r0; g2 += e[2] * r0; g3 += e[3] * r0;
-            g0 += e[4] * r1; g1 += e[5] * r1; g2 += e[6] * r1; g3 += e[7] * r1;
-            g0 += e[8] * r2; g1 += e[9] * r2; g2 += e[10] * r2; g3 += e[11] * r2;
-            g0 += e[12] * r3; g1 += e[13] * r3; g2 += e[14] * r3; g3 += e[15] * r3;
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              const T *e = s_eab + 4 * (g * ntab + t);
+              acc[g][0] += e[0] * r0; acc[g][1] += e[1] * r0;
+              acc[g][2] += e[2] * r0; acc[g][3] += e[3] * r0;
+              acc[g][0] += e[4] * r1; acc[g][1] += e[5] * r1;
+              acc[g][2] += e[6] * r1; acc[g][3] += e[7] * r1;
+              acc[g][0] += e[8] * r2; acc[g][1] += e[9] * r2;
+              acc[g][2] += e[10] * r2; acc[g][3] += e[11] * r2;
+              acc[g][0] += e[12] * r3; acc[g][1] += e[13] * r3;
+              acc[g][2] += e[14] * r3; acc[g][3] += e[15] * r3;
+            }
           }
-          for (; t < ab.ntab; ++t) {
+          for (; t < ntab; ++t) {
             const T r = R[(int64_t)hs[t] * ld + kl];
-            g0 += s_eab[4 * t + 0] * r;
-            g1 += s_eab[4 * t + 1] * r;
-            g2 += s_eab[4 * t + 2] * r;
-            g3 += s_eab[4 * t + 3] * r;
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              const T *e = s_eab + 4 * (g * ntab + t);
+              acc[g][0] += e[0] * r; acc[g][1] += e[1] * r;
+              acc[g][2] += e[2] * r; acc[g][3] += e[3] * r;
+            }
           }
         }
-        T *gp = G + (int64_t)(4 * tp) * ld + kl;
-        gp[0] = sh.gm[4 * tp + 0] ? g0 : (T)0;
-        gp[ld] = sh.gm[4 * tp + 1] ? g1 : (T)0;
-        gp[2 * ld] = sh.gm[4 * tp + 2] ? g2 : (T)0;
-        gp[3 * ld] = sh.gm[4 * tp + 3] ? g3 : (T)0;
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+          if (g >= na) break;
+          const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
+          T *gp = G + (int64_t)(4 * (g * ntg + tp)) * ld + kl;
+          gp[0] = m[0] ? acc[g][0] : (T)0;
+          gp[ld] = m[1] ? acc[g][1] : (T)0;
+          gp[2 * ld] = m[2] ? acc[g][2] : (T)0;
+          gp[3 * ld] = m[3] ? acc[g][3] : (T)0;
+        }
       }
     }
-    double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
-    const double2 (&dab)[2][4] = sh.dab;
-    for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
-      const CDCall cd = a.cds[ci];
-      const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
-      const double pabcd = pab1 * pcd1;
-      for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
-        int kl = kl0 + lane;
-        if (kl >= maxcd) continue;
-        // second contraction (ERIFAST :12809-13119): a_s = sum Z E_CD G,
-        // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
-        T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-        T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
-#pragma unroll 4
-        for (int tp = 0; tp < cd.ntcd; ++tp) {
-          // masked E(CD) times Z (fold_kernel)
-          T e11r, e11i, e21r, e21i;
-          ldcd(a, cd.eoff + (int64_t)tp * maxcd + kl, e11r, e11i, e21r, e21i);
-          const T *gp = G + (int64_t)(4 * tp) * ld + kl;
-          T g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
-          // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
-          // set 3: E21cd x G21
-          ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
-          br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
-          ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
-          br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
-          ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
-          br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
-          ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
-          br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
-        }
-        // raw sums in FP64; RR(n) is formed where the digestion needs it
-        Raw raw;
-        for (int k = 0; k < 4; ++k) {
-          raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
-        }
-        raw.pab = pab1;
-        raw.pabcd = pabcd;
-        if (a.rr_out) {
-#pragma unroll
-          for (int n = 0; n < 16; ++n) {
-            double re, im;
-            if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
-            a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+    for (int g = 0; g < na; ++g) {
+      const ABCombo ab = a.abs[a0 + g];
+      const T *Gg = G + (int64_t)(4 * g * ntg) * ld;
+      double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
+      const double2 (&dab)[2][4] = sh.dab[g];
+      for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
+        const CDCall cd = a.cds[ci];
+        const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+        const double pabcd = pab1 * pcd1;
+        for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
+          int kl = kl0 + lane;
+          if (kl >= maxcd) continue;
+          // second contraction (ERIFAST :12809-13119): a_s = sum Z E_CD G,
+          // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
+          T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+          T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+  #pragma unroll 4
+          for (int tp = 0; tp < cd.ntcd; ++tp) {
+            // masked E(CD) times Z (fold_kernel)
+            T e11r, e11i, e21r, e21i;
+            ldcd(a, cd.eoff + (int64_t)tp * maxcd + kl, e11r, e11i, e21r, e21i);
+            const T *gp = Gg + (int64_t)(4 * tp) * ld + kl;
+            T g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
+            // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
+            // set 3: E21cd x G21
+            ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
+            br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
+            ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
+            br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
+            ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
+            br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
+            ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
+            br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
           }
-          continue;
+          // raw sums in FP64; RR(n) is formed where the digestion needs it
+          Raw raw;
+          for (int k = 0; k < 4; ++k) {
+            raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
+          }
+          raw.pab = pab1;
+          raw.pabcd = pabcd;
+          if (a.rr_out) {
+  #pragma unroll
+            for (int n = 0; n < 16; ++n) {
+              double re, im;
+              if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
+              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+            }
+            continue;
+          }
+          // digestion (CLMMTG / CLMMTZ)
+          const int idx[4] = {ib, jb, kl / nD, kl % nD};
+          const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          int flg = cd.iflg;
+          // ITOG: screening only when an SS pair is involved (ITT > 1, :12096)
+          if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
+            flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
+          if (a.isym != 0)
+            digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+                         cd.pcd[1], dab, accAB, xv, a.gdir);
+          else
+            digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+                          cd.pcd[1], dab, accAB, xv, a.gdir);
         }
-        // digestion (CLMMTG / CLMMTZ)
-        const int idx[4] = {ib, jb, kl / nD, kl % nD};
-        const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                               {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        int flg = cd.iflg;
-        // ITOG: screening only when an SS pair is involved (ITT > 1, :12096)
-        if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
-          flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
-        if (a.isym != 0)
-          digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], dab, accAB, xv, a.gdir);
-        else
-          digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], dab, accAB, xv, a.gdir);
+        if (!a.rr_out) {
+          const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.gxch, nd,
+                         lane);
+        }
       }
       if (!a.rr_out) {
-        const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.gxch, nd,
-                       lane);
-      }
-    }
-    if (!a.rr_out) {
-      // batches 1 and 2 write G(A sA, B sB)(i,j): reduce over the wave
-      for (int s = 0; s < 4; ++s) {
-        double re = wave_sum(accAB[s][0]);
-        double im = wave_sum(accAB[s][1]);
-        if (lane == 0 && (re != 0.0 || im != 0.0)) {
-          int orr = ab.fa[s >> 1] + ib, occ = ab.fb[s & 1] + jb;
-          atomic_add2(&a.gdir[(int64_t)occ * nd + orr], re, im);
+        // batches 1 and 2 write G(A sA, B sB)(i,j): reduce over the wave
+        for (int s = 0; s < 4; ++s) {
+          double re = wave_sum(accAB[s][0]);
+          double im = wave_sum(accAB[s][1]);
+          if (lane == 0 && (re != 0.0 || im != 0.0)) {
+            int orr = ab.fa[s >> 1] + ib, occ = ab.fb[s & 1] + jb;
+            atomic_add2(&a.gdir[(int64_t)occ * nd + orr], re, im);
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
+    a0 += na;
   }
 }
 

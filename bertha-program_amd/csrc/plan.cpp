@@ -617,6 +617,13 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
             }
             cl.ab_end = (int)P.abs.size();
             if (cl.ab_end == cl.ab_begin) continue;
+            // combinations of one shape (N(L_AB), N(L_CD) of G) side by side,
+            // so the kernel's first contraction can share each R load among
+            // several of them; only the order of accumulation changes
+            std::stable_sort(P.abs.begin() + cl.ab_begin, P.abs.begin() + cl.ab_end,
+                             [](const ABCombo &x, const ABCombo &y) {
+                               return x.ntab != y.ntab ? x.ntab > y.ntab : x.ntg > y.ntg;
+                             });
             P.classes.push_back(cl);
             P.max_lfl = std::max(P.max_lfl, cl.lamfl);
             P.max_maxcd = std::max(P.max_maxcd, cl.maxcd);
