@@ -157,14 +157,17 @@ class DIIS:
         if errmax > ERRCOM or kss <= 2:
             return fock
         b = self.bsav[:kss, :kss]
-        # min |B c|^2 subject to sum c = 1: KKT system of the normal equations
-        kkt = np.zeros((kss + 1, kss + 1), dtype=np.complex128)
-        kkt[:kss, :kss] = b.conj().T @ b
-        kkt[:kss, kss] = 1.0
-        kkt[kss, :kss] = 1.0
-        rhs = np.zeros(kss + 1, dtype=np.complex128)
-        rhs[kss] = 1.0
-        c = np.linalg.lstsq(kkt, rhs, rcond=None)[0][:kss]
+        # ZGGLSE's problem min |B c| subject to sum c = 1, by the null-space
+        # method: c = c0 + N y with c0 = 1/kss and N an orthonormal basis of
+        # {x : sum x = 0}, y the least-squares solution of B N y = -B c0.
+        # Unlike the normal equations it does not square B's condition
+        # number, and it is invariant to B's scale (B ~ |error|^2 -> 0 near
+        # convergence).
+        q, _ = np.linalg.qr(np.ones((kss, 1), dtype=np.complex128), mode="complete")
+        nsp = q[:, 1:]
+        c0 = np.full(kss, 1.0 / kss, dtype=np.complex128)
+        y = np.linalg.lstsq(b @ nsp, -(b @ c0), rcond=None)[0]
+        c = c0 + nsp @ y
         out = torch.zeros_like(fock)
         for m in range(kss):
             out = out + complex(c[m]) * self.fck[m]

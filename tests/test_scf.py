@@ -50,6 +50,51 @@ def test_ne_scf_on_oracle(bdf, orc, scf):
     assert np.abs(eig - ref).max() < 0.05
 
 
+def test_diis_coefficients_are_the_constrained_minimum(scf):
+    """The C-DIIS step solves ZGGLSE's problem min |B c| subject to sum c = 1
+    (DIIS, bertha-scf.f:4078); for a nonsingular B^H B its closed form is
+    c = M^-1 1 / (1^T M^-1 1), M = B^H B (evaluated here in 40-digit
+    arithmetic).  The extrapolated Fock matrix must be that combination, also
+    when the errors are tiny (B ~ |error|^2), where normal equations of the
+    constrained problem lose their accuracy."""
+    import torch
+    rng = np.random.default_rng(5)
+    n, depth = 6, 6
+    d = scf.DIIS(depth)
+    ovlp = torch.eye(n, dtype=torch.complex128)
+    focks, errs = [], []
+    for it in range(1, depth + 1):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        f = torch.from_numpy(a + a.conj().T) * 1e-3
+        p = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        dens = torch.from_numpy(p + p.conj().T) * 1e-2
+        out = d(it, f, dens, ovlp)
+        focks.append(f)
+        errs.append((f @ dens.conj() @ ovlp - ovlp @ dens.conj() @ f).numpy())
+    # errors are small (below ERRCOM), so the last call extrapolated
+    assert max(np.abs(e).max() for e in errs) < scf.ERRCOM
+    import mpmath
+    mpmath.mp.dps = 40
+    b = mpmath.matrix([[mpmath.mpc(complex(np.sum(errs[m].conj() * errs[k])))
+                        for k in range(depth)] for m in range(depth)])
+    mm = b.H * b
+    w = mpmath.lu_solve(mm, mpmath.matrix([1] * depth))
+    tot = sum(w[i] for i in range(depth))
+    c = [complex(w[i] / tot) for i in range(depth)]
+    expect = sum(c[m] * focks[m] for m in range(depth))
+    assert torch.abs(out - expect).max() < 1e-9 * torch.abs(expect).max()
+
+
+def test_damping_is_off_once_diis_engages(bdf, orc, scf):
+    """Density damping only acts before the DIIS error first reaches ERRCOM:
+    the damped Ne solve reaches the same state as the undamped one."""
+    s = bdf.systems.ne_atom()
+    r0 = scf.run_scf(s, 10, OracleBackend(bdf, orc, s))
+    r1 = scf.run_scf(s, 10, OracleBackend(bdf, orc, s), damping=0.5)
+    assert r0.converged and r1.converged
+    assert abs(r0.energy - r1.energy) < 1e-8
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("hmlt", ["DHFR", "DHFB"])
 def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
