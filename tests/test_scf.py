@@ -95,6 +95,36 @@ def test_damping_is_off_once_diis_engages(bdf, orc, scf):
     assert abs(r0.energy - r1.energy) < 1e-8
 
 
+def test_superposed_density_places_atom_blocks(bdf, orc, scf):
+    """The atomic guess puts each atom's LL, LS, SL and SS blocks at the
+    molecule's rows and columns of that centre (FOCKIND is centre-major,
+    bertha-scf.f:2605-2645): checked against the oracle's labels."""
+    import torch
+    S = bdf.systems
+    ev = S.even_tempered
+    sh = [[ev(0.2, 3.0, 3), ev(0.4, 3.0, 2)], [ev(0.1, 2.5, 2)]]
+    mol = S.System("pair", np.array([[0, 0, 0.0], [0, 0, 2.0]]), sh, shape="DIATOM",
+                   charges=[4.0, 2.0])
+    atoms = [S.System(f"a{c}", np.zeros((1, 3)), [sh[c]], shape="ATOMIC") for c in range(2)]
+    rng = np.random.default_rng(1)
+    dens = [torch.from_numpy(rng.standard_normal((a.ndim, a.ndim)) + 0j) for a in atoms]
+    d = scf.superposed_density(mol, dens).numpy()
+    O = orc.Oracle(mol)
+    icn, kqn, mqn = O.labels()
+    nskp = O.nskp
+    for c in range(2):
+        la = np.where(icn == c + 1)[0]
+        m = len(la)
+        rows = np.concatenate([la, nskp + la])
+        assert np.array_equal(d[np.ix_(rows, rows)], dens[c].numpy())
+        # nothing couples the two atoms
+        other = np.setdiff1d(np.arange(mol.ndim), rows)
+        assert not np.any(d[np.ix_(rows, other)])
+        # the atom alone has the same label sequence as the molecule's centre
+        ia, ka, ma = orc.Oracle(atoms[c]).labels()
+        assert np.array_equal(ka[:m], kqn[la]) and np.array_equal(ma[:m], mqn[la])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("hmlt", ["DHFR", "DHFB"])
 def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
