@@ -414,6 +414,7 @@ static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, dou
   a.intsym = e->intsym;
   a.gaunt = e->gaunt;
   a.stale = dp.stale.p;
+  a.Ecd = e->Ecd.p;
   a.rr_out = rr_out;
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   HIPCHK(launch_breit(a, e->nblocks, e->stream));

@@ -94,6 +94,7 @@ struct BreitArgs {
   int isym, intsym, gaunt;
   uint8_t *stale;   // per class: column flags of the last R batch (BIIFAST's IRC)
   double2 *rr_out;  // test hook: RR[16][maxcd] of the first call instead of digesting
+  const double4 *Ecd;  // E(CD) with column masks and ERIFAST's Z sign applied
 };
 
 struct OneArgs {

@@ -21,6 +21,10 @@ __constant__ uint32_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits (32-bi
 // for t, tp < N(kMaxLab); rows padded to a multiple of 4 for 16-byte scalar loads
 constexpr int kHsRow = (ntuv(kMaxLab) + 3) / 4 * 4;
 __constant__ int c_hsum[ntuv(kMaxLab) * kHsRow];
+// unit steps in Hermite index space: c_hstep[h][d] = index(tuv(h) + e_d)
+// (d = 0..2; -1 beyond order kMaxLfl) and c_hstep[h][4 + d] = index(tuv(h) -
+// e_d) (-1 where that component is zero); rows of 8 words
+__constant__ int c_hstep[ntuv(kMaxLfl)][8];
 __constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
 __constant__ float c_rcpf[2 * kMaxLfl + 2 * 60 + 4];
 
@@ -1043,7 +1047,6 @@ struct BreitShared {
   uint8_t cflag[ntuv(kMaxLfl)];
   uint8_t gflag[ntuv(kMaxLfl)];
   uint8_t gm[ntuv(2 * kMaxL + 1) * 12];
-  uint8_t km[ntuv(2 * kMaxL + 1) * 12];
   double xacc[kXAccMax];  // exchange accumulators (XView)
 };
 
@@ -1119,13 +1122,11 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
           const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
           double g[4] = {0.0, 0.0, 0.0, 0.0};
           if (gm[0] | gm[1] | gm[2] | gm[3]) {
-            int t2, u2, v2;
-            dec(tp, t2, u2, v2);
+            // Hermite indices from the constant tables: h = tuv + tuv',
+            // h2 = h + e_ix, h1 = h2 + e_jx, h3 = h2 - e_jx
+            const int *hs = c_hsum + tp * kHsRow;
             for (int t = 0; t < ab.ntab; ++t) {
-              int t1, u1, v1;
-              dec(t, t1, u1, v1);
-              const int s3[3] = {t1 + t2, u1 + u2, v1 + v2};
-              const int h = hidx(s3[0], s3[1], s3[2]);
+              const int h = hs[t];
               if (sh.cflag[h]) {
                 double r = R[(int64_t)h * ld + kl];
                 const double *e = &sh.eab[ix][4 * t];
@@ -1133,22 +1134,21 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                 for (int c = 0; c < 4; ++c) g[c] += e[c] * r;
               }
               if (a.gaunt) continue;
+              const uint32_t s3 = c_tuv[h];
+              const int h2 = c_hstep[h][ix];
               for (int jx = 0; jx < 3; ++jx) {
                 const double *e = &sh.eab[jx][4 * t];
                 if (!any4(e)) continue;
-                int e1[3] = {s3[0], s3[1], s3[2]};
-                e1[ix] += 1;
-                const int h2 = hidx(e1[0], e1[1], e1[2]);
-                int e3[3] = {e1[0], e1[1], e1[2]};
-                e1[jx] += 1;
-                e3[jx] -= 1;
-                const int h1 = hidx(e1[0], e1[1], e1[2]);
-                const bool ok3 = e3[0] >= 0 && e3[1] >= 0 && e3[2] >= 0;
-                const int h3 = ok3 ? hidx(e3[0], e3[1], e3[2]) : 0;
+                const int h1 = c_hstep[h2][jx];
+                const int h3 = c_hstep[h2][4 + jx];
+                const bool ok3 = h3 >= 0;
                 int fl = sh.gflag[h1] + sh.gflag[h2] + (ok3 ? sh.gflag[h3] : 0);
                 if (fl == 0) continue;
                 double tv = R[(int64_t)h1 * ld + kl] * half_ia - R[(int64_t)h2 * ld + kl] * pq[jx];
-                if (ok3) tv += R[(int64_t)h3 * ld + kl] * (double)s3[jx];
+                if (ok3) {
+                  const int s3j = (int)((s3 >> (10 - 5 * jx)) & 31);  // (t+t')_jx
+                  tv += R[(int64_t)h3 * ld + kl] * (double)s3j;
+                }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) g[c] -= e[c] * tv;
               }
@@ -1161,9 +1161,6 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
     }
     for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
       const BrCD cd = a.cds[ci];
-      __syncthreads();
-      for (int t = lane; t < 12 * cd.ntcd; t += 64) sh.km[t] = a.masks[cd.kmask + t];
-      __syncthreads();
       const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
       const double pabcd = pab1 * pcd1;
       for (int kl = lane; kl < maxcd; kl += 64) {
@@ -1171,15 +1168,9 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
         double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
         for (int ix = 0; ix < 3; ++ix)
           for (int tp = 0; tp < cd.ntcd; ++tp) {
-            double4 e = a.E[cd.eoff[ix] + (int64_t)tp * maxcd + kl];
-            int t2, u2, v2;
-            dec(tp, t2, u2, v2);
-            double z = ((t2 + u2 + v2) & 1) ? -1.0 : 1.0;
-            const uint8_t *km = &sh.km[(tp * 3 + ix) * 4];
-            double e11r = km[0] ? z * e.x : 0.0;
-            double e11i = km[1] ? z * e.y : 0.0;
-            double e21r = km[2] ? z * e.z : 0.0;
-            double e21i = km[3] ? z * e.w : 0.0;
+            // masked E(CD) times Z (fold_kernel; EILSCD0's flags)
+            const double4 e = a.Ecd[cd.eoff[ix] + (int64_t)tp * maxcd + kl];
+            const double e11r = e.x, e11i = e.y, e21r = e.z, e21i = e.w;
             const double *gp = G + (int64_t)(ix * 4 * ab.ntg + tp) * ld + kl;
             const int64_t gs = (int64_t)ab.ntg * ld;
             double g11r = gp[0], g11i = gp[gs], g21r = gp[2 * gs], g21i = gp[3 * gs];
@@ -1465,6 +1456,24 @@ hipError_t upload_constants() {
       hs[tp * kHsRow + t] = v;
     }
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_hsum), hs, sizeof(hs));
+  if (e != hipSuccess) return e;
+  static int st[ntuv(kMaxLfl)][8];
+  auto index_of = [](int T, int U, int V) {
+    const int L = T + U + V;
+    return L * (L + 1) * (L + 2) / 6 + T * (L + 1) - T * (T - 1) / 2 + U;
+  };
+  for (int h = 0; h < ntuv(kMaxLfl); ++h) {
+    const int c[3] = {(int)(tuv[h] >> 10), (int)((tuv[h] >> 5) & 31), (int)(tuv[h] & 31)};
+    for (int d = 0; d < 3; ++d) {
+      int up[3] = {c[0], c[1], c[2]}, dn[3] = {c[0], c[1], c[2]};
+      up[d] += 1;
+      dn[d] -= 1;
+      st[h][d] = c[0] + c[1] + c[2] + 1 <= kMaxLfl ? index_of(up[0], up[1], up[2]) : -1;
+      st[h][4 + d] = dn[d] >= 0 ? index_of(dn[0], dn[1], dn[2]) : -1;
+    }
+    st[h][3] = st[h][7] = -1;
+  }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_hstep), st, sizeof(st));
   if (e != hipSuccess) return e;
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
   float rcpf[2 * kMaxLfl + 2 * 60 + 4];

