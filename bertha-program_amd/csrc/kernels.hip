@@ -1117,47 +1117,89 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                       (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q - Py,
                       (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q - Pz};
       double half_ia = 0.5 / (p * q / (p + q));
-      for (int ix = 0; ix < 3; ++ix)
-        for (int tp = 0; tp < ab.ntg; ++tp) {
+      for (int tp = 0; tp < ab.ntg; ++tp) {
+        // the three Pauli components ix share every R value they read: R at
+        // h = tuv + tuv', h + e_i, h + e_i + e_j and h + e_i - e_j
+        bool act[3];
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix) {
           const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
-          double g[4] = {0.0, 0.0, 0.0, 0.0};
-          if (gm[0] | gm[1] | gm[2] | gm[3]) {
-            // Hermite indices from the constant tables: h = tuv + tuv',
-            // h2 = h + e_ix, h1 = h2 + e_jx, h3 = h2 - e_jx
-            const int *hs = c_hsum + tp * kHsRow;
-            for (int t = 0; t < ab.ntab; ++t) {
-              const int h = hs[t];
-              if (sh.cflag[h]) {
-                double r = R[(int64_t)h * ld + kl];
+          act[ix] = (gm[0] | gm[1] | gm[2] | gm[3]) != 0;
+        }
+        double g[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        if (act[0] || act[1] || act[2]) {
+          const int *hs = c_hsum + tp * kHsRow;
+          for (int t = 0; t < ab.ntab; ++t) {
+            const int h = hs[t];
+            const bool fh = sh.cflag[h] != 0;
+            const double r0 = R[(int64_t)h * ld + kl];
+            if (fh) {
+#pragma unroll
+              for (int ix = 0; ix < 3; ++ix) {
+                if (!act[ix]) continue;
                 const double *e = &sh.eab[ix][4 * t];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) g[c] += e[c] * r;
+                for (int c = 0; c < 4; ++c) g[ix][c] += e[c] * r0;
               }
-              if (a.gaunt) continue;
-              const uint32_t s3 = c_tuv[h];
-              const int h2 = c_hstep[h][ix];
+            }
+            if (a.gaunt) continue;
+            // gauge term T = R(h+e_ix+e_jx)/(2 alpha) - PQ_jx R(h+e_ix) +
+            // (t+t')_jx R(h+e_ix-e_jx), skipped where all three columns are
+            // flagged off (BIIFAST :18492-18810)
+            const uint32_t s3 = c_tuv[h];
+            const int sj[3] = {(int)(s3 >> 10), (int)((s3 >> 5) & 31), (int)(s3 & 31)};
+            int hp[3], hpp[3][3], hpm[3][3];
+            double rp[3], rpp[3][3], rpm[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              hp[i] = c_hstep[h][i];
+              rp[i] = R[(int64_t)hp[i] * ld + kl];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) {
+                if (j < i) {
+                  hpp[i][j] = hpp[j][i];
+                  rpp[i][j] = rpp[j][i];
+                } else {
+                  hpp[i][j] = c_hstep[hp[i]][j];
+                  rpp[i][j] = R[(int64_t)hpp[i][j] * ld + kl];
+                }
+                if (i == j) {
+                  hpm[i][j] = h;
+                  rpm[i][j] = r0;
+                } else {
+                  hpm[i][j] = c_hstep[hp[i]][4 + j];  // -1 where (t+t')_j = 0
+                  rpm[i][j] = hpm[i][j] >= 0 ? R[(int64_t)hpm[i][j] * ld + kl] : 0.0;
+                }
+              }
+#pragma unroll
+            for (int ix = 0; ix < 3; ++ix) {
+              if (!act[ix]) continue;
+#pragma unroll
               for (int jx = 0; jx < 3; ++jx) {
                 const double *e = &sh.eab[jx][4 * t];
                 if (!any4(e)) continue;
-                const int h1 = c_hstep[h2][jx];
-                const int h3 = c_hstep[h2][4 + jx];
-                const bool ok3 = h3 >= 0;
-                int fl = sh.gflag[h1] + sh.gflag[h2] + (ok3 ? sh.gflag[h3] : 0);
+                const bool ok3 = hpm[ix][jx] >= 0;
+                int fl = sh.gflag[hpp[ix][jx]] + sh.gflag[hp[ix]] + (ok3 ? sh.gflag[hpm[ix][jx]] : 0);
                 if (fl == 0) continue;
-                double tv = R[(int64_t)h1 * ld + kl] * half_ia - R[(int64_t)h2 * ld + kl] * pq[jx];
-                if (ok3) {
-                  const int s3j = (int)((s3 >> (10 - 5 * jx)) & 31);  // (t+t')_jx
-                  tv += R[(int64_t)h3 * ld + kl] * (double)s3j;
-                }
+                double tv = rpp[ix][jx] * half_ia - rp[ix] * pq[jx];
+                if (ok3) tv += rpm[ix][jx] * (double)sj[jx];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) g[c] -= e[c] * tv;
+                for (int c = 0; c < 4; ++c) g[ix][c] -= e[c] * tv;
               }
             }
           }
+        }
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix) {
+          const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
 #pragma unroll
           for (int c = 0; c < 4; ++c)
-            G[(int64_t)((ix * 4 + c) * ab.ntg + tp) * ld + kl] = gm[c] ? g[c] : 0.0;
+            G[(int64_t)((ix * 4 + c) * ab.ntg + tp) * ld + kl] = gm[c] ? g[ix][c] : 0.0;
         }
+      }
     }
     for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
       const BrCD cd = a.cds[ci];
