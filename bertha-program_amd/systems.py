@@ -21,7 +21,8 @@ class System:
     shape: str = "NOSYMM"           # ATOMIC | LINEAR | DIATOM | NOSYMM
     hmlt: str = "DHFR"              # DHFR (Dirac-Coulomb) | DHFB (+ Breit)
     charges: Sequence[float] = ()
-    nucleus: str = "POINT"          # POINT | GAUSS (NMDL, bertha-scf.f:1182-1204)
+    nucleus: object = "POINT"       # POINT | GAUSS, or one of them per centre (NMDL,
+                                    # bertha-scf.f:1182-1204)
     masses: Sequence[float] = ()    # mass numbers A (Gaussian nuclei)
 
     @property
@@ -60,10 +61,15 @@ def nuclear_exponents(system: "System"):
     """Gaussian-nucleus exponents xi = 3/(2 R_rms^2) per centre with the
     reference's RMS-radius rule (bertha-scf.f:1182-1204); None for point
     nuclei."""
-    if system.nucleus.upper() == "POINT":
+    models = ([system.nucleus] * system.ncnt if isinstance(system.nucleus, str)
+              else list(system.nucleus))
+    if all(m.upper() == "POINT" for m in models):
         return None
     xi = []
-    for z, a in zip(system.charges, system.masses):
+    for z, a, m in zip(system.charges, system.masses, models):
+        if m.upper() == "POINT":
+            xi.append(0.0)  # xi <= 0: point nucleus
+            continue
         if int(z) == 1:
             rfm = PROTON_RADIUS_FM
         elif z < 90.0:
