@@ -644,9 +644,6 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   CDCall cd;
   cd.eoff = e->T.pairs[pcd].off;
   cd.ntcd = ntcd;
-  cd.kmask = (int)dp.P.masks.size();
-  for (int t = 0; t < ntcd; ++t)
-    for (int c = 0; c < 4; ++c) dp.P.masks.push_back(e->eflags[e->T.flag_off[pcd] + 4 * t + c]);
   for (int s = 0; s < 2; ++s) cd.fc[s] = cd.fd[s] = 0;
   cd.iflg = 0;
   int mq[4];
@@ -839,14 +836,8 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
       }
     BrCD cd;
     cd.ntcd = ntcd;
-    cd.kmask = (int)dp.P.masks.size();
     for (int ix = 0; ix < 3; ++ix)
       cd.eoff[ix] = e->T.pairs[e->T.pair_index.at(Etabs::key(ic[2], ic[3], kidx[2], kidx[3], midx[2], midx[3], kLSx + ix))].off;
-    for (int t = 0; t < ntcd; ++t)
-      for (int ix = 0; ix < 3; ++ix) {
-        int pr = e->T.pair_index.at(Etabs::key(ic[2], ic[3], kidx[2], kidx[3], midx[2], midx[3], kLSx + ix));
-        for (int c = 0; c < 4; ++c) dp.P.masks.push_back(e->eflags[e->T.flag_off[pr] + 4 * t + c]);
-      }
     for (int s = 0; s < 2; ++s) cd.fc[s] = cd.fd[s] = 0;
     cd.iflg = 0;
     int mq[4];

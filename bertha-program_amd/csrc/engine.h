@@ -82,7 +82,6 @@ struct ABCombo {
 struct CDCall {
   int64_t eoff;     // E(CD) double4 offset
   int ntcd;
-  int kmask;        // byte offset of the E(CD) column mask [ntcd][4]
   int fc[2], fd[2]; // Fock offsets of C, D for sign slots
   int iflg;         // bit b set: batch b+1 active
   int8_t pab[2], pcd[2];  // [0] same-slot phase (PAB1), [1] PAB2
@@ -135,7 +134,6 @@ struct BrAB {
 struct BrCD {
   int64_t eoff[3];
   int ntcd;
-  int kmask;        // bytes [ntcd][3][4]: E(CD) column flags (EILSCD0)
   int fc[2], fd[2];
   int iflg;         // bit b set: exchange batch b+1 active
   int8_t pab[2], pcd[2];

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <limits>
 #include <map>
 #include <tuple>
 
@@ -396,6 +397,20 @@ static int ncntrs(int a, int b, int c, int d) {
 // Items of a plan: every (class, ij), its AB combinations cut into
 // consecutive runs of cost at most max(total / 8192, one combination), then
 // ordered by cost, largest first.
+// The kernels address the plan's tables with 32-bit offsets and counts
+// (ABCombo::gmask, cd_begin/cd_end, item indices): refuse a plan that does
+// not fit them rather than let an offset wrap.
+template <class PlanT>
+static std::string check_offsets(const PlanT &P) {
+  const size_t lim = (size_t)std::numeric_limits<int>::max();
+  if (P.masks.size() > lim || P.cds.size() > lim || P.abs.size() > lim || P.items.size() > lim ||
+      P.classes.size() > lim)
+    return "plan exceeds the 32-bit offsets of its tables (masks " +
+           std::to_string(P.masks.size()) + " bytes, " + std::to_string(P.cds.size()) +
+           " calls, " + std::to_string(P.items.size()) + " items)";
+  return "";
+}
+
 static void make_items(const std::vector<ClassDesc> &classes, const std::vector<double> &abcost,
                        const std::vector<double> &rcost, std::vector<WorkItem> &items) {
   double total = 0.0;
@@ -577,9 +592,6 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
                                 int pcd_pair = T.pair_index.at(Etabs::key(cc, cd, kc, kd, mc, md, it2 == 4));
                                 call.eoff = T.pairs[pcd_pair].off;
                                 call.ntcd = ntuv(lc + ld + (it2 == 4 ? 2 : 0));
-                                call.kmask = (int)P.masks.size();
-                                for (int t = 0; t < call.ntcd; ++t)
-                                  for (int c = 0; c < 4; ++c) P.masks.push_back((uint8_t)kmask_at(pcd_pair, t, c));
                                 call.fc[0] = B.lrge_at(cc, kc, 2 * mc - 1) + naddcd;
                                 call.fc[1] = B.lrge_at(cc, kc, 2 * mc) + naddcd;
                                 call.fd[0] = B.lrge_at(cd, kd, 2 * md - 1) + naddcd;
@@ -668,7 +680,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
     }
   }
   make_items(P.classes, abcost, rcost, P.items);
-  return "";
+  return check_offsets(P);
 }
 
 // ---------------------------------------------------------------------------
@@ -792,12 +804,6 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
                             for (int ix = 0; ix < 3; ++ix)
                               call.eoff[ix] = T.pairs[ls_pair(cc, cd, kc, kd, mc, md, ix)].off;
                             call.ntcd = ntcd;
-                            call.kmask = (int)P.masks.size();
-                            for (int t = 0; t < ntcd; ++t)
-                              for (int ix = 0; ix < 3; ++ix) {
-                                int pr = ls_pair(cc, cd, kc, kd, mc, md, ix);
-                                for (int c = 0; c < 4; ++c) P.masks.push_back((uint8_t)flag_at(pr, t, c));
-                              }
                             // IT2 = 2: C large, D small
                             call.fc[0] = B.lrge_at(cc, kc, 2 * mc - 1);
                             call.fc[1] = B.lrge_at(cc, kc, 2 * mc);
@@ -875,7 +881,7 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
     }
   }
   make_items(P.classes, abcost, rcost, P.items);
-  return "";
+  return check_offsets(P);
 }
 
 }  // namespace bdf

@@ -44,4 +44,20 @@ def small_systems(bdf):
     fsh = S.System("f-shell-pair", np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 1.9]]),
                    [[ev(0.2, 3.0, 2), ev(0.3, 3.0, 2), ev(0.4, 3.0, 1), ev(0.5, 3.0, 1)],
                     [ev(0.1, 3.0, 2), ev(0.3, 3.0, 1)]], shape="NOSYMM")
-    return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh}
+    # four distinct centres: the only systems whose quartets (AB|CD) can put
+    # every label on its own centre; "ring4" is planar (z = 0), like the ring
+    # workload
+    quad = S.System("four-centre", np.array([[0.0, 0.0, 0.0], [1.5, 0.2, 0.4], [-0.4, 1.6, -0.3],
+                                             [0.9, -1.2, 1.3]]),
+                    [[ev(0.1, 2.8, 3), ev(0.2, 2.6, 2)],
+                     [ev(0.08, 3.0, 3), ev(0.3, 2.5, 1)],
+                     [ev(0.06, 3.0, 2), ev(0.3, 2.5, 1)],
+                     [ev(0.07, 3.0, 2)]], shape="NOSYMM")
+    ring4 = S.System("planar-four", np.array([[1.3, 0.0, 0.0], [0.05, 1.35, 0.0], [-1.32, 0.02, 0.0],
+                                              [0.0, -1.28, 0.0]]),
+                     [[ev(0.1, 2.8, 3), ev(0.2, 2.6, 1), ev(0.5, 2.5, 1)],
+                      [ev(0.08, 3.0, 2), ev(0.3, 2.5, 1)],
+                      [ev(0.1, 2.8, 3), ev(0.2, 2.6, 1)],
+                      [ev(0.08, 3.0, 2)]], shape="NOSYMM")
+    return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh, "quad": quad,
+            "ring4": ring4}

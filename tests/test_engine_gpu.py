@@ -70,7 +70,7 @@ def test_eri_batches_match_oracle(bdf, orc, systems, name):
             assert np.abs(rg[use] - ro[use]).max() <= tol, (icnt, kidx, midx, ibas, jbas, itn)
 
 
-@pytest.mark.parametrize("name", ["ne", "diat", "off", "tri", "fsh"])
+@pytest.mark.parametrize("name", ["ne", "diat", "off", "tri", "fsh", "quad", "ring4"])
 def test_clmfast_matches_oracle(bdf, orc, systems, name):
     s = systems[name]
     print(f"[clmfast] {name}", flush=True)
@@ -129,7 +129,7 @@ def test_schwarz_tables_match_oracle(bdf, orc, systems, name):
     assert np.abs(q ** 2 - qo ** 2).max() < 1e-12
 
 
-@pytest.mark.parametrize("name", ["diat", "tri"])
+@pytest.mark.parametrize("name", ["diat", "tri", "quad", "ring4"])
 def test_screened_clmfast_matches_oracle(bdf, orc, systems, name):
     s = systems[name]
     print(f"[schwarz build] {name}", flush=True)
@@ -147,7 +147,7 @@ def _e2(d, g, x):
     return 0.5 * np.real(np.sum(d.T * (g - x)))
 
 
-@pytest.mark.parametrize("name", ["off", "tri"])
+@pytest.mark.parametrize("name", ["off", "tri", "quad", "ring4"])
 def test_mixed_precision_split(bdf, orc, systems, name):
     """FP32 items below the split threshold: a zero threshold reproduces the
     FP64 build exactly; a threshold above every bound runs everything in
