@@ -104,6 +104,8 @@ struct bdf_engine {
   int schwarz = 0, have_tables = 0;
   double sens = 1.0e-12;
   DevBuf<double> gdsc, qbnd;
+  DevBuf<int> partner;          // the other sign slot of every row (Schwarz tables)
+  DevBuf<double> gslot, dslot;  // slot maxima of GDSC and |D| (SCHWARZ's direct groups)
   std::vector<double> qbnd_host;
   // FP32/FP64 split
   int mixed = 0;
@@ -231,6 +233,33 @@ static int make_tables(bdf_engine *e) {
   da.qbnd = e->qbnd.p;
   da.ndim = B.ndim;
   HIPCHK(launch_diag(da, e->stream));
+  // (held until the stream synchronises below: the upload is asynchronous)
+  std::vector<int> partner;
+  if (e->schwarz) {
+    // row r's other sign slot: the (-|m|, +|m|) rows of one (centre, kappa,
+    // |m|) block, large and small
+    partner.assign(B.ndim, -1);
+    for (int c = 0; c < B.ncnt; ++c)
+      for (int k = 1; k <= 2 * B.lmax[c] + 1; ++k) {
+        const int kq = B.kappa_of(c, k), l = kq < 0 ? -kq - 1 : kq;
+        for (int m = 1; m <= std::abs(kq); ++m)
+          for (int t = 0; t < 2; ++t) {
+            const int lo = B.lrge_at(c, k, 2 * m - 1) + t * B.nskp;
+            const int hi = B.lrge_at(c, k, 2 * m) + t * B.nskp;
+            for (int i = 0; i < B.nfn(c, l); ++i) {
+              partner[lo + i] = hi + i;
+              partner[hi + i] = lo + i;
+            }
+          }
+      }
+    for (int r = 0; r < B.ndim; ++r)
+      if (partner[r] < 0 || partner[r] >= B.ndim)
+        return fail(BDF_EINVAL, "Schwarz tables: row without a sign-slot partner");
+    HIPCHK(e->partner.upload(partner, e->stream));
+    HIPCHK(e->gslot.alloc(nn));
+    HIPCHK(e->dslot.alloc(nn));
+    HIPCHK(launch_slot_tables(e->gdsc.p, nullptr, e->partner.p, B.ndim, e->gslot.p, e->stream));
+  }
   e->qbnd_host.resize(nn);
   HIPCHK(hipMemcpyAsync(e->qbnd_host.data(), e->qbnd.p, nn * sizeof(double),
                         hipMemcpyDeviceToHost, e->stream));
@@ -305,6 +334,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   else
     HIPCHK(launch_zero(rr_out, nullptr, 0, e->counter.p, e->dmax.p, e->stream));
   if (mixed) HIPCHK(launch_dmax(d_dent, (int64_t)nd * nd, e->dmax.p, e->stream));
+  if (e->schwarz && !rr_out)
+    HIPCHK(launch_slot_tables(nullptr, d_dent, e->partner.p, nd, e->dslot.p, e->stream));
   FockArgs a;
   a.classes = dp.classes.p;
   a.abs = dp.abs.p;
@@ -326,6 +357,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.intsym = e->intsym;
   a.rr_out = rr_out;
   a.gdsc = e->gdsc.p;
+  a.gslot = e->gslot.p;
+  a.dslot = e->dslot.p;
   a.sens = e->sens;
   a.schwarz = (e->schwarz && !rr_out) ? 1 : 0;
   a.nskp = e->B.nskp;
@@ -1003,7 +1036,8 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   add(e->Ecd32.n, sizeof(float4));
   add(e->flags.n, 1);
   add(e->slab.n, sizeof(double));
-  add(e->gdsc.n + e->qbnd.n, sizeof(double));
+  add(e->gdsc.n + e->qbnd.n + e->gslot.n + e->dslot.n, sizeof(double));
+  add(e->partner.n, sizeof(int));
   add(e->dent.n + e->gdir.n + e->gxch.n, sizeof(double2));
   add(e->pairs.n, sizeof(EPair));
   add(e->terms.n, sizeof(ETerm));

@@ -47,6 +47,9 @@ struct FockArgs {
   double2 *rr_out;      // test hook: write RR[16][maxcd] instead of digesting
   // SCHWARZ masks (SCHWRZ): GDSC table, sensitivity, large-block size
   const double *gdsc;
+  // the largest GDSC and |D| over the four sign slots of each block pair,
+  // stored at the pair's (-|m|, -|m|) element (slot_tables_kernel)
+  const double *gslot, *dslot;
   double sens;
   int schwarz, nskp;
   // Schwarz split: items with qitem * max|D| < split run in FP32 on E32
@@ -124,6 +127,10 @@ hipError_t launch_etab(const EtabArgs &a, hipStream_t s);
 hipError_t launch_eflag(const EflagArgs &a, hipStream_t s);
 hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
                        unsigned long long *dmax, hipStream_t s);
+// out(r, c) = max over the sign slots {r, partner[r]} x {c, partner[c]} of
+// G (when G is given) or of |D|, for every (r, c)
+hipError_t launch_slot_tables(const double *G, const double2 *D, const int *partner, int nd,
+                              double *out, hipStream_t s);
 hipError_t launch_to_f32(const double4 *src, float4 *dst, int64_t n, hipStream_t s);
 hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hipStream_t s);
 // FP64 instance on s; with a.mixed also the FP32 instance on s32 (slab

@@ -577,47 +577,33 @@ __device__ __forceinline__ void digest(const Raw &raw,
 // in the reference's order: direct (1,2) with D(C,D), (3,4) with D(A,B);
 // exchange (8,9) with D(A,D), (5) with D(B,C), (7,11) with D(A,C), (6,10)
 // with D(B,D); the first group above SENS ends the test.
-__device__ __forceinline__ double slot_max(const double *G, int nd, const int (&r)[2], int ir,
-                                           const int (&c)[2], int ic) {
-  double x = 0.0;
-#pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int w = 0; w < 2; ++w) x = fmax(x, G[(int64_t)(c[w] + ic) * nd + r[u] + ir]);
-  return x;
+// gslot and dslot hold, at the (-|m|, -|m|) element of every block pair, the
+// largest GDSC and |D| over its four sign slots (slot_tables_kernel), so each
+// bound and density factor is one load.  gab and dab are the (A,B) factors,
+// fixed for one primitive pair ij of one AB combination.
+__device__ __forceinline__ int64_t slot_at(const int (&off)[4][2], const int (&idx)[4], int r,
+                                           int c, int nd) {
+  return (int64_t)(off[c][0] + idx[c]) * nd + off[r][0] + idx[r];
 }
-__device__ __forceinline__ double slot_absmax(const double2 *D, int nd, const int (&r)[2], int ir,
-                                              const int (&c)[2], int ic) {
-  double x = 0.0;
-#pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int w = 0; w < 2; ++w) {
-      double2 d = D[(int64_t)(c[w] + ic) * nd + r[u] + ir];
-      x = fmax(x, hypot(d.x, d.y));
-    }
-  return x;
-}
-__device__ int schwarz_mask(const double *G, const double2 *D, int nd, const int (&off)[4][2],
-                            const int (&idx)[4], double sens) {
-  const double gdir = slot_max(G, nd, off[0], idx[0], off[1], idx[1]) *
-                      slot_max(G, nd, off[2], idx[2], off[3], idx[3]);
-  const double gx1 = slot_max(G, nd, off[1], idx[1], off[2], idx[2]) *
-                     slot_max(G, nd, off[0], idx[0], off[3], idx[3]);
-  const double gx2 = slot_max(G, nd, off[1], idx[1], off[3], idx[3]) *
-                     slot_max(G, nd, off[0], idx[0], off[2], idx[2]);
+__device__ int schwarz_mask(const double *gslot, const double *dslot, int nd,
+                            const int (&off)[4][2], const int (&idx)[4], double sens, double gab,
+                            double dab) {
+  const int64_t cd = slot_at(off, idx, 2, 3, nd);
+  const double gdir = gab * gslot[cd];
   int mask = 0x7ff;
-  if (gdir * slot_absmax(D, nd, off[2], idx[2], off[3], idx[3]) > sens) return mask;
+  if (gdir * dslot[cd] > sens) return mask;
   mask &= ~0x003;
-  if (gdir * slot_absmax(D, nd, off[0], idx[0], off[1], idx[1]) > sens) return mask;
+  if (gdir * dab > sens) return mask;
   mask &= ~0x00c;
-  if (gx1 * slot_absmax(D, nd, off[0], idx[0], off[3], idx[3]) > sens) return mask;
+  const double gx1 = gslot[slot_at(off, idx, 1, 2, nd)] * gslot[slot_at(off, idx, 0, 3, nd)];
+  if (gx1 * dslot[slot_at(off, idx, 0, 3, nd)] > sens) return mask;
   mask &= ~0x180;
-  if (gx1 * slot_absmax(D, nd, off[1], idx[1], off[2], idx[2]) > sens) return mask;
+  if (gx1 * dslot[slot_at(off, idx, 1, 2, nd)] > sens) return mask;
   mask &= ~0x010;
-  if (gx2 * slot_absmax(D, nd, off[0], idx[0], off[2], idx[2]) > sens) return mask;
+  const double gx2 = gslot[slot_at(off, idx, 1, 3, nd)] * gslot[slot_at(off, idx, 0, 2, nd)];
+  if (gx2 * dslot[slot_at(off, idx, 0, 2, nd)] > sens) return mask;
   mask &= ~0x440;
-  if (gx2 * slot_absmax(D, nd, off[1], idx[1], off[3], idx[3]) > sens) return mask;
+  if (gx2 * dslot[slot_at(off, idx, 1, 3, nd)] > sens) return mask;
   return 0;
 }
 
@@ -872,6 +858,13 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       const T *Gg = G + (int64_t)(4 * g * ntg) * ld;
       double accAB[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};  // batches 1,2 output
       const double2 (&dab)[2][4] = sh.dab[g];
+      // SCHWARZ's (A,B) factors for this ij (the same for every call and kl)
+      double s_gab = 0.0, s_dab = 0.0;
+      if (a.schwarz) {
+        const int64_t e = (int64_t)(ab.fb[0] + jb) * nd + ab.fa[0] + ib;
+        s_gab = a.gslot[e];
+        s_dab = a.dslot[e];
+      }
       for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
         const CDCall cd = a.cds[ci];
         const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
@@ -924,7 +917,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           int flg = cd.iflg;
           // ITOG: screening only when an SS pair is involved (ITT > 1, :12096)
           if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
-            flg &= schwarz_mask(a.gdsc, Dm, nd, off, idx, a.sens);
+            flg &= schwarz_mask(a.gslot, a.dslot, nd, off, idx, a.sens, s_gab, s_dab);
           if (a.isym != 0)
             digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                          cd.pcd[1], dab, accAB, xv, a.gdir);
@@ -1549,6 +1542,30 @@ hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
   int64_t nb = (n + bs - 1) / bs;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(zero_kernel, dim3((unsigned)nb), dim3(bs), 0, s, x, n, y, counter, dmax);
+  return hipGetLastError();
+}
+__global__ void slot_tables_kernel(const double *G, const double2 *D, const int *partner, int nd,
+                                   double *out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)nd * nd) return;
+  const int c = (int)(g / nd), r = (int)(g - (int64_t)c * nd);
+  const int rr[2] = {r, partner[r]}, cc[2] = {c, partner[c]};
+  double x = 0.0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const int64_t k = (int64_t)cc[w] * nd + rr[u];
+      x = fmax(x, G ? G[k] : hypot(D[k].x, D[k].y));
+    }
+  out[g] = x;
+}
+hipError_t launch_slot_tables(const double *G, const double2 *D, const int *partner, int nd,
+                              double *out, hipStream_t s) {
+  const int64_t n = (int64_t)nd * nd;
+  const int bs = 256;
+  hipLaunchKernelGGL(slot_tables_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, G,
+                     D, partner, nd, out);
   return hipGetLastError();
 }
 __global__ void to_f32_kernel(const double4 *src, float4 *dst, int64_t n) {
