@@ -16,6 +16,13 @@ Legs:
   cpu       the oracle (C restatement of the reference path) timed on the
             host's cores on the same build, and the GPU result checked
             against it (max |dG|, |dE_2e|)
+  serve     Poisson-arriving DHF solves through the continuous-batching
+            server: p50/p99 latency and solves/s
+
+The 12-centre ring (NDIM 1368) runs as
+  python bench.py --workload ring --schwarz --density atoms --no-cpu-baseline --serve 0
+(SCHWARZ masks at SENS 1e-12, superposed atomic densities; its oracle build
+takes tens of minutes, hence no CPU leg).
 """
 from __future__ import annotations
 
@@ -49,6 +56,44 @@ def profiled_traffic(workload: str):
     return None
 
 
+def atoms_density(pkg, system):
+    """Superposed densities of the system's atoms, each solved alone in its
+    basis with the bare-nucleus Hamiltonian: Z electrons in the lowest
+    positive-energy Kramers pairs, an odd electron shared by the last pair.
+    Hermitian, positive and normalised (elements below 1), unlike a random
+    test density; centres with the same charge and basis share one solve."""
+    import importlib
+
+    import torch
+    scf = importlib.import_module(pkg.__name__ + ".scf")
+    cache = {}
+    blocks = []
+    for c in range(system.ncnt):
+        z = float(system.charges[c])
+        key = (z, tuple(tuple(float(x) for x in e) for e in system.shells[c]))
+        if key not in cache:
+            atom = pkg.systems.System("atom", np.zeros((1, 3)), [system.shells[c]],
+                                      shape="ATOMIC", charges=[z])
+            be = scf.EngineBackend(pkg, atom)
+            try:
+                ovlp, hnuc, hkin = be.one_electron(137.035999084)
+                nskp = be.nskp
+            finally:
+                be.close()
+            _, coef, finite = scf.solve_fock(hnuc + hkin, scf.orthogonaliser(ovlp))
+            if not bool(finite):
+                raise RuntimeError("atomic guess: eigensolver returned non-finite vectors")
+            nel = int(round(z))
+            full, odd = nel // 2, nel % 2
+            occ = coef[:, nskp:nskp + 2 * (full + odd)]
+            w = torch.ones(occ.shape[1], dtype=torch.float64, device=occ.device)
+            if odd:
+                w[-2:] = 0.5
+            cache[key] = (occ.conj() * w) @ occ.T
+        blocks.append(cache[key])
+    return scf.superposed_density(system, blocks)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +103,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--stage", type=int, default=3, choices=[1, 2, 3],
                     help="integral-inclusion stage of every leg (ILEV; 3 = full Dirac-Coulomb)")
+    ap.add_argument("--schwarz", action="store_true",
+                    help="SCHWARZ masks (SENS 1e-12) on every leg, the oracle's included")
+    ap.add_argument("--density", default="random", choices=["random", "atoms"],
+                    help="random Hermitian test density, or superposed bare-nucleus atoms")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--split", type=float, default=0.1,
@@ -83,12 +132,15 @@ def main() -> int:
     if args.workload == "tlf":
         kw["hmlt"] = "DHFR"
     system = S.config_system(args.workload, **kw)
-    eng = pkg.Engine(system)
+    eng = pkg.Engine(system, schwarz=args.schwarz)
     n = eng.ndim
     flops, quartets, items = eng.plan_info(args.stage)
-    dens = S.random_density(n, seed=11, real=(system.shape != "NOSYMM"))
-
     dev = torch.device("cuda:0")
+    if args.density == "atoms":
+        dens = np.asfortranarray(atoms_density(pkg, system).cpu().numpy())
+    else:
+        dens = S.random_density(n, seed=11, real=(system.shape != "NOSYMM"))
+
     if args.copy_calibration > 0:
         nbytes = args.copy_calibration << 20
         # a fill kernel (writes nbytes) and an elementwise scale (reads and
@@ -141,7 +193,8 @@ def main() -> int:
     # mixed-precision leg: same build with FP32 items below the Schwarz split
     mixed = None
     if args.split > 0:
-        eng_m = pkg.Engine(system, precision="mixed", split_threshold=args.split)
+        eng_m = pkg.Engine(system, precision="mixed", split_threshold=args.split,
+                           schwarz=args.schwarz)
         stream_m = torch.cuda.ExternalStream(eng_m.stream(), device=dev)
         gm_t = torch.zeros_like(g_t)
         xm_t = torch.zeros_like(x_t)
@@ -158,13 +211,17 @@ def main() -> int:
         ev1.record(stream_m)
         torch.cuda.synchronize()
         ms_m = ev0.elapsed_time(ev1) / args.steps
+        build_m()
+        torch.cuda.synchronize()
+        kms_m = eng_m.stats()[1]
         gm = gm_t.cpu().numpy().T
         xm = xm_t.cpu().numpy().T
         de_m = 0.5 * np.real(np.sum(dens * ((gm - xm) - (g - x))))
         mixed = {"value": 1000.0 / ms_m, "ms_per_step": ms_m, "split_threshold": args.split,
                  "abs_dE2e_vs_fp64_Eh": float(abs(de_m)),
                  "max_abs_dG_vs_fp64": float(max(np.abs(gm - g).max(), np.abs(xm - x).max())),
-                 "speedup_vs_fp64": ms_per_step / ms_m}
+                 "speedup_vs_fp64": ms_per_step / ms_m,
+                 "quartets_per_s": quartets / (kms_m * 1e-3)}
         eng_m.close()
 
     cpu = None
@@ -175,7 +232,7 @@ def main() -> int:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         orc = pyoracle.Oracle(system)
         t0 = time.perf_counter()
-        go, xo = orc.clmfast(dens, args.stage, nthreads=threads)
+        go, xo = orc.clmfast(dens, args.stage, nthreads=threads, schwarz=args.schwarz)
         tc = time.perf_counter() - t0
         cpu = {"value": 1.0 / tc, "unit": "Fock builds/s", "cores": threads, "kind": "port",
                "sample": f"one full stage-{args.stage} Dirac-Coulomb build of the {system.name} workload "
@@ -221,12 +278,16 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32 (Schwarz split)" if headline else "f64",
-        "data": "synthetic Hermitian density; even-tempered basis and seeded geometry of the workload",
+        "data": ("superposed bare-nucleus atomic densities" if args.density == "atoms"
+                 else "synthetic Hermitian density") +
+                "; even-tempered basis and seeded geometry of the workload",
         "config": {"workload": system.name, "ndim": n,
                    "precision": (f"mixed, FP32 below split {args.split}" if headline else "fp64"),
-                   "stage": args.stage, "quartets_per_build": quartets, "work_items": items,
+                   "stage": args.stage, "schwarz": args.schwarz, "density": args.density,
+                   "quartets_per_build": quartets, "work_items": items,
                    "host_wall_s": wall},
-        "fp64": {"value": value, "ms_per_step": ms_per_step},
+        "fp64": {"value": value, "ms_per_step": ms_per_step,
+                 "quartets_per_s": quartets / (kernel_ms * 1e-3)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic(args.workload),
