@@ -394,6 +394,10 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
 }
 
 static int make_breit_plan(bdf_engine *e, int ilev) {
+  // with INTSYM off BRTFAST also loops over SL component pairs (IT = 3,
+  // :17139-17147) from tables EQSAVE does not build for it; HFSCF always runs
+  // it with INTSYM on (:5215)
+  if (!e->intsym) return fail(BDF_EUNSUPP, "Breit with intsym = 0 is not supported");
   DevBreitPlan &dp = e->bplans[ilev - 1];
   if (dp.built) return BDF_OK;
   std::string err;

@@ -1429,7 +1429,10 @@ __global__ void complete_kernel(CompleteArgs a) {
   if (g >= (int64_t)n * n) return;
   int i = (int)(g % n), j = (int)(g / n);
   if (i > j) return;
-  if (a.intsym && a.icn[i] == a.icn[j] && a.kqn[i] == a.kqn[j] && abs(a.mqn[i]) == abs(a.mqn[j]))
+  // INTSYM off: every quartet was digested directly and nothing is completed
+  // (:12174-12177 fall through to 401); on: diagonal sub-blocks are complete
+  if (!a.intsym ||
+      (a.icn[i] == a.icn[j] && a.kqn[i] == a.kqn[j] && abs(a.mqn[i]) == abs(a.mqn[j])))
     return;
   int ndim = a.ndim;
   int k = i + n, l = j + n;

@@ -129,7 +129,8 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4],
    stage ilev (multi-centre quartets only at ilev 3, :17254).  BDIR stays
    zero for closed shells; BXCH holds the LS blocks and their conjugate
    transpose SL (:17646-17664).  Needs an engine created with
-   BDF_HMLT_DHFB.  deno/wdir/wxch (open shells) must be NULL. */
+   BDF_HMLT_DHFB and intsym = 1 (BDF_EUNSUPP otherwise, as for open shells:
+   deno/wdir/wxch must be NULL). */
 int bdf_breit(bdf_engine *e, int ilev, const double _Complex *dent,
               const double _Complex *deno, int ld, double _Complex *bdir,
               double _Complex *bxch, double _Complex *wdir, double _Complex *wxch);

@@ -1764,8 +1764,10 @@ static void complete(const orc_sys *s, zc *gdir, zc *gxch) {
   for (int j = 0; j < ndim - nskp; ++j)
     for (int i = 0; i <= j; ++i) {
       int k = i + nskp, l = j + nskp;
-      if (s->intsym && s->labicn[i] == s->labicn[j] && s->labkqn[i] == s->labkqn[j] &&
-          abs(s->labmqn[i]) == abs(s->labmqn[j]))
+      /* INTSYM off: every quartet was digested directly, nothing to complete;
+         on: the diagonal sub-blocks are complete already */
+      if (!s->intsym || (s->labicn[i] == s->labicn[j] && s->labkqn[i] == s->labkqn[j] &&
+                         abs(s->labmqn[i]) == abs(s->labmqn[j])))
         continue;
       G(gdir, i, j) += conj(G(gdir, j, i)); G(gdir, j, i) = conj(G(gdir, i, j));
       G(gxch, i, j) += conj(G(gxch, j, i)); G(gxch, j, i) = conj(G(gxch, i, j));

@@ -94,6 +94,26 @@ def test_clmfast_racah1_matches_oracle(bdf, orc, systems):
     assert np.abs(x - xo).max() < FOCK_ABS
 
 
+@pytest.mark.parametrize("name", ["off", "quad"])
+def test_clmfast_intsym_off_matches_oracle(bdf, orc, systems, name):
+    s = systems[name]
+    print(f"[intsym off] {name}", flush=True)
+    d = bdf.systems.random_density(s.ndim, seed=12)
+    with bdf.Engine(s, intsym=False) as eng:
+        g, x = eng.clmfast(d, 3)
+    go, xo = orc.Oracle(s, intsym=False).clmfast(d, 3)
+    assert np.abs(g - go).max() < FOCK_ABS and np.abs(x - xo).max() < FOCK_ABS
+
+
+def test_breit_rejects_intsym_off(bdf, systems):
+    s0 = systems["off"]
+    s = bdf.systems.System(s0.name, s0.xyz, s0.shells, s0.shape, hmlt="DHFB")
+    d = bdf.systems.random_density(s.ndim, seed=13)
+    with bdf.Engine(s, intsym=False) as eng:
+        with pytest.raises(bdf.BerthaError, match="intsym"):
+            eng.breit(d, 3)
+
+
 def test_repeated_builds_agree(bdf, systems):
     s = systems["tri"]
     d = bdf.systems.random_density(s.ndim, seed=1)

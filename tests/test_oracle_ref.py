@@ -66,6 +66,22 @@ def test_clmfast_racah1_skips_one_centre(bdf, orc, ref_ok):
     assert np.abs(xr - xo).max() < 1e-12
 
 
+@pytest.mark.parametrize("name", ["diat", "off"])
+def test_clmfast_intsym_off_matches_reference(bdf, orc, ref_ok, name):
+    """INTSYM off: every quartet digested by batches 1 and 5 and no Hermitian
+    completion (:12174-12177); equal to the INTSYM-on build."""
+    s = small_systems(bdf)[name]
+    R = orc.Reference(s, intsym=False)
+    O = orc.Oracle(s, intsym=False)
+    d = bdf.systems.random_density(R.ndim, seed=12, real=(s.shape != "NOSYMM"))
+    gr, xr = R.clmfast(d, 3)
+    go, xo = O.clmfast(d, 3)
+    assert np.abs(gr - go).max() < 1e-12
+    assert np.abs(xr - xo).max() < 1e-12
+    g1, x1 = orc.Oracle(s).clmfast(d, 3)
+    assert np.abs(g1 - go).max() < 1e-12 and np.abs(x1 - xo).max() < 1e-12
+
+
 @pytest.mark.parametrize("name", ["ne", "off"])
 def test_erifast_batches_match_reference(bdf, orc, ref_ok, name):
     s = small_systems(bdf)[name]
