@@ -168,7 +168,8 @@ __global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
 
 // E(CD) copy for ERIFAST's second contraction (:12809-13119): the column
 // masks and the sign Z = (-1)^(t'+u'+v') applied once per table entry
-// instead of once per quartet; an FP32 copy of it for the mixed build.
+// instead of once per quartet, pairs in kl = l * nC + k order; an FP32 copy
+// of it for the mixed build.
 __global__ void __launch_bounds__(256) fold_kernel(EflagArgs a) {
   const int p = blockIdx.x;
   const EPair P = a.pairs[p];
@@ -183,9 +184,13 @@ __global__ void __launch_bounds__(256) fold_kernel(EflagArgs a) {
     const double4 e = a.E[P.off + w];
     const double4 o = make_double4(f[0] ? z * e.x : 0.0, f[1] ? z * e.y : 0.0,
                                    f[2] ? z * e.z : 0.0, f[3] ? z * e.w : 0.0);
-    a.Ecd[P.off + w] = o;
-    if (a.Ecd32)
-      a.Ecd32[P.off + w] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    // pair index transposed, kl = l * nC + k: the kernels give consecutive
+    // lanes consecutive k, so the digestion's G(C,D) atomics and D(C,D) loads
+    // touch consecutive matrix rows
+    const int ij = (int)(w - (int64_t)n * maxab), i = ij / P.nB, j = ij - i * P.nB;
+    const int64_t dst = P.off + (int64_t)n * maxab + j * P.nA + i;
+    a.Ecd[dst] = o;
+    if (a.Ecd32) a.Ecd32[dst] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
   }
 }
 
@@ -705,7 +710,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   T *s_eab = reinterpret_cast<T *>(sh.eab);
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
-  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int nB = cl.nbas[1], nC = cl.nbas[2], nD = cl.nbas[3];
   const int ib = it.ij / nB, jb = it.ij % nB;
   const XView xv{xacc, cl.nbas[2], nD};
   const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
@@ -721,7 +726,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
     int kl = kl0 + lane;
     if (kl < maxcd) {
-      int kb = kl / nD, lb = kl % nD;
+      int kb = kl % nC, lb = kl / nC;
       double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
       double q = ec + ed;
       double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
@@ -745,7 +750,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
     int kl = kl0 + lane;
     if (kl < maxcd) {
-      int kb = kl / nD, lb = kl % nD;
+      int kb = kl % nC, lb = kl / nC;
       double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
       T pre = (T)(2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
       for (int col = 0; col < nt; ++col) {
@@ -906,12 +911,12 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
             for (int n = 0; n < 16; ++n) {
               double re, im;
               if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
-              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+              a.rr_out[(int64_t)n * maxcd + (kl % nC) * nD + kl / nC] = make_double2(re, im);
             }
             continue;
           }
           // digestion (CLMMTG / CLMMTZ)
-          const int idx[4] = {ib, jb, kl / nD, kl % nD};
+          const int idx[4] = {ib, jb, kl % nC, kl / nC};
           const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
           int flg = cd.iflg;
@@ -986,7 +991,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 // every kl of a class, to order L
 __device__ void r_columns(const ClassDesc &cl, const double *exps, int ij, double *R, double *F,
                           int ld, int L, int lane) {
-  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int nB = cl.nbas[1], nC = cl.nbas[2], nD = cl.nbas[3];
   const int ib = ij / nB, jb = ij % nB;
   const double ea = exps[cl.exp[0] + ib], eb = exps[cl.exp[1] + jb];
   const double p = ea + eb;
@@ -994,7 +999,7 @@ __device__ void r_columns(const ClassDesc &cl, const double *exps, int ij, doubl
   const double Py = (cl.xyz[0][1] * ea + cl.xyz[1][1] * eb) / p;
   const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
   for (int kl = lane; kl < cl.maxcd; kl += 64) {
-    int kb = kl / nD, lb = kl % nD;
+    int kb = kl % nC, lb = kl / nC;
     double ec = exps[cl.exp[2] + kb], ed = exps[cl.exp[3] + lb];
     double q = ec + ed;
     double Qx = (cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q;
@@ -1061,7 +1066,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   double *G = F + (int64_t)(kMaxLfl + 1) * ld;
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
-  const int nB = cl.nbas[1], nD = cl.nbas[3];
+  const int nB = cl.nbas[1], nC = cl.nbas[2], nD = cl.nbas[3];
   const XView xv{sh.xacc, cl.nbas[2], nD};
   const int ib = it.ij / nB, jb = it.ij % nB;
   const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
@@ -1080,7 +1085,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   // RCTTFL = BFAC * PRE * R for every column; the flags are applied where
   // the reference tests them
   for (int kl = lane; kl < maxcd; kl += 64) {
-    int kb = kl / nD, lb = kl % nD;
+    int kb = kl % nC, lb = kl / nC;
     double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
     double pre = bfac * (2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
     for (int col = 0; col < nt; ++col) R[(int64_t)col * ld + kl] *= pre;
@@ -1103,7 +1108,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
     for (int t = lane; t < nt; t += 64) sh.gflag[t] = gsrc[t];
     __syncthreads();
     for (int kl = lane; kl < maxcd; kl += 64) {
-      int kb = kl / nD, lb = kl % nD;
+      int kb = kl % nC, lb = kl / nC;
       double ec = a.exps[cl.exp[2] + kb], ed = a.exps[cl.exp[3] + lb];
       double q = ec + ed;
       double pq[3] = {(cl.xyz[2][0] * ec + cl.xyz[3][0] * ed) / q - Px,
@@ -1230,12 +1235,12 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
             for (int n = 0; n < 16; ++n) {
               double re, im;
               if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
-              a.rr_out[(int64_t)n * maxcd + kl] = make_double2(re, im);
+              a.rr_out[(int64_t)n * maxcd + (kl % nC) * nD + kl / nC] = make_double2(re, im);
             }
           }
           continue;
         }
-        const int idx[4] = {ib, jb, kl / nD, kl % nD};
+        const int idx[4] = {ib, jb, kl % nC, kl / nC};
         const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
         // exchange batches only: the direct batches' accumulators and D(A,B) stay unused
