@@ -7,6 +7,10 @@ SCF settings the reference reads with it (closed-shell count NCLS, level
 shifts SHLEV, starting stage ILEV, READIN), so that the engine's SCF driver
 and serving loop can be fed the reference's own decks.
 
+solve() runs a deck on the device and, given a directory, reads and writes
+the deck's wavefunction file (<name>_<HMLT>.wfn, the reference's WFNFL) in
+the reference's record layout (scf.read_wfn / scf.write_wfn).
+
 Scope: the HFSCF tree's Dirac Hamiltonians DHFR (Coulomb), DHFB (Coulomb and
 Breit) and DHFP (Coulomb SCF, Breit added once at the end); POINT nuclei and
 Gaussian ones (GAUSS, and FERMI/UNIFM as the reference runs them without a
@@ -23,6 +27,7 @@ leaves energies unchanged.
 from __future__ import annotations
 
 import dataclasses
+import os
 import re
 from typing import List
 
@@ -222,20 +227,42 @@ def read_deck(path: str) -> Deck:
         return parse_deck(f.read())
 
 
-def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, **engine_kw):
+def wfn_path(deck: Deck, directory: str) -> str:
+    """WFNFL: <directory>/<name>_<HMLT>.wfn (the reference's
+    'output/'//MOLCL//'_'//HMLT//'.wfn', bertha-scf.f:241)."""
+    return os.path.join(directory, f"{deck.name}_{deck.hmlt}.wfn")
+
+
+def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, wfn_dir=None,
+          **engine_kw):
     """HFSCF of a deck on the device (DHFR/DHFB), or a DHFR solve followed by
     one Breit build with the converged density (DHFP).  Returns
-    (SCFResult, Breit energy added after the SCF or None)."""
+    (SCFResult, Breit energy added after the SCF or None).
+
+    With wfn_dir, the wavefunction file of the deck is used as the reference
+    uses it: a RESUME deck (READIN) whose file exists starts from the density
+    of its spinors (stage 3 from the first iteration, bertha-scf.f:534-536,
+    597-604), and the converged eigenvalues and spinors are written back."""
+    import torch
+
     from . import scf as _scf
+    dens0 = None
+    path = wfn_path(deck, wfn_dir) if wfn_dir is not None else None
+    if path is not None and deck.readin and os.path.exists(path):
+        _, coef = _scf.read_wfn(path, deck.system.ndim)
+        nskp = deck.system.ndim // 2
+        dens0 = _scf.density_from_coefficients(torch.from_numpy(coef), nskp, deck.nocc)
     be = _scf.EngineBackend(pkg, deck.system, **engine_kw)
     try:
         res = _scf.run_scf(deck.system, deck.nocc, be, shifts=deck.shifts, ilev=deck.ilev,
-                           max_iter=max_iter, verbose=verbose)
+                           max_iter=max_iter, verbose=verbose, dens0=dens0)
     finally:
         be.close()
+    if path is not None:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        _scf.write_wfn(path, res.eigenvalues, res.coefficients)
     if deck.hmlt != "DHFP":
         return res, None
-    import torch
     sb = dataclasses.replace(deck.system, hmlt="DHFB")
     bb = _scf.EngineBackend(pkg, sb, **engine_kw)
     try:

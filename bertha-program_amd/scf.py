@@ -24,6 +24,7 @@ column-major arrays are exchanged through transposed copies.
 from __future__ import annotations
 
 import dataclasses
+import re
 from typing import List, Optional
 
 import numpy as np
@@ -65,8 +66,9 @@ class SCFResult:
     converged: bool
     history: List[float]
     stages: List[int]
-    eigenvalues: torch.Tensor
+    eigenvalues: torch.Tensor       # level shift taken off the virtuals, as written to WFNFL
     density: torch.Tensor
+    coefficients: Optional[torch.Tensor] = None  # COEF(J, I): column I is spinor I
 
 
 class EngineBackend:
@@ -275,9 +277,11 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
             fock = fock + bdir - bxch
         e = energies(system, dens, hkin, hnuc, gdir, gxch, bdir, bxch)
         etot = e.total
+        shifted = 0.0
         if it != 1 and coef is not None:
             a = ovlp @ coef[:, nskp + nocc:]
             fock = fock + shlv * (a @ a.conj().T)
+            shifted = shlv
         nmlev[ilev] += 1
         fock = diis(nmlev[ilev], fock, dens, ovlp)
         eig, coef, finite = solve_fock(fock, li)
@@ -324,8 +328,82 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
         elif wedn[-1] < ENRGTOL and dnrm < DSTYTOL:
             converged = True
             break
+    if eig is not None and shifted:
+        # the virtuals' level shift comes off before EIGN is written (:5570-5575)
+        eig = eig.clone()
+        eig[nskp + nocc:] -= shifted
     return SCFResult(energy=e.total, energies=e, iterations=it, converged=converged,
-                     history=history, stages=stages, eigenvalues=eig, density=dens)
+                     history=history, stages=stages, eigenvalues=eig, density=dens,
+                     coefficients=coef)
+
+
+def density_from_coefficients(coef, nskp: int, nocc: int):
+    """DENSTY's closed-shell density of the aufbau occupation: the nocc
+    spinors above the nskp negative-energy ones, D_ij = sum_o conj(C_io) C_jo."""
+    occ = coef[:, nskp:nskp + nocc]
+    return occ.conj() @ occ.T
+
+
+def write_wfn(path: str, eigenvalues, coef) -> None:
+    """The reference's wavefunction file (WFNFL, bertha-scf.f:5578-5583): one
+    list-directed record per spinor I, EIGN(I) followed by COEF(J,I) for
+    J = 1..NDIM as (re,im) pairs, 17 significant digits."""
+    w = np.asarray(eigenvalues.detach().cpu() if hasattr(eigenvalues, "detach") else eigenvalues,
+                   dtype=np.float64)
+    c = np.asarray(coef.detach().cpu() if hasattr(coef, "detach") else coef, dtype=np.complex128)
+    n = c.shape[0]
+    if c.shape != (n, n) or w.shape != (n,):
+        raise ValueError("write_wfn: eigenvalues (ndim,) and coefficients (ndim, ndim) expected")
+    with open(path, "w") as f:
+        for i in range(n):
+            col = c[:, i]
+            f.write(f" {w[i]:.17G} " + " ".join(f"({z.real:.17G},{z.imag:.17G})" for z in col)
+                    + "\n")
+
+
+# a list-directed value: a complex constant (re,im) or a real
+_WFN_TOKEN = re.compile(r"\(\s*([^,()\s]+)\s*,\s*([^,()\s]+)\s*\)|([^,\s()]+)")
+
+
+def read_wfn(path: str, ndim: int):
+    """Read WFNFL as the reference does (bertha-scf.f:597-604): NDIM
+    list-directed records READ(8,*) EIGN(I),(COEF(J,I),J=1,NDIM); a record may
+    continue over several lines, values may be separated by blanks or commas
+    and use D exponents.  Returns (eigenvalues (ndim,), coefficients
+    (ndim, ndim)) as numpy arrays."""
+    def num(t):
+        try:
+            return float(t.replace("D", "E").replace("d", "e"))
+        except ValueError:
+            raise ValueError(f"read_wfn: not a number: {t!r}") from None
+
+    w = np.zeros(ndim)
+    c = np.zeros((ndim, ndim), dtype=np.complex128)
+    with open(path) as f:
+        lines = iter(f.read().splitlines())
+        for i in range(ndim):
+            # a record continues over lines until it holds 1 + NDIM values; a
+            # complex constant may itself be split over two lines (Fortran
+            # runtimes wrap list-directed output)
+            buf, vals = "", []
+            while len(vals) < ndim + 1:
+                try:
+                    buf += " " + next(lines)
+                except StopIteration:
+                    raise ValueError(f"read_wfn: file ends inside record {i + 1}") from None
+                if buf.count("(") > buf.count(")"):
+                    continue
+                vals = []
+                for m in _WFN_TOKEN.finditer(buf):
+                    if m.group(3) is not None:
+                        vals.append(num(m.group(3)))
+                    else:
+                        vals.append(complex(num(m.group(1)), num(m.group(2))))
+            if isinstance(vals[0], complex):
+                raise ValueError(f"read_wfn: record {i + 1} starts with a complex value")
+            w[i] = vals[0]
+            c[:, i] = vals[1:ndim + 1]
+    return w, c
 
 
 def superposed_density(system: System, atom_densities) -> torch.Tensor:

@@ -168,3 +168,24 @@ def test_deck_solve_matches_the_driver(bdf, deck):
     assert abs(res_p.energy - res.energy) < 1e-8
     # the oracle's Breit build of its converged Ne density gives 0.0166147 Eh
     assert breit_p is not None and abs(breit_p - 0.0166147) < 1e-6
+
+
+@pytest.mark.gpu
+def test_deck_resumes_from_its_wavefunction_file(bdf, deck, tmp_path):
+    """The solve writes <name>_<HMLT>.wfn; the same deck with RESUME (READIN)
+    starts from it and converges at once to the same energy."""
+    import os
+    scf = importlib.import_module(bdf.__name__ + ".scf")
+    print("[deck] Ne write and resume", flush=True)
+    d = deck.parse_deck(NE_DECK)
+    res, _ = deck.solve(bdf, d, wfn_dir=str(tmp_path))
+    path = deck.wfn_path(d, str(tmp_path))
+    assert res.converged and os.path.basename(path) == "Ne_DHFR.wfn" and os.path.exists(path)
+    w, c = scf.read_wfn(path, d.system.ndim)
+    assert np.array_equal(w, res.eigenvalues.cpu().numpy())
+    assert np.array_equal(c, res.coefficients.cpu().numpy())
+    resume = deck.parse_deck(NE_DECK.replace(".FALSE.", ".TRUE.", 1))
+    assert resume.readin
+    r2, _ = deck.solve(bdf, resume, wfn_dir=str(tmp_path))
+    assert r2.converged and abs(r2.energy - res.energy) < 1e-8
+    assert r2.iterations <= 4 < res.iterations
