@@ -27,6 +27,7 @@ takes tens of minutes, hence no CPU leg).
 from __future__ import annotations
 
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -249,7 +250,14 @@ def main() -> int:
         import importlib
         sv = importlib.import_module(pkg.__name__ + ".serve")
         reqs = sv.request_mix(args.serve, seed=args.seed, rate=args.serve_rate)
-        server = sv.Server(pkg, slots=args.serve_slots)
+        finished = itertools.count(1)
+
+        def progress(r):
+            k = next(finished)  # atomic under the GIL: workers call this concurrently
+            if k % 32 == 0:
+                print(f"serve: {k} of {len(reqs)} solves done", file=sys.stderr, flush=True)
+
+        server = sv.Server(pkg, slots=args.serve_slots, on_result=progress)
         t0 = time.perf_counter()
         res = server.run(reqs)
         span = time.perf_counter() - t0

@@ -84,8 +84,10 @@ class Server:
 
     def __init__(self, pkg, slots: int = 4, hbm_bytes: Optional[int] = None,
                  workgroups: int = 0, device: str = "cuda:0", max_iter: int = 100,
-                 damping: float = 0.5, atomic_guess: bool = True):
+                 damping: float = 0.5, atomic_guess: bool = True, on_result=None):
         self.pkg = pkg
+        # called with each Result as its solve finishes (from a worker thread)
+        self.on_result = on_result
         # molecules start from the superposed densities of their atoms in the
         # same basis (closed-shell atoms only), cached across requests
         self.atomic_guess = atomic_guess
@@ -200,6 +202,8 @@ class Server:
             with self._lock:
                 self.results.append(res)
             self._release(need)
+            if self.on_result is not None:
+                self.on_result(res)
 
     def run(self, requests: Sequence[Request]) -> List[Result]:
         """Serve requests at their arrival times; returns results by id."""
