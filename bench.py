@@ -20,9 +20,10 @@ Legs:
             server: p50/p99 latency and solves/s
 
 The 12-centre ring (NDIM 1368) runs as
-  python bench.py --workload ring --schwarz --density atoms --no-cpu-baseline --serve 0
-(SCHWARZ masks at SENS 1e-12, superposed atomic densities; its oracle build
-takes tens of minutes, hence no CPU leg).
+  python bench.py --workload ring --schwarz rigorous --density atoms --no-cpu-baseline --serve 0
+(Schwarz screening at SENS 1e-12 with every dropped term below it,
+superposed atomic densities; its oracle build takes tens of minutes, hence no
+CPU leg).
 """
 from __future__ import annotations
 
@@ -104,8 +105,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--stage", type=int, default=3, choices=[1, 2, 3],
                     help="integral-inclusion stage of every leg (ILEV; 3 = full Dirac-Coulomb)")
-    ap.add_argument("--schwarz", action="store_true",
-                    help="SCHWARZ masks (SENS 1e-12) on every leg, the oracle's included")
+    ap.add_argument("--schwarz", nargs="?", const="reference", default=None,
+                    choices=["reference", "rigorous"],
+                    help="SCHWARZ masks at SENS 1e-12 on every leg, the oracle's included: the "
+                         "reference's (GDSC bounds) or rigorous (Q(a,b) Q(c,d) on every group)")
     ap.add_argument("--density", default="random", choices=["random", "atoms"],
                     help="random Hermitian test density, or superposed bare-nucleus atoms")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -133,7 +136,8 @@ def main() -> int:
     if args.workload == "tlf":
         kw["hmlt"] = "DHFR"
     system = S.config_system(args.workload, **kw)
-    eng = pkg.Engine(system, schwarz=args.schwarz)
+    schwarz = {None: False, "reference": True, "rigorous": "rigorous"}[args.schwarz]
+    eng = pkg.Engine(system, schwarz=schwarz)
     n = eng.ndim
     flops, quartets, items = eng.plan_info(args.stage)
     dev = torch.device("cuda:0")
@@ -195,7 +199,7 @@ def main() -> int:
     mixed = None
     if args.split > 0:
         eng_m = pkg.Engine(system, precision="mixed", split_threshold=args.split,
-                           schwarz=args.schwarz)
+                           schwarz=schwarz)
         stream_m = torch.cuda.ExternalStream(eng_m.stream(), device=dev)
         gm_t = torch.zeros_like(g_t)
         xm_t = torch.zeros_like(x_t)
@@ -233,7 +237,7 @@ def main() -> int:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         orc = pyoracle.Oracle(system)
         t0 = time.perf_counter()
-        go, xo = orc.clmfast(dens, args.stage, nthreads=threads, schwarz=args.schwarz)
+        go, xo = orc.clmfast(dens, args.stage, nthreads=threads, schwarz=schwarz)
         tc = time.perf_counter() - t0
         cpu = {"value": 1.0 / tc, "unit": "Fock builds/s", "cores": threads, "kind": "port",
                "sample": f"one full stage-{args.stage} Dirac-Coulomb build of the {system.name} workload "

@@ -109,6 +109,14 @@ def library() -> _C.CDLL:
     return _lib
 
 
+def _schwarz_mode(schwarz) -> int:
+    """False/0: off; True/1: the reference's SCHWARZ masks (GDSC);
+    "rigorous"/2: the masks with every group bounded by Q(a,b) Q(c,d)."""
+    if schwarz == "rigorous" or schwarz == 2:
+        return 2
+    return 1 if schwarz else 0
+
+
 def _check(rc: int) -> None:
     if rc != BDF_OK:
         raise BerthaError(rc, library().bdf_last_error().decode())
@@ -123,7 +131,7 @@ class Engine:
     """
 
     def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
-                 device: int = 0, schwarz: bool = False, sens: float = 0.0,
+                 device: int = 0, schwarz=False, sens: float = 0.0,
                  precision: str = "fp64", split_threshold: float = 0.0, gaunt: bool = False,
                  workgroups: int = 0):
         lib = library()
@@ -137,7 +145,7 @@ class Engine:
                     SHAPES[system.shape], HMLTS[system.hmlt])
         prec = {"fp64": 0, "mixed": 1}[precision]
         o = _Options(1 if intsym else 0, 1 if racah1 else 0, prec, split_threshold, device,
-                     1 if schwarz else 0, sens, 1 if gaunt else 0, workgroups)
+                     _schwarz_mode(schwarz), sens, 1 if gaunt else 0, workgroups)
         h = _C.c_void_p()
         nd = _C.c_int()
         ns = _C.c_int()

@@ -258,7 +258,10 @@ static int make_tables(bdf_engine *e) {
     HIPCHK(e->partner.upload(partner, e->stream));
     HIPCHK(e->gslot.alloc(nn));
     HIPCHK(e->dslot.alloc(nn));
-    HIPCHK(launch_slot_tables(e->gdsc.p, nullptr, e->partner.p, B.ndim, e->gslot.p, e->stream));
+    // mode 1 bounds with GDSC as the reference; mode 2 with the Cauchy-Schwarz
+    // factors Q
+    HIPCHK(launch_slot_tables(e->schwarz == 2 ? e->qbnd.p : e->gdsc.p, nullptr, e->partner.p,
+                              B.ndim, e->gslot.p, e->stream));
   }
   e->qbnd_host.resize(nn);
   HIPCHK(hipMemcpyAsync(e->qbnd_host.data(), e->qbnd.p, nn * sizeof(double),
@@ -360,7 +363,7 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.gslot = e->gslot.p;
   a.dslot = e->dslot.p;
   a.sens = e->sens;
-  a.schwarz = (e->schwarz && !rr_out) ? 1 : 0;
+  a.schwarz = rr_out ? 0 : e->schwarz;
   a.nskp = e->B.nskp;
   a.mixed = mixed ? 1 : 0;
   a.split = e->split;
@@ -485,7 +488,8 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   e->isym = sys->shape;
   e->intsym = o.intsym ? 1 : 0;
   e->racah1 = o.racah1 ? 1 : 0;
-  e->schwarz = o.schwarz ? 1 : 0;
+  if (o.schwarz < 0 || o.schwarz > 2) return fail(BDF_EINVAL, "bdf_create: schwarz must be 0, 1 or 2");
+  e->schwarz = o.schwarz;
   e->sens = o.sens > 0.0 ? o.sens : 1.0e-12;
   e->mixed = o.precision == BDF_PREC_MIXED;
   e->breit = sys->hmlt == BDF_HMLT_DHFB;

@@ -590,9 +590,11 @@ __device__ __forceinline__ int64_t slot_at(const int (&off)[4][2], const int (&i
                                            int c, int nd) {
   return (int64_t)(off[c][0] + idx[c]) * nd + off[r][0] + idx[r];
 }
+// With rigorous set (mode 2) gslot holds the Cauchy-Schwarz factors Q and
+// every group is bounded by Q(A,B) Q(C,D), a bound of |(ab|cd)| itself.
 __device__ int schwarz_mask(const double *gslot, const double *dslot, int nd,
                             const int (&off)[4][2], const int (&idx)[4], double sens, double gab,
-                            double dab) {
+                            double dab, bool rigorous) {
   const int64_t cd = slot_at(off, idx, 2, 3, nd);
   const double gdir = gab * gslot[cd];
   int mask = 0x7ff;
@@ -600,12 +602,14 @@ __device__ int schwarz_mask(const double *gslot, const double *dslot, int nd,
   mask &= ~0x003;
   if (gdir * dab > sens) return mask;
   mask &= ~0x00c;
-  const double gx1 = gslot[slot_at(off, idx, 1, 2, nd)] * gslot[slot_at(off, idx, 0, 3, nd)];
+  const double gx1 =
+      rigorous ? gdir : gslot[slot_at(off, idx, 1, 2, nd)] * gslot[slot_at(off, idx, 0, 3, nd)];
   if (gx1 * dslot[slot_at(off, idx, 0, 3, nd)] > sens) return mask;
   mask &= ~0x180;
   if (gx1 * dslot[slot_at(off, idx, 1, 2, nd)] > sens) return mask;
   mask &= ~0x010;
-  const double gx2 = gslot[slot_at(off, idx, 1, 3, nd)] * gslot[slot_at(off, idx, 0, 2, nd)];
+  const double gx2 =
+      rigorous ? gdir : gslot[slot_at(off, idx, 1, 3, nd)] * gslot[slot_at(off, idx, 0, 2, nd)];
   if (gx2 * dslot[slot_at(off, idx, 0, 2, nd)] > sens) return mask;
   mask &= ~0x440;
   if (gx2 * dslot[slot_at(off, idx, 1, 3, nd)] > sens) return mask;
@@ -922,7 +926,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           int flg = cd.iflg;
           // ITOG: screening only when an SS pair is involved (ITT > 1, :12096)
           if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
-            flg &= schwarz_mask(a.gslot, a.dslot, nd, off, idx, a.sens, s_gab, s_dab);
+            flg &= schwarz_mask(a.gslot, a.dslot, nd, off, idx, a.sens, s_gab, s_dab, a.schwarz == 2);
           if (a.isym != 0)
             digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                          cd.pcd[1], dab, accAB, xv, a.gdir);

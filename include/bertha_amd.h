@@ -76,8 +76,12 @@ typedef struct {
                              Q(ij) max Q(kl) max|D| is below it runs in FP32
                              (<= 0: 0.1); Q = sqrt|(ab|ba)| */
   int device;          /* HIP device ordinal */
-  int schwarz;         /* SCHWRZ: SCHWARZ masks from CLMDIAG bounds (reference
-                          default 0, bertha-scf.f:5257) */
+  int schwarz;         /* SCHWRZ (reference default 0, bertha-scf.f:5257):
+                          1 = SCHWARZ masks from CLMDIAG's GDSC, as the
+                          reference; 2 = the same decision tree with every
+                          group bounded by Q(a,b) Q(c,d), Q = sqrt|(ab|ba)|,
+                          so no dropped term exceeds sens (GDSC is no bound
+                          for complex spinor quartets) */
   double sens;         /* SENS of the masks; <= 0 selects the reference's 1e-12
                           (bertha-scf.f:11567) */
   int gaunt;           /* GAUNT2: Gaunt interaction instead of Breit in

@@ -133,8 +133,10 @@ struct orc_sys {
   double *e0ll[4], *e0ss[4];
   long n0ll, n0ss;
   long *iad0ll, *iad0ss;
-  /* Schwarz screening (SCHWRZ): GDSC from CLMDIAG, SENS */
+  /* Schwarz screening (SCHWRZ): GDSC from CLMDIAG, SENS; mode 2 bounds every
+     group by the Cauchy-Schwarz factors qbnd (orc_pairbound) instead */
   double *gdsc; /* [ndim][ndim] column-major, NULL until computed */
+  double *qbnd; /* [ndim][ndim], mode 2 only */
   int schwarz;
   double sens;
   /* EILS tables (EILSGN): plane 4*(ix-1)+c, c = Re/Im E11, Re/Im E21 */
@@ -1399,7 +1401,10 @@ static double slot_absmax(const zc *m, int n, const int r[2], int ir, const int 
    so the groups after it stay on. */
 static void schwarz_masks(const orc_sys *s, const digest_t *d, const zc *dent, int *msk) {
   const int n = s->ndim;
-  const double *G = s->gdsc;
+  /* mode 2 (not the reference's): every group bounded by Q(A,B) Q(C,D), which
+     bounds |(ab|cd)| for every quartet, so no dropped term exceeds SENS */
+  const int rigorous = s->schwarz == 2;
+  const double *G = rigorous ? s->qbnd : s->gdsc;
   const double sens = s->sens;
   const int i = d->ibas - 1, j = d->jbas - 1;
   const int *A = d->nlab[0], *B = d->nlab[1], *C = d->nlab[2], *D = d->nlab[3];
@@ -1408,8 +1413,10 @@ static void schwarz_masks(const orc_sys *s, const digest_t *d, const zc *dent, i
   for (int k = 0; k < d->nbas[2]; ++k)
     for (int l = 0; l < d->nbas[3]; ++l, ++m) {
       const double gdir = gab * slot_max(G, n, C, k, D, l);
-      const double gx1 = slot_max(G, n, B, j, C, k) * slot_max(G, n, A, i, D, l);
-      const double gx2 = slot_max(G, n, B, j, D, l) * slot_max(G, n, A, i, C, k);
+      const double gx1 =
+          rigorous ? gdir : slot_max(G, n, B, j, C, k) * slot_max(G, n, A, i, D, l);
+      const double gx2 =
+          rigorous ? gdir : slot_max(G, n, B, j, D, l) * slot_max(G, n, A, i, C, k);
       const double xl[6] = {gdir * slot_absmax(dent, n, C, k, D, l),
                             gdir * slot_absmax(dent, n, A, i, B, j),
                             gx1 * slot_absmax(dent, n, A, i, D, l),
@@ -1547,7 +1554,11 @@ void orc_set_schwarz(orc_sys *s, int on, double sens) {
   s->drop_count = 0.0;
   s->schwarz = on;
   s->sens = sens > 0.0 ? sens : 1.0e-12; /* CLMFAST DATA SENS (11567) */
-  if (on && !s->gdsc) orc_clmdiag(s, NULL);
+  if (on == 1 && !s->gdsc) orc_clmdiag(s, NULL);
+  if (on == 2 && !s->qbnd) {
+    s->qbnd = malloc(sizeof(double) * s->ndim * s->ndim);
+    orc_pairbound(s, s->qbnd);
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2627,6 +2638,7 @@ void orc_destroy(orc_sys *s) {
   for (int c = 0; c < 4; ++c) { free(s->e0ll[c]); free(s->e0ss[c]); }
   free(s->iad0ll); free(s->iad0ss);
   free(s->gdsc);
+  free(s->qbnd);
   for (int c = 0; c < 12; ++c) free(s->eils[c]);
   free(s->iadls);
   free(s);

@@ -95,8 +95,10 @@ void orc_clmdiag(orc_sys *s, double *gdsc);
    routine) */
 void orc_pairbound(orc_sys *s, double *out);
 
-/* SCHWRZ toggle for orc_clmfast*: SCHWARZ masks (bertha-scf.f:37889) with
-   sensitivity sens (<= 0: the reference's 1e-12) */
+/* SCHWRZ toggle for orc_clmfast*: on = 1, SCHWARZ masks (bertha-scf.f:37889)
+   from CLMDIAG's GDSC; on = 2, the same decision tree with every group bounded
+   by the Cauchy-Schwarz factors of orc_pairbound (not the reference's: no
+   dropped term exceeds sens); sensitivity sens (<= 0: the reference's 1e-12) */
 void orc_set_schwarz(orc_sys *s, int on, double sens);
 
 /* largest |term| among the digestion terms the masks dropped since the last

@@ -188,7 +188,8 @@ class Oracle:
         d = np.asfortranarray(dent, dtype=np.complex128)
         g = np.zeros((n, n), dtype=np.complex128, order="F")
         x = np.zeros((n, n), dtype=np.complex128, order="F")
-        _oracle_lib().orc_set_schwarz(self._h, 1 if schwarz else 0, sens)
+        mode = 2 if schwarz == "rigorous" else (1 if schwarz else 0)
+        _oracle_lib().orc_set_schwarz(self._h, mode, sens)
         if nthreads > 1:
             _oracle_lib().orc_clmfast_mt(self._h, ilev, d.ctypes.data, g.ctypes.data,
                                          x.ctypes.data, nthreads)

@@ -120,3 +120,26 @@ def test_schwarz_mask_limits(bdf, orc, systems, name):
         O.clmfast(d, 3, schwarz=True, sens=sens)
         counts.append(O.schwarz_stats()[1])
     assert counts == sorted(counts) and counts[-1] > counts[0]
+
+
+@pytest.mark.parametrize("name", ["diat", "tri", "quad", "ring4"])
+def test_rigorous_masks_drop_nothing_above_sens(bdf, orc, name):
+    """Mode 2 bounds every group by Q(A,B) Q(C,D) >= |(ab|cd)|: no dropped
+    digestion term exceeds SENS, while the reference's GDSC masks drop terms
+    many orders above it (GDSC is no bound for complex spinor quartets);
+    and the rigorous build stays within (dropped terms) x SENS of the
+    unscreened one."""
+    s = small_systems(bdf)[name]
+    O = orc.Oracle(s)
+    d = bdf.systems.random_density(O.ndim, seed=3, real=(s.shape != "NOSYMM"))
+    g0, x0 = O.clmfast(d, 3)
+    counts = []
+    for sens in (1e-12, 1e-6, 1e-3):
+        g2, x2 = O.clmfast(d, 3, schwarz="rigorous", sens=sens)
+        drop_max, drop_count = O.schwarz_stats()
+        assert drop_max <= sens
+        assert max(np.abs(g2 - g0).max(), np.abs(x2 - x0).max()) <= drop_count * sens
+        counts.append(drop_count)
+    assert counts == sorted(counts) and counts[-1] > 0
+    O.clmfast(d, 3, schwarz=True, sens=1e-12)
+    assert O.schwarz_stats()[0] > 1e-6
