@@ -583,8 +583,9 @@ __device__ __forceinline__ void digest(const Raw &raw,
 // exchange (8,9) with D(A,D), (5) with D(B,C), (7,11) with D(A,C), (6,10)
 // with D(B,D); the first group above SENS ends the test.
 // gslot and dslot hold, at the (-|m|, -|m|) element of every block pair, the
-// largest GDSC and |D| over its four sign slots (slot_tables_kernel), so each
-// bound and density factor is one load.  gab and dab are the (A,B) factors,
+// largest GDSC (the Cauchy-Schwarz factor Q in mode 2) and |D| over its four
+// sign slots (slot_tables_kernel), so each bound and density factor is one
+// load.  gab and dab are the (A,B) factors,
 // fixed for one primitive pair ij of one AB combination.
 __device__ __forceinline__ int64_t slot_at(const int (&off)[4][2], const int (&idx)[4], int r,
                                            int c, int nd) {
