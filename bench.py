@@ -111,6 +111,8 @@ def main() -> int:
                          "reference's (GDSC bounds) or rigorous (Q(a,b) Q(c,d) on every group)")
     ap.add_argument("--density", default="random", choices=["random", "atoms"],
                     help="random Hermitian test density, or superposed bare-nucleus atoms")
+    ap.add_argument("--breit", type=int, default=0, metavar="K",
+                    help="also time K Breit builds (BRTFAST) of the same density (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--split", type=float, default=0.1,
@@ -229,6 +231,29 @@ def main() -> int:
                  "quartets_per_s": quartets / (kms_m * 1e-3)}
         eng_m.close()
 
+    # Breit leg: BRTFAST of the same density on an engine with the LS tables
+    breit = None
+    if args.breit > 0:
+        import dataclasses
+        eng_b = pkg.Engine(dataclasses.replace(system, hmlt="DHFB"))
+        bflops, bquartets, bitems = eng_b.breit_plan_info(args.stage)
+        b_t = torch.zeros_like(g_t)
+        xb_t = torch.zeros_like(x_t)
+        eng_b.breit_device(d_t.data_ptr(), b_t.data_ptr(), xb_t.data_ptr(), args.stage)
+        torch.cuda.synchronize()
+        bms = []
+        for _ in range(args.breit):
+            eng_b.breit_device(d_t.data_ptr(), b_t.data_ptr(), xb_t.data_ptr(), args.stage)
+            torch.cuda.synchronize()
+            bms.append(eng_b.stats()[1])
+            print(f"breit: build {len(bms)} of {args.breit}, {bms[-1]:.1f} ms", file=sys.stderr,
+                  flush=True)
+        bmean = float(np.mean(bms))
+        breit = {"builds": args.breit, "kernel_ms": bmean, "flops_per_build": bflops,
+                 "quartets_per_build": bquartets, "work_items": bitems,
+                 "tflops": bflops / (bmean * 1e-3) / 1e12}
+        eng_b.close()
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -308,6 +333,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "parity": parity,
         "mixed": mixed,
+        "breit": breit,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
