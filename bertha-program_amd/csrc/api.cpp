@@ -58,6 +58,7 @@ struct DevBuf {
 // device copy of one stage's plan
 struct DevPlan {
   int built = 0;
+  int xacc_n = 0, blocks = 1;  // exchange-accumulator doubles, workgroups per launch
   Plan P;
   DevBuf<double> qitem;  // Schwarz factor of each work item (mixed precision)
   DevBuf<ClassDesc> classes;
@@ -96,7 +97,9 @@ struct bdf_engine {
   DevPlan plans[3];
   DevBuf<int> counter;
   DevBuf<double> slab;
-  int nblocks = 0;
+  int nblocks = 0;        // the caller's persistent workgroups (user_blocks)
+  int cus = 0, user_blocks = 0;
+  int slab_blocks = 0;    // workgroups the scratch holds
   int64_t slab_stride = 0;
   int slab_ld = 0, slab_rows = 0;
   DevBuf<double2> dent, gdir, gxch;
@@ -125,17 +128,32 @@ struct bdf_engine {
 
 static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double> &q);
 
-// per-workgroup scratch: R rows, Boys rows and g_rows G rows, each ld
-// doubles (ld = largest kl count rounded to the wavefront)
-static int ensure_slab(bdf_engine *e, int maxcd, int g_rows) {
+// persistent workgroups of a launch: the caller's count (bdf_options.
+// workgroups), else per_cu per compute unit
+static int blocks_for(const bdf_engine *e, int per_cu) {
+  return e->user_blocks ? e->nblocks : e->cus * per_cu;
+}
+
+// exchange-accumulator doubles of a plan's largest class (the Coulomb
+// kernel's dynamic LDS)
+static int plan_xacc(const Plan &P) {
+  int n = 0;
+  for (const ClassDesc &cl : P.classes) n = std::max(n, 2 * (12 * cl.nbas[3] + 16 * cl.nbas[2]));
+  return n;
+}
+
+// per-workgroup scratch for `blocks` workgroups: R rows, Boys rows and g_rows
+// G rows, each ld doubles (ld = largest kl count rounded to the wavefront)
+static int ensure_slab(bdf_engine *e, int maxcd, int g_rows, int blocks) {
   int ld = ((std::max(maxcd, 1) + 63) / 64) * 64;
   int rows = ntuv(kMaxLfl) + kMaxLfl + 1 + g_rows;
-  if (ld > e->slab_ld || rows > e->slab_rows || !e->slab.p) {
+  if (ld > e->slab_ld || rows > e->slab_rows || blocks > e->slab_blocks || !e->slab.p) {
     e->slab_ld = std::max(ld, e->slab_ld);
     e->slab_rows = std::max(rows, e->slab_rows);
+    e->slab_blocks = std::max(blocks, e->slab_blocks);
     e->slab_stride = (int64_t)e->slab_rows * e->slab_ld;
     // the mixed build's FP32 instance runs concurrently on a second set of blocks
-    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->nblocks * (e->mixed ? 2 : 1)));
+    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->slab_blocks * (e->mixed ? 2 : 1)));
   }
   return BDF_OK;
 }
@@ -155,7 +173,10 @@ static int make_plan(bdf_engine *e, int ilev) {
     item_factors(e, dp.P, q);
     HIPCHK(dp.qitem.upload(q, e->stream));
   }
-  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * ntuv(kMaxLab));
+  // four workgroups per CU for each wave per SIMD the kernel runs at
+  dp.xacc_n = plan_xacc(dp.P);
+  dp.blocks = blocks_for(e, 4 * fock_waves(dp.xacc_n));
+  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * ntuv(kMaxLab), dp.blocks);
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -372,10 +393,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.E32 = e->E32.p;
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
-  a.xacc_n = 0;
-  for (const ClassDesc &cl : dp.P.classes)
-    a.xacc_n = std::max(a.xacc_n, 2 * (12 * cl.nbas[3] + 16 * cl.nbas[2]));
-  int nb = std::min(e->nblocks, std::max(1, a.nitems));
+  a.xacc_n = plan_xacc(dp.P);
+  const int nb = std::min(blocks_for(e, 4 * fock_waves(a.xacc_n)), std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));
   HIPCHK(hipEventRecord(e->ev2, e->stream));
@@ -417,7 +436,7 @@ static int make_breit_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)std::max(dp.P.nflag, 1)));
-  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1));
+  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1), blocks_for(e, 8));
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -457,7 +476,7 @@ static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, dou
   a.Ecd = e->Ecd.p;
   a.rr_out = rr_out;
   HIPCHK(hipEventRecord(e->ev0, e->stream));
-  HIPCHK(launch_breit(a, e->nblocks, e->stream));
+  HIPCHK(launch_breit(a, blocks_for(e, 8), e->stream));
   HIPCHK(hipEventRecord(e->ev2, e->stream));
   HIPCHK(hipEventRecord(e->ev1, e->stream));
   e->last_quartets = dp.P.quartets;
@@ -562,7 +581,9 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     HIPCHK(e->dmax.alloc(1));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, e->device));
-    e->nblocks = o.workgroups > 0 ? o.workgroups : prop.multiProcessorCount * 8;
+    e->cus = prop.multiProcessorCount;
+    e->user_blocks = o.workgroups > 0;
+    e->nblocks = o.workgroups;
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->mixed) {
       HIPCHK(e->E32.alloc((size_t)e->T.total));
@@ -704,7 +725,7 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
-  int src = ensure_slab(e, mcd, 4 * ntuv(kMaxLab));
+  int src = ensure_slab(e, mcd, 4 * ntuv(kMaxLab), 1);
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
@@ -904,7 +925,7 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)dp.P.nflag));
-  int src = ensure_slab(e, mcd, 12 * ntcd);
+  int src = ensure_slab(e, mcd, 12 * ntcd, 1);
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));

@@ -511,11 +511,12 @@ __device__ __forceinline__ void lds_add4(const XView &x, bool kbatch, int which,
 
 // digestion of one RR batch element (CLMMTG :13902-14392, CLMMTZ
 // :13196-13514 when REAL): batches 1-2 into registers (G(A,B) is fixed by
-// ij), 3-4 straight to global memory, exchange batches into LDS.  The density
-// loads of a pair of batches are issued together so their latencies
-// overlap; batches 3-4 read D(A,B)(i,j), which is the same for every kl and
-// comes in preloaded (dab).
-template <bool REAL>
+// ij), 3-4 straight to global memory, exchange batches into LDS.  With PAIR
+// the density loads of two batches are issued together so their latencies
+// overlap; without, each batch loads just before it uses them, which keeps
+// the kernel within the registers of three waves per SIMD.  Batches 3-4 read
+// D(A,B)(i,j), which is the same for every kl and comes in preloaded (dab).
+template <bool REAL, bool PAIR>
 __device__ __forceinline__ void digest(const Raw &raw,
                                        const int (&off)[4][2], const int (&idx)[4],
                                        const double2 *__restrict__ Dm, int nd, int iflg,
@@ -524,9 +525,13 @@ __device__ __forceinline__ void digest(const Raw &raw,
                                        double (&accAB)[4][2], const XView &xa, double2 *gdir) {
   if (!intsym) iflg &= 0x11;
   {
-    double2 d0[4], d1[4];
-    if (iflg & 1) load_d<0, REAL>(off, idx, Dm, nd, d0);
-    if (iflg & 2) load_d<1, REAL>(off, idx, Dm, nd, d1);
+    // PAIR: the density loads of batches 1 and 2 issued together (latencies
+    // overlap, more registers live)
+    double2 dp0[4], dp1[4];
+    if (PAIR) {
+      if (iflg & 1) load_d<0, REAL>(off, idx, Dm, nd, dp0);
+      if (iflg & 2) load_d<1, REAL>(off, idx, Dm, nd, dp1);
+    }
     if (iflg & 12) {
       double o[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
       if (iflg & 4) batch_terms<2, REAL>(raw, dab[0], pab1, pab2, pcd1, pcd2, o);
@@ -539,8 +544,14 @@ __device__ __forceinline__ void digest(const Raw &raw,
                     o[s][1]);
       }
     }
-    if (iflg & 1) batch_terms<0, REAL>(raw, d0, pab1, pab2, pcd1, pcd2, accAB);
-    if (iflg & 2) batch_terms<1, REAL>(raw, d1, pab1, pab2, pcd1, pcd2, accAB);
+    if (iflg & 1) {
+      if (!PAIR) load_d<0, REAL>(off, idx, Dm, nd, dp0);
+      batch_terms<0, REAL>(raw, dp0, pab1, pab2, pcd1, pcd2, accAB);
+    }
+    if (iflg & 2) {
+      if (!PAIR) load_d<1, REAL>(off, idx, Dm, nd, dp1);
+      batch_terms<1, REAL>(raw, dp1, pab1, pab2, pcd1, pcd2, accAB);
+    }
   }
 #define XL(B, D) \
   if (iflg & (1 << B)) load_d<B, REAL>(off, idx, Dm, nd, D);
@@ -550,29 +561,34 @@ __device__ __forceinline__ void digest(const Raw &raw,
     batch_terms<B, REAL>(raw, D, pab1, pab2, pcd1, pcd2, o);             \
     lds_add4<REAL>(xa, KB, W, POS, o);                                   \
   }
-  {
-    double2 d4[4], d5[4];
-    XL(4, d4) XL(5, d5)
-    XB(4, d4, false, 0, idx[3])
-    XB(5, d5, true, 0, idx[2])
+  if (PAIR) {
+    {
+      double2 d4[4], d5[4];
+      XL(4, d4) XL(5, d5)
+      XB(4, d4, false, 0, idx[3])
+      XB(5, d5, true, 0, idx[2])
+    }
+    {
+      double2 d6[4], d7[4];
+      XL(6, d6) XL(7, d7)
+      XB(6, d6, false, 1, idx[3])
+      XB(7, d7, true, 1, idx[2])
+    }
+    {
+      double2 d8[4], d9[4];
+      XL(8, d8) XL(9, d9)
+      XB(8, d8, true, 2, idx[2])
+      XB(9, d9, true, 3, idx[2])
+    }
+  } else {
+    { double2 d4[4]; XL(4, d4) XB(4, d4, false, 0, idx[3]) }
+    { double2 d5[4]; XL(5, d5) XB(5, d5, true, 0, idx[2]) }
+    { double2 d6[4]; XL(6, d6) XB(6, d6, false, 1, idx[3]) }
+    { double2 d7[4]; XL(7, d7) XB(7, d7, true, 1, idx[2]) }
+    { double2 d8[4]; XL(8, d8) XB(8, d8, true, 2, idx[2]) }
+    { double2 d9[4]; XL(9, d9) XB(9, d9, true, 3, idx[2]) }
   }
-  {
-    double2 d6[4], d7[4];
-    XL(6, d6) XL(7, d7)
-    XB(6, d6, false, 1, idx[3])
-    XB(7, d7, true, 1, idx[2])
-  }
-  {
-    double2 d8[4], d9[4];
-    XL(8, d8) XL(9, d9)
-    XB(8, d8, true, 2, idx[2])
-    XB(9, d9, true, 3, idx[2])
-  }
-  {
-    double2 d10[4];
-    XL(10, d10)
-    XB(10, d10, false, 2, idx[3])
-  }
+  { double2 d10[4]; XL(10, d10) XB(10, d10, false, 2, idx[3]) }
 #undef XL
 #undef XB
 }
@@ -702,7 +718,7 @@ static_assert(sizeof(FockShared) + 16 + 2 * (12 * 28 + 16 * 28) * sizeof(double)
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
 // E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
 // accumulates in FP64.
-template <typename T>
+template <typename T, int W>
 __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, FockShared &sh,
                           double *xacc, int lane) {
   const ClassDesc &cl = a.classes[it.cls];
@@ -818,10 +834,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           }
         }
         if (any) {
-          // R rows from the sum table, four independent loads in flight
+          // R rows from the sum table, independent loads in flight: four with
+          // two waves per SIMD, two with three (more would spill)
           const int *hs = c_hsum + tp * kHsRow;
           int t = 0;
-          for (; t + 4 <= ntab; t += 4) {
+          for (; W == 2 && t + 4 <= ntab; t += 4) {
             const T r0 = R[(int64_t)hs[t] * ld + kl];
             const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
             const T r2 = R[(int64_t)hs[t + 2] * ld + kl];
@@ -838,6 +855,19 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
               acc[g][2] += e[10] * r2; acc[g][3] += e[11] * r2;
               acc[g][0] += e[12] * r3; acc[g][1] += e[13] * r3;
               acc[g][2] += e[14] * r3; acc[g][3] += e[15] * r3;
+            }
+          }
+          for (; W != 2 && t + 2 <= ntab; t += 2) {
+            const T r0 = R[(int64_t)hs[t] * ld + kl];
+            const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              const T *e = s_eab + 4 * (g * ntab + t);
+              acc[g][0] += e[0] * r0; acc[g][1] += e[1] * r0;
+              acc[g][2] += e[2] * r0; acc[g][3] += e[3] * r0;
+              acc[g][0] += e[4] * r1; acc[g][1] += e[5] * r1;
+              acc[g][2] += e[6] * r1; acc[g][3] += e[7] * r1;
             }
           }
           for (; t < ntab; ++t) {
@@ -929,10 +959,10 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           if (a.schwarz && (ab.fa[0] >= a.nskp || cd.fc[0] >= a.nskp))
             flg &= schwarz_mask(a.gslot, a.dslot, nd, off, idx, a.sens, s_gab, s_dab, a.schwarz == 2);
           if (a.isym != 0)
-            digest<true>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+            digest<true, W == 2>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                          cd.pcd[1], dab, accAB, xv, a.gdir);
           else
-            digest<false>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
+            digest<false, W == 2>(raw, off, idx, Dm, nd, flg, a.intsym, pab1, cd.pab[1], pcd1,
                           cd.pcd[1], dab, accAB, xv, a.gdir);
         }
         if (!a.rr_out) {
@@ -964,9 +994,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 // split on, an item whose largest possible quartet contribution
 // qitem * max|D| is below the split threshold belongs to the FP32 instance
 // of the kernel, every other item to the FP64 instance; the two instances
-// keep their own counters and register allocations.
-template <typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) fock_kernel(FockArgs a) {
+// keep their own counters and register allocations.  W waves per SIMD:
+// three where the plan's LDS lets twelve workgroups share a CU, else two
+// with more loads in flight per wave (fock_waves).
+template <typename T, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, W))) fock_kernel(FockArgs a) {
   __shared__ FockShared sh;
   __shared__ int s_item;
   extern __shared__ double xacc[];  // a.xacc_n doubles (launch_fock)
@@ -985,7 +1017,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     if (idx >= a.nitems) break;
     const bool low = a.mixed && a.qitem[idx] * dmax < a.split;
     if (low != kLow) continue;
-    fock_item<T>(a, a.items[idx], slab, sh, xacc, lane);
+    fock_item<T, W>(a, a.items[idx], slab, sh, xacc, lane);
   }
 }
 
@@ -1252,10 +1284,10 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
         double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
         const double2 dab_unused[2][4] = {};
         if (a.isym != 0)
-          digest<true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<true, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
                        cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
         else
-          digest<false>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+          digest<false, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
                         cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
       }
       if (!a.rr_out) {
@@ -1599,11 +1631,17 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
   hipLaunchKernelGGL(dmax_kernel, dim3(256), dim3(256), 0, s, d, n, out);
   return hipGetLastError();
 }
-hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
-                       hipEvent_t fork, hipEvent_t join) {
+int fock_waves(int xacc_n) {
+  const size_t lds = sizeof(FockShared) + sizeof(int) + (size_t)xacc_n * sizeof(double);
+  return lds <= 160 * 1024 / 12 ? 3 : 2;
+}
+
+template <int W>
+static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
+                                hipEvent_t fork, hipEvent_t join) {
+  const size_t lds = a.xacc_n * sizeof(double);
   if (!a.mixed) {
-    hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s,
-                       a);
+    hipLaunchKernelGGL((fock_kernel<double, W>), dim3(nblocks), dim3(64), lds, s, a);
     return hipGetLastError();
   }
   // the FP32 instance runs beside the FP64 one on its own stream, with the
@@ -1611,16 +1649,20 @@ hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_
   hipError_t e = hipEventRecord(fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(s32, fork, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fock_kernel<double>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s,
-                     a);
+  hipLaunchKernelGGL((fock_kernel<double, W>), dim3(nblocks), dim3(64), lds, s, a);
   FockArgs b = a;
   b.slab = a.slab + (int64_t)nblocks * a.slab_stride;
-  hipLaunchKernelGGL(fock_kernel<float>, dim3(nblocks), dim3(64), a.xacc_n * sizeof(double), s32,
-                     b);
+  hipLaunchKernelGGL((fock_kernel<float, W>), dim3(nblocks), dim3(64), lds, s32, b);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipEventRecord(join, s32);
   if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
   return e;
+}
+
+hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
+                       hipEvent_t fork, hipEvent_t join) {
+  return fock_waves(a.xacc_n) == 3 ? launch_fock_w<3>(a, nblocks, s, s32, fork, join)
+                                   : launch_fock_w<2>(a, nblocks, s, s32, fork, join);
 }
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
   if (a.nitems == 0) return hipSuccess;

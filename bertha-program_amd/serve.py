@@ -94,10 +94,10 @@ class Server:
         self._atoms = {}
         self.slots = slots
         # persistent workgroups per solve's engine (0: the engine's default,
-        # eight per CU): every solve may use the whole device, and solves in
+        # twelve per CU): every solve may use the whole device, and solves in
         # flight together share it through the hardware queues
         self.workgroups = workgroups
-        self._wg_count = workgroups or 8 * torch.cuda.get_device_properties(
+        self._wg_count = workgroups or 12 * torch.cuda.get_device_properties(
             torch.device(device)).multi_processor_count
         self.device = torch.device(device)
         if hbm_bytes is None:
