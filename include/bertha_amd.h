@@ -86,9 +86,11 @@ typedef struct {
                           (bertha-scf.f:11567) */
   int gaunt;           /* GAUNT2: Gaunt interaction instead of Breit in
                           bdf_breit (reference default 0, bertha-scf.f:5246) */
-  int workgroups;      /* persistent workgroups of the quartet kernels (0: 8
-                          per compute unit); fewer shrink the per-workgroup
-                          scratch when several engines share the device */
+  int workgroups;      /* persistent workgroups of the quartet kernels (0: 12
+                          per compute unit for Coulomb builds whose LDS allows
+                          three waves per SIMD, else 8); fewer shrink the
+                          per-workgroup scratch when several engines share
+                          the device */
 } bdf_options;
 
 typedef struct bdf_engine bdf_engine;
