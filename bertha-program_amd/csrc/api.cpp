@@ -118,6 +118,7 @@ struct bdf_engine {
   DevBreitPlan bplans[3];
   DevBuf<float4> E32;
   DevBuf<double4> Ecd;  // masked, signed E(CD) of the second contraction
+  DevBuf<double4> Eab;  // E with EPSAB applied: the Breit kernel's AB side
   DevBuf<float4> Ecd32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -474,6 +475,7 @@ static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, dou
   a.gaunt = e->gaunt;
   a.stale = dp.stale.p;
   a.Ecd = e->Ecd.p;
+  a.Eab = e->Eab.p;
   a.rr_out = rr_out;
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   HIPCHK(launch_breit(a, blocks_for(e, 8), e->stream));
@@ -560,6 +562,10 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     ea.normS = e->normS.p;
     ea.E = e->E.p;
     HIPCHK(launch_etab(ea, e->stream));
+    if (e->breit) {
+      HIPCHK(e->Eab.alloc((size_t)e->T.total));
+      HIPCHK(launch_eps_filter(e->E.p, e->Eab.p, e->T.total, e->stream));
+    }
     EflagArgs fa;
     fa.pairs = e->pairs.p;
     fa.npairs = ea.npairs;
@@ -1062,6 +1068,7 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   add(e->E.n, sizeof(double4));
   add(e->E32.n, sizeof(float4));
   add(e->Ecd.n, sizeof(double4));
+  add(e->Eab.n, sizeof(double4));
   add(e->Ecd32.n, sizeof(float4));
   add(e->flags.n, 1);
   add(e->slab.n, sizeof(double));

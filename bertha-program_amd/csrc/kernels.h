@@ -98,6 +98,7 @@ struct BreitArgs {
   uint8_t *stale;   // per class: column flags of the last R batch (BIIFAST's IRC)
   double2 *rr_out;  // test hook: RR[16][maxcd] of the first call instead of digesting
   const double4 *Ecd;  // E(CD) with column masks and ERIFAST's Z sign applied
+  const double4 *Eab;  // E with EPSAB applied (launch_eps_filter), the AB side
 };
 
 struct OneArgs {
@@ -131,6 +132,7 @@ hipError_t launch_zero(double2 *x, double2 *y, int64_t n, int *counter,
 // G (when G is given) or of |D|, for every (r, c)
 hipError_t launch_slot_tables(const double *G, const double2 *D, const int *partner, int nd,
                               double *out, hipStream_t s);
+hipError_t launch_eps_filter(const double4 *src, double4 *dst, int64_t n, hipStream_t s);
 hipError_t launch_to_f32(const double4 *src, float4 *dst, int64_t n, hipStream_t s);
 hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hipStream_t s);
 // FP64 instance on s; with a.mixed also the FP32 instance on s32 (slab

@@ -1078,7 +1078,6 @@ __global__ void __launch_bounds__(64) breit_flags_kernel(BreitArgs a) {
 }
 
 struct BreitShared {
-  double eab[3][ntuv(2 * kMaxL + 1) * 4];
   uint8_t cflag[ntuv(kMaxLfl)];
   uint8_t gflag[ntuv(kMaxLfl)];
   uint8_t gm[ntuv(2 * kMaxL + 1) * 12];
@@ -1114,6 +1113,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
   const int maxab = cl.nbas[0] * nB;
   const double bfac = a.gaunt ? -1.0 : -0.5;
+  const int ijs = __builtin_amdgcn_readfirstlane(it.ij);  // uniform: E(AB) by scalar loads
 
   r_columns(cl, a.exps, it.ij, R, F, ld, L, lane);
   __syncthreads();
@@ -1131,15 +1131,6 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
 
   for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
     const BrAB ab = a.abs[abi];
-    for (int w = lane; w < 3 * ab.ntab; w += 64) {
-      int ix = w / ab.ntab, t = w % ab.ntab;
-      double4 e = a.E[ab.eoff[ix] + (int64_t)t * maxab + it.ij];
-      // EPSAB = 1e-16 (:18436-18470)
-      sh.eab[ix][4 * t + 0] = fabs(e.x) > 1.0e-16 ? e.x : 0.0;
-      sh.eab[ix][4 * t + 1] = fabs(e.y) > 1.0e-16 ? e.y : 0.0;
-      sh.eab[ix][4 * t + 2] = fabs(e.z) > 1.0e-16 ? e.z : 0.0;
-      sh.eab[ix][4 * t + 3] = fabs(e.w) > 1.0e-16 ? e.w : 0.0;
-    }
     for (int t = lane; t < 12 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
     const uint8_t *gsrc = abi == cl.ab_begin ? sh.cflag : a.stale + cl.flagoff;
     for (int t = lane; t < nt; t += 64) sh.gflag[t] = gsrc[t];
@@ -1168,13 +1159,20 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
             const int h = hs[t];
             const bool fh = sh.cflag[h] != 0;
             const double r0 = R[(int64_t)h * ld + kl];
+            // E(AB) of the three components at t, EPSAB applied (the same
+            // address in every lane: scalar loads)
+            double e3[3][4];
+#pragma unroll
+            for (int ix = 0; ix < 3; ++ix) {
+              const double4 v = a.Eab[ab.eoff[ix] + (int64_t)t * maxab + ijs];
+              e3[ix][0] = v.x; e3[ix][1] = v.y; e3[ix][2] = v.z; e3[ix][3] = v.w;
+            }
             if (fh) {
 #pragma unroll
               for (int ix = 0; ix < 3; ++ix) {
                 if (!act[ix]) continue;
-                const double *e = &sh.eab[ix][4 * t];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) g[ix][c] += e[c] * r0;
+                for (int c = 0; c < 4; ++c) g[ix][c] += e3[ix][c] * r0;
               }
             }
             if (a.gaunt) continue;
@@ -1214,7 +1212,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
               if (!act[ix]) continue;
 #pragma unroll
               for (int jx = 0; jx < 3; ++jx) {
-                const double *e = &sh.eab[jx][4 * t];
+                const double *e = e3[jx];
                 if (!any4(e)) continue;
                 const bool ok3 = hpm[ix][jx] >= 0;
                 int fl = sh.gflag[hpp[ix][jx]] + sh.gflag[hp[ix]] + (ok3 ? sh.gflag[hpm[ix][jx]] : 0);
@@ -1310,7 +1308,7 @@ __global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
   for (;;) {
     if (lane == 0) s_item = atomicAdd(a.counter, 1);
     __syncthreads();
-    int idx = s_item;
+    int idx = __builtin_amdgcn_readfirstlane(s_item);  // the same in every lane
     __syncthreads();
     if (idx >= a.nitems) break;
     breit_item(a, a.items[idx], slab, sh, lane);
@@ -1611,6 +1609,23 @@ hipError_t launch_slot_tables(const double *G, const double2 *D, const int *part
   const int bs = 256;
   hipLaunchKernelGGL(slot_tables_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, G,
                      D, partner, nd, out);
+  return hipGetLastError();
+}
+// E with ERIFAST's EPSAB cut (|e| <= 1e-16 -> 0, :12445; BIIFAST :18436-18470)
+// applied once, for kernels that read E(AB) straight from memory
+__global__ void eps_filter_kernel(const double4 *src, double4 *dst, int64_t n) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) {
+    const double4 v = src[g];
+    dst[g] = make_double4(fabs(v.x) > 1.0e-16 ? v.x : 0.0, fabs(v.y) > 1.0e-16 ? v.y : 0.0,
+                          fabs(v.z) > 1.0e-16 ? v.z : 0.0, fabs(v.w) > 1.0e-16 ? v.w : 0.0);
+  }
+}
+hipError_t launch_eps_filter(const double4 *src, double4 *dst, int64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  int bs = 256;
+  hipLaunchKernelGGL(eps_filter_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, src,
+                     dst, n);
   return hipGetLastError();
 }
 __global__ void to_f32_kernel(const double4 *src, float4 *dst, int64_t n) {
