@@ -136,11 +136,15 @@ static int blocks_for(const bdf_engine *e, int per_cu) {
 }
 
 // exchange-accumulator doubles of a plan's largest class (the Coulomb
-// kernel's dynamic LDS)
-static int plan_xacc(const Plan &P) {
+// kernel's dynamic LDS): four sign slots of (re, im) per entry, or on the
+// real path (ISYM 1, 2) the two diagonal slots' real parts
+static int plan_xacc(const Plan &P, int isym) {
   int n = 0;
-  for (const ClassDesc &cl : P.classes) n = std::max(n, 2 * (12 * cl.nbas[3] + 16 * cl.nbas[2]));
-  return n;
+  for (const ClassDesc &cl : P.classes) {
+    const int nC = cl.nbas[2], nD = cl.nbas[3];
+    n = std::max(n, isym != 0 ? 6 * nD + 8 * nC : 2 * (12 * nD + 16 * nC));
+  }
+  return n + (n & 1);  // even: 16-byte alignment of what follows
 }
 
 // per-workgroup scratch for `blocks` workgroups: R rows, Boys rows and g_rows
@@ -175,8 +179,8 @@ static int make_plan(bdf_engine *e, int ilev) {
     HIPCHK(dp.qitem.upload(q, e->stream));
   }
   // four workgroups per CU for each wave per SIMD the kernel runs at
-  dp.xacc_n = plan_xacc(dp.P);
-  dp.blocks = blocks_for(e, 4 * fock_waves(dp.xacc_n));
+  dp.xacc_n = plan_xacc(dp.P, e->isym);
+  dp.blocks = blocks_for(e, 4 * fock_waves(dp.xacc_n, e->isym));
   int rc = ensure_slab(e, dp.P.max_maxcd, 4 * ntuv(kMaxLab), dp.blocks);
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -394,8 +398,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.E32 = e->E32.p;
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
-  a.xacc_n = plan_xacc(dp.P);
-  const int nb = std::min(blocks_for(e, 4 * fock_waves(a.xacc_n)), std::max(1, a.nitems));
+  a.xacc_n = plan_xacc(dp.P, e->isym);
+  const int nb = std::min(blocks_for(e, 4 * fock_waves(a.xacc_n, e->isym)), std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));
   HIPCHK(hipEventRecord(e->ev2, e->stream));

@@ -485,16 +485,18 @@ __device__ __forceinline__ void batch_terms(const Raw &raw, const double2 (&d)[4
   }
 }
 
-// LDS accumulators of the exchange batches of one call, complex, packed for
-// the class at hand: the batches whose output varies only with l (5, 7, 11)
-// as [3][4 slots][nD], then those varying only with k (6, 8, 9, 10) as
-// [4][4 slots][nC]; 2 * (12 nD + 16 nC) doubles.
+// LDS accumulators of the exchange batches of one call, packed for the class
+// at hand: the batches whose output varies only with l (5, 7, 11) as
+// [3][slots][nD], then those varying only with k (6, 8, 9, 10) as
+// [4][slots][nC].  Complex path: four sign slots, (re, im) each,
+// 2 (12 nD + 16 nC) doubles; real path (ISYM 1, 2; CLMMTZ keeps the real parts
+// of the sign-diagonal slots only): slots 0 and 3, one double each,
+// 6 nD + 8 nC doubles.
 struct XView {
   double *p;
   int nC, nD;
 };
 constexpr int kXAccMax = 2 * (12 * kMaxPrim + 16 * kMaxPrim);  // doubles, any class
-__device__ __forceinline__ int xacc_doubles(int nC, int nD) { return 2 * (12 * nD + 16 * nC); }
 
 template <bool REAL>
 __device__ __forceinline__ void lds_add4(const XView &x, bool kbatch, int which, int pos,
@@ -502,10 +504,17 @@ __device__ __forceinline__ void lds_add4(const XView &x, bool kbatch, int which,
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     if (REAL && (s == 1 || s == 2)) continue;
-    double *d = kbatch ? x.p + 2 * (12 * x.nD + (which * 4 + s) * x.nC + pos)
-                       : x.p + 2 * ((which * 4 + s) * x.nD + pos);
-    if (o[s][0] != 0.0) unsafeAtomicAdd(d, o[s][0]);
-    if (!REAL && o[s][1] != 0.0) unsafeAtomicAdd(d + 1, o[s][1]);
+    if (REAL) {
+      const int si = s == 3;
+      double *d = kbatch ? x.p + 6 * x.nD + (which * 2 + si) * x.nC + pos
+                         : x.p + (which * 2 + si) * x.nD + pos;
+      if (o[s][0] != 0.0) unsafeAtomicAdd(d, o[s][0]);
+    } else {
+      double *d = kbatch ? x.p + 2 * (12 * x.nD + (which * 4 + s) * x.nC + pos)
+                         : x.p + 2 * ((which * 4 + s) * x.nD + pos);
+      if (o[s][0] != 0.0) unsafeAtomicAdd(d, o[s][0]);
+      if (o[s][1] != 0.0) unsafeAtomicAdd(d + 1, o[s][1]);
+    }
   }
 }
 
@@ -658,26 +667,29 @@ __device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, float &x, flo
 
 // Flush the exchange accumulators of one call into the exchange matrix (one
 // atomic per output element instead of one per quartet), then clear them.
+template <bool REAL>
 __device__ void flush_exchange(const XView &x, int flg, const int (&offs)[4][2], int ib, int jb,
                                double2 *out, int nd, int lane) {
   __syncthreads();
   const int nC = x.nC, nD = x.nD;
-  const int nl = 12 * nD, ntot = nl + 16 * nC;
+  constexpr int ns = REAL ? 2 : 4;  // sign slots kept
+  const int nl = 3 * ns * nD, ntot = nl + 4 * ns * nC;
   for (int w = lane; w < ntot; w += 64) {
     int b, s, pos;
     if (w < nl) {
-      const int li = w / (4 * nD), r = w - li * 4 * nD;
+      const int li = w / (ns * nD), r = w - li * ns * nD;
       s = r / nD;
       pos = r - s * nD;
       b = li == 0 ? 4 : (li == 1 ? 6 : 10);
     } else {
-      const int ki = (w - nl) / (4 * nC), r = (w - nl) - ki * 4 * nC;
+      const int ki = (w - nl) / (ns * nC), r = (w - nl) - ki * ns * nC;
       s = r / nC;
       pos = r - s * nC;
       b = ki == 0 ? 5 : ki + 6;
     }
+    if (REAL) s = s ? 3 : 0;
     if (!((flg >> b) & 1)) continue;
-    double *v = x.p + 2 * w;
+    double *v = REAL ? x.p + w : x.p + 2 * w;
     int row, col;
     const int rs = s >> 1, cs = s & 1;
     switch (b) {
@@ -689,9 +701,10 @@ __device__ void flush_exchange(const XView &x, int flg, const int (&offs)[4][2],
       case 9: row = offs[2][rs] + pos; col = offs[0][cs] + ib; break;
       default: row = offs[3][rs] + pos; col = offs[1][cs] + jb; break;
     }
-    if (v[0] != 0.0 || v[1] != 0.0) atomic_add2(&out[(int64_t)col * nd + row], v[0], v[1]);
+    const double im = REAL ? 0.0 : v[1];
+    if (v[0] != 0.0 || im != 0.0) atomic_add2(&out[(int64_t)col * nd + row], v[0], im);
     v[0] = 0.0;
-    v[1] = 0.0;
+    if (!REAL) v[1] = 0.0;
   }
   __syncthreads();
 }
@@ -968,8 +981,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         if (!a.rr_out) {
           const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                   {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-          flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.gxch, nd,
-                         lane);
+          const int xf = a.intsym ? cd.iflg : (cd.iflg & 0x11);
+          if (a.isym != 0)
+            flush_exchange<true>(xv, xf, offs, ib, jb, a.gxch, nd, lane);
+          else
+            flush_exchange<false>(xv, xf, offs, ib, jb, a.gxch, nd, lane);
         }
       }
       if (!a.rr_out) {
@@ -995,8 +1011,8 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 // qitem * max|D| is below the split threshold belongs to the FP32 instance
 // of the kernel, every other item to the FP64 instance; the two instances
 // keep their own counters and register allocations.  W waves per SIMD:
-// three where the plan's LDS lets twelve workgroups share a CU, else two
-// with more loads in flight per wave (fock_waves).
+// three on the complex path where the plan's LDS lets twelve workgroups
+// share a CU, else two with more loads in flight per wave (fock_waves).
 template <typename T, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, W))) fock_kernel(FockArgs a) {
   __shared__ FockShared sh;
@@ -1291,7 +1307,11 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
       if (!a.rr_out) {
         const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        flush_exchange(xv, a.intsym ? cd.iflg : (cd.iflg & 0x11), offs, ib, jb, a.bxch, nd, lane);
+        const int xf = a.intsym ? cd.iflg : (cd.iflg & 0x11);
+        if (a.isym != 0)
+          flush_exchange<true>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
+        else
+          flush_exchange<false>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
       }
     }
     __syncthreads();
@@ -1646,7 +1666,11 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
   hipLaunchKernelGGL(dmax_kernel, dim3(256), dim3(256), 0, s, d, n, out);
   return hipGetLastError();
 }
-int fock_waves(int xacc_n) {
+int fock_waves(int xacc_n, int isym) {
+  // the real path (ISYM 1, 2) digests half the terms, so its time goes to the
+  // first contraction, which gains more from four R loads in flight than from
+  // a third wave (TlF: 1.30 s with two waves, 1.33 s with three)
+  if (isym != 0) return 2;
   const size_t lds = sizeof(FockShared) + sizeof(int) + (size_t)xacc_n * sizeof(double);
   return lds <= 160 * 1024 / 12 ? 3 : 2;
 }
@@ -1676,8 +1700,8 @@ static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, h
 
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
                        hipEvent_t fork, hipEvent_t join) {
-  return fock_waves(a.xacc_n) == 3 ? launch_fock_w<3>(a, nblocks, s, s32, fork, join)
-                                   : launch_fock_w<2>(a, nblocks, s, s32, fork, join);
+  return fock_waves(a.xacc_n, a.isym) == 3 ? launch_fock_w<3>(a, nblocks, s, s32, fork, join)
+                                           : launch_fock_w<2>(a, nblocks, s, s32, fork, join);
 }
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
   if (a.nitems == 0) return hipSuccess;
