@@ -49,6 +49,11 @@ extern "C" {
 #define BDF_PREC_FP64 0  /* every quartet in FP64 */
 #define BDF_PREC_MIXED 1 /* Schwarz-split FP32/FP64 (see bdf_options) */
 
+/* bdf_options.schwarz */
+#define BDF_SCHWARZ_OFF 0        /* no screening (the reference's default) */
+#define BDF_SCHWARZ_REFERENCE 1  /* SCHWARZ masks from CLMDIAG's GDSC */
+#define BDF_SCHWARZ_RIGOROUS 2   /* same masks, every group bounded by Q(a,b) Q(c,d) */
+
 /* Basis and geometry, as CARDIN fills COMMON /BSET/ (bertha-scf.f:180-182):
    centre c has angular momenta 0..lmax[c]; nfnc[c*lstride + l] exponents for
    shell l, stored at exps[(c*lstride + l)*mbs + i]. */

@@ -10,6 +10,7 @@ module bertha_amd
   integer(c_int), parameter :: BDF_SHAPE_NOSYMM = 0, BDF_SHAPE_LINEAR = 1, BDF_SHAPE_ATOMIC = 2
   integer(c_int), parameter :: BDF_HMLT_DHFR = 1, BDF_HMLT_DHFB = 2
   integer(c_int), parameter :: BDF_PREC_FP64 = 0, BDF_PREC_MIXED = 1
+  integer(c_int), parameter :: BDF_SCHWARZ_OFF = 0, BDF_SCHWARZ_REFERENCE = 1, BDF_SCHWARZ_RIGOROUS = 2
 
   type, bind(c) :: bdf_system
     integer(c_int) :: ncnt
