@@ -119,7 +119,10 @@ def main() -> int:
                     help="Schwarz split threshold of the mixed-precision leg (0: skip the leg)")
     ap.add_argument("--serve", type=int, default=32,
                     help="requests of the served-SCF leg (0: skip the leg)")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=8.0,
+                    help="Poisson arrivals per second of the served-SCF leg (above capacity: throughput)")
+    ap.add_argument("--serve-sustained-rate", type=float, default=1.2,
+                    help="arrival rate of a second served-SCF leg below capacity (latency; 0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a kernel of known traffic that "
@@ -275,30 +278,40 @@ def main() -> int:
 
     # served-SCF leg: DHF solves of atoms and diatomics arriving over time
     serving = None
+    serving_sustained = None
     if args.serve > 0:
         import importlib
         sv = importlib.import_module(pkg.__name__ + ".serve")
-        reqs = sv.request_mix(args.serve, seed=args.seed, rate=args.serve_rate)
-        finished = itertools.count(1)
 
-        def progress(r):
-            k = next(finished)  # atomic under the GIL: workers call this concurrently
-            if k % 32 == 0:
-                print(f"serve: {k} of {len(reqs)} solves done", file=sys.stderr, flush=True)
+        def serve_leg(rate):
+            reqs = sv.request_mix(args.serve, seed=args.seed, rate=rate)
+            finished = itertools.count(1)
 
-        server = sv.Server(pkg, slots=args.serve_slots, on_result=progress)
-        t0 = time.perf_counter()
-        res = server.run(reqs)
-        span = time.perf_counter() - t0
-        p50, p99 = sv.percentiles(res)
-        serving = {"requests": len(res), "p50_latency_s": p50, "p99_latency_s": p99,
-                   "solves_per_s": len(res) / span, "wall_s": span,
-                   "converged": sum(1 for r in res if r.converged),
-                   "arrival_rate_per_s": args.serve_rate, "slots": args.serve_slots,
-                   "peak_concurrent": server.peak_active,
-                   "ndim_range": [min(r.ndim for r in res), max(r.ndim for r in res)],
-                   "mix": "60% atoms Z in {2,4,10,12,18,20,36,54}, 40% diatomics of the "
-                          "same elements, R ~ U(2.5, 8.0) bohr, Poisson arrivals"}
+            def progress(r):
+                k = next(finished)  # atomic under the GIL: workers call this concurrently
+                if k % 32 == 0:
+                    print(f"serve: {k} of {len(reqs)} solves done (rate {rate}/s)",
+                          file=sys.stderr, flush=True)
+
+            server = sv.Server(pkg, slots=args.serve_slots, on_result=progress)
+            t0 = time.perf_counter()
+            res = server.run(reqs)
+            span = time.perf_counter() - t0
+            p50, p99 = sv.percentiles(res)
+            return {"requests": len(res), "p50_latency_s": p50, "p99_latency_s": p99,
+                    "solves_per_s": len(res) / span, "wall_s": span,
+                    "converged": sum(1 for r in res if r.converged),
+                    "arrival_rate_per_s": rate, "slots": args.serve_slots,
+                    "peak_concurrent": server.peak_active,
+                    "ndim_range": [min(r.ndim for r in res), max(r.ndim for r in res)],
+                    "mix": "60% atoms Z in {2,4,10,12,18,20,36,54}, 40% diatomics of the "
+                           "same elements, R ~ U(2.5, 8.0) bohr, Poisson arrivals"}
+
+        # arrivals faster than the solves (throughput), then below capacity
+        # (latency of a stream the server keeps up with)
+        serving = serve_leg(args.serve_rate)
+        if args.serve_sustained_rate > 0:
+            serving_sustained = serve_leg(args.serve_sustained_rate)
 
     # headline: the mixed build when it meets the metric's |dE| < 1 microhartree
     # against the FP64 build, else the FP64 build
@@ -335,6 +348,7 @@ def main() -> int:
         "mixed": mixed,
         "breit": breit,
         "serving": serving,
+        "serving_sustained": serving_sustained,
     }
     print(json.dumps(out), flush=True)
     eng.close()
