@@ -123,7 +123,7 @@ def main() -> int:
                     help="Poisson arrivals per second of the served-SCF leg (above capacity: throughput)")
     ap.add_argument("--serve-sustained-rate", type=float, default=1.2,
                     help="arrival rate of a second served-SCF leg below capacity (latency; 0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a kernel of known traffic that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters in a profiled run)")

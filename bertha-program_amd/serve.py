@@ -82,7 +82,7 @@ def estimate_bytes(system: System, workgroups: int) -> int:
 class Server:
     """Continuous-batching DHF server on one device."""
 
-    def __init__(self, pkg, slots: int = 4, hbm_bytes: Optional[int] = None,
+    def __init__(self, pkg, slots: int = 8, hbm_bytes: Optional[int] = None,
                  workgroups: int = 0, device: str = "cuda:0", max_iter: int = 100,
                  damping: float = 0.5, atomic_guess: bool = True, on_result=None):
         self.pkg = pkg
