@@ -17,7 +17,9 @@ Legs:
             host's cores on the same build, and the GPU result checked
             against it (max |dG|, |dE_2e|)
   serve     Poisson-arriving DHF solves through the continuous-batching
-            server: p50/p99 latency and solves/s
+            server: solves/s with arrivals above capacity, then p50/p99
+            latency with arrivals below it (serving_sustained)
+  breit     (--breit K) K Breit builds of the same density, kernel time
 
 The 12-centre ring (NDIM 1368) runs as
   python bench.py --workload ring --schwarz rigorous --density atoms --no-cpu-baseline --serve 0
