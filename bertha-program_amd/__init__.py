@@ -110,11 +110,16 @@ def library() -> _C.CDLL:
 
 
 def _schwarz_mode(schwarz) -> int:
-    """False/0: off; True/1: the reference's SCHWARZ masks (GDSC);
-    "rigorous"/2: the masks with every group bounded by Q(a,b) Q(c,d)."""
-    if schwarz == "rigorous" or schwarz == 2:
-        return 2
-    return 1 if schwarz else 0
+    """False/0/None: off; True/1/"reference": the reference's SCHWARZ masks
+    (GDSC); "rigorous"/2: the masks with every group bounded by Q(a,b) Q(c,d)."""
+    modes = {"reference": 1, "rigorous": 2}
+    if isinstance(schwarz, str):
+        if schwarz not in modes:
+            raise ValueError(f"schwarz must be one of {sorted(modes)}, False or True, not {schwarz!r}")
+        return modes[schwarz]
+    if schwarz is None or schwarz in (0, 1, 2):  # bools are 0 and 1
+        return int(schwarz or 0)
+    raise ValueError(f"schwarz must be 0, 1 or 2, not {schwarz!r}")
 
 
 def _check(rc: int) -> None:

@@ -41,3 +41,14 @@ def test_fortran_interface_module_names_every_entry_point():
     text = open(path).read().lower()
     for n in declared_symbols():
         assert f"name='{n}'" in text or f'name="{n}"' in text, n
+
+
+def test_schwarz_mode_names(bdf):
+    """Engine(schwarz=...) maps onto bdf_options.schwarz and refuses the rest."""
+    from importlib import import_module
+    m = import_module(bdf.__name__)
+    assert [m._schwarz_mode(v) for v in (False, None, 0, True, 1, "reference", 2, "rigorous")] == \
+        [0, 0, 0, 1, 1, 1, 2, 2]
+    for bad in (3, -1, "strict", 1.5):
+        with pytest.raises(ValueError):
+            m._schwarz_mode(bad)
