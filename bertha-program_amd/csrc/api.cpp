@@ -178,10 +178,12 @@ static int make_plan(bdf_engine *e, int ilev) {
     item_factors(e, dp.P, q);
     HIPCHK(dp.qitem.upload(q, e->stream));
   }
-  // four workgroups per CU for each wave per SIMD the kernel runs at
+  // four workgroups per CU for each wave per SIMD the kernel runs at, G rows
+  // for its combination groups
   dp.xacc_n = plan_xacc(dp.P, e->isym);
-  dp.blocks = blocks_for(e, 4 * fock_waves(dp.xacc_n, e->isym));
-  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * ntuv(kMaxLab), dp.blocks);
+  const FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
+  dp.blocks = blocks_for(e, 4 * fl.waves);
+  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * fl.cap * ntuv(kMaxLab), dp.blocks);
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -399,7 +401,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
   a.xacc_n = plan_xacc(dp.P, e->isym);
-  const int nb = std::min(blocks_for(e, 4 * fock_waves(a.xacc_n, e->isym)), std::max(1, a.nitems));
+  const int nb = std::min(blocks_for(e, 4 * fock_launch_config(a.xacc_n, e->isym).waves),
+                          std::max(1, a.nitems));
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));
   HIPCHK(hipEventRecord(e->ev2, e->stream));

@@ -137,10 +137,15 @@ hipError_t launch_to_f32(const double4 *src, float4 *dst, int64_t n, hipStream_t
 hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hipStream_t s);
 // FP64 instance on s; with a.mixed also the FP32 instance on s32 (slab
 // blocks nblocks..2*nblocks-1), forked from and joined back into s
-// waves per SIMD launch_fock runs the Coulomb kernel at for a plan whose
-// exchange accumulators take xacc_n doubles of LDS, on the real (isym != 0)
-// or complex path (3 or 2; workgroups per CU four times that)
-int fock_waves(int xacc_n, int isym);
+// How launch_fock runs the Coulomb kernel for a plan whose exchange
+// accumulators take xacc_n doubles of LDS, on the real (isym != 0) or complex
+// path: waves per SIMD (3 or 2; workgroups per CU four times that) and the
+// capacity multiple of its combination groups (1 or 2; the slab's G region
+// holds cap * 4 * N(kMaxLab) rows)
+struct FockLaunch {
+  int waves, cap;
+};
+FockLaunch fock_launch_config(int xacc_n, int isym);
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
                        hipEvent_t fork, hipEvent_t join);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);

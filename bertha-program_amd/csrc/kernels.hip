@@ -712,27 +712,31 @@ __device__ void flush_exchange(const XView &x, int flg, const int (&offs)[4][2],
 // AB combinations whose first contraction shares the R loads
 constexpr int kGroup = 4;
 
-// Shared-memory state of one workgroup
+// Shared-memory state of one workgroup.  C sets the capacity of a group of
+// AB combinations sharing the first contraction's R loads: sum of their
+// N(L_AB) and of their N(L_CD) up to C * N(kMaxLab), so C = 2 lets two
+// combinations of the largest orders (N = 165, f-f SS pairs) share them.
+template <int C>
 struct FockShared {
   union {
-    uint8_t cflag[ntuv(kMaxLfl)];  // R column flags (until the prefactor pass)
-    double eab[ntuv(kMaxLab) * 4];  // E(AB) rows of the current AB combinations
+    uint8_t cflag[ntuv(kMaxLfl)];       // R column flags (until the prefactor pass)
+    double eab[C * ntuv(kMaxLab) * 4];  // E(AB) rows of the current AB combinations
   };
-  uint8_t gm[ntuv(kMaxLab) * 4];    // their G masks
-  double2 dab[kGroup][2][4];         // their D(A,B)(i,j), D(B,A)(j,i) slots
+  uint8_t gm[C * ntuv(kMaxLab) * 4];    // their G masks
+  double2 dab[kGroup][2][4];             // their D(A,B)(i,j), D(B,A)(j,i) slots
 };
 // the exchange accumulators follow in dynamic LDS, sized for the plan's
 // largest class: eight workgroups per CU fit up to 28 primitives per shell
 // (seven at the 32-primitive limit)
-static_assert(sizeof(FockShared) + 16 + 2 * (12 * 28 + 16 * 28) * sizeof(double) <=
+static_assert(sizeof(FockShared<1>) + 16 + 2 * (12 * 28 + 16 * 28) * sizeof(double) <=
                   160 * 1024 / 8,
               "fock_kernel's LDS exceeds 1/8 of a CU's");
 
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
 // E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
 // accumulates in FP64.
-template <typename T, int W>
-__device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, FockShared &sh,
+template <typename T, int W, int C>
+__device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, FockShared<C> &sh,
                           double *xacc, int lane) {
   const ClassDesc &cl = a.classes[it.cls];
   // slab rows packed at the class's own kl count, so the rows a workgroup
@@ -801,8 +805,8 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
     const ABCombo ab0 = a.abs[a0];
     const int ntab = ab0.ntab, ntg = ab0.ntg;
     int na = 1;
-    while (na < kGroup && a0 + na < it.ab_hi && (na + 1) * ntab <= ntuv(kMaxLab) &&
-           (na + 1) * ntg <= ntuv(kMaxLab) && a.abs[a0 + na].ntab == ntab &&
+    while (na < kGroup && a0 + na < it.ab_hi && (na + 1) * ntab <= C * ntuv(kMaxLab) &&
+           (na + 1) * ntg <= C * ntuv(kMaxLab) && a.abs[a0 + na].ntab == ntab &&
            a.abs[a0 + na].ntg == ntg)
       ++na;
     for (int w = lane; w < na * ntab; w += 64) {
@@ -1012,10 +1016,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 // of the kernel, every other item to the FP64 instance; the two instances
 // keep their own counters and register allocations.  W waves per SIMD:
 // three on the complex path where the plan's LDS lets twelve workgroups
-// share a CU, else two with more loads in flight per wave (fock_waves).
-template <typename T, int W>
+// share a CU, else two with more loads in flight per wave and, where eight
+// workgroups still fit, C = 2 (fock_launch_config).
+template <typename T, int W, int C>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, W))) fock_kernel(FockArgs a) {
-  __shared__ FockShared sh;
+  __shared__ FockShared<C> sh;
   __shared__ int s_item;
   extern __shared__ double xacc[];  // a.xacc_n doubles (launch_fock)
   constexpr bool kLow = sizeof(T) == sizeof(float);
@@ -1033,7 +1038,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, W)))
     if (idx >= a.nitems) break;
     const bool low = a.mixed && a.qitem[idx] * dmax < a.split;
     if (low != kLow) continue;
-    fock_item<T, W>(a, a.items[idx], slab, sh, xacc, lane);
+    fock_item<T, W, C>(a, a.items[idx], slab, sh, xacc, lane);
   }
 }
 
@@ -1666,21 +1671,24 @@ hipError_t launch_dmax(const double2 *d, int64_t n, unsigned long long *out, hip
   hipLaunchKernelGGL(dmax_kernel, dim3(256), dim3(256), 0, s, d, n, out);
   return hipGetLastError();
 }
-int fock_waves(int xacc_n, int isym) {
-  // the real path (ISYM 1, 2) digests half the terms, so its time goes to the
-  // first contraction, which gains more from four R loads in flight than from
-  // a third wave (TlF: 1.30 s with two waves, 1.33 s with three)
-  if (isym != 0) return 2;
-  const size_t lds = sizeof(FockShared) + sizeof(int) + (size_t)xacc_n * sizeof(double);
-  return lds <= 160 * 1024 / 12 ? 3 : 2;
+FockLaunch fock_launch_config(int xacc_n, int isym) {
+  const size_t dyn = sizeof(int) + (size_t)xacc_n * sizeof(double);
+  // three waves on the complex path where twelve workgroups fit; the real
+  // path (ISYM 1, 2) digests half the terms, so its time goes to the first
+  // contraction, which gains more from four R loads in flight than from a
+  // third wave (TlF: 1.30 s with two waves, 1.33 s with three)
+  if (isym == 0 && sizeof(FockShared<1>) + dyn <= 160 * 1024 / 12) return {3, 1};
+  // two waves: double group capacity while eight workgroups still fit
+  if (sizeof(FockShared<2>) + dyn <= 160 * 1024 / 8) return {2, 2};
+  return {2, 1};
 }
 
-template <int W>
+template <int W, int C>
 static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
                                 hipEvent_t fork, hipEvent_t join) {
   const size_t lds = a.xacc_n * sizeof(double);
   if (!a.mixed) {
-    hipLaunchKernelGGL((fock_kernel<double, W>), dim3(nblocks), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((fock_kernel<double, W, C>), dim3(nblocks), dim3(64), lds, s, a);
     return hipGetLastError();
   }
   // the FP32 instance runs beside the FP64 one on its own stream, with the
@@ -1688,10 +1696,10 @@ static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, h
   hipError_t e = hipEventRecord(fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(s32, fork, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((fock_kernel<double, W>), dim3(nblocks), dim3(64), lds, s, a);
+  hipLaunchKernelGGL((fock_kernel<double, W, C>), dim3(nblocks), dim3(64), lds, s, a);
   FockArgs b = a;
   b.slab = a.slab + (int64_t)nblocks * a.slab_stride;
-  hipLaunchKernelGGL((fock_kernel<float, W>), dim3(nblocks), dim3(64), lds, s32, b);
+  hipLaunchKernelGGL((fock_kernel<float, W, C>), dim3(nblocks), dim3(64), lds, s32, b);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipEventRecord(join, s32);
   if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
@@ -1700,8 +1708,10 @@ static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, h
 
 hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
                        hipEvent_t fork, hipEvent_t join) {
-  return fock_waves(a.xacc_n, a.isym) == 3 ? launch_fock_w<3>(a, nblocks, s, s32, fork, join)
-                                           : launch_fock_w<2>(a, nblocks, s, s32, fork, join);
+  const FockLaunch c = fock_launch_config(a.xacc_n, a.isym);
+  if (c.waves == 3) return launch_fock_w<3, 1>(a, nblocks, s, s32, fork, join);
+  if (c.cap == 2) return launch_fock_w<2, 2>(a, nblocks, s, s32, fork, join);
+  return launch_fock_w<2, 1>(a, nblocks, s, s32, fork, join);
 }
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s) {
   if (a.nitems == 0) return hipSuccess;
