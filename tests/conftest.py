@@ -44,6 +44,11 @@ def small_systems(bdf):
     fsh = S.System("f-shell-pair", np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 1.9]]),
                    [[ev(0.2, 3.0, 2), ev(0.3, 3.0, 2), ev(0.4, 3.0, 1), ev(0.5, 3.0, 1)],
                     [ev(0.1, 3.0, 2), ev(0.3, 3.0, 1)]], shape="NOSYMM")
+    # the same shells on the z axis: the real path, whose SS f-f combinations
+    # (165 Hermite rows each) are the largest the Coulomb kernel groups
+    fdiat = S.System("f-shell-diatomic", np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.9]]),
+                     [[ev(0.2, 3.0, 2), ev(0.3, 3.0, 2), ev(0.4, 3.0, 1), ev(0.5, 3.0, 1)],
+                      [ev(0.1, 3.0, 2), ev(0.3, 3.0, 1)]], shape="DIATOM")
     # four distinct centres: the only systems whose quartets (AB|CD) can put
     # every label on its own centre; "ring4" is planar (z = 0), like the ring
     # workload
@@ -59,5 +64,5 @@ def small_systems(bdf):
                       [ev(0.08, 3.0, 2), ev(0.3, 2.5, 1)],
                       [ev(0.1, 2.8, 3), ev(0.2, 2.6, 1)],
                       [ev(0.08, 3.0, 2)]], shape="NOSYMM")
-    return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh, "quad": quad,
-            "ring4": ring4}
+    return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh, "fdiat": fdiat,
+            "quad": quad, "ring4": ring4}

@@ -70,7 +70,7 @@ def test_eri_batches_match_oracle(bdf, orc, systems, name):
             assert np.abs(rg[use] - ro[use]).max() <= tol, (icnt, kidx, midx, ibas, jbas, itn)
 
 
-@pytest.mark.parametrize("name", ["ne", "diat", "off", "tri", "fsh", "quad", "ring4"])
+@pytest.mark.parametrize("name", ["ne", "diat", "off", "tri", "fsh", "fdiat", "quad", "ring4"])
 def test_clmfast_matches_oracle(bdf, orc, systems, name):
     s = systems[name]
     print(f"[clmfast] {name}", flush=True)
