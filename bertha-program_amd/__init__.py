@@ -101,6 +101,8 @@ def library() -> _C.CDLL:
         lib.bdf_breit_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
                                             _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_memory_info.argtypes = [P, _C.POINTER(_C.c_size_t)]
+        lib.bdf_launch_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_int), _C.POINTER(_C.c_int),
+                                        _C.POINTER(_C.c_int), _C.POINTER(_C.c_size_t)]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -287,6 +289,15 @@ class Engine:
         n = _C.c_long()
         _check(library().bdf_plan_info(self._h, ilev, _C.byref(f), _C.byref(q), _C.byref(n)))
         return f.value, q.value, n.value
+
+    def launch_info(self, ilev: int = 3):
+        """(waves per SIMD, group capacity, workgroups, scratch bytes) of the
+        stage's Coulomb build."""
+        w, c, g = _C.c_int(), _C.c_int(), _C.c_int()
+        b = _C.c_size_t()
+        _check(library().bdf_launch_info(self._h, ilev, _C.byref(w), _C.byref(c), _C.byref(g),
+                                         _C.byref(b)))
+        return w.value, c.value, g.value, b.value
 
     def device_bytes(self) -> int:
         """Device memory held by the engine."""

@@ -173,8 +173,9 @@ static int make_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
+  // (held until the stream synchronises below: the upload is asynchronous)
+  std::vector<double> q;
   if (e->mixed) {
-    std::vector<double> q;
     item_factors(e, dp.P, q);
     HIPCHK(dp.qitem.upload(q, e->stream));
   }
@@ -511,12 +512,12 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(BDF_EDEVICE, "bdf_create: no HIP device");
   if (o.device < 0 || o.device >= ndev) return fail(BDF_EINVAL, "bdf_create: bad device ordinal");
+  if (o.schwarz < 0 || o.schwarz > 2) return fail(BDF_EINVAL, "bdf_create: schwarz must be 0, 1 or 2");
   bdf_engine *e = new bdf_engine();
   e->device = o.device;
   e->isym = sys->shape;
   e->intsym = o.intsym ? 1 : 0;
   e->racah1 = o.racah1 ? 1 : 0;
-  if (o.schwarz < 0 || o.schwarz > 2) return fail(BDF_EINVAL, "bdf_create: schwarz must be 0, 1 or 2");
   e->schwarz = o.schwarz;
   e->sens = o.sens > 0.0 ? o.sens : 1.0e-12;
   e->mixed = o.precision == BDF_PREC_MIXED;
@@ -685,6 +686,8 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   cl.maxcd = mcd;
   cl.mcnt = mcnt;
   cl.flagoff = 0;
+  cl.sym = axis_symmetry(cl);
+  cl.pad = 0;
   cl.ab_begin = 0;
   cl.ab_end = 1;
   ABCombo ab;
@@ -698,7 +701,7 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   ab.nb = cl.nbas[1];
   ab.cd_begin = 0;
   ab.cd_end = 1;
-  ab.pad = 0;
+  ab.lab = lq[0] + lq[1] + (itn[0] == 4 ? 2 : 0);
   // G mask: aggregated column flags of the (C,D,lc,ld) class for this itn
   std::vector<int> agg(4 * ntcd, 0);
   int lc = lq[2], ld = lq[3];
@@ -875,6 +878,8 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   cl.maxcd = mcd;
   cl.mcnt = 0;
   cl.flagoff = 0;
+  cl.sym = axis_symmetry(cl);
+  cl.pad = 0;
   cl.ab_begin = 0;
   cl.ab_end = 1;
   int ntcd = ntuv(lamcd);
@@ -1118,6 +1123,22 @@ int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long
   if (flops) *flops = P.flops;
   if (quartets) *quartets = P.quartets;
   if (items) *items = (long)P.items.size();
+  return BDF_OK;
+}
+
+int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgroups,
+                    size_t *scratch_bytes) {
+  if (!e) return fail(BDF_EINVAL, "bdf_launch_info: null engine");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_launch_info: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  const DevPlan &dp = e->plans[ilev - 1];
+  const FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
+  if (waves) *waves = fl.waves;
+  if (cap) *cap = fl.cap;
+  if (workgroups) *workgroups = dp.blocks;
+  if (scratch_bytes) *scratch_bytes = e->slab.n * sizeof(double);
   return BDF_OK;
 }
 

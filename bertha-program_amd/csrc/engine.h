@@ -64,6 +64,12 @@ struct ClassDesc {
   int ab_begin, ab_end;
   int mcnt;
   int flagoff;      // Breit: offset of the class's gauge-term column flags
+  // bit d (x = 1, y = 2, z = 4) set when all four centres have coordinate d
+  // exactly 0: then P_d = Q_d = 0 exactly and R_tuv vanishes exactly unless
+  // the d-th Hermite index is even, so contraction terms with odd index sums
+  // along d are skipped without changing any sum
+  int sym;
+  int pad;
 };
 
 // One (kappa_A, kappa_B, |m_A|, |m_B|, IT1) combination of a class.
@@ -75,7 +81,7 @@ struct ABCombo {
   int fa[2], fb[2]; // Fock offsets of A, B for sign slots (0 = -|m|)
   int cd_begin, cd_end;
   int nb;           // primitives of B (ij -> i, j)
-  int pad;
+  int lab;          // L_AB: ntab = N(lab)
 };
 
 // One ERIFAST call (CD combination and IT2) with its digestion flags.

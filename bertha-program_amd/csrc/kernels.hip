@@ -25,6 +25,17 @@ __constant__ int c_hsum[ntuv(kMaxLab) * kHsRow];
 // (d = 0..2; -1 beyond order kMaxLfl) and c_hstep[h][4 + d] = index(tuv(h) -
 // e_d) (-1 where that component is zero); rows of 8 words
 __constant__ int c_hstep[ntuv(kMaxLfl)][8];
+// Hermite indices t < N(kMaxLab) grouped by parity class under an axis mask
+// m (ClassDesc::sym): class(t) = (t_x & 1 | (t_y & 1) << 1 | (t_z & 1) << 2) & m.
+// For the bra order L, c_tperm[m][L][pos] lists the t < N(L) class by class,
+// ascending inside a class, class c at positions [c_coff[m][L][c],
+// c_coff[m][L][c + 1]); g_hsp[m][tp][k] = index(tuv(t_k) + tuv(tp)) for the
+// k-th t of tp's own class.  With m = 0 these are the identity and c_hsum.
+// R(tuv + tuv') vanishes exactly unless t and tp share a class, so the first
+// contraction of tp runs over that class only, in ascending t as before.
+__constant__ int16_t c_tperm[8][kMaxLab + 1][ntuv(kMaxLab)];
+__constant__ uint8_t c_coff[8][kMaxLab + 1][9];
+__device__ int g_hsp[8][ntuv(kMaxLab)][kHsRow];
 __constant__ double c_rcp[2 * kMaxLfl + 2 * 60 + 4];  // 1/(2k+1)
 __constant__ float c_rcpf[2 * kMaxLfl + 2 * 60 + 4];
 
@@ -37,6 +48,11 @@ __device__ __forceinline__ void dec(int n, int &t, int &u, int &v) {
   t = x >> 10;
   u = (x >> 5) & 31;
   v = x & 31;
+}
+// parity class of Hermite index n (bit d: the d-th component is odd)
+__device__ __forceinline__ int hpar(int n) {
+  const uint32_t x = c_tuv[n];
+  return (int)(((x >> 10) & 1) | (((x >> 5) & 1) << 1) | ((x & 1) << 2));
 }
 
 // ---------------------------------------------------------------------------
@@ -333,12 +349,15 @@ __device__ void boys_ref(T t, int L, T *f, int ld) {
 // R^(n)_{000} = (-2a)^n F_n and R^(n)_{t+1,u,v} = X R^(n+1)_{tuv} +
 // t R^(n+1)_{t-1,u,v} (McMurchie-Davidson), descending levels so each slot is
 // overwritten only after the lower order has been read.
-template <typename T>
-__device__ void rints(T *r, const T *f, int ld, int L, T alpha, T X, T Y, T Z) {
-  T m2a = -2 * alpha;
-  T pw = 1;
+// The seeds (-2a)^n F_n are formed in the precision of f (FP64 for every
+// item: the FP32 items of the mixed build run the recursion in FP32 from
+// FP64 Boys values).
+template <typename T, typename FT>
+__device__ void rints(T *r, const FT *f, int ld, int L, FT alpha, T X, T Y, T Z) {
+  const FT m2a = -2 * alpha;
+  FT pw = 1;
   for (int k = 0; k < L; ++k) pw *= m2a;
-  r[0] = pw * f[L * ld];
+  r[0] = (T)(pw * f[L * ld]);
   for (int n = L - 1; n >= 0; --n) {
     for (int lev = L - n; lev >= 1; --lev) {
       for (int t = lev; t >= 0; --t)
@@ -360,7 +379,7 @@ __device__ void rints(T *r, const T *f, int ld, int L, T alpha, T X, T Y, T Z) {
     }
     pw = 1;
     for (int k = 0; k < n; ++k) pw *= m2a;
-    r[0] = pw * f[n * ld];
+    r[0] = (T)(pw * f[n * ld]);
   }
 }
 
@@ -731,6 +750,12 @@ struct FockShared {
 static_assert(sizeof(FockShared<1>) + 16 + 2 * (12 * 28 + 16 * 28) * sizeof(double) <=
                   160 * 1024 / 8,
               "fock_kernel's LDS exceeds 1/8 of a CU's");
+// the capacity-2 instance on the real path (ISYM 1, 2: 6 nD + 8 nC
+// accumulator doubles) keeps eight workgroups per CU at any shell size; on
+// the complex path fock_launch_config takes it only where it fits
+static_assert(sizeof(FockShared<2>) + 16 + (6 * kMaxPrim + 8 * kMaxPrim) * sizeof(double) <=
+                  160 * 1024 / 8,
+              "fock_kernel<.,2,2>'s LDS exceeds 1/8 of a CU's on the real path");
 
 // One work item (class, ij) in precision T: R-integrals for every kl, the two
 // E.R.E contractions (ERIFAST) and the digestion into GDIR/GXCH, which always
@@ -742,9 +767,13 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   // slab rows packed at the class's own kl count, so the rows a workgroup
   // touches occupy as few cache lines as possible
   const int ld = cl.maxcd;
-  T *R = reinterpret_cast<T *>(slab);            // [rows][ld]
-  T *F = R + (int64_t)ntuv(kMaxLfl) * ld;         // [kMaxLfl+1][ld]
-  T *G = F + (int64_t)(kMaxLfl + 1) * ld;         // [ntg*4][ld]
+  // R [rows][ld] in T, Boys values F [kMaxLfl+1][ld] in FP64 (8-byte
+  // aligned), G [ntg*4][ld] in T: within the slab's rows * ld doubles
+  char *base = reinterpret_cast<char *>(slab);
+  T *R = reinterpret_cast<T *>(base);
+  const int64_t fo = ((int64_t)ntuv(kMaxLfl) * ld * (int64_t)sizeof(T) + 7) & ~(int64_t)7;
+  double *F = reinterpret_cast<double *>(base + fo);
+  T *G = reinterpret_cast<T *>(base + fo + (int64_t)(kMaxLfl + 1) * ld * 8);
   T *s_eab = reinterpret_cast<T *>(sh.eab);
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
@@ -758,6 +787,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   const double Pz = (cl.xyz[0][2] * ea + cl.xyz[1][2] * eb) / p;
   const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
   const int maxab = cl.nbas[0] * nB;
+  const int sym = cl.sym;
 
   // R-integrals for every kl of this ij, then the reference's column test
   // (sum over kl of |R| > 1e-10 before the prefactor, :12660-12675)
@@ -773,8 +803,8 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       double alpha = p * q / (p + q);
       double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
       double Targ = alpha * (X * X + Y * Y + Z * Z);
-      boys_ref<T>((T)Targ, L, F + kl, ld);
-      rints<T>(R + kl, F + kl, ld, L, (T)alpha, (T)X, (T)Y, (T)Z);
+      boys_ref<double>(Targ, L, F + kl, ld);
+      rints<T, double>(R + kl, F + kl, ld, L, alpha, (T)X, (T)Y, (T)Z);
     }
   }
   __syncthreads();
@@ -810,7 +840,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
            a.abs[a0 + na].ntg == ntg)
       ++na;
     for (int w = lane; w < na * ntab; w += 64) {
-      const int g = w / ntab, t = w - g * ntab;
+      const int g = w / ntab, t = c_tperm[sym][ab0.lab][w - g * ntab];
       T ex, ey, ez, ew;
       lde(a, a.abs[a0 + g].eoff + (int64_t)t * maxab + it.ij, ex, ey, ez, ew);
       // E(AB) entries of magnitude <= 1e-16 are skipped by ERIFAST (:12445)
@@ -852,10 +882,15 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         }
         if (any) {
           // R rows from the sum table, independent loads in flight: four with
-          // two waves per SIMD, two with three (more would spill)
-          const int *hs = c_hsum + tp * kHsRow;
+          // two waves per SIMD, two with three (more would spill); only the t
+          // of tp's parity class (the other R rows are exact zeros)
+          const int pc = hpar(tp) & sym;
+          const int tb = c_coff[sym][ab0.lab][pc];
+          const int nc = c_coff[sym][ab0.lab][pc + 1] - tb;
+          const int *hs = &g_hsp[sym][tp][0];
+          const T *eb = s_eab + 4 * tb;
           int t = 0;
-          for (; W == 2 && t + 4 <= ntab; t += 4) {
+          for (; W == 2 && t + 4 <= nc; t += 4) {
             const T r0 = R[(int64_t)hs[t] * ld + kl];
             const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
             const T r2 = R[(int64_t)hs[t + 2] * ld + kl];
@@ -863,7 +898,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
               if (g >= na) break;
-              const T *e = s_eab + 4 * (g * ntab + t);
+              const T *e = eb + 4 * (g * ntab + t);
               acc[g][0] += e[0] * r0; acc[g][1] += e[1] * r0;
               acc[g][2] += e[2] * r0; acc[g][3] += e[3] * r0;
               acc[g][0] += e[4] * r1; acc[g][1] += e[5] * r1;
@@ -874,25 +909,25 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
               acc[g][2] += e[14] * r3; acc[g][3] += e[15] * r3;
             }
           }
-          for (; W != 2 && t + 2 <= ntab; t += 2) {
+          for (; W != 2 && t + 2 <= nc; t += 2) {
             const T r0 = R[(int64_t)hs[t] * ld + kl];
             const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
               if (g >= na) break;
-              const T *e = s_eab + 4 * (g * ntab + t);
+              const T *e = eb + 4 * (g * ntab + t);
               acc[g][0] += e[0] * r0; acc[g][1] += e[1] * r0;
               acc[g][2] += e[2] * r0; acc[g][3] += e[3] * r0;
               acc[g][0] += e[4] * r1; acc[g][1] += e[5] * r1;
               acc[g][2] += e[6] * r1; acc[g][3] += e[7] * r1;
             }
           }
-          for (; t < ntab; ++t) {
+          for (; t < nc; ++t) {
             const T r = R[(int64_t)hs[t] * ld + kl];
 #pragma unroll
             for (int g = 0; g < kGroup; ++g) {
               if (g >= na) break;
-              const T *e = s_eab + 4 * (g * ntab + t);
+              const T *e = eb + 4 * (g * ntab + t);
               acc[g][0] += e[0] * r; acc[g][1] += e[1] * r;
               acc[g][2] += e[2] * r; acc[g][3] += e[3] * r;
             }
@@ -1066,7 +1101,7 @@ __device__ void r_columns(const ClassDesc &cl, const double *exps, int ij, doubl
     double alpha = p * q / (p + q);
     double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
     boys_ref<double>(alpha * (X * X + Y * Y + Z * Z), L, F + kl, ld);
-    rints<double>(R + kl, F + kl, ld, L, alpha, X, Y, Z);
+    rints<double, double>(R + kl, F + kl, ld, L, alpha, X, Y, Z);
   }
 }
 
@@ -1135,6 +1170,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   const int maxab = cl.nbas[0] * nB;
   const double bfac = a.gaunt ? -1.0 : -0.5;
   const int ijs = __builtin_amdgcn_readfirstlane(it.ij);  // uniform: E(AB) by scalar loads
+  const int sym = cl.sym;
 
   r_columns(cl, a.exps, it.ij, R, F, ld, L, lane);
   __syncthreads();
@@ -1174,11 +1210,28 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
           act[ix] = (gm[0] | gm[1] | gm[2] | gm[3]) != 0;
         }
         double g[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        // On the class's symmetric axes (ClassDesc::sym) R(h') vanishes
+        // exactly unless h' is even there, so with dl the parity class of
+        // h = tuv + tuv' on those axes: the direct term of ix needs dl = 0
+        // and the gauge term of (ix, jx) needs dl = (e_ix + e_jx) mod 2
+        // (its PQ_jx R(h + e_ix) part is zero when jx is symmetric, where
+        // PQ_jx = 0).  pairs[dl] holds the (ix, jx) bits 3 ix + jx that can
+        // contribute; a t whose dl has none is skipped.
+        int pairs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+          for (int jx = 0; jx < 3; ++jx)
+            if (act[ix] && (!a.gaunt || ix == jx))
+              pairs[((1 << ix) ^ (1 << jx)) & sym] |= 1 << (3 * ix + jx);
         if (act[0] || act[1] || act[2]) {
           const int *hs = c_hsum + tp * kHsRow;
           for (int t = 0; t < ab.ntab; ++t) {
             const int h = hs[t];
-            const bool fh = sh.cflag[h] != 0;
+            const int dl = hpar(h) & sym;
+            const int pm = pairs[dl];
+            if (pm == 0) continue;
+            const bool fh = dl == 0 && sh.cflag[h] != 0;
             const double r0 = R[(int64_t)h * ld + kl];
             // E(AB) of the three components at t, EPSAB applied (the same
             // address in every lane: scalar loads)
@@ -1199,7 +1252,8 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
             if (a.gaunt) continue;
             // gauge term T = R(h+e_ix+e_jx)/(2 alpha) - PQ_jx R(h+e_ix) +
             // (t+t')_jx R(h+e_ix-e_jx), skipped where all three columns are
-            // flagged off (BIIFAST :18492-18810)
+            // flagged off (BIIFAST :18492-18810); R values are loaded only
+            // for the pairs in pm
             const uint32_t s3 = c_tuv[h];
             const int sj[3] = {(int)(s3 >> 10), (int)((s3 >> 5) & 31), (int)(s3 & 31)};
             int hp[3], hpp[3][3], hpm[3][3];
@@ -1207,7 +1261,9 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               hp[i] = c_hstep[h][i];
-              rp[i] = R[(int64_t)hp[i] * ld + kl];
+              // the middle part of (i, j) for a non-symmetric j
+              const int need = pm & (7 << (3 * i)) & ~(sym * (1 << (3 * i)));
+              rp[i] = need ? R[(int64_t)hp[i] * ld + kl] : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -1218,21 +1274,23 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                   rpp[i][j] = rpp[j][i];
                 } else {
                   hpp[i][j] = c_hstep[hp[i]][j];
-                  rpp[i][j] = R[(int64_t)hpp[i][j] * ld + kl];
+                  const bool need = (pm >> (3 * i + j) & 1) || (pm >> (3 * j + i) & 1);
+                  rpp[i][j] = need ? R[(int64_t)hpp[i][j] * ld + kl] : 0.0;
                 }
                 if (i == j) {
                   hpm[i][j] = h;
                   rpm[i][j] = r0;
                 } else {
                   hpm[i][j] = c_hstep[hp[i]][4 + j];  // -1 where (t+t')_j = 0
-                  rpm[i][j] = hpm[i][j] >= 0 ? R[(int64_t)hpm[i][j] * ld + kl] : 0.0;
+                  const bool need = (pm >> (3 * i + j) & 1) && hpm[i][j] >= 0;
+                  rpm[i][j] = need ? R[(int64_t)hpm[i][j] * ld + kl] : 0.0;
                 }
               }
 #pragma unroll
             for (int ix = 0; ix < 3; ++ix) {
-              if (!act[ix]) continue;
 #pragma unroll
               for (int jx = 0; jx < 3; ++jx) {
+                if (!(pm >> (3 * ix + jx) & 1)) continue;
                 const double *e = e3[jx];
                 if (!any4(e)) continue;
                 const bool ok3 = hpm[ix][jx] >= 0;
@@ -1400,7 +1458,7 @@ __device__ void vnuc1(const OneArgs &a, int64_t eoff, int maxab, int ij, int lam
     }
     double X = px - a.xyz[3 * iz], Y = py - a.xyz[3 * iz + 1], Z = pz - a.xyz[3 * iz + 2];
     boys_ref<double>(alpha * (X * X + Y * Y + Z * Z), lam, f, 1);
-    rints<double>(r, f, 1, lam, alpha, X, Y, Z);
+    rints<double, double>(r, f, 1, lam, alpha, X, Y, Z);
     for (int t = 0; t < nt; ++t) {
       double4 e = a.E[eoff + (int64_t)t * maxab + ij];
       v11.x -= pnc * e.x * r[t];
@@ -1578,6 +1636,42 @@ hipError_t upload_constants() {
   }
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_hstep), st, sizeof(st));
   if (e != hipSuccess) return e;
+  // parity-class orderings of the bra's Hermite indices (c_tperm, c_coff,
+  // g_hsp): see their declarations
+  {
+    constexpr int NT = ntuv(kMaxLab);
+    static int16_t perm[8][kMaxLab + 1][NT];
+    static uint8_t coff[8][kMaxLab + 1][9];
+    static int hsp[8][NT][kHsRow];
+    auto par = [&](int n) {
+      const uint32_t x = tuv[n];
+      return (int)(((x >> 10) & 1) | (((x >> 5) & 1) << 1) | ((x & 1) << 2));
+    };
+    for (int m = 0; m < 8; ++m) {
+      for (int L = 0; L <= kMaxLab; ++L) {
+        const int nt = ntuv(L);
+        int pos = 0;
+        for (int c = 0; c < 8; ++c) {
+          coff[m][L][c] = (uint8_t)pos;
+          for (int t = 0; t < nt; ++t)
+            if ((par(t) & m) == c) perm[m][L][pos++] = (int16_t)t;
+        }
+        coff[m][L][8] = (uint8_t)pos;
+        for (int t = pos; t < NT; ++t) perm[m][L][t] = 0;
+      }
+      for (int tp = 0; tp < NT; ++tp) {
+        const int c = par(tp) & m;
+        int k = 0;
+        for (int t = 0; t < NT; ++t)
+          if ((par(t) & m) == c) hsp[m][tp][k++] = hs[tp * kHsRow + t];
+        for (; k < kHsRow; ++k) hsp[m][tp][k] = 0;
+      }
+    }
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_tperm), perm, sizeof(perm));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_coff), coff, sizeof(coff));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_hsp), hsp, sizeof(hsp));
+    if (e != hipSuccess) return e;
+  }
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
   float rcpf[2 * kMaxLfl + 2 * 60 + 4];
   for (int k = 0; k < (int)(sizeof(rcp) / sizeof(double)); ++k) {

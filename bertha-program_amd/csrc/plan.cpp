@@ -27,6 +27,14 @@ int Basis::kappa_of(int c, int k) const {  // k is 1-based KAPA index
   return (k % 2 == 0) ? l : -l - 1;
 }
 
+int axis_symmetry(const ClassDesc &cl) {
+  int m = 0;
+  for (int d = 0; d < 3; ++d)
+    if (cl.xyz[0][d] == 0.0 && cl.xyz[1][d] == 0.0 && cl.xyz[2][d] == 0.0 && cl.xyz[3][d] == 0.0)
+      m |= 1 << d;
+  return m;
+}
+
 std::string build_basis(const bdf_system &sys, Basis &B) {
   if (sys.ncnt <= 0 || !sys.xyz || !sys.lmax || !sys.nfnc || !sys.exps)
     return "bdf_system: null or empty arrays";
@@ -488,6 +496,8 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
             cl.maxcd = cl.nbas[2] * cl.nbas[3];
             cl.mcnt = mcnt;
             cl.flagoff = 0;
+            cl.sym = axis_symmetry(cl);
+            cl.pad = 0;
             cl.ab_begin = (int)P.abs.size();
             // aggregated E(CD) flags of the (C,D,lc,ld) class (E0LLCD0/E0SSCD0)
             int ntss = ntuv(lc + ld + 2);
@@ -529,7 +539,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
                       ab.fb[1] = B.lrge_at(cb, kb, 2 * mb) + naddab;
                       ab.nb = cl.nbas[1];
                       ab.cd_begin = (int)P.cds.size();
-                      ab.pad = 0;
+                      ab.lab = la + lb + (it1 == 4 ? 2 : 0);
                       int first_it2 = 0;
                       for (int nc = (lc == 0); nc <= 1; ++nc) {
                         int kc = 2 * lc + nc, kqc = B.kappa_of(cc, kc), jqc = 2 * std::abs(kqc) - 1;
@@ -728,6 +738,8 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
             cl.maxcd = cl.nbas[2] * cl.nbas[3];
             cl.mcnt = mcnt;
             cl.flagoff = P.nflag;
+            cl.sym = axis_symmetry(cl);
+            cl.pad = 0;
             cl.ab_begin = (int)P.abs.size();
             int ntcd = ntuv(lamcd);
             // EILSCD0 (:17997): E(CD) flags summed over the class, [tuv][ix][c]
@@ -852,7 +864,11 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
   // (ij, class); first contraction per (ij, AB combination) 3 x (8 flops of
   // the Gaunt term + 3 x 14 of the gauge term) per (tuv, tuv', kl); second
   // contraction 3 x 32 (3 x 8 real) per (tuv', kl) and call; digestion 128
-  // (8 real) flops per active batch and element.
+  // (8 real) flops per active batch and element.  The first contraction's
+  // count is BIIFAST's statements per (IX, ICD, IAB, N) (bertha-scf.f
+  // :18700-18810): Gaunt term G += E R for four components, 4 x 2 = 8; per
+  // JX of the gauge term T1 = R 0.5 / APH (2), T2 = R PQ (1), T3 = R RTP
+  // (1), T = T1 - T2 + T3 (2) and G -= E T for four components (8): 14.
   for (const ClassDesc &cl : P.classes) {
     double maxab = (double)cl.nbas[0] * cl.nbas[1], maxcd = cl.maxcd;
     double L = cl.lamfl;

@@ -46,6 +46,8 @@ struct Etabs {
 };
 
 std::string build_basis(const bdf_system &sys, Basis &B);
+// ClassDesc::sym of a class from its centres' coordinates
+int axis_symmetry(const ClassDesc &cl);
 void build_etables(const Basis &B, Etabs &T, bool with_ls);
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                        int isym, int intsym, int racah1, int ilev, Plan &P);

@@ -189,6 +189,20 @@ def energies(system: System, dens, hkin, hnuc, gdir, gxch, bdir=None, bxch=None)
     return e
 
 
+def density(coef, nskp: int, nocc: int):
+    """DENSTY (bertha-scf.f:3580), closed shell: D_ij = sum over the nocc
+    lowest positive-energy spinors o of conj(C_io) C_jo (columns nskp+o)."""
+    occ = coef[:, nskp:nskp + nocc]
+    return occ.conj() @ occ.T
+
+
+def level_shift(fock, ovlp, coef, nskp: int, nocc: int, shlv: float):
+    """LEVSHFT (bertha-scf.f:3641): F + shlv sum_v (S C_v)(S C_v)^H over the
+    virtual spinors v > nskp + nocc."""
+    a = ovlp @ coef[:, nskp + nocc:]
+    return fock + shlv * (a @ a.conj().T)
+
+
 def orthogonaliser(ovlp):
     """L^-1 of ZHEGV's Cholesky factor S = L L^H, from the lower triangle of S
     (once per geometry: S does not change between iterations)."""
@@ -289,15 +303,13 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
         etot = e.total
         shifted = 0.0
         if it != 1 and coef is not None:
-            a = ovlp @ coef[:, nskp + nocc:]
-            fock = fock + shlv * (a @ a.conj().T)
+            fock = level_shift(fock, ovlp, coef, nskp, nocc, shlv)
             shifted = shlv
         nmlev[ilev] += 1
         fock = diis(nmlev[ilev], fock, dens, ovlp)
         eig, coef, finite = solve_fock(fock, li)
-        occ = coef[:, nskp:nskp + nocc]
         dprev = dens
-        dens = occ.conj() @ occ.T  # D_ij = sum_o conj(C_io) C_jo
+        dens = density(coef, nskp, nocc)
         if it > 1 and diis.last_errmax <= ERRCOM:
             damping = 0.0  # DIIS has engaged: no more damping in this solve
         if damping > 0.0 and it > 1:

@@ -55,11 +55,34 @@ class Result:
         return self.finish - self.arrival
 
 
-def estimate_bytes(system: System, workgroups: int) -> int:
+def _launch_shape(system: System, maxn: int):
+    """(workgroups per CU, slab rows) of the engine's Coulomb build, as the
+    engine chooses them: R rows N(16) = 969, Boys rows 17 and 4 * cap * N(8)
+    G rows; the real path (DIATOM/ATOMIC) runs two waves per SIMD at group
+    capacity 2, the complex path three waves at capacity 1 while its
+    exchange accumulators 2 (12 n + 16 n) doubles fit 1/12 of the LDS
+    (n <= 16 primitives per shell), two waves at capacity 2 up to n = 18 and
+    at capacity 1 beyond.  A DHFB engine's Breit scratch holds 12 N(7) G
+    rows at eight workgroups per CU."""
+    base = 969 + 17
+    if system.shape in ("DIATOM", "LINEAR", "ATOMIC"):
+        per_cu, rows = 8, base + 4 * 2 * 165
+    elif maxn <= 16:
+        per_cu, rows = 12, base + 4 * 165
+    elif maxn <= 18:
+        per_cu, rows = 8, base + 4 * 2 * 165
+    else:
+        per_cu, rows = 8, base + 4 * 165
+    if system.hmlt == "DHFB":
+        per_cu, rows = 8, max(rows, base + 12 * 120)
+    return per_cu, rows
+
+
+def estimate_bytes(system: System, workgroups: int = 0, cus: int = 256) -> int:
     """Device footprint of one solve before its engine exists: E tables,
-    per-workgroup scratch and the SCF matrices."""
+    per-workgroup scratch (for `workgroups` persistent workgroups, or the
+    engine's default count on `cus` compute units) and the SCF matrices."""
     words = 0
-    lmax_all = 0
     maxn = 1
     for a in range(system.ncnt):
         for b in range(system.ncnt):
@@ -71,12 +94,14 @@ def estimate_bytes(system: System, workgroups: int) -> int:
                     for extra in ((0, 2, 1, 1, 1) if system.hmlt == "DHFB" else (0, 2)):
                         L = la + lb + extra
                         words += nc * na * nb * (L + 1) * (L + 2) * (L + 3) // 6
-            lmax_all = max(lmax_all, system.lmax(a))
-            maxn = max([maxn] + [system.nfnc_of(a, l) for l in range(system.lmax(a) + 1)])
+        maxn = max([maxn] + [system.nfnc_of(a, l) for l in range(system.lmax(a) + 1)])
+    # E, its folded E(CD) copy (double4 each)
+    table_bytes = 2 * 32 * words
     ld = ((maxn * maxn + 63) // 64) * 64
-    rows = 2426 if system.hmlt == "DHFB" else 1646
+    per_cu, rows = _launch_shape(system, maxn)
+    wg = workgroups or per_cu * cus
     n = system.ndim
-    return 32 * words + 8 * rows * ld * workgroups + _SCF_MATRICES * 16 * n * n
+    return table_bytes + 8 * rows * ld * wg + _SCF_MATRICES * 16 * n * n
 
 
 class Server:
@@ -97,8 +122,7 @@ class Server:
         # twelve per CU): every solve may use the whole device, and solves in
         # flight together share it through the hardware queues
         self.workgroups = workgroups
-        self._wg_count = workgroups or 12 * torch.cuda.get_device_properties(
-            torch.device(device)).multi_processor_count
+        self._cus = torch.cuda.get_device_properties(torch.device(device)).multi_processor_count
         self.device = torch.device(device)
         if hbm_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -167,6 +191,17 @@ class Server:
             dens.append(d)
         return _scf.superposed_density(system, dens)
 
+    def _need(self, system: System) -> int:
+        """Admission footprint: the solve's engine plus, for a molecule
+        started from atomic densities, the largest atom engine that _guess
+        may create while the molecule's engine is alive."""
+        need = estimate_bytes(system, self.workgroups, self._cus)
+        if self.atomic_guess and system.ncnt > 1:
+            atoms = [System("atom", np.zeros((1, 3)), [system.shells[c]], shape="ATOMIC",
+                            charges=[float(system.charges[c])]) for c in range(system.ncnt)]
+            need += max(estimate_bytes(a, self.workgroups, self._cus) for a in atoms)
+        return need
+
     def _release(self, need: int):
         with self._lock:
             self._used -= need
@@ -217,7 +252,7 @@ class Server:
             delay = req.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            need = estimate_bytes(req.system, self._wg_count)
+            need = self._need(req.system)
             with self._lock:
                 self._pending.append((req, need))
                 self._lock.notify_all()

@@ -217,6 +217,13 @@ int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms);
    and work items.  No counterpart in the reference. */
 int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items);
 
+/* How the stage-ilev Coulomb build is launched: waves per SIMD of its
+   quartet kernel (2 or 3), the capacity multiple of its AB-combination
+   groups (1 or 2), persistent workgroups, and the engine's scratch in
+   bytes.  No counterpart in the reference. */
+int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgroups,
+                    size_t *scratch_bytes);
+
 void bdf_destroy(bdf_engine *e);
 const char *bdf_last_error(void);
 

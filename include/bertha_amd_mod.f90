@@ -162,6 +162,15 @@ module bertha_amd
       type(c_ptr), value :: eng
     end subroutine
 
+    integer(c_int) function bdf_launch_info(eng, ilev, waves, cap, workgroups, scratch_bytes) &
+        bind(c, name='bdf_launch_info')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev
+      integer(c_int), intent(out) :: waves, cap, workgroups
+      integer(c_size_t), intent(out) :: scratch_bytes
+    end function
+
     type(c_ptr) function bdf_last_error() bind(c, name='bdf_last_error')
       import :: c_ptr
     end function

@@ -941,6 +941,12 @@ void orc_rmake(const double *qp, const double *aph, int n, int lam, double *rc) 
 /* ------------------------------------------------------------------------ */
 typedef struct {
   int ieab, igab, ncd;
+  /* timing samples (orc_clmfast_sample): only the class numbered cls_sel of
+     a range (-1: all) and only primitive pairs ij = (ibas-1)*nB + jbas-1
+     with ij % stride == 0 are evaluated; calls counts ERIFAST calls made and
+     calls_all those of the unsampled build */
+  int cls_sel, stride;
+  long calls, calls_all;
   int nab_max;         /* size of irij */
   unsigned char *irij; /* [ibas][jbas] */
   /* /ICDC/ flags, [itn 0:LL 1:SS][icd] and [itn][ncd][mc][md][icd] */
@@ -965,7 +971,10 @@ typedef struct {
 #define KMIDX(itn, ncd, mc, md, icd, nte) \
   (((((long)(itn) * 4 + (ncd)) * MKPX + (mc)) * MKPX + (md)) * (nte) + (icd))
 
-static void ws_init(eri_ws *w) { memset(w, 0, sizeof(*w)); }
+static void ws_init(eri_ws *w) {
+  memset(w, 0, sizeof(*w));
+  w->cls_sel = -1;
+}
 static void ws_free(eri_ws *w) {
   free(w->irij); free(w->icdr11); free(w->icdi11); free(w->icdr21); free(w->icdi21);
   for (int c = 0; c < 4; ++c) { free(w->km[c]); free(w->iab[c]); free(w->gcd[c]); }
@@ -1399,13 +1408,18 @@ static double slot_absmax(const zc *m, int n, const int r[2], int ir, const int 
    (3,4) with D(A,B); exchange (8,9) with D(A,D), (5) with D(B,C), (7,11) with
    D(A,C), (6,10) with D(B,D) -- and the first group above SENS ends the test,
    so the groups after it stay on. */
+static void schwarz_masks_with(int n, const double *G, int rigorous, double sens,
+                               const digest_t *d, const zc *dent, int *msk);
 static void schwarz_masks(const orc_sys *s, const digest_t *d, const zc *dent, int *msk) {
-  const int n = s->ndim;
   /* mode 2 (not the reference's): every group bounded by Q(A,B) Q(C,D), which
      bounds |(ab|cd)| for every quartet, so no dropped term exceeds SENS */
   const int rigorous = s->schwarz == 2;
-  const double *G = rigorous ? s->qbnd : s->gdsc;
-  const double sens = s->sens;
+  schwarz_masks_with(s->ndim, rigorous ? s->qbnd : s->gdsc, rigorous, s->sens, d, dent, msk);
+}
+
+/* the decision tree of SCHWARZ on an explicit bound table G (n x n) */
+static void schwarz_masks_with(int n, const double *G, int rigorous, double sens,
+                               const digest_t *d, const zc *dent, int *msk) {
   const int i = d->ibas - 1, j = d->jbas - 1;
   const int *A = d->nlab[0], *B = d->nlab[1], *C = d->nlab[2], *D = d->nlab[3];
   const double gab = slot_max(G, n, A, i, B, j);
@@ -1544,6 +1558,21 @@ void orc_pairbound(orc_sys *s, double *out) {
         }
 }
 
+void orc_schwarz_probe(const orc_sys *s, const double *g, const double *dent, const int nlab[8],
+                       int ibas, int jbas, int nb3, int nb4, double sens, int *msk) {
+  digest_t d;
+  memset(&d, 0, sizeof(d));
+  for (int k = 0; k < 4; ++k) {
+    d.nlab[k][0] = nlab[2 * k];
+    d.nlab[k][1] = nlab[2 * k + 1];
+  }
+  d.ibas = ibas;
+  d.jbas = jbas;
+  d.nbas[2] = nb3;
+  d.nbas[3] = nb4;
+  schwarz_masks_with(s->ndim, g, 0, sens, &d, (const zc *)dent, msk);
+}
+
 void orc_schwarz_stats(const orc_sys *s, double *drop_max, double *drop_count) {
   *drop_max = s->drop_max;
   *drop_count = s->drop_count;
@@ -1610,10 +1639,12 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
           q.mcnt = ncntrs(ca, cb, cc, cd);
           q.ilev = ilev;
           if (q.mcnt == 1 && s->racah1) continue;
+          int cls_idx = 0;
           for (int la = 0; la <= (s->nkap[ca] - 1) / 2; ++la)
           for (int lb = 0; lb <= (s->nkap[cb] - 1) / 2; ++lb)
           for (int lc = 0; lc <= (s->nkap[cc] - 1) / 2; ++lc)
           for (int ld = 0; ld <= (s->nkap[cd] - 1) / 2; ++ld) {
+            if (w->cls_sel >= 0 && cls_idx++ != w->cls_sel) continue;
             int lq[4] = {la, lb, lc, ld};
             for (int k = 0; k < 4; ++k) {
               q.lqn[k] = lq[k];
@@ -1739,6 +1770,11 @@ static void clmfast_range(orc_sys *s, int ilev, const zc *dent, zc *gdir, zc *gx
                                     if (ilev == 1 && itt > 1 && q.mcnt != 1) continue;
                                     if (ilev == 2 && itt > 3 && q.mcnt != 1) continue;
                                     int itn[2] = {it1, it2};
+                                    ++w->calls_all;
+                                    if (w->stride > 1 &&
+                                        ((ibas - 1) * q.nbas[1] + jbas - 1) % w->stride != 0)
+                                      continue;
+                                    ++w->calls;
                                     erifast(s, w, &q, ibas, jbas, itn, rr);
                                     dg.ibas = ibas; dg.jbas = jbas;
                                     for (int k = 0; k < 4; ++k) dg.nbas[k] = q.nbas[k];
@@ -1848,6 +1884,71 @@ void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir, doub
     free(gd); free(gx);
   }
   complete(s, (zc *)gdir, (zc *)gxch);
+}
+
+/* CLMFAST as a timing sample: work items are single classes (A,B,C,D,
+   la,lb,lc,ld) under dynamic scheduling, and only every stride-th primitive
+   pair ij of each class is evaluated (stride 1: the whole build).  Returns
+   the fraction of the build's ERIFAST calls made (every call of a class
+   costs the same, so this is the fraction of the work).  The matrices are
+   the sample's Hermitian-completed partial sums: a result only for
+   stride 1. */
+double orc_clmfast_sample(orc_sys *s, int ilev, const double *dent, double *gdir, double *gxch,
+                          int nthreads, int stride) {
+  int ndim = s->ndim;
+  long nn = (long)ndim * ndim;
+  s->record = 0;
+  memset(gdir, 0, sizeof(double) * 2 * nn);
+  memset(gxch, 0, sizeof(double) * 2 * nn);
+  const zc *d = (const zc *)dent;
+  int npair = 0;
+  int pa[4096], pb[4096];
+  for (int a = 0; a < s->ncnt; ++a) {
+    int nb = s->intsym ? a + 1 : s->ncnt;
+    for (int b = 0; b < nb; ++b) { pa[npair] = a; pb[npair] = b; ++npair; }
+  }
+  int ncls = 1;
+  for (int c = 0; c < s->ncnt; ++c) {
+    int nl = (s->nkap[c] - 1) / 2 + 1;
+    if (nl > ncls) ncls = nl;
+  }
+  ncls = ncls * ncls * ncls * ncls;  /* upper bound of classes per quadruple */
+  long nq = (long)npair * s->ncnt * s->ncnt * ncls;
+  long calls = 0, calls_all = 0;
+  if (nthreads < 1) nthreads = 1;
+  zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
+#pragma omp parallel num_threads(nthreads) reduction(+ : calls, calls_all)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    gd[tid] = calloc(nn, sizeof(zc));
+    gx[tid] = calloc(nn, sizeof(zc));
+    eri_ws w;
+    ws_init(&w);
+    w.stride = stride;
+#pragma omp for schedule(dynamic, 1)
+    for (long q = 0; q < nq; ++q) {
+      long qq = q / ncls;
+      w.cls_sel = (int)(q % ncls);
+      int p = (int)(qq / ((long)s->ncnt * s->ncnt));
+      int c = (int)((qq / s->ncnt) % s->ncnt), dd = (int)(qq % s->ncnt);
+      clmfast_range(s, ilev, d, gd[tid], gx[tid], pa[p], pa[p] + 1, pb[p], c, dd, &w);
+    }
+    calls += w.calls;
+    calls_all += w.calls_all;
+    ws_free(&w);
+  }
+  zc *G = (zc *)gdir, *X = (zc *)gxch;
+  for (int t = 0; t < nthreads; ++t) {
+    if (!gd[t]) continue;
+    for (long k = 0; k < nn; ++k) { G[k] += gd[t][k]; X[k] += gx[t][k]; }
+    free(gd[t]); free(gx[t]);
+  }
+  free(gd); free(gx);
+  complete(s, G, X);
+  return calls_all > 0 ? (double)calls / (double)calls_all : 1.0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2160,8 +2261,16 @@ static void biifast(orc_sys *s, brt_ws *w, const bquartet_t *q, int ibas, int jb
 /* BRTFAST loop nest for centre A in [a_lo, a_hi) and optional B, C, D
    selections; digestion as BRTMTG/BRTMTZ: exchange batches 5, 6, 7 and,
    for IQL != IQR, 9, 10, 11 into BXCH (closed shell: no direct term) */
+/* smp (may be NULL): a timing sample as orc_clmfast_sample's -- only class
+   number smp->cls_sel of the range and every smp->stride-th primitive pair,
+   with the BIIFAST calls made and those of the whole build counted */
+typedef struct {
+  int cls_sel, stride;
+  long calls, calls_all;
+} brt_sample;
+
 static void brtfast_range(orc_sys *s, int ilev, const zc *dent, zc *bdir, zc *bxch, int a_lo,
-                          int a_hi, int b_sel, int c_sel, int d_sel) {
+                          int a_hi, int b_sel, int c_sel, int d_sel, brt_sample *smp) {
   int ndim = s->ndim;
   int isym = s->isym;
   int *index = calloc((long)s->ncnt * 2 * MKPX * (2 * MKPX + 1), sizeof(int));
@@ -2185,10 +2294,12 @@ static void brtfast_range(orc_sys *s, int ilev, const zc *dent, zc *bdir, zc *bx
           if (mcnt != 1 && ilev < 3) continue;
           bquartet_t q;
           for (int k = 0; k < 4; ++k) { q.icnt[k] = ic[k] + 1; q.xyz[k] = s->xyz + 3 * ic[k]; }
+          int cls_idx = 0;
           for (int la = 0; la <= (s->nkap[ca] - 1) / 2; ++la)
           for (int lb = 0; lb <= (s->nkap[cb] - 1) / 2; ++lb)
           for (int lc = 0; lc <= (s->nkap[cc] - 1) / 2; ++lc)
           for (int ld = 0; ld <= (s->nkap[cd] - 1) / 2; ++ld) {
+            if (smp && smp->cls_sel >= 0 && cls_idx++ != smp->cls_sel) continue;
             int lq[4] = {la, lb, lc, ld};
             for (int k = 0; k < 4; ++k) {
               q.lqn[k] = lq[k];
@@ -2290,6 +2401,13 @@ static void brtfast_range(orc_sys *s, int ilev, const zc *dent, zc *bdir, zc *bx
                                 dg.nlab[2][1] = LRGE(s, cc, kc, 2 * mc);
                                 dg.nlab[3][0] = LRGE(s, cd, kd, 2 * md - 1) + s->nskp;
                                 dg.nlab[3][1] = LRGE(s, cd, kd, 2 * md) + s->nskp;
+                                if (smp) {
+                                  ++smp->calls_all;
+                                  if (smp->stride > 1 &&
+                                      ((ibas - 1) * q.nbas[1] + jbas - 1) % smp->stride != 0)
+                                    continue;
+                                  ++smp->calls;
+                                }
                                 biifast(s, &w, &q, ibas, jbas, rr);
                                 int iflg[11] = {0, 0, 0, 0, 1, 1, 1, 0, 0, 0, 0};
                                 if (iql != iqr) iflg[8] = iflg[9] = iflg[10] = 1;
@@ -2334,7 +2452,7 @@ void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir, doub
   const zc *d = (const zc *)dent;
   if (nthreads <= 1) {
     for (int a = 0; a < s->ncnt; ++a)
-      brtfast_range(s, ilev, d, (zc *)bdir, (zc *)bxch, a, a + 1, -1, -1, -1);
+      brtfast_range(s, ilev, d, (zc *)bdir, (zc *)bxch, a, a + 1, -1, -1, -1, NULL);
   } else {
     long nq = (long)s->ncnt * s->ncnt * s->ncnt * s->ncnt;
     zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
@@ -2351,7 +2469,7 @@ void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir, doub
         int n = s->ncnt;
         int a = (int)(k / ((long)n * n * n)), b = (int)((k / ((long)n * n)) % n);
         int c = (int)((k / n) % n), dd = (int)(k % n);
-        brtfast_range(s, ilev, d, gd[tid], gx[tid], a, a + 1, b, c, dd);
+        brtfast_range(s, ilev, d, gd[tid], gx[tid], a, a + 1, b, c, dd, NULL);
       }
     }
     zc *B = (zc *)bdir, *X = (zc *)bxch;
@@ -2363,6 +2481,57 @@ void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir, doub
     free(gd); free(gx);
   }
   complete_breit(s, (zc *)bdir, (zc *)bxch);
+}
+
+/* BRTFAST as a timing sample, as orc_clmfast_sample: returns the fraction
+   of the build's BIIFAST calls made. */
+double orc_brtfast_sample(orc_sys *s, int ilev, const double *dent, double *bdir, double *bxch,
+                          int nthreads, int stride) {
+  int ndim = s->ndim, n = s->ncnt;
+  long nn = (long)ndim * ndim;
+  build_ls_tables(s);
+  memset(bdir, 0, sizeof(double) * 2 * nn);
+  memset(bxch, 0, sizeof(double) * 2 * nn);
+  const zc *d = (const zc *)dent;
+  int ncls = 1;
+  for (int c = 0; c < n; ++c) {
+    int nl = (s->nkap[c] - 1) / 2 + 1;
+    if (nl > ncls) ncls = nl;
+  }
+  ncls = ncls * ncls * ncls * ncls;
+  long nq = (long)n * n * n * n * ncls;
+  long calls = 0, calls_all = 0;
+  if (nthreads < 1) nthreads = 1;
+  zc **gd = calloc(nthreads, sizeof(zc *)), **gx = calloc(nthreads, sizeof(zc *));
+#pragma omp parallel num_threads(nthreads) reduction(+ : calls, calls_all)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    gd[tid] = calloc(nn, sizeof(zc));
+    gx[tid] = calloc(nn, sizeof(zc));
+    brt_sample smp = {-1, stride, 0, 0};
+#pragma omp for schedule(dynamic, 1)
+    for (long k = 0; k < nq; ++k) {
+      long kk = k / ncls;
+      smp.cls_sel = (int)(k % ncls);
+      int a = (int)(kk / ((long)n * n * n)), b = (int)((kk / ((long)n * n)) % n);
+      int c = (int)((kk / n) % n), dd = (int)(kk % n);
+      brtfast_range(s, ilev, d, gd[tid], gx[tid], a, a + 1, b, c, dd, &smp);
+    }
+    calls += smp.calls;
+    calls_all += smp.calls_all;
+  }
+  zc *B = (zc *)bdir, *X = (zc *)bxch;
+  for (int t = 0; t < nthreads; ++t) {
+    if (!gd[t]) continue;
+    for (long k = 0; k < nn; ++k) { B[k] += gd[t][k]; X[k] += gx[t][k]; }
+    free(gd[t]); free(gx[t]);
+  }
+  free(gd); free(gx);
+  complete_breit(s, B, X);
+  return calls_all > 0 ? (double)calls / (double)calls_all : 1.0;
 }
 
 void orc_brtfast(orc_sys *s, int ilev, const double *dent, double *bdir, double *bxch) {

@@ -66,12 +66,24 @@ void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
 void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,
                     double *gxch, int nthreads);
 
+/* CLMFAST as a timing sample (bench.py's CPU baseline): classes as work
+   items on nthreads threads, only every stride-th primitive pair ij of each
+   class evaluated; returns the fraction of the build's ERIFAST calls made.
+   gdir/gxch receive the sample's partial sums (not a result). */
+double orc_clmfast_sample(orc_sys *s, int ilev, const double *dent, double *gdir, double *gxch,
+                          int nthreads, int stride);
+
 /* BRTFAST (bertha-scf.f:17060): closed-shell Breit matrices BDIR (zero for
    closed shells) and BXCH, LS blocks and their conjugate SL blocks */
 void orc_brtfast(orc_sys *s, int ilev, const double *dent, double *bdir,
                  double *bxch);
 void orc_brtfast_mt(orc_sys *s, int ilev, const double *dent, double *bdir,
                     double *bxch, int nthreads);
+/* BRTFAST as a timing sample, as orc_clmfast_sample (fraction of the
+   build's BIIFAST calls made). */
+double orc_brtfast_sample(orc_sys *s, int ilev, const double *dent, double *bdir, double *bxch,
+                          int nthreads, int stride);
+
 /* BIIFAST batch (bertha-scf.f:18337) for one (LS|LS) block quartet at a
    fresh class; rr planes as orc_erifast */
 int orc_biifast(orc_sys *s, const int icnt[4], const int kidx[4],
@@ -86,6 +98,14 @@ void orc_set_gaunt(orc_sys *s, int on);
    Gaussian-nucleus exponent (<= 0: point nucleus); cv the speed of light. */
 void orc_oneel(orc_sys *s, const double *znuc, const double *xi, double cv,
                double *ovlp, double *hnuc, double *hkin);
+
+/* SCHWARZ (bertha-scf.f:37889) for one ERIFAST call on an explicit bound
+   table g (ndim x ndim real, column-major) and density dent (complex
+   column-major): nlab = NA1, NA2, NB1, NB2, NC1, NC2, ND1, ND2 (0-based
+   block offsets), ibas/jbas 1-based, nb3 x nb4 kl pairs; msk[m] bit b keeps
+   batch b+1, 0 drops the kl. */
+void orc_schwarz_probe(const orc_sys *s, const double *g, const double *dent, const int nlab[8],
+                       int ibas, int jbas, int nb3, int nb4, double sens, int *msk);
 
 /* CLMDIAG (bertha-scf.f:38126): GDSC, ndim x ndim real column-major */
 void orc_clmdiag(orc_sys *s, double *gdsc);

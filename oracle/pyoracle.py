@@ -46,10 +46,16 @@ def _oracle_lib():
         lib.orc_erifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, C.c_int, P, P]
         lib.orc_clmfast.argtypes = [P, C.c_int, P, P, P]
         lib.orc_clmfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
+        lib.orc_brtfast_sample.restype = C.c_double
+        lib.orc_brtfast_sample.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_int]
+        lib.orc_clmfast_sample.restype = C.c_double
+        lib.orc_clmfast_sample.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_int]
         lib.orc_clmdiag.argtypes = [P, P]
         lib.orc_pairbound.argtypes = [P, P]
         lib.orc_set_schwarz.argtypes = [P, C.c_int, C.c_double]
         lib.orc_schwarz_stats.argtypes = [P, P, P]
+        lib.orc_schwarz_probe.argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_double, P]
         lib.orc_brtfast.argtypes = [P, C.c_int, P, P, P]
         lib.orc_brtfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
         lib.orc_set_gaunt.argtypes = [P, C.c_int]
@@ -170,6 +176,18 @@ class Oracle:
             _oracle_lib().orc_brtfast(self._h, ilev, d.ctypes.data, b.ctypes.data, x.ctypes.data)
         return b, x
 
+    def brtfast_sample(self, dent, ilev=3, nthreads=1, stride=1, gaunt=False):
+        """Timing sample of BRTFAST, as clmfast_sample: (fraction of the
+        build's BIIFAST calls made, BDIR, BXCH)."""
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        b = np.zeros((n, n), dtype=np.complex128, order="F")
+        x = np.zeros((n, n), dtype=np.complex128, order="F")
+        _oracle_lib().orc_set_gaunt(self._h, 1 if gaunt else 0)
+        frac = _oracle_lib().orc_brtfast_sample(self._h, ilev, d.ctypes.data, b.ctypes.data,
+                                                x.ctypes.data, nthreads, stride)
+        return frac, b, x
+
     def pairbound(self):
         """sqrt|(ab|ba)| table (ndim x ndim real)."""
         g = np.zeros((self.ndim, self.ndim), order="F")
@@ -182,6 +200,30 @@ class Oracle:
         a, b = C.c_double(), C.c_double()
         _oracle_lib().orc_schwarz_stats(self._h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def schwarz_probe(self, g, dent, nlab, ibas, jbas, nb3, nb4, sens):
+        """SCHWARZ's masks for one ERIFAST call on bound table g: int[nb3*nb4],
+        bit b keeps batch b+1, 0 drops the kl (kl = k * nb4 + l)."""
+        gf = np.asfortranarray(g, dtype=np.float64)
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        lab = np.ascontiguousarray(nlab, dtype=np.int32)
+        msk = np.zeros(nb3 * nb4, dtype=np.int32)
+        _oracle_lib().orc_schwarz_probe(self._h, gf.ctypes.data, d.ctypes.data, lab.ctypes.data,
+                                        ibas, jbas, nb3, nb4, sens, msk.ctypes.data)
+        return msk
+
+    def clmfast_sample(self, dent, ilev=3, nthreads=1, stride=1):
+        """Timing sample of CLMFAST (classes as work items, every stride-th
+        primitive pair ij): (fraction of the build's ERIFAST calls made,
+        GDIR, GXCH partial sums; the whole build for stride 1)."""
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        g = np.zeros((n, n), dtype=np.complex128, order="F")
+        x = np.zeros((n, n), dtype=np.complex128, order="F")
+        _oracle_lib().orc_set_schwarz(self._h, 0, 0.0)
+        frac = _oracle_lib().orc_clmfast_sample(self._h, ilev, d.ctypes.data, g.ctypes.data,
+                                                x.ctypes.data, nthreads, stride)
+        return frac, g, x
 
     def clmfast(self, dent, ilev=3, nthreads=1, schwarz=False, sens=0.0):
         n = self.ndim
@@ -306,6 +348,65 @@ class Reference:
                              C.c_int(1 if gauss else 0), *[C.c_void_p(o.ctypes.data) for o in out])
         m = [o.reshape(n, n, order="F") for o in out]
         return m[0] + 1j * m[1], m[2] + 1j * m[3], m[4] + 1j * m[5]
+
+    def schwarz_masks(self, g, dent, nlab, ibas, jbas, nb3, nb4, sens):
+        """The reference's SCHWARZ on bound table g (passed as its GDIR
+        argument) and density dent: (IMTX[11][nb3*nb4], ISCR[nb3*nb4], MAXN)."""
+        n = self.ndim
+        gf = np.asfortranarray(g, dtype=np.float64)
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        dre = np.asfortranarray(d.real)
+        dim = np.asfortranarray(d.imag)
+        lab = np.ascontiguousarray(nlab, dtype=np.int32)
+        m = nb3 * nb4
+        imtx = np.zeros(11 * m, dtype=np.int32)
+        iscr = np.zeros(m, dtype=np.int32)
+        maxn = C.c_int()
+        assert gf.shape == (n, n)
+        _ref_lib().ref_schwarz_masks(C.c_void_p(gf.ctypes.data), C.c_void_p(dre.ctypes.data),
+                                     C.c_void_p(dim.ctypes.data), C.c_void_p(lab.ctypes.data),
+                                     C.c_int(ibas), C.c_int(jbas), C.c_int(nb3), C.c_int(nb4),
+                                     C.c_double(sens), C.c_void_p(imtx.ctypes.data),
+                                     C.c_void_p(iscr.ctypes.data), C.byref(maxn))
+        return imtx.reshape(11, m), iscr, maxn.value
+
+    @staticmethod
+    def _planes(m):
+        m = np.asfortranarray(m, dtype=np.complex128)
+        return np.asfortranarray(m.real), np.asfortranarray(m.imag)
+
+    def densty(self, coef, icls):
+        """DENSTY with COEF and closed-shell spinors icls (1-based, counted
+        from NSKP): DENT."""
+        n = self.ndim
+        cre, cim = self._planes(coef)
+        ic = np.ascontiguousarray(icls, dtype=np.int32)
+        dre, dim = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        _ref_lib().ref_densty(C.c_void_p(cre.ctypes.data), C.c_void_p(cim.ctypes.data),
+                              C.c_int(ic.size), C.c_void_p(ic.ctypes.data),
+                              C.c_void_p(dre.ctypes.data), C.c_void_p(dim.ctypes.data))
+        return dre + 1j * dim
+
+    def levshft(self, fock, ovlp, coef, nocc, shlv):
+        """LEVSHFT: the shifted FOCK."""
+        fre, fim = self._planes(fock)
+        ore, oim = self._planes(ovlp)
+        cre, cim = self._planes(coef)
+        _ref_lib().ref_levshft(*[C.c_void_p(x.ctypes.data) for x in (fre, fim, ore, oim, cre, cim)],
+                               C.c_int(nocc), C.c_double(shlv))
+        return fre + 1j * fim
+
+    def enrgies(self, dent, hkin, hnuc, fock, gdir, gxch, bdir, bxch, enuc, breit):
+        """ENRGIES, closed shell: (ETOT, EONE, EHKN, EHNC, EGDR, EGXC, EBDR,
+        EBXC)."""
+        planes = []
+        for m in (dent, hkin, hnuc, fock, gdir, gxch, bdir, bxch):
+            planes += list(self._planes(m))
+        a = np.concatenate([p.reshape(-1, order="F") for p in planes])
+        out = np.zeros(8)
+        _ref_lib().ref_enrgies(C.c_void_p(a.ctypes.data), C.c_double(enuc),
+                               C.c_int(2 if breit else 1), C.c_void_p(out.ctypes.data))
+        return out
 
     def clmdiag(self):
         """CLMDIAG's GDSC (ndim x ndim real)."""

@@ -84,6 +84,24 @@ def test_clmfast_matches_oracle(bdf, orc, systems, name):
             assert np.abs(x - xo).max() < FOCK_ABS, (name, ilev, np.abs(x - xo).max())
 
 
+@pytest.mark.parametrize("nprim,shape", [(16, (3, 1)), (17, (2, 2)), (20, (2, 1))])
+def test_clmfast_wide_shells_match_oracle(bdf, orc, nprim, shape):
+    """The complex path's launch shapes by the exchange accumulators' LDS:
+    three waves per SIMD up to 16 primitives per shell, two waves at group
+    capacity 2 for 17-18 and at capacity 1 beyond."""
+    ev = bdf.systems.even_tempered
+    s = bdf.systems.System(f"wide-{nprim}", np.array([[0.1, -0.2, 0.3], [1.2, 0.9, -1.1]]),
+                           [[ev(0.05, 2.2, nprim), ev(0.2, 2.5, 1)], [ev(0.1, 3.0, 2)]],
+                           shape="NOSYMM")
+    print(f"[wide] {nprim} primitives", flush=True)
+    d = bdf.systems.random_density(s.ndim, seed=nprim)
+    with bdf.Engine(s) as eng:
+        assert eng.launch_info(3)[:2] == shape
+        g, x = eng.clmfast(d, 3)
+    go, xo = orc.Oracle(s).clmfast(d, 3)
+    assert np.abs(g - go).max() < FOCK_ABS and np.abs(x - xo).max() < FOCK_ABS
+
+
 def test_clmfast_racah1_matches_oracle(bdf, orc, systems):
     s = systems["tri"]
     d = bdf.systems.random_density(s.ndim, seed=9)

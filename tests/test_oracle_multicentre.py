@@ -66,3 +66,36 @@ def test_translated_four_centres_agree(orc, bdf):
     g, x = orc.Oracle(s).clmfast(d, 3)
     gt, xt = orc.Oracle(t).clmfast(d, 3)
     assert np.abs(gt - g).max() < 1e-11 and np.abs(xt - x).max() < 1e-11
+
+
+@pytest.mark.parametrize("name", ["diat", "tri", "quad"])
+def test_sampled_oracle_build_is_the_build_at_stride_one(bdf, orc, name):
+    """bench.py's CPU baseline times orc_clmfast_sample (classes as work
+    items, every stride-th primitive pair): at stride 1 it is the oracle's
+    CLMFAST build, and a larger stride reports the fraction of ERIFAST calls
+    it made."""
+    from conftest import small_systems
+    s = small_systems(bdf)[name]
+    O = orc.Oracle(s)
+    d = bdf.systems.random_density(s.ndim, seed=4, real=(s.shape != "NOSYMM"))
+    g, x = O.clmfast(d, 3)
+    frac, g1, x1 = O.clmfast_sample(d, 3, nthreads=4, stride=1)
+    assert frac == 1.0
+    assert np.abs(g - g1).max() < 1e-13 and np.abs(x - x1).max() < 1e-13
+    frac3, _, _ = O.clmfast_sample(d, 3, nthreads=4, stride=3)
+    assert 1.0 / 3.0 - 1e-12 <= frac3 < 0.6
+
+
+@pytest.mark.parametrize("name", ["diat", "off"])
+def test_sampled_oracle_breit_is_the_build_at_stride_one(bdf, orc, name):
+    from conftest import small_systems
+    s0 = small_systems(bdf)[name]
+    s = bdf.systems.System(s0.name, s0.xyz, s0.shells, s0.shape, hmlt="DHFB")
+    O = orc.Oracle(s)
+    d = bdf.systems.random_density(s.ndim, seed=4, real=(s.shape != "NOSYMM"))
+    b, x = O.brtfast(d, 3)
+    frac, b1, x1 = O.brtfast_sample(d, 3, nthreads=4, stride=1)
+    assert frac == 1.0
+    assert np.abs(b - b1).max() < 1e-13 and np.abs(x - x1).max() < 1e-13
+    frac3, _, _ = O.brtfast_sample(d, 3, nthreads=4, stride=3)
+    assert 1.0 / 3.0 - 1e-12 <= frac3 < 0.6

@@ -9,9 +9,12 @@ cannot produce it in a fresh process: ERI loops N = 1..MAXN with MAXN of
 The (ab|ba) bound table is pinned by the Cauchy-Schwarz inequality of the
 Coulomb metric on random ERIFAST batches, which it must satisfy and saturate.
 
-The SCHWARZ masks (:37889) have no runnable reference counterpart; they are
-checked only for monotonic growth of the dropped set with SENS: parity
-unpinned against the reference.
+The SCHWARZ masks (:37889) are pinned to the reference's own SCHWARZ,
+called through oracle/ref/ref_driver.f with the pinned GDSC injected as its
+bound-table argument (the reference's CLMDIAG cannot fill it, above): for
+random block quartets, primitive pairs, densities and SENS from 1e-12 to 1,
+IMTX, ISCR and MAXN equal the oracle's masks bit for bit.  Monotonic growth
+of the dropped set with SENS is checked on whole builds as well.
 """
 import numpy as np
 import pytest
@@ -143,3 +146,47 @@ def test_rigorous_masks_drop_nothing_above_sens(bdf, orc, name):
     assert counts == sorted(counts) and counts[-1] > 0
     O.clmfast(d, 3, schwarz=True, sens=1e-12)
     assert O.schwarz_stats()[0] > 1e-6
+
+
+@pytest.mark.parametrize("name", ["ne", "diat", "off", "fsh"])
+def test_schwarz_masks_match_reference(bdf, orc, systems, name):
+    if not orc.reference_available():
+        pytest.skip("oracle/_ref not built")
+    s = systems[name]
+    R = orc.Reference(s)
+    O = orc.Oracle(s)
+    g = O.clmdiag()  # pinned to the reference's ERIFAST (test above)
+    lab = O.labels()
+    rng = np.random.default_rng(11)
+    real = s.shape != "NOSYMM"
+    d = bdf.systems.random_density(O.ndim, seed=8, real=real)
+    # elements spread over decades, so that every group of the decision
+    # tree decides somewhere in the SENS sweep
+    d = d * np.exp(rng.uniform(-12.0, 0.0, size=d.shape))
+    d = np.asfortranarray(0.5 * (d + d.conj().T))
+    seen_partial = seen_drop = seen_keep = 0
+    for _ in range(24):
+        blk = _random_block(rng, s, 4)
+        itn = [int(rng.choice([1, 4])), int(rng.choice([1, 4]))]
+        nlab = []
+        for k, (c, _, kq, mq) in enumerate(blk):
+            base = O.nskp if itn[k // 2] == 4 else 0
+            nlab += [_block_start(lab, c, kq, -(2 * mq - 1)) + base,
+                     _block_start(lab, c, kq, 2 * mq - 1) + base]
+        nf = [s.nfnc_of(b[0] - 1, b[1] // 2) for b in blk]
+        i, j = int(rng.integers(1, nf[0] + 1)), int(rng.integers(1, nf[1] + 1))
+        for sens in (1e-12, 1e-9, 1e-6, 1e-4, 1e-2, 1.0):
+            imtx, iscr, maxn = R.schwarz_masks(g, d, nlab, i, j, nf[2], nf[3], sens)
+            msk = O.schwarz_probe(g, d, nlab, i, j, nf[2], nf[3], sens)
+            ref_bits = (imtx * (1 << np.arange(11))[:, None]).sum(axis=0)
+            # a dropped kl (ISCR = 0) has every IMTX bit cleared, which is
+            # the oracle's msk = 0
+            assert np.array_equal(iscr, (msk != 0).astype(np.int32)), (name, sens)
+            assert np.array_equal(np.where(iscr == 1, ref_bits, 0), msk), (name, sens)
+            assert maxn == int((msk != 0).sum())
+            seen_drop += int((msk == 0).sum())
+            seen_keep += int((msk == 0x7ff).sum())
+            seen_partial += int(((msk != 0) & (msk != 0x7ff)).sum())
+    # the comparison reached every outcome of the decision tree
+    assert seen_drop > 0 and seen_keep > 0 and seen_partial > 0, (seen_drop, seen_keep,
+                                                                   seen_partial)

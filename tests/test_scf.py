@@ -39,6 +39,55 @@ def test_one_electron_oracle_matches_reference(bdf, orc, name):
             assert np.abs(r - g).max() <= 1e-14 * np.abs(r).max()
 
 
+def _hermitian(rng, n, scale=1.0):
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return scale * (a + a.conj().T)
+
+
+@pytest.mark.parametrize("hmlt", ["DHFR", "DHFB"])
+def test_driver_steps_match_reference(bdf, orc, scf, hmlt):
+    """The driver's density, level shift and energies against the
+    reference's own DENSTY (bertha-scf.f:3580), LEVSHFT (:3641) and ENRGIES
+    (:3698), called through oracle/ref/ref_driver.f on random matrices of a
+    two-centre system's dimensions."""
+    if not orc.reference_available():
+        pytest.skip("oracle/_ref/libbertha_ref.so not built (reference sources absent)")
+    import torch
+    base = small_systems(bdf)["diat"]
+    s = bdf.systems.System(base.name, base.xyz, base.shells, base.shape, hmlt=hmlt,
+                           charges=[3.0, 1.0])
+    R = orc.Reference(s)
+    n, nskp, nocc = R.ndim, R.nskp, 4
+    rng = np.random.default_rng(7)
+    coef = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    tc = torch.from_numpy(coef)
+    # DENSTY
+    d_ref = R.densty(coef, np.arange(1, nocc + 1))
+    d = scf.density(tc, nskp, nocc).numpy()
+    assert np.abs(d - d_ref).max() <= 1e-14 * np.abs(d_ref).max()
+    # LEVSHFT
+    fock = _hermitian(rng, n)
+    ovlp = _hermitian(rng, n, 0.1) + n * np.eye(n)
+    f_ref = R.levshft(fock, ovlp, coef, nocc, 0.7)
+    f = scf.level_shift(torch.from_numpy(fock), torch.from_numpy(ovlp), tc, nskp, nocc,
+                        0.7).numpy()
+    assert np.abs(f - f_ref).max() <= 1e-13 * np.abs(f_ref).max()
+    # ENRGIES
+    dens = _hermitian(rng, n, 0.05)
+    mats = [_hermitian(rng, n) for _ in range(7)]
+    hkin, hnuc, fk, gd, gx, bd, bx = mats
+    breit = hmlt == "DHFB"
+    e_ref = R.enrgies(dens, hkin, hnuc, fk, gd, gx, bd, bx, bdf.systems.nuclear_repulsion(s),
+                      breit)
+    T = [torch.from_numpy(m) for m in (dens, hkin, hnuc, gd, gx, bd, bx)]
+    e = scf.energies(s, T[0], T[1], T[2], T[3], T[4], *(T[5:] if breit else (None, None)))
+    got = [e.total, e.kinetic + e.potential, e.kinetic, e.potential, e.coulomb_direct,
+           e.coulomb_exchange, e.breit_direct, e.breit_exchange]
+    if not breit:
+        e_ref[6:] = 0.0  # ENRGIES returns before the Breit terms for DHFR
+    assert np.allclose(got, e_ref, rtol=1e-13, atol=1e-11), (got, e_ref.tolist())
+
+
 def test_ne_scf_on_oracle(bdf, orc, scf):
     s = bdf.systems.ne_atom()
     r = scf.run_scf(s, 10, OracleBackend(bdf, orc, s))
@@ -48,6 +97,30 @@ def test_ne_scf_on_oracle(bdf, orc, scf):
     eig = r.eigenvalues.numpy()[s.ndim // 2:s.ndim // 2 + 10]
     ref = [-32.77, -32.77, -1.936, -1.936, -0.853, -0.853, -0.848, -0.848, -0.848, -0.848]
     assert np.abs(eig - ref).max() < 0.05
+
+
+def test_converged_state_is_stationary(bdf, orc, scf):
+    """At convergence the Fock matrix built from the final density commutes
+    with it (the DIIS error F P S - S P F, P = conj(D), bertha-scf.f:3883)
+    and the energy is E_nuc + 1/2 sum D (H + F) (ENRGIES, :3745-3782)."""
+    import torch
+    s = bdf.systems.ne_atom()
+    be = OracleBackend(bdf, orc, s)
+    r = scf.run_scf(s, 10, be)
+    assert r.converged
+    ovlp, hnuc, hkin = be.one_electron(137.035999084)
+    d = r.density
+    gdir, gxch = be.coulomb(d, 3)
+    h = hnuc + hkin
+    f = h + gdir - gxch
+    p = d.conj()
+    err = f @ p @ ovlp - ovlp @ p @ f
+    assert float(err.abs().max()) < 1e-7
+    e = bdf.systems.nuclear_repulsion(s) + 0.5 * float(torch.sum(d * (h + f)).real)
+    assert abs(e - r.energy) < 1e-9
+    # and the density is idempotent in the metric: P S P = P, trace 10
+    assert float((p @ ovlp @ p - p).abs().max()) < 1e-10
+    assert abs(float(torch.sum(d * ovlp).real) - 10.0) < 1e-10
 
 
 def test_diis_coefficients_are_the_constrained_minimum(scf):
@@ -148,3 +221,24 @@ def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
         be.close()
     assert ro.converged and rg.converged
     assert abs(rg.energy - ro.energy) < 1e-8, (rg.energy, ro.energy)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workload", ["ne", "water"])
+def test_mixed_precision_scf_energy(bdf, scf, workload):
+    """The metric's bar at the level of a converged solve: the SCF on the
+    mixed-precision engine (FP32 below the Schwarz split) lands within 1 uEh
+    of the FP64 SCF, for the Ne atom and the water workload (NDIM 380)."""
+    s = bdf.systems.config_system(workload, **({} if workload == "ne" else {"seed": 0}))
+    nocc = int(round(sum(s.charges)))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[scf] {workload} {prec}", flush=True)
+        be = scf.EngineBackend(bdf, s, precision=prec)
+        try:
+            out[prec] = scf.run_scf(s, nocc, be)
+        finally:
+            be.close()
+    assert out["fp64"].converged and out["mixed"].converged
+    assert abs(out["mixed"].energy - out["fp64"].energy) < 1e-6, (
+        out["mixed"].energy, out["fp64"].energy)
