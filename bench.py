@@ -140,6 +140,9 @@ def main() -> int:
                          "bounds) or rigorous (Q(a,b) Q(c,d) on every group)")
     ap.add_argument("--density", default="atoms", choices=["random", "atoms"],
                     help="superposed bare-nucleus atoms, or a random Hermitian test density")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every build from the host instead of replaying captured HIP "
+                         "graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=0,
                     help="oracle sample: every S-th primitive pair (0: by workload)")
@@ -176,7 +179,8 @@ def main() -> int:
     schwarz = {None: False, "reference": True, "rigorous": "rigorous"}[args.schwarz]
     dev = torch.device("cuda:0")
     t_setup = time.perf_counter()
-    eng = pkg.Engine(system, schwarz=schwarz)
+    graph = not args.no_graph
+    eng = pkg.Engine(system, schwarz=schwarz, graph=graph)
     n = eng.ndim
     flops_c, quartets_c, items_c = eng.plan_info(args.stage)
     flops_b = quartets_b = 0.0
@@ -226,7 +230,9 @@ def main() -> int:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms = ev0.elapsed_time(ev1) / args.steps
-        # each quartet kernel's own duration (events bracket only it)
+        # each quartet kernel's own duration (events bracket only it in a
+        # build launched from the host)
+        e.set_launch(graph=False)
         e.clmfast_device(d_t.data_ptr(), g_t.data_ptr(), x_t.data_ptr(), args.stage)
         kc = e.stats()[1]
         kb = 0.0
@@ -234,6 +240,7 @@ def main() -> int:
             e.breit_device(d_t.data_ptr(), b_t.data_ptr(), bx_t.data_ptr(), args.stage)
             kb = e.stats()[1]
         torch.cuda.synchronize()
+        e.set_launch(graph=graph)
         print(f"bench: {label} {ms:.1f} ms/step (coulomb kernel {kc:.1f} ms, breit kernel "
               f"{kb:.1f} ms)", file=sys.stderr, flush=True)
         mats = [m.cpu().numpy().T for m in (g_t, x_t, b_t, bx_t)]
@@ -248,12 +255,14 @@ def main() -> int:
         return 0.5 * float(np.real(v))
 
     e64 = e2(g, x, b, bx)
+    launch = {"hip_graph": graph, "settings": eng.launch,
+              "tuned": eng.tuning is not None}
     eng.close()
 
     mixed = None
     if args.split > 0:
         eng_m = pkg.Engine(system, precision="mixed", split_threshold=args.split,
-                           schwarz=schwarz)
+                           schwarz=schwarz, graph=graph)
         ms_m, _, kc_m, kb_m, (gm, xm, bm, bxm) = timed_leg(eng_m, "mixed")
         eng_m.close()
         dE = abs(e2(gm, xm, bm, bxm) - e64)
@@ -375,7 +384,7 @@ def main() -> int:
                    "precision": (f"mixed, Coulomb FP32 below split {args.split}" if headline
                                  else "fp64"),
                    "quartets_per_build": quartets_c + quartets_b, "work_items": items_c,
-                   "host_wall_s_fp64": wall64},
+                   "host_wall_s_fp64": wall64, "launch": launch},
         "fp64": {"value": 1000.0 / ms64, "ms_per_step": ms64, "coulomb_kernel_ms": kc64,
                  "breit_kernel_ms": kb64, "two_electron_energy_Eh": e64,
                  "coulomb_tflops": flops_c / (kc64 * 1e-3) / 1e12,

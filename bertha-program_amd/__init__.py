@@ -54,6 +54,16 @@ class _System(_C.Structure):
     ]
 
 
+class _Launch(_C.Structure):
+    _fields_ = [
+        ("graph", _C.c_int),
+        ("waves", _C.c_int),
+        ("cap", _C.c_int),
+        ("per_cu", _C.c_int),
+        ("breit_per_cu", _C.c_int),
+    ]
+
+
 class _Options(_C.Structure):
     _fields_ = [
         ("intsym", _C.c_int),
@@ -79,6 +89,14 @@ def library() -> _C.CDLL:
             raise RuntimeError(
                 f"{LIBPATH} is missing: build it with `make -C bertha-program_amd` "
                 "(or __graft_entry__.build())")
+        # torch ships its own HIP runtime under the same soname as the
+        # system's: whichever loads first serves the whole process, and torch
+        # finds no GPU through the system's.  Load torch's first where torch
+        # is installed, so that the engine and torch share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = _C.CDLL(LIBPATH)
         P = _C.c_void_p
         lib.bdf_last_error.restype = _C.c_char_p
@@ -101,6 +119,7 @@ def library() -> _C.CDLL:
         lib.bdf_breit_plan_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
                                             _C.POINTER(_C.c_double), _C.POINTER(_C.c_long)]
         lib.bdf_memory_info.argtypes = [P, _C.POINTER(_C.c_size_t)]
+        lib.bdf_set_launch.argtypes = [P, _C.POINTER(_Launch)]
         lib.bdf_launch_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_int), _C.POINTER(_C.c_int),
                                         _C.POINTER(_C.c_int), _C.POINTER(_C.c_size_t)]
         lib.bdf_stream.argtypes = [P]
@@ -140,7 +159,7 @@ class Engine:
     def __init__(self, system: System, intsym: bool = True, racah1: bool = False,
                  device: int = 0, schwarz=False, sens: float = 0.0,
                  precision: str = "fp64", split_threshold: float = 0.0, gaunt: bool = False,
-                 workgroups: int = 0):
+                 workgroups: int = 0, tuned: bool = True, graph: bool = False):
         lib = library()
         self.system = system
         xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
@@ -160,6 +179,20 @@ class Engine:
         self._h = h
         self.ndim = nd.value
         self.nskp = ns.value
+        # launch settings (bdf_set_launch), with the autotuner's shapes for
+        # this kind of system (tune.py) unless tuned is False
+        self._launch = {"graph": False, "waves": 0, "cap": 0, "per_cu": 0, "breit_per_cu": 0}
+        self.tuning = None
+        upd = {"graph": bool(graph)}
+        if tuned:
+            from . import tune as _tune
+            entry = _tune.lookup(system)
+            if entry:
+                upd.update(waves=entry["waves"], cap=entry["cap"], per_cu=entry["per_cu"],
+                           breit_per_cu=entry.get("breit_per_cu", 0))
+                self.tuning = entry
+        if any(upd.values()):
+            self.set_launch(**upd)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -289,6 +322,23 @@ class Engine:
         n = _C.c_long()
         _check(library().bdf_plan_info(self._h, ilev, _C.byref(f), _C.byref(q), _C.byref(n)))
         return f.value, q.value, n.value
+
+    def set_launch(self, **kw) -> None:
+        """bdf_set_launch: graph (HIP-graph replay of builds), waves, cap,
+        per_cu, breit_per_cu (the quartet kernels' launch shapes, 0: the
+        engine's choice); settings not named keep their current values."""
+        bad = set(kw) - set(self._launch)
+        if bad:
+            raise TypeError(f"unknown launch settings {sorted(bad)}")
+        new = dict(self._launch, **kw)
+        lo = _Launch(1 if new["graph"] else 0, new["waves"], new["cap"], new["per_cu"],
+                     new["breit_per_cu"])
+        _check(library().bdf_set_launch(self._h, _C.byref(lo)))
+        self._launch = new
+
+    @property
+    def launch(self) -> dict:
+        return dict(self._launch)
 
     def launch_info(self, ilev: int = 3):
         """(waves per SIMD, group capacity, workgroups, scratch bytes) of the

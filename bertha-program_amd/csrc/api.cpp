@@ -59,6 +59,7 @@ struct DevBuf {
 struct DevPlan {
   int built = 0;
   int xacc_n = 0, blocks = 1;  // exchange-accumulator doubles, workgroups per launch
+  FockLaunch fl{2, 1};         // waves per SIMD and group capacity of the launch
   Plan P;
   DevBuf<double> qitem;  // Schwarz factor of each work item (mixed precision)
   DevBuf<ClassDesc> classes;
@@ -77,6 +78,14 @@ struct DevBreitPlan {
   DevBuf<uint8_t> masks;
   DevBuf<WorkItem> items;
   DevBuf<uint8_t> stale;
+};
+
+// One captured build: the launches of a Coulomb (kind 0) or Breit (kind 1)
+// build of stage ilev on fixed device buffers, replayed as a HIP graph.
+struct GraphEntry {
+  int kind, ilev;
+  const void *d, *g, *x;
+  hipGraphExec_t exec;
 };
 
 struct bdf_engine {
@@ -125,7 +134,24 @@ struct bdf_engine {
   hipStream_t stream32 = nullptr;  // FP32 instance of the mixed build
   hipEvent_t fork = nullptr, join = nullptr;
   double last_ms = 0.0, last_quartets = 0.0;
+  // launch settings (bdf_set_launch): 0 = the engine's choice
+  int tune_waves = 0, tune_cap = 0, tune_per_cu = 0, tune_breit_per_cu = 0;
+  int use_graph = 0;
+  std::vector<GraphEntry> graphs;
 };
+
+// where one build's per-workgroup scratch lives: base, doubles per
+// workgroup, workgroups it holds
+struct SlabView {
+  double *p;
+  int64_t stride;
+  int blocks;
+};
+
+static void drop_graphs(bdf_engine *e) {
+  for (GraphEntry &g : e->graphs) (void)hipGraphExecDestroy(g.exec);
+  e->graphs.clear();
+}
 
 static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double> &q);
 
@@ -157,10 +183,33 @@ static int ensure_slab(bdf_engine *e, int maxcd, int g_rows, int blocks) {
     e->slab_rows = std::max(rows, e->slab_rows);
     e->slab_blocks = std::max(blocks, e->slab_blocks);
     e->slab_stride = (int64_t)e->slab_rows * e->slab_ld;
+    // captured builds hold the old scratch address
+    drop_graphs(e);
     // the mixed build's FP32 instance runs concurrently on a second set of blocks
     HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->slab_blocks * (e->mixed ? 2 : 1)));
   }
   return BDF_OK;
+}
+
+// the Coulomb kernel's launch shape for a plan: the engine's choice by LDS
+// (fock_launch_config), with the settings of bdf_set_launch applied
+static FockLaunch fock_config(const bdf_engine *e, const DevPlan &dp) {
+  FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
+  if (e->tune_waves) fl.waves = e->tune_waves;
+  if (e->tune_cap) fl.cap = e->tune_cap;
+  if (e->tune_waves == 3 && !e->tune_cap) fl.cap = 1;
+  return fl;
+}
+
+// workgroups and scratch of a built plan for its launch shape
+static int size_launch(bdf_engine *e, DevPlan &dp) {
+  dp.fl = fock_config(e, dp);
+  dp.blocks = blocks_for(e, e->tune_per_cu ? e->tune_per_cu : 4 * dp.fl.waves);
+  return ensure_slab(e, dp.P.max_maxcd, 4 * dp.fl.cap * ntuv(kMaxLab), dp.blocks);
+}
+
+static int breit_blocks(const bdf_engine *e) {
+  return blocks_for(e, e->tune_breit_per_cu ? e->tune_breit_per_cu : 8);
 }
 
 static int make_plan(bdf_engine *e, int ilev) {
@@ -182,9 +231,7 @@ static int make_plan(bdf_engine *e, int ilev) {
   // four workgroups per CU for each wave per SIMD the kernel runs at, G rows
   // for its combination groups
   dp.xacc_n = plan_xacc(dp.P, e->isym);
-  const FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
-  dp.blocks = blocks_for(e, 4 * fl.waves);
-  int rc = ensure_slab(e, dp.P.max_maxcd, 4 * fl.cap * ntuv(kMaxLab), dp.blocks);
+  int rc = size_launch(e, dp);
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -356,11 +403,12 @@ static void item_factors(const bdf_engine *e, const Plan &P, std::vector<double>
   }
 }
 
-static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_gdir,
-                    double2 *d_gxch, double2 *rr_out, DevPlan *override_plan) {
-  DevPlan &dp = override_plan ? *override_plan : e->plans[ilev - 1];
+// The launches of one Coulomb build on the engine's stream; with timing,
+// ev0 and ev2 bracket the quartet kernel(s).
+static int enqueue_fock(bdf_engine *e, DevPlan &dp, const double2 *d_dent, double2 *d_gdir,
+                        double2 *d_gxch, double2 *rr_out, bool mixed, bool timing,
+                        const SlabView &sv) {
   int nd = e->B.ndim;
-  const bool mixed = e->mixed && !rr_out && !override_plan;
   if (!rr_out)
     HIPCHK(launch_zero(d_gdir, d_gxch, (int64_t)nd * nd, e->counter.p, e->dmax.p, e->stream));
   else
@@ -382,8 +430,8 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.gdir = d_gdir;
   a.gxch = d_gxch;
   a.ndim = nd;
-  a.slab = e->slab.p;
-  a.slab_stride = e->slab_stride;
+  a.slab = sv.p;
+  a.slab_stride = sv.stride;
   a.ld = e->slab_ld;
   a.isym = e->isym;
   a.intsym = e->intsym;
@@ -402,11 +450,11 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
   a.xacc_n = plan_xacc(dp.P, e->isym);
-  const int nb = std::min(blocks_for(e, 4 * fock_launch_config(a.xacc_n, e->isym).waves),
-                          std::max(1, a.nitems));
-  HIPCHK(hipEventRecord(e->ev0, e->stream));
-  if (a.nitems > 0) HIPCHK(launch_fock(a, nb, e->stream, e->stream32, e->fork, e->join));
-  HIPCHK(hipEventRecord(e->ev2, e->stream));
+  const FockLaunch fl = dp.fl;
+  const int nb = std::min(sv.blocks, std::max(1, a.nitems));
+  if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
+  if (a.nitems > 0) HIPCHK(launch_fock(a, fl, nb, e->stream, e->stream32, e->fork, e->join));
+  if (timing) HIPCHK(hipEventRecord(e->ev2, e->stream));
   if (!rr_out) {
     CompleteArgs c;
     c.gdir = d_gdir;
@@ -419,8 +467,55 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
     c.intsym = e->intsym;
     HIPCHK(launch_complete(c, e->stream));
   }
+  return BDF_OK;
+}
+
+// Replays the captured build of (kind, ilev, buffers), capturing it first if
+// it is new; enqueue(timing) issues the build's launches.  ev0 and ev2/ev1
+// bracket the whole graph.
+template <class F>
+static int run_graph(bdf_engine *e, int kind, int ilev, const void *d, const void *g,
+                     const void *x, F enqueue) {
+  GraphEntry *hit = nullptr;
+  for (GraphEntry &ge : e->graphs)
+    if (ge.kind == kind && ge.ilev == ilev && ge.d == d && ge.g == g && ge.x == x) hit = &ge;
+  if (!hit) {
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    if (rc != BDF_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ce);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(ie);
+    e->graphs.push_back({kind, ilev, d, g, x, exec});
+    hit = &e->graphs.back();
+  }
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  HIPCHK(hipGraphLaunch(hit->exec, e->stream));
+  HIPCHK(hipEventRecord(e->ev2, e->stream));
   HIPCHK(hipEventRecord(e->ev1, e->stream));
+  return BDF_OK;
+}
+
+static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_gdir,
+                    double2 *d_gxch, double2 *rr_out, DevPlan *override_plan) {
+  DevPlan &dp = override_plan ? *override_plan : e->plans[ilev - 1];
+  const bool mixed = e->mixed && !rr_out && !override_plan;
   e->last_quartets = dp.P.quartets;
+  const SlabView sv{e->slab.p, e->slab_stride, dp.blocks};
+  if (e->use_graph && !rr_out && !override_plan)
+    return run_graph(e, 0, ilev, d_dent, d_gdir, d_gxch, [&](bool timing) {
+      return enqueue_fock(e, dp, d_dent, d_gdir, d_gxch, nullptr, mixed, timing, sv);
+    });
+  int rc = enqueue_fock(e, dp, d_dent, d_gdir, d_gxch, rr_out, mixed, true, sv);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
   return BDF_OK;
 }
 
@@ -445,15 +540,15 @@ static int make_breit_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)std::max(dp.P.nflag, 1)));
-  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1), blocks_for(e, 8));
+  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1), breit_blocks(e));
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
   return BDF_OK;
 }
 
-static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, double2 *d_bdir,
-                     double2 *d_bxch, double2 *rr_out) {
+static int enqueue_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent,
+                         double2 *d_bdir, double2 *d_bxch, double2 *rr_out, bool timing) {
   int nd = e->B.ndim;
   if (!rr_out)
     HIPCHK(launch_zero(d_bdir, d_bxch, (int64_t)nd * nd, e->counter.p, nullptr, e->stream));
@@ -485,11 +580,22 @@ static int run_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent, dou
   a.Ecd = e->Ecd.p;
   a.Eab = e->Eab.p;
   a.rr_out = rr_out;
-  HIPCHK(hipEventRecord(e->ev0, e->stream));
-  HIPCHK(launch_breit(a, blocks_for(e, 8), e->stream));
-  HIPCHK(hipEventRecord(e->ev2, e->stream));
-  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
+  HIPCHK(launch_breit(a, breit_blocks(e), e->stream));
+  if (timing) HIPCHK(hipEventRecord(e->ev2, e->stream));
+  return BDF_OK;
+}
+
+static int run_breit(bdf_engine *e, DevBreitPlan &dp, int ilev, const double2 *d_dent,
+                     double2 *d_bdir, double2 *d_bxch, double2 *rr_out) {
   e->last_quartets = dp.P.quartets;
+  if (e->use_graph && !rr_out)
+    return run_graph(e, 1, ilev, d_dent, d_bdir, d_bxch, [&](bool timing) {
+      return enqueue_breit(e, dp, d_dent, d_bdir, d_bxch, nullptr, timing);
+    });
+  int rc = enqueue_breit(e, dp, d_dent, d_bdir, d_bxch, rr_out, true);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
   return BDF_OK;
 }
 
@@ -741,7 +847,11 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   HIPCHK(dp.cds.upload(dp.P.cds, e->stream));
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
-  int src = ensure_slab(e, mcd, 4 * ntuv(kMaxLab), 1);
+  dp.P.ilev = ilev;
+  dp.xacc_n = plan_xacc(dp.P, e->isym);
+  dp.fl = fock_config(e, dp);
+  dp.blocks = 1;
+  int src = ensure_slab(e, mcd, 4 * dp.fl.cap * ntuv(kMaxLab), 1);
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
@@ -818,7 +928,7 @@ int bdf_breit_device(bdf_engine *e, int ilev, const void *d_dent, void *d_bdir, 
   HIPCHK(hipSetDevice(e->device));
   int rc = make_breit_plan(e, ilev);
   if (rc != BDF_OK) return rc;
-  return run_breit(e, e->bplans[ilev - 1], (const double2 *)d_dent, (double2 *)d_bdir,
+  return run_breit(e, e->bplans[ilev - 1], ilev, (const double2 *)d_dent, (double2 *)d_bdir,
                    (double2 *)d_bxch, nullptr);
 }
 
@@ -947,7 +1057,7 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
-  int rc = run_breit(e, dp, nullptr, nullptr, nullptr, out.p);
+  int rc = run_breit(e, dp, 3, nullptr, nullptr, nullptr, out.p);
   if (rc != BDF_OK) return rc;
   HIPCHK(hipMemcpyAsync(rr, out.p, sizeof(double2) * 16 * mcd, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -1134,11 +1244,40 @@ int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgrou
   int rc = make_plan(e, ilev);
   if (rc != BDF_OK) return rc;
   const DevPlan &dp = e->plans[ilev - 1];
-  const FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
-  if (waves) *waves = fl.waves;
-  if (cap) *cap = fl.cap;
+  if (waves) *waves = dp.fl.waves;
+  if (cap) *cap = dp.fl.cap;
   if (workgroups) *workgroups = dp.blocks;
   if (scratch_bytes) *scratch_bytes = e->slab.n * sizeof(double);
+  return BDF_OK;
+}
+
+int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
+  if (!e || !l) return fail(BDF_EINVAL, "bdf_set_launch: null argument");
+  if (l->waves != 0 && l->waves != 2 && l->waves != 3)
+    return fail(BDF_EINVAL, "bdf_set_launch: waves must be 0, 2 or 3");
+  if (l->cap < 0 || l->cap > 2) return fail(BDF_EINVAL, "bdf_set_launch: cap must be 0, 1 or 2");
+  if (l->waves == 3 && l->cap == 2)
+    return fail(BDF_EINVAL, "bdf_set_launch: three waves run at capacity 1 only");
+  if (l->per_cu < 0 || l->per_cu > 32 || l->breit_per_cu < 0 || l->breit_per_cu > 32)
+    return fail(BDF_EINVAL, "bdf_set_launch: workgroups per CU outside 0..32");
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
+  e->use_graph = l->graph ? 1 : 0;
+  e->tune_waves = l->waves;
+  e->tune_cap = l->cap;
+  e->tune_per_cu = l->per_cu;
+  e->tune_breit_per_cu = l->breit_per_cu;
+  for (DevPlan &dp : e->plans)
+    if (dp.built) {
+      int rc = size_launch(e, dp);
+      if (rc != BDF_OK) return rc;
+    }
+  for (DevBreitPlan &bp : e->bplans)
+    if (bp.built) {
+      int rc = ensure_slab(e, bp.P.max_maxcd, 12 * std::max(bp.P.max_ntg, 1), breit_blocks(e));
+      if (rc != BDF_OK) return rc;
+    }
   return BDF_OK;
 }
 
@@ -1146,6 +1285,7 @@ void bdf_destroy(bdf_engine *e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
   if (e->ev2) (void)hipEventDestroy(e->ev2);

@@ -146,8 +146,10 @@ struct FockLaunch {
   int waves, cap;
 };
 FockLaunch fock_launch_config(int xacc_n, int isym);
-hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
-                       hipEvent_t fork, hipEvent_t join);
+// the instantiated shapes: {3, 1}, {2, 2}, {2, 1}
+bool fock_launch_valid(FockLaunch c);
+hipError_t launch_fock(const FockArgs &a, FockLaunch c, int nblocks, hipStream_t s,
+                       hipStream_t s32, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
 hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s);

@@ -16,6 +16,7 @@
 
 namespace bdf {
 
+
 __constant__ uint32_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits (32-bit words: scalar loads)
 // R row of a sum of Hermite indices: c_hsum[tp * kHsRow + t] = hidx(tuv(t) + tuv(tp))
 // for t, tp < N(kMaxLab); rows padded to a multiple of 4 for 16-byte scalar loads
@@ -1800,9 +1801,13 @@ static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, h
   return e;
 }
 
-hipError_t launch_fock(const FockArgs &a, int nblocks, hipStream_t s, hipStream_t s32,
-                       hipEvent_t fork, hipEvent_t join) {
-  const FockLaunch c = fock_launch_config(a.xacc_n, a.isym);
+bool fock_launch_valid(FockLaunch c) {
+  return (c.waves == 3 && c.cap == 1) || (c.waves == 2 && (c.cap == 1 || c.cap == 2));
+}
+
+hipError_t launch_fock(const FockArgs &a, FockLaunch c, int nblocks, hipStream_t s,
+                       hipStream_t s32, hipEvent_t fork, hipEvent_t join) {
+  if (!fock_launch_valid(c)) return hipErrorInvalidValue;
   if (c.waves == 3) return launch_fock_w<3, 1>(a, nblocks, s, s32, fork, join);
   if (c.cap == 2) return launch_fock_w<2, 2>(a, nblocks, s, s32, fork, join);
   return launch_fock_w<2, 1>(a, nblocks, s, s32, fork, join);

@@ -217,6 +217,27 @@ int bdf_last_stats(bdf_engine *e, double *quartets, double *kernel_ms);
    and work items.  No counterpart in the reference. */
 int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long *items);
 
+/* Launch settings of an engine (no counterpart in the reference).  graph:
+   1 = each Coulomb or Breit build is captured once per (stage, device
+   buffers) as a HIP graph and replayed on later calls with the same
+   buffers (bdf_coulomb and bdf_breit reuse one set of buffers); the
+   captured launches are the uncaptured ones, so results agree to the
+   rounding of the kernels' atomic accumulation.  waves (2 or 3 waves per
+   SIMD), cap (group capacity 1 or 2; three waves run at capacity 1) and
+   per_cu (persistent workgroups per compute unit) set the Coulomb kernel's
+   launch shape, breit_per_cu the Breit kernel's workgroups per CU; 0 keeps
+   the engine's choice.  Takes effect from the next build; captured builds
+   are dropped. */
+typedef struct {
+  int graph;
+  int waves;
+  int cap;
+  int per_cu;
+  int breit_per_cu;
+} bdf_launch;
+
+int bdf_set_launch(bdf_engine *e, const bdf_launch *l);
+
 /* How the stage-ilev Coulomb build is launched: waves per SIMD of its
    quartet kernel (2 or 3), the capacity multiple of its AB-combination
    groups (1 or 2), persistent workgroups, and the engine's scratch in

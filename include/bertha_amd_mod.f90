@@ -26,6 +26,10 @@ module bertha_amd
     integer(c_int) :: gaunt, workgroups
   end type bdf_options
 
+  type, bind(c) :: bdf_launch
+    integer(c_int) :: graph, waves, cap, per_cu, breit_per_cu
+  end type bdf_launch
+
   interface
     integer(c_int) function bdf_create(sys, opt, eng, ndim, nskp) bind(c, name='bdf_create')
       import :: c_int, c_ptr, bdf_system, bdf_options
@@ -161,6 +165,12 @@ module bertha_amd
       import :: c_ptr
       type(c_ptr), value :: eng
     end subroutine
+
+    integer(c_int) function bdf_set_launch(eng, l) bind(c, name='bdf_set_launch')
+      import :: c_int, c_ptr, bdf_launch
+      type(c_ptr), value :: eng
+      type(bdf_launch), intent(in) :: l
+    end function
 
     integer(c_int) function bdf_launch_info(eng, ilev, waves, cap, workgroups, scratch_bytes) &
         bind(c, name='bdf_launch_info')

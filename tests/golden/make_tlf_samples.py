@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Writes tests/golden/tlf_samples.npz: the oracle's Dirac-Coulomb builds of
 the heavy-diatomic workload (Tl 28s24p18d12f + F 14s9p4d on the z axis,
-NDIM 1340, Gaussian nuclei) at stages 1 and 3, at 2000 sampled matrix
-elements, with E_J and E_K of each build.
+NDIM 1340, Gaussian nuclei) at stages 1 and 3, and its stage-3 Breit build,
+at 2000 sampled matrix elements, with E_J and E_K of each Coulomb build and
+E_B of the Breit build.
 
 The reference build holds two centres and 26 exponents per shell
 (parameters.h), so Tl's 28 s functions are beyond it and nothing of the
@@ -20,7 +21,8 @@ Density: Tl and F solved alone in their basis with the bare-nucleus
 Hamiltonian of their Gaussian nuclei, Z electrons in the lowest
 positive-energy Kramers pairs (Tl's odd electron shared by the last pair),
 superposed on the molecule.  The occupied spinors and occupations are
-stored, not the matrices.  About ten minutes of 8 cores.
+stored, not the matrices.  About half an hour of 8 cores (the Breit build
+takes two thirds of it).
 Run:  python tests/golden/make_tlf_samples.py
 """
 import os
@@ -85,7 +87,9 @@ def main():
     threads = int(os.environ.get("ORACLE_THREADS", min(16, os.cpu_count() or 1)))
     for ilev in (1, 3):
         t = time.time()
-        gd, gx = orc.clmfast(dens, ilev, nthreads=threads)
+        # the whole build (stride 1), classes as work items
+        frac, gd, gx = orc.clmfast_sample(dens, ilev, nthreads=threads, stride=1)
+        assert frac == 1.0
         print(f"stage {ilev}: oracle build {time.time() - t:.0f} s", flush=True)
         out[f"s{ilev}_ej"] = 0.5 * np.real(np.sum(dens * gd))
         out[f"s{ilev}_ek"] = 0.5 * np.real(np.sum(dens * gx))
@@ -105,6 +109,15 @@ def main():
         print(f"stage {ilev}: max|GXCH - GXCH^H| {asym.max():.3e}, max|GXCH| "
               f"{np.abs(gx).max():.3e}, max|GDIR - GDIR^H| "
               f"{np.abs(gd - gd.conj().T).max():.3e}", flush=True)
+    # the Breit build (BRTFAST) of the same density on a DHFB system
+    sb = pkg.systems.config_system("tlf", seed=0, hmlt="DHFB")
+    orb = pyoracle.Oracle(sb)
+    t = time.time()
+    frac, bd, bx = orb.brtfast_sample(dens, 3, nthreads=threads, stride=1)
+    assert frac == 1.0
+    print(f"breit: oracle build {time.time() - t:.0f} s", flush=True)
+    out["b3_eb"] = 0.5 * np.real(np.sum(dens * (bd - bx)))
+    out["b3_x"] = bx[idx[:, 0], idx[:, 1]]
     np.savez_compressed(os.path.join(HERE, "tlf_samples.npz"), **out)
 
 

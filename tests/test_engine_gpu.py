@@ -310,3 +310,51 @@ def test_ring_workload_matches_oracle_samples(bdf):
             assert np.abs(x[i, j] - gold[ref + "_x"]).max() < tol, label
         assert abs(0.5 * np.real(np.sum(d * g)) - gold[ref + "_ej"]) < 1e-8, label
         assert abs(0.5 * np.real(np.sum(d * x)) - gold[ref + "_ek"]) < 1e-8, label
+
+
+@pytest.mark.parametrize("name,kw", [("tri", {}), ("fdiat", {}), ("off", {"precision": "mixed"})])
+def test_graph_builds_equal_stream_builds(bdf, systems, name, kw):
+    """Builds replayed from a captured HIP graph (bdf_set_launch graph = 1)
+    equal the stream builds: the first call captures, later calls replay;
+    replays are interleaved with another stage's graph, host work and a
+    synchronisation, and every replay is checked."""
+    s = systems[name]
+    d = bdf.systems.random_density(s.ndim, seed=31, real=(s.shape != "NOSYMM"))
+    with bdf.Engine(s, **kw) as eng:
+        g0, x0 = eng.clmfast(d, 3)
+        g1, x1 = eng.clmfast(d, 1)
+        eng.set_launch(graph=True)
+        print(f"[graph] {name} {kw}", flush=True)
+        for rep in range(3):
+            g, x = eng.clmfast(d, 3)
+            for a, b in ((g, g0), (x, x0)):
+                assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max()), rep
+            gs, xs = eng.clmfast(d, 1)
+            assert np.abs(gs - g1).max() <= 1e-12 * max(1.0, np.abs(g1).max())
+            _ = np.linalg.norm(g)  # host work between replays
+        eng.set_launch(graph=False)
+        g, x = eng.clmfast(d, 3)
+        assert np.abs(g - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
+
+
+@pytest.mark.parametrize("shape", [(3, 1, 12), (2, 1, 8), (2, 2, 8), (2, 1, 4)])
+def test_forced_launch_shapes_match_oracle(bdf, orc, systems, shape):
+    """Every launch shape the autotuner may choose gives the oracle's build,
+    on the complex path (three-centre) and the real path (f-shell diatomic)."""
+    waves, cap, per_cu = shape
+    for name in ("tri", "fdiat"):
+        s = systems[name]
+        d = bdf.systems.random_density(s.ndim, seed=5, real=(s.shape != "NOSYMM"))
+        with bdf.Engine(s, tuned=False) as eng:
+            eng.set_launch(waves=waves, cap=cap, per_cu=per_cu)
+            assert eng.launch_info(3)[:2] == (waves, cap)
+            g, x = eng.clmfast(d, 3)
+        go, xo = orc.Oracle(s).clmfast(d, 3)
+        assert np.abs(g - go).max() < FOCK_ABS and np.abs(x - xo).max() < FOCK_ABS, (name, shape)
+
+
+def test_set_launch_rejects_bad_shapes(bdf, systems):
+    with bdf.Engine(systems["ne"]) as eng:
+        for bad in ({"waves": 4}, {"cap": 3}, {"waves": 3, "cap": 2}, {"per_cu": 40}):
+            with pytest.raises(bdf.BerthaError):
+                eng.set_launch(**bad)

@@ -132,20 +132,25 @@ def test_tlf_workload_matches_oracle_samples(bdf, tlf_gold, tlf_density):
     assert abs(_e2(d, gm, xm) - _e2(d, g, x)) < 1e-6
 
 
-def test_tlf_breit_properties(bdf, tlf_density):
+def test_tlf_breit_matches_oracle_samples(bdf, tlf_gold, tlf_density):
     s = bdf.systems.config_system("tlf", seed=0, hmlt="DHFB")
     d = tlf_density
     n, ns = s.ndim, s.ndim // 2
+    i, j = tlf_gold["idx"][:, 0], tlf_gold["idx"][:, 1]
     with bdf.Engine(s) as eng:
         print("[tlf] breit", flush=True)
         b, x = eng.breit(d, 3)
         b2, x2 = eng.breit(d, 3)
     assert np.abs(x2 - x).max() <= 1e-12 * np.abs(x).max()
+    assert np.abs(x[i, j] - tlf_gold["b3_x"]).max() < FOCK_ABS
+    eb = 0.5 * float(np.real(np.sum(d * (b - x))))
+    assert abs(eb - tlf_gold["b3_eb"]) < 1e-8
     assert np.abs(b).max() == 0.0  # closed shell: BDIR untouched
-    # LS holds the build, SL its conjugate transpose (BRTFAST :17646-17664);
-    # LL and SS stay zero
-    assert np.abs(x[:ns, :ns]).max() == 0.0 and np.abs(x[ns:, ns:]).max() == 0.0
+    # SL is the conjugate transpose of LS (BRTFAST :17646-17664); the
+    # exchange batches also reach the LL (A, C large) and SS (B, D small)
+    # blocks
     assert np.abs(x[ns:, :ns] - x[:ns, ns:].conj().T).max() == 0.0
+    assert np.abs(x[:ns, :ns]).max() > 0.0 and np.abs(x[ns:, ns:]).max() > 0.0
     assert np.abs(x[:ns, ns:]).max() > 1e-6
     eb = 0.5 * float(np.real(np.sum(d * x)))
     assert np.isfinite(eb)
