@@ -104,6 +104,7 @@ def library() -> _C.CDLL:
                                    _C.POINTER(P), _C.POINTER(_C.c_int), _C.POINTER(_C.c_int)]
         lib.bdf_coulomb.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
         lib.bdf_coulomb_device.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_coulomb_device_static.argtypes = [P, _C.c_int, P, P, P]
         lib.bdf_eri_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int, P, _C.c_int]
         lib.bdf_etable.argtypes = [P, _C.c_int, _C.c_int, _C.c_int, _C.c_int, _C.c_int,
                                    _C.c_int, _C.c_int, P, _C.c_long, _C.POINTER(_C.c_long)]
@@ -226,6 +227,11 @@ class Engine:
     def clmfast_device(self, d_dent: int, d_gdir: int, d_gxch: int, ilev: int = 3) -> None:
         """Enqueue a build on device pointers (e.g. torch tensors' data_ptr())."""
         _check(library().bdf_coulomb_device(self._h, ilev, d_dent, d_gdir, d_gxch))
+
+    def clmfast_device_static(self, d_dent: int, d_gdir: int, d_gxch: int, ilev: int = 3) -> None:
+        """bdf_coulomb_device_static: the same build without allocating (the
+        stage's plan must exist: plan_info(ilev) builds it)."""
+        _check(library().bdf_coulomb_device_static(self._h, ilev, d_dent, d_gdir, d_gxch))
 
     def erifast(self, icnt, kidx, midx, ibas: int, jbas: int, itn, ilev: int = 3):
         """RR(N,1..16) of one ERIFAST batch, shape (16, maxcd) complex."""

@@ -138,6 +138,8 @@ struct bdf_engine {
   int tune_waves = 0, tune_cap = 0, tune_per_cu = 0, tune_breit_per_cu = 0;
   int use_graph = 0;
   std::vector<GraphEntry> graphs;
+  // this device's static scratch (bdf_coulomb_device_static)
+  double *static_scratch = nullptr;
 };
 
 // where one build's per-workgroup scratch lives: base, doubles per
@@ -637,6 +639,8 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
   }
   int rc = [&]() -> int {
     HIPCHK(hipSetDevice(e->device));
+    // set-up of this device's static scratch (once per device)
+    HIPCHK(static_scratch(e->device, &e->static_scratch));
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
@@ -730,6 +734,39 @@ int bdf_coulomb_device(bdf_engine *e, int ilev, const void *d_dent, void *d_gdir
   if (rc != BDF_OK) return rc;
   return run_fock(e, ilev, (const double2 *)d_dent, (double2 *)d_gdir, (double2 *)d_gxch, nullptr,
                   nullptr);
+}
+
+int bdf_coulomb_device_static(bdf_engine *e, int ilev, const void *d_dent, void *d_gdir,
+                              void *d_gxch) {
+  if (!e || !d_dent || !d_gdir || !d_gxch)
+    return fail(BDF_EINVAL, "bdf_coulomb_device_static: null argument");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_coulomb_device_static: ilev outside 1..3");
+  DevPlan &dp = e->plans[ilev - 1];
+  if (!dp.built)
+    return fail(BDF_EINVAL, "bdf_coulomb_device_static: the stage's plan is not built "
+                            "(bdf_plan_info or bdf_coulomb_device builds it)");
+  if (!e->static_scratch) return fail(BDF_EDEVICE, "bdf_coulomb_device_static: no scratch");
+  HIPCHK(hipSetDevice(e->device));
+  // this plan's own per-workgroup rows at its kl row length; the mixed
+  // build's FP32 instance takes a second set of workgroups
+  const int64_t ld = ((std::max(dp.P.max_maxcd, 1) + 63) / 64) * 64;
+  const int64_t rows = ntuv(kMaxLfl) + kMaxLfl + 1 + 4 * dp.fl.cap * ntuv(kMaxLab);
+  const int64_t stride = rows * ld;
+  const int64_t sets = e->mixed ? 2 : 1;
+  const int64_t fit = (int64_t)kStaticScratchDoubles / (stride * sets);
+  if (fit < 1)
+    return fail(BDF_ELIMIT, "bdf_coulomb_device_static: one workgroup's scratch (" +
+                                std::to_string(stride * sets * 8) +
+                                " bytes) exceeds the static scratch (" +
+                                std::to_string(kStaticScratchDoubles * 8) + " bytes)");
+  const SlabView sv{e->static_scratch, stride, (int)std::min<int64_t>(dp.blocks, fit)};
+  e->last_quartets = dp.P.quartets;
+  const bool mixed = e->mixed != 0;
+  int rc = enqueue_fock(e, dp, (const double2 *)d_dent, (double2 *)d_gdir, (double2 *)d_gxch,
+                        nullptr, mixed, true, sv);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  return BDF_OK;
 }
 
 int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent, const double _Complex *deno,

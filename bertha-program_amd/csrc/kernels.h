@@ -123,6 +123,13 @@ struct CompleteArgs {
   int ndim, nskp, intsym;
 };
 
+// the allocation-free build's scratch: 256 MiB per device, in kernels.hip
+constexpr size_t kStaticScratchDoubles = (size_t(256) << 20) / sizeof(double);
+constexpr int kMaxDevices = 64;
+// device address of that scratch on `device`, which must be current; looked
+// up once per device (hipGetSymbolAddress) and kept
+hipError_t static_scratch(int device, double **ptr);
+
 hipError_t upload_constants();
 hipError_t launch_etab(const EtabArgs &a, hipStream_t s);
 hipError_t launch_eflag(const EflagArgs &a, hipStream_t s);

@@ -11,11 +11,34 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.h"
 
 namespace bdf {
 
+// Scratch of the allocation-free Coulomb build (bdf_coulomb_device_static):
+// the per-workgroup R, Boys and G rows live here instead of in the engine's
+// own slab.  Each GPU the library runs on holds one copy for the life of the
+// process; every such build on that GPU shares it.
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices];
+
+hipError_t static_scratch(int device, double **ptr) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    // the array's device address on the current device (the caller has made
+    // `device` current); host code never passes the symbol itself
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_dev[device];
+  return hipSuccess;
+}
 
 __constant__ uint32_t c_tuv[ntuv(kMaxLfl)];  // (t,u,v) packed 5:5:5 bits (32-bit words: scalar loads)
 // R row of a sum of Hermite indices: c_hsum[tp * kHsRow + t] = hidx(tuv(t) + tuv(tp))

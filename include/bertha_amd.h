@@ -125,6 +125,23 @@ int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent,
 int bdf_coulomb_device(bdf_engine *e, int ilev, const void *d_dent,
                        void *d_gdir, void *d_gxch);
 
+/* bdf_coulomb_device without allocating: no hipMalloc, hipFree or other
+   allocating call.  No counterpart in the reference; it is for callers that
+   cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid it).  The quartet kernel's
+   per-workgroup scratch comes from a static 256 MiB array per GPU instead
+   of the engine's own, which bounds its workgroups (fewer than
+   bdf_coulomb_device uses for systems with long primitive-pair lists, so it
+   is slower there); a plan whose single workgroup needs more than 256 MiB
+   (twice its rows in a mixed-precision engine) is refused with BDF_ELIMIT.
+   The stage's plan must be built beforehand (bdf_plan_info or a
+   bdf_coulomb_device call of the same stage).  Every call on one GPU
+   shares that GPU's array: calls that overlap on one GPU (engines on
+   different host threads or streams) race.  Enqueued on the engine's
+   stream, not synchronised, never captured as a graph. */
+int bdf_coulomb_device_static(bdf_engine *e, int ilev, const void *d_dent,
+                              void *d_gdir, void *d_gxch);
+
 /* Test hook mirroring ERIFAST's output (bertha-scf.f:12269-13132) for one
    block quartet: centres icnt[4] (1-based), kappa indices kidx[4] (1-based
    position in KAPA, as KA/KB/KC/KD in CLMFAST :11771-11856), |mj| indices

@@ -56,6 +56,13 @@ module bertha_amd
       integer(c_int), value :: ilev
     end function
 
+    integer(c_int) function bdf_coulomb_device_static(eng, ilev, d_dent, d_gdir, d_gxch) &
+        bind(c, name='bdf_coulomb_device_static')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, d_dent, d_gdir, d_gxch
+      integer(c_int), value :: ilev
+    end function
+
     integer(c_int) function bdf_eri_batch(eng, icnt, kidx, midx, ibas, jbas, itn, ilev, rr, maxcd) &
         bind(c, name='bdf_eri_batch')
       import :: c_int, c_ptr, c_double_complex
