@@ -108,9 +108,11 @@ struct bdf_engine {
   DevBuf<double> slab;
   int nblocks = 0;        // the caller's persistent workgroups (user_blocks)
   int cus = 0, user_blocks = 0;
-  int slab_blocks = 0;    // workgroups the scratch holds
-  int64_t slab_stride = 0;
-  int slab_ld = 0, slab_rows = 0;
+  // scratch rows per workgroup and workgroups of the Coulomb (0) and Breit
+  // (1) kernels, at one row length slab_ld; the Coulomb kernel of a mixed
+  // engine takes a second set for its FP32 instance
+  int slab_ld = 0;
+  int slab_rows[2] = {0, 0}, slab_blocks[2] = {0, 0};
   DevBuf<double2> dent, gdir, gxch;
   // Schwarz tables (GDSC of CLMDIAG and the (ab|ba) bound), built on demand
   int schwarz = 0, have_tables = 0;
@@ -136,6 +138,7 @@ struct bdf_engine {
   double last_ms = 0.0, last_quartets = 0.0;
   // launch settings (bdf_set_launch): 0 = the engine's choice
   int tune_waves = 0, tune_cap = 0, tune_per_cu = 0, tune_breit_per_cu = 0;
+  int tune_breit_group = 0;
   int use_graph = 0;
   std::vector<GraphEntry> graphs;
   // this device's static scratch (bdf_coulomb_device_static)
@@ -175,20 +178,31 @@ static int plan_xacc(const Plan &P, int isym) {
   return n + (n & 1);  // even: 16-byte alignment of what follows
 }
 
-// per-workgroup scratch for `blocks` workgroups: R rows, Boys rows and g_rows
-// G rows, each ld doubles (ld = largest kl count rounded to the wavefront)
-static int ensure_slab(bdf_engine *e, int maxcd, int g_rows, int blocks) {
+// doubles per workgroup of kernel kind (0 Coulomb, 1 Breit)
+static int64_t slab_stride(const bdf_engine *e, int kind) {
+  return (int64_t)e->slab_rows[kind] * e->slab_ld;
+}
+
+// per-workgroup scratch of kernel kind for `blocks` workgroups: R rows, Boys
+// rows and g_rows G rows, each ld doubles (ld = largest kl count rounded to
+// the wavefront); the allocation serves the larger of the two kinds
+static int ensure_slab(bdf_engine *e, int kind, int maxcd, int g_rows, int blocks) {
   int ld = ((std::max(maxcd, 1) + 63) / 64) * 64;
   int rows = ntuv(kMaxLfl) + kMaxLfl + 1 + g_rows;
-  if (ld > e->slab_ld || rows > e->slab_rows || blocks > e->slab_blocks || !e->slab.p) {
+  if (ld > e->slab_ld || rows > e->slab_rows[kind] || blocks > e->slab_blocks[kind] ||
+      !e->slab.p) {
     e->slab_ld = std::max(ld, e->slab_ld);
-    e->slab_rows = std::max(rows, e->slab_rows);
-    e->slab_blocks = std::max(blocks, e->slab_blocks);
-    e->slab_stride = (int64_t)e->slab_rows * e->slab_ld;
-    // captured builds hold the old scratch address
-    drop_graphs(e);
+    e->slab_rows[kind] = std::max(rows, e->slab_rows[kind]);
+    e->slab_blocks[kind] = std::max(blocks, e->slab_blocks[kind]);
     // the mixed build's FP32 instance runs concurrently on a second set of blocks
-    HIPCHK(e->slab.alloc((size_t)e->slab_stride * e->slab_blocks * (e->mixed ? 2 : 1)));
+    const size_t need = std::max<size_t>(
+        (size_t)slab_stride(e, 0) * e->slab_blocks[0] * (e->mixed ? 2 : 1),
+        (size_t)slab_stride(e, 1) * e->slab_blocks[1]);
+    if (need > e->slab.n || !e->slab.p) {
+      // captured builds hold the old scratch address
+      drop_graphs(e);
+      HIPCHK(e->slab.alloc(need));
+    }
   }
   return BDF_OK;
 }
@@ -207,11 +221,16 @@ static FockLaunch fock_config(const bdf_engine *e, const DevPlan &dp) {
 static int size_launch(bdf_engine *e, DevPlan &dp) {
   dp.fl = fock_config(e, dp);
   dp.blocks = blocks_for(e, e->tune_per_cu ? e->tune_per_cu : 4 * dp.fl.waves);
-  return ensure_slab(e, dp.P.max_maxcd, 4 * dp.fl.cap * ntuv(kMaxLab), dp.blocks);
+  return ensure_slab(e, 0, dp.P.max_maxcd, 4 * dp.fl.cap * ntuv(kMaxLab), dp.blocks);
 }
 
 static int breit_blocks(const bdf_engine *e) {
   return blocks_for(e, e->tune_breit_per_cu ? e->tune_breit_per_cu : 8);
+}
+
+// AB combinations the Breit kernel contracts together
+static int breit_group(const bdf_engine *e) {
+  return e->tune_breit_group ? e->tune_breit_group : 1;
 }
 
 static int make_plan(bdf_engine *e, int ilev) {
@@ -510,7 +529,7 @@ static int run_fock(bdf_engine *e, int ilev, const double2 *d_dent, double2 *d_g
   DevPlan &dp = override_plan ? *override_plan : e->plans[ilev - 1];
   const bool mixed = e->mixed && !rr_out && !override_plan;
   e->last_quartets = dp.P.quartets;
-  const SlabView sv{e->slab.p, e->slab_stride, dp.blocks};
+  const SlabView sv{e->slab.p, slab_stride(e, 0), dp.blocks};
   if (e->use_graph && !rr_out && !override_plan)
     return run_graph(e, 0, ilev, d_dent, d_gdir, d_gxch, [&](bool timing) {
       return enqueue_fock(e, dp, d_dent, d_gdir, d_gxch, nullptr, mixed, timing, sv);
@@ -542,7 +561,8 @@ static int make_breit_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)std::max(dp.P.nflag, 1)));
-  int rc = ensure_slab(e, dp.P.max_maxcd, 12 * std::max(dp.P.max_ntg, 1), breit_blocks(e));
+  int rc = ensure_slab(e, 1, dp.P.max_maxcd, 12 * breit_group(e) * std::max(dp.P.max_ntg, 1),
+                       breit_blocks(e));
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -573,8 +593,9 @@ static int enqueue_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent,
   a.ndim = nd;
   a.nskp = e->B.nskp;
   a.slab = e->slab.p;
-  a.slab_stride = e->slab_stride;
+  a.slab_stride = slab_stride(e, 1);
   a.ld = e->slab_ld;
+  a.group = rr_out ? 1 : breit_group(e);
   a.isym = e->isym;
   a.intsym = e->intsym;
   a.gaunt = e->gaunt;
@@ -888,7 +909,7 @@ int bdf_eri_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   dp.xacc_n = plan_xacc(dp.P, e->isym);
   dp.fl = fock_config(e, dp);
   dp.blocks = 1;
-  int src = ensure_slab(e, mcd, 4 * dp.fl.cap * ntuv(kMaxLab), 1);
+  int src = ensure_slab(e, 0, mcd, 4 * dp.fl.cap * ntuv(kMaxLab), 1);
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
@@ -1090,7 +1111,7 @@ int bdf_bii_batch(bdf_engine *e, const int icnt[4], const int kidx[4], const int
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)dp.P.nflag));
-  int src = ensure_slab(e, mcd, 12 * ntcd, 1);
+  int src = ensure_slab(e, 1, mcd, 12 * ntcd, 1);
   if (src != BDF_OK) return src;
   DevBuf<double2> out;
   HIPCHK(out.alloc((size_t)16 * mcd));
@@ -1297,6 +1318,8 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
     return fail(BDF_EINVAL, "bdf_set_launch: three waves run at capacity 1 only");
   if (l->per_cu < 0 || l->per_cu > 32 || l->breit_per_cu < 0 || l->breit_per_cu > 32)
     return fail(BDF_EINVAL, "bdf_set_launch: workgroups per CU outside 0..32");
+  if (l->breit_group != 0 && l->breit_group != 1 && l->breit_group != 2 && l->breit_group != 4)
+    return fail(BDF_EINVAL, "bdf_set_launch: breit_group must be 0, 1, 2 or 4");
   HIPCHK(hipSetDevice(e->device));
   HIPCHK(hipStreamSynchronize(e->stream));
   drop_graphs(e);
@@ -1305,6 +1328,7 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
   e->tune_cap = l->cap;
   e->tune_per_cu = l->per_cu;
   e->tune_breit_per_cu = l->breit_per_cu;
+  e->tune_breit_group = l->breit_group;
   for (DevPlan &dp : e->plans)
     if (dp.built) {
       int rc = size_launch(e, dp);
@@ -1312,7 +1336,8 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
     }
   for (DevBreitPlan &bp : e->bplans)
     if (bp.built) {
-      int rc = ensure_slab(e, bp.P.max_maxcd, 12 * std::max(bp.P.max_ntg, 1), breit_blocks(e));
+      int rc = ensure_slab(e, 1, bp.P.max_maxcd,
+                           12 * breit_group(e) * std::max(bp.P.max_ntg, 1), breit_blocks(e));
       if (rc != BDF_OK) return rc;
     }
   return BDF_OK;

@@ -99,6 +99,7 @@ struct BreitArgs {
   double2 *rr_out;  // test hook: RR[16][maxcd] of the first call instead of digesting
   const double4 *Ecd;  // E(CD) with column masks and ERIFAST's Z sign applied
   const double4 *Eab;  // E with EPSAB applied (launch_eps_filter), the AB side
+  int group;           // AB combinations contracted together: 1, 2 or 4
 };
 
 struct OneArgs {
@@ -159,6 +160,9 @@ hipError_t launch_fock(const FockArgs &a, FockLaunch c, int nblocks, hipStream_t
                        hipStream_t s32, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_complete(const CompleteArgs &a, hipStream_t s);
 hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
+// AB combinations of one class that breit_kernel contracts together
+// (BreitArgs::group: 1, 2 or 4; its scratch holds 12 N(L_CD) G rows for each)
+constexpr int kMaxBreitGroup = 4;
 hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_oneel(const OneArgs &a, hipStream_t s);
 

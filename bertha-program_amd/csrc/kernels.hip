@@ -1157,12 +1157,16 @@ __global__ void __launch_bounds__(64) breit_flags_kernel(BreitArgs a) {
   }
 }
 
+// LDS of a Breit workgroup contracting GB AB combinations together; at
+// GB = 1 eight workgroups fit a CU's 160 KiB
+template <int GB>
 struct BreitShared {
   uint8_t cflag[ntuv(kMaxLfl)];
   uint8_t gflag[ntuv(kMaxLfl)];
-  uint8_t gm[ntuv(2 * kMaxL + 1) * 12];
+  uint8_t gm[GB][ntuv(2 * kMaxL + 1) * 12];
   double xacc[kXAccMax];  // exchange accumulators (XView)
 };
+static_assert(8 * sizeof(BreitShared<1>) <= 160 * 1024, "Breit LDS limits occupancy");
 
 __device__ __forceinline__ int any4(const double *e) {
   return e[0] != 0.0 || e[1] != 0.0 || e[2] != 0.0 || e[3] != 0.0;
@@ -1173,8 +1177,9 @@ __device__ __forceinline__ int any4(const double *e) {
 // gauge term T = R(+e_ix+e_jx)/(2 alpha) - PQ_jx R(+e_ix) +
 // (t+t')_jx R(+e_ix-e_jx) (BIIFAST :18492-18810), RR = sum_ix E^ix_CD G^ix
 // (:18830-19150) and the exchange digestion into BXCH.
+template <int kBreitGroup>
 __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
-                           BreitShared &sh, int lane) {
+                           BreitShared<kBreitGroup> &sh, int lane) {
   const ClassDesc &cl = a.classes[it.cls];
   const int ld = cl.maxcd;  // slab rows packed at the class's kl count
   double *R = slab;
@@ -1210,10 +1215,24 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   }
   __syncthreads();
 
-  for (int abi = it.ab_lo; abi < it.ab_hi; ++abi) {
-    const BrAB ab = a.abs[abi];
-    for (int t = lane; t < 12 * ab.ntg; t += 64) sh.gm[t] = a.masks[ab.gmask + t];
-    const uint8_t *gsrc = abi == cl.ab_begin ? sh.cflag : a.stale + cl.flagoff;
+  // AB combinations in groups of up to kBreitGroup: every combination of a
+  // class has the same N(L_AB) and N(L_CD), reads the same R values and the
+  // same gauge factors T, and differs only in E(AB).  The class's first
+  // combination reads the fresh R-column flags (BIIFAST's IRC of this pair),
+  // every later one the class's stale flags, so groups never mix the two.
+  for (int a0 = it.ab_lo; a0 < it.ab_hi;) {
+    const int na = a0 == cl.ab_begin ? 1 : min(kBreitGroup, it.ab_hi - a0);
+    const int ntg = a.abs[a0].ntg, ntab = a.abs[a0].ntab;
+    int64_t eoff[kBreitGroup][3];
+#pragma unroll
+    for (int g = 0; g < kBreitGroup; ++g)
+#pragma unroll
+      for (int ix = 0; ix < 3; ++ix) eoff[g][ix] = g < na ? a.abs[a0 + g].eoff[ix] : 0;
+    for (int g = 0; g < na; ++g) {
+      const int gmo = a.abs[a0 + g].gmask;
+      for (int t = lane; t < 12 * ntg; t += 64) sh.gm[g][t] = a.masks[gmo + t];
+    }
+    const uint8_t *gsrc = a0 == cl.ab_begin ? sh.cflag : a.stale + cl.flagoff;
     for (int t = lane; t < nt; t += 64) sh.gflag[t] = gsrc[t];
     __syncthreads();
     for (int kl = lane; kl < maxcd; kl += 64) {
@@ -1224,33 +1243,43 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
                       (cl.xyz[2][1] * ec + cl.xyz[3][1] * ed) / q - Py,
                       (cl.xyz[2][2] * ec + cl.xyz[3][2] * ed) / q - Pz};
       double half_ia = 0.5 / (p * q / (p + q));
-      for (int tp = 0; tp < ab.ntg; ++tp) {
+      for (int tp = 0; tp < ntg; ++tp) {
         // the three Pauli components ix share every R value they read: R at
         // h = tuv + tuv', h + e_i, h + e_i + e_j and h + e_i - e_j
-        bool act[3];
+        bool act[kBreitGroup][3], any[3] = {false, false, false};
 #pragma unroll
-        for (int ix = 0; ix < 3; ++ix) {
-          const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
-          act[ix] = (gm[0] | gm[1] | gm[2] | gm[3]) != 0;
-        }
-        double g[3][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        for (int g = 0; g < kBreitGroup; ++g)
+#pragma unroll
+          for (int ix = 0; ix < 3; ++ix) {
+            const uint8_t *gm = &sh.gm[g][(tp * 3 + ix) * 4];
+            act[g][ix] = g < na && (gm[0] | gm[1] | gm[2] | gm[3]) != 0;
+            any[ix] = any[ix] || act[g][ix];
+          }
+        double gv[kBreitGroup][3][4];
+#pragma unroll
+        for (int g = 0; g < kBreitGroup; ++g)
+#pragma unroll
+          for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) gv[g][ix][c] = 0.0;
         // On the class's symmetric axes (ClassDesc::sym) R(h') vanishes
         // exactly unless h' is even there, so with dl the parity class of
         // h = tuv + tuv' on those axes: the direct term of ix needs dl = 0
         // and the gauge term of (ix, jx) needs dl = (e_ix + e_jx) mod 2
         // (its PQ_jx R(h + e_ix) part is zero when jx is symmetric, where
         // PQ_jx = 0).  pairs[dl] holds the (ix, jx) bits 3 ix + jx that can
-        // contribute; a t whose dl has none is skipped.
+        // contribute to some combination of the group; a t whose dl has none
+        // is skipped.
         int pairs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int ix = 0; ix < 3; ++ix)
 #pragma unroll
           for (int jx = 0; jx < 3; ++jx)
-            if (act[ix] && (!a.gaunt || ix == jx))
+            if (any[ix] && (!a.gaunt || ix == jx))
               pairs[((1 << ix) ^ (1 << jx)) & sym] |= 1 << (3 * ix + jx);
-        if (act[0] || act[1] || act[2]) {
+        if (any[0] || any[1] || any[2]) {
           const int *hs = c_hsum + tp * kHsRow;
-          for (int t = 0; t < ab.ntab; ++t) {
+          for (int t = 0; t < ntab; ++t) {
             const int h = hs[t];
             const int dl = hpar(h) & sym;
             const int pm = pairs[dl];
@@ -1259,19 +1288,24 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
             const double r0 = R[(int64_t)h * ld + kl];
             // E(AB) of the three components at t, EPSAB applied (the same
             // address in every lane: scalar loads)
-            double e3[3][4];
+            double e3[kBreitGroup][3][4];
 #pragma unroll
-            for (int ix = 0; ix < 3; ++ix) {
-              const double4 v = a.Eab[ab.eoff[ix] + (int64_t)t * maxab + ijs];
-              e3[ix][0] = v.x; e3[ix][1] = v.y; e3[ix][2] = v.z; e3[ix][3] = v.w;
-            }
-            if (fh) {
+            for (int g = 0; g < kBreitGroup; ++g)
 #pragma unroll
               for (int ix = 0; ix < 3; ++ix) {
-                if (!act[ix]) continue;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) g[ix][c] += e3[ix][c] * r0;
+                double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (g < na) v = a.Eab[eoff[g][ix] + (int64_t)t * maxab + ijs];
+                e3[g][ix][0] = v.x; e3[g][ix][1] = v.y; e3[g][ix][2] = v.z; e3[g][ix][3] = v.w;
               }
+            if (fh) {
+#pragma unroll
+              for (int g = 0; g < kBreitGroup; ++g)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) {
+                  if (!act[g][ix]) continue;
+#pragma unroll
+                  for (int c = 0; c < 4; ++c) gv[g][ix][c] += e3[g][ix][c] * r0;
+                }
             }
             if (a.gaunt) continue;
             // gauge term T = R(h+e_ix+e_jx)/(2 alpha) - PQ_jx R(h+e_ix) +
@@ -1315,98 +1349,111 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
 #pragma unroll
               for (int jx = 0; jx < 3; ++jx) {
                 if (!(pm >> (3 * ix + jx) & 1)) continue;
-                const double *e = e3[jx];
-                if (!any4(e)) continue;
                 const bool ok3 = hpm[ix][jx] >= 0;
                 int fl = sh.gflag[hpp[ix][jx]] + sh.gflag[hp[ix]] + (ok3 ? sh.gflag[hpm[ix][jx]] : 0);
                 if (fl == 0) continue;
                 double tv = rpp[ix][jx] * half_ia - rp[ix] * pq[jx];
                 if (ok3) tv += rpm[ix][jx] * (double)sj[jx];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) g[ix][c] -= e[c] * tv;
+                for (int g = 0; g < kBreitGroup; ++g) {
+                  const double *e = e3[g][jx];
+                  if (!act[g][ix] || !any4(e)) continue;
+#pragma unroll
+                  for (int c = 0; c < 4; ++c) gv[g][ix][c] -= e[c] * tv;
+                }
               }
             }
           }
         }
 #pragma unroll
-        for (int ix = 0; ix < 3; ++ix) {
-          const uint8_t *gm = &sh.gm[(tp * 3 + ix) * 4];
+        for (int g = 0; g < kBreitGroup; ++g) {
+          if (g >= na) break;
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
-            G[(int64_t)((ix * 4 + c) * ab.ntg + tp) * ld + kl] = gm[c] ? g[ix][c] : 0.0;
+          for (int ix = 0; ix < 3; ++ix) {
+            const uint8_t *gm = &sh.gm[g][(tp * 3 + ix) * 4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              G[(int64_t)(((g * 3 + ix) * 4 + c) * ntg + tp) * ld + kl] = gm[c] ? gv[g][ix][c] : 0.0;
+          }
         }
       }
     }
-    for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
-      const BrCD cd = a.cds[ci];
-      const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
-      const double pabcd = pab1 * pcd1;
-      for (int kl = lane; kl < maxcd; kl += 64) {
-        double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-        double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
-        for (int ix = 0; ix < 3; ++ix)
-          for (int tp = 0; tp < cd.ntcd; ++tp) {
-            // masked E(CD) times Z (fold_kernel; EILSCD0's flags)
-            const double4 e = a.Ecd[cd.eoff[ix] + (int64_t)tp * maxcd + kl];
-            const double e11r = e.x, e11i = e.y, e21r = e.z, e21i = e.w;
-            const double *gp = G + (int64_t)(ix * 4 * ab.ntg + tp) * ld + kl;
-            const int64_t gs = (int64_t)ab.ntg * ld;
-            double g11r = gp[0], g11i = gp[gs], g21r = gp[2 * gs], g21i = gp[3 * gs];
-            ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
-            br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
-            ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
-            br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
-            ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
-            br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
-            ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
-            br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
-          }
-        Raw raw;
-        for (int k = 0; k < 4; ++k) {
-          raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
-        }
-        raw.pab = pab1;
-        raw.pabcd = pabcd;
-        if (a.rr_out) {
-          if (ci == 0) {
-#pragma unroll
-            for (int n = 0; n < 16; ++n) {
-              double re, im;
-              if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
-              a.rr_out[(int64_t)n * maxcd + (kl % nC) * nD + kl / nC] = make_double2(re, im);
+    for (int g = 0; g < na; ++g) {
+      const BrAB ab = a.abs[a0 + g];
+      const double *Gg = G + (int64_t)(g * 12 * ntg) * ld;
+      for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
+        const BrCD cd = a.cds[ci];
+        const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+        const double pabcd = pab1 * pcd1;
+        for (int kl = lane; kl < maxcd; kl += 64) {
+          double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+          double br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+          for (int ix = 0; ix < 3; ++ix)
+            for (int tp = 0; tp < cd.ntcd; ++tp) {
+              // masked E(CD) times Z (fold_kernel; EILSCD0's flags)
+              const double4 e = a.Ecd[cd.eoff[ix] + (int64_t)tp * maxcd + kl];
+              const double e11r = e.x, e11i = e.y, e21r = e.z, e21i = e.w;
+              const double *gp = Gg + (int64_t)(ix * 4 * ntg + tp) * ld + kl;
+              const int64_t gs = (int64_t)ntg * ld;
+              double g11r = gp[0], g11i = gp[gs], g21r = gp[2 * gs], g21i = gp[3 * gs];
+              ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
+              br[0] += e11r * g11r + e11i * g11i; bi[0] += e11i * g11r - e11r * g11i;
+              ar[1] += e21r * g11r - e21i * g11i; ai[1] += e21r * g11i + e21i * g11r;
+              br[1] += e21r * g11r + e21i * g11i; bi[1] += e21i * g11r - e21r * g11i;
+              ar[2] += e11r * g21r - e11i * g21i; ai[2] += e11r * g21i + e11i * g21r;
+              br[2] += e11r * g21r + e11i * g21i; bi[2] += e11i * g21r - e11r * g21i;
+              ar[3] += e21r * g21r - e21i * g21i; ai[3] += e21r * g21i + e21i * g21r;
+              br[3] += e21r * g21r + e21i * g21i; bi[3] += e21i * g21r - e21r * g21i;
             }
+          Raw raw;
+          for (int k = 0; k < 4; ++k) {
+            raw.ar[k] = ar[k]; raw.ai[k] = ai[k]; raw.br[k] = br[k]; raw.bi[k] = bi[k];
           }
-          continue;
+          raw.pab = pab1;
+          raw.pabcd = pabcd;
+          if (a.rr_out) {
+            if (ci == 0) {
+#pragma unroll
+              for (int n = 0; n < 16; ++n) {
+                double re, im;
+                if (a.isym != 0) rrv<true>(raw, n, re, im); else rrv<false>(raw, n, re, im);
+                a.rr_out[(int64_t)n * maxcd + (kl % nC) * nD + kl / nC] = make_double2(re, im);
+              }
+            }
+            continue;
+          }
+          const int idx[4] = {ib, jb, kl % nC, kl / nC};
+          const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                 {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          // exchange batches only: the direct batches' accumulators and D(A,B) stay unused
+          double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+          const double2 dab_unused[2][4] = {};
+          if (a.isym != 0)
+            digest<true, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+                               cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
+          else
+            digest<false, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
+                                cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
         }
-        const int idx[4] = {ib, jb, kl % nC, kl / nC};
-        const int off[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                               {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        // exchange batches only: the direct batches' accumulators and D(A,B) stay unused
-        double acc_unused[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        const double2 dab_unused[2][4] = {};
-        if (a.isym != 0)
-          digest<true, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                       cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
-        else
-          digest<false, true>(raw, off, idx, Dm, nd, cd.iflg, a.intsym, pab1, cd.pab[1], pcd1,
-                        cd.pcd[1], dab_unused, acc_unused, xv, a.bdir);
-      }
-      if (!a.rr_out) {
-        const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
-                                {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
-        const int xf = a.intsym ? cd.iflg : (cd.iflg & 0x11);
-        if (a.isym != 0)
-          flush_exchange<true>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
-        else
-          flush_exchange<false>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
+        if (!a.rr_out) {
+          const int offs[4][2] = {{ab.fa[0], ab.fa[1]}, {ab.fb[0], ab.fb[1]},
+                                  {cd.fc[0], cd.fc[1]}, {cd.fd[0], cd.fd[1]}};
+          const int xf = a.intsym ? cd.iflg : (cd.iflg & 0x11);
+          if (a.isym != 0)
+            flush_exchange<true>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
+          else
+            flush_exchange<false>(xv, xf, offs, ib, jb, a.bxch, nd, lane);
+        }
       }
     }
     __syncthreads();
+    a0 += na;
   }
 }
 
+template <int GB>
 __global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
-  __shared__ BreitShared sh;
+  __shared__ BreitShared<GB> sh;
   __shared__ int s_item;
   const int lane = threadIdx.x;
   double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
@@ -1418,7 +1465,7 @@ __global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
     int idx = __builtin_amdgcn_readfirstlane(s_item);  // the same in every lane
     __syncthreads();
     if (idx >= a.nitems) break;
-    breit_item(a, a.items[idx], slab, sh, lane);
+    breit_item<GB>(a, a.items[idx], slab, sh, lane);
   }
 }
 
@@ -1846,9 +1893,17 @@ hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s) {
   int nf = std::max(1, std::min(nblocks, a.nclasses));
   if (a.nclasses > 0)
     hipLaunchKernelGGL(breit_flags_kernel, dim3(nf), dim3(64), 0, s, a);
-  if (a.nitems > 0)
-    hipLaunchKernelGGL(breit_kernel, dim3(std::max(1, std::min(nblocks, a.nitems))), dim3(64), 0,
-                       s, a);
+  if (a.nitems > 0) {
+    const dim3 grid(std::max(1, std::min(nblocks, a.nitems)));
+    if (a.group == 1)
+      hipLaunchKernelGGL(breit_kernel<1>, grid, dim3(64), 0, s, a);
+    else if (a.group == 2)
+      hipLaunchKernelGGL(breit_kernel<2>, grid, dim3(64), 0, s, a);
+    else if (a.group == 4)
+      hipLaunchKernelGGL(breit_kernel<4>, grid, dim3(64), 0, s, a);
+    else
+      return hipErrorInvalidValue;
+  }
   if (!a.rr_out) {
     int64_t n = (int64_t)a.nskp * (a.ndim - a.nskp);
     int bs = 256;

@@ -3,7 +3,7 @@
 For a system, times the stage-3 Coulomb build under every launch shape of
 fock_kernel (waves per SIMD, group capacity, persistent workgroups per CU)
 and, for Dirac-Coulomb-Breit systems, the Breit build under every workgroup
-count of breit_kernel, and records the fastest in tuned_launch.json beside
+count and AB-combination group of breit_kernel, and records the fastest in tuned_launch.json beside
 this file, keyed by the system's signature: integral path (real for
 ATOMIC/DIATOM/LINEAR, complex otherwise), Hamiltonian, largest primitive
 count per shell and largest l.  Engine(..., tuned=True) applies the entry
@@ -31,7 +31,7 @@ TABLE = os.path.join(HERE, "tuned_launch.json")
 # their registers allow and below
 COULOMB_SHAPES = [(3, 1, 12), (3, 1, 8), (2, 2, 8), (2, 1, 8), (2, 2, 6), (2, 1, 6),
                   (2, 2, 4)]
-BREIT_PER_CU = [8, 6, 4]
+BREIT_SHAPES = [(8, 1), (6, 1), (8, 2), (4, 4)]  # (workgroups per CU, AB group)
 
 
 def signature(system) -> str:
@@ -94,16 +94,19 @@ def tune(pkg, system, reps: int = 3, log=print) -> dict:
              "tuned": time.strftime("%Y-%m-%d")}
     if system.hmlt.upper() == "DHFB":
         breit = {}
-        for per_cu in BREIT_PER_CU:
-            eng.set_launch(waves=best[0], cap=best[1], per_cu=best[2], breit_per_cu=per_cu)
+        for per_cu, group in BREIT_SHAPES:
+            eng.set_launch(waves=best[0], cap=best[1], per_cu=best[2], breit_per_cu=per_cu,
+                           breit_group=group)
             ms = _time(eng, lambda: eng.breit_device(d.data_ptr(), g.data_ptr(), x.data_ptr(), 3),
                        max(1, reps - 1))
-            breit[per_cu] = ms
-            log(f"tune {signature(system)}: breit_kernel per CU {per_cu}: {ms:.2f} ms")
+            breit[(per_cu, group)] = ms
+            log(f"tune {signature(system)}: breit_kernel per CU {per_cu} group {group}: "
+                f"{ms:.2f} ms")
         bbest = min(breit, key=breit.get)
-        entry.update({"breit_per_cu": bbest, "breit_ms": breit[bbest]})
+        entry.update({"breit_per_cu": bbest[0], "breit_group": bbest[1], "breit_ms": breit[bbest]})
         entry["default"]["breit_per_cu"] = 8
-        entry["default"]["breit_ms"] = breit.get(8)
+        entry["default"]["breit_group"] = 1
+        entry["default"]["breit_ms"] = breit.get((8, 1))
     eng.close()
     return entry
 

@@ -242,15 +242,17 @@ int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long
    rounding of the kernels' atomic accumulation.  waves (2 or 3 waves per
    SIMD), cap (group capacity 1 or 2; three waves run at capacity 1) and
    per_cu (persistent workgroups per compute unit) set the Coulomb kernel's
-   launch shape, breit_per_cu the Breit kernel's workgroups per CU; 0 keeps
-   the engine's choice.  Takes effect from the next build; captured builds
-   are dropped. */
+   launch shape, breit_per_cu the Breit kernel's workgroups per CU and
+   breit_group the number of AB combinations it contracts together (1, 2 or
+   4; its scratch grows with it); 0 keeps the engine's choice.  Takes effect
+   from the next build; captured builds are dropped. */
 typedef struct {
   int graph;
   int waves;
   int cap;
   int per_cu;
   int breit_per_cu;
+  int breit_group;
 } bdf_launch;
 
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l);
