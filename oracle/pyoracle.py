@@ -426,6 +426,20 @@ class Reference:
                                *[C.c_void_p(o.ctypes.data) for o in out])
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
 
+    def clmfast_racah(self, dent, ilev=3):
+        """CLMFAST followed by COULOMB1 for every centre, as HFSCF runs them
+        with RACAH1 on (construct the Reference with racah1=True)."""
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        dre = np.asfortranarray(d.real)
+        dim = np.asfortranarray(d.imag)
+        out = [np.zeros((n, n), order="F") for _ in range(4)]
+        _ref_lib().ref_schwarz(C.c_int(0))
+        _ref_lib().ref_clmfast_racah(C.c_int(ilev), C.c_void_p(dre.ctypes.data),
+                                     C.c_void_p(dim.ctypes.data),
+                                     *[C.c_void_p(o.ctypes.data) for o in out])
+        return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
     def funfm(self, t, lam, itype):
         t = np.ascontiguousarray(t, dtype=np.float64)
         out = np.zeros((lam + 1) * t.size)
