@@ -106,6 +106,10 @@ def library() -> _C.CDLL:
         lib.bdf_coulomb.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
         lib.bdf_coulomb_device.argtypes = [P, _C.c_int, P, P, P]
         lib.bdf_coulomb_device_static.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_coulomb1.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
+        lib.bdf_coulomb1_device.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_racah_info.argtypes = [P, _C.POINTER(_C.c_double), _C.POINTER(_C.c_long),
+                                       _C.POINTER(_C.c_long)]
         lib.bdf_eri_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int, P, _C.c_int]
         lib.bdf_etable.argtypes = [P, _C.c_int, _C.c_int, _C.c_int, _C.c_int, _C.c_int,
                                    _C.c_int, _C.c_int, P, _C.c_long, _C.POINTER(_C.c_long)]
@@ -235,6 +239,39 @@ class Engine:
         """bdf_coulomb_device_static: the same build without allocating (the
         stage's plan must exist: plan_info(ilev) builds it)."""
         _check(library().bdf_coulomb_device_static(self._h, ilev, d_dent, d_gdir, d_gxch))
+
+    def coulomb1(self, dent: _np.ndarray, gdir: _np.ndarray, gxch: _np.ndarray,
+                 exact_beta: bool = False):
+        """bdf_coulomb1: every centre's COULOMB1 on top of a CLMFAST result
+        (gdir, gxch of an engine with racah1=True); returns the new matrices."""
+        n = self.ndim
+        d = _np.asfortranarray(dent, dtype=_np.complex128)
+        g = _np.array(gdir, dtype=_np.complex128, order="F")
+        x = _np.array(gxch, dtype=_np.complex128, order="F")
+        if d.shape != (n, n) or g.shape != (n, n) or x.shape != (n, n):
+            raise ValueError(f"matrices must be {n}x{n}")
+        _check(library().bdf_coulomb1(self._h, 1 if exact_beta else 0, d.ctypes.data, None, n,
+                                      g.ctypes.data, x.ctypes.data, None, None))
+        return g, x
+
+    def coulomb1_device(self, d_dent: int, d_gdir: int, d_gxch: int,
+                        exact_beta: bool = False) -> None:
+        _check(library().bdf_coulomb1_device(self._h, 1 if exact_beta else 0, d_dent, d_gdir,
+                                             d_gxch))
+
+    def clmfast_racah(self, dent: _np.ndarray, ilev: int = 3, exact_beta: bool = False):
+        """HFSCF's Coulomb build with RACAH1 on: CLMFAST without one-centre
+        quartets, then COULOMB1 for every centre (engine built with
+        racah1=True)."""
+        g, x = self.clmfast(dent, ilev)
+        return self.coulomb1(dent, g, x, exact_beta)
+
+    def racah_info(self):
+        """(digestion terms per build, work items, uncovered beta reads) of
+        the one-centre Racah build."""
+        t, i, u = _C.c_double(), _C.c_long(), _C.c_long()
+        _check(library().bdf_racah_info(self._h, _C.byref(t), _C.byref(i), _C.byref(u)))
+        return t.value, i.value, u.value
 
     def erifast(self, icnt, kidx, midx, ibas: int, jbas: int, itn, ilev: int = 3):
         """RR(N,1..16) of one ERIFAST batch, shape (16, maxcd) complex."""

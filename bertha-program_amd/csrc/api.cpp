@@ -80,6 +80,18 @@ struct DevBreitPlan {
   DevBuf<uint8_t> stale;
 };
 
+// device copy of the one-centre Racah plan
+struct DevRacah {
+  int built = 0;
+  RacahPlan P;
+  DevBuf<RcBlock> blocks;
+  DevBuf<RcSeg> segs;
+  DevBuf<RcCombo> combos;
+  DevBuf<RcEntry> entries;
+  DevBuf<RcOp> ops;
+  DevBuf<RcItem> items;
+};
+
 // One captured build: the launches of a Coulomb (kind 0) or Breit (kind 1)
 // build of stage ilev on fixed device buffers, replayed as a HIP graph.
 struct GraphEntry {
@@ -127,6 +139,7 @@ struct bdf_engine {
   int breit = 0;  // DHFB: LS tables and the Breit plans
   int gaunt = 0;  // GAUNT2
   DevBreitPlan bplans[3];
+  DevRacah racah;  // COULOMB1 replacement (bdf_coulomb1)
   DevBuf<float4> E32;
   DevBuf<double4> Ecd;  // masked, signed E(CD) of the second contraction
   DevBuf<double4> Eab;  // E with EPSAB applied: the Breit kernel's AB side
@@ -256,6 +269,27 @@ static int make_plan(bdf_engine *e, int ilev) {
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
+  return BDF_OK;
+}
+
+static int make_racah(bdf_engine *e) {
+  DevRacah &r = e->racah;
+  if (r.built) return BDF_OK;
+  std::string err;
+  try {
+    err = build_racah_plan(e->B, e->isym, r.P);
+  } catch (const std::bad_alloc &) {
+    err = "racah plan: out of host memory";
+  }
+  if (!err.empty()) return fail(BDF_ELIMIT, err);
+  HIPCHK(r.blocks.upload(r.P.blocks, e->stream));
+  HIPCHK(r.segs.upload(r.P.segs, e->stream));
+  HIPCHK(r.combos.upload(r.P.combos, e->stream));
+  HIPCHK(r.entries.upload(r.P.entries, e->stream));
+  HIPCHK(r.ops.upload(r.P.ops, e->stream));
+  HIPCHK(r.items.upload(r.P.items, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  r.built = 1;
   return BDF_OK;
 }
 
@@ -810,6 +844,72 @@ int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent, const doub
   HIPCHK(hipMemcpy2DAsync(gxch, ld * sizeof(double2), e->gxch.p, nd * sizeof(double2),
                           nd * sizeof(double2), nd, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent, void *d_gdir,
+                        void *d_gxch) {
+  if (!e || !d_dent || !d_gdir || !d_gxch) return fail(BDF_EINVAL, "bdf_coulomb1_device: null argument");
+  if (exact_beta != 0 && exact_beta != 1)
+    return fail(BDF_EINVAL, "bdf_coulomb1_device: exact_beta must be 0 or 1");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_racah(e);
+  if (rc != BDF_OK) return rc;
+  const DevRacah &r = e->racah;
+  RacahArgs a;
+  a.blocks = r.blocks.p;
+  a.segs = r.segs.p;
+  a.combos = r.combos.p;
+  a.entries = r.entries.p;
+  a.ops = r.ops.p;
+  a.items = r.items.p;
+  a.nitems = (int)r.P.items.size();
+  a.exps = e->exps.p;
+  a.normL = e->normL.p;
+  a.normS = e->normS.p;
+  a.dent = (const double2 *)d_dent;
+  a.gdir = (double2 *)d_gdir;
+  a.gxch = (double2 *)d_gxch;
+  a.ndim = e->B.ndim;
+  a.isym = e->isym;
+  a.exact = exact_beta;
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  HIPCHK(launch_racah(a, e->stream));
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  return BDF_OK;
+}
+
+int bdf_coulomb1(bdf_engine *e, int exact_beta, const double _Complex *dent,
+                 const double _Complex *deno, int ld, double _Complex *gdir, double _Complex *gxch,
+                 double _Complex *qdir, double _Complex *qxch) {
+  if (!e || !dent || !gdir || !gxch) return fail(BDF_EINVAL, "bdf_coulomb1: null argument");
+  if (deno || qdir || qxch) return fail(BDF_EUNSUPP, "bdf_coulomb1: open shells are not supported");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_coulomb1: ld < ndim");
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(e->dent.alloc((size_t)nd * nd));
+  HIPCHK(e->gdir.alloc((size_t)nd * nd));
+  HIPCHK(e->gxch.alloc((size_t)nd * nd));
+  const size_t row = nd * sizeof(double2), lrow = ld * sizeof(double2);
+  HIPCHK(hipMemcpy2DAsync(e->dent.p, row, dent, lrow, row, nd, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpy2DAsync(e->gdir.p, row, gdir, lrow, row, nd, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpy2DAsync(e->gxch.p, row, gxch, lrow, row, nd, hipMemcpyHostToDevice, e->stream));
+  int rc = bdf_coulomb1_device(e, exact_beta, e->dent.p, e->gdir.p, e->gxch.p);
+  if (rc != BDF_OK) return rc;
+  HIPCHK(hipMemcpy2DAsync(gdir, lrow, e->gdir.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpy2DAsync(gxch, lrow, e->gxch.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_racah_info(bdf_engine *e, double *terms, long *items, long *uncovered) {
+  if (!e) return fail(BDF_EINVAL, "bdf_racah_info: null engine");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_racah(e);
+  if (rc != BDF_OK) return rc;
+  if (terms) *terms = e->racah.P.terms;
+  if (items) *items = (long)e->racah.P.items.size();
+  if (uncovered) *uncovered = (long)e->racah.P.uncovered;
   return BDF_OK;
 }
 

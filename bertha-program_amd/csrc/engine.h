@@ -157,6 +157,76 @@ struct BreitPlan {
   double flops = 0.0;
 };
 
+// ---- one-centre Racah plan (COULOMB1, bertha-scf.f:21540) ----------------
+// Tensor orders of one LQN block: NU runs over one parity class of
+// [max(|la-lb|,|lc-ld|), min(la+lb,lc+ld)], at most kMaxL+1 values.
+constexpr int kRcMaxNu = kMaxL + 1;
+constexpr int kRcMaxNuf = 2 * kMaxL;  // largest tensor order
+
+// One LQN block (la <= lb, lc <= ld, la(la+1)/2+lb <= lc(lc+1)/2+ld) of one
+// centre, with the tensor orders TNSCLM1 allows (25551).
+struct RcBlock {
+  int l[4], nb[4], exp[4];  // l, primitives and exponent offsets of A, B, C, D
+  int nunum, nui, nuf, nvals;
+  int nus[kRcMaxNu];
+  double cc[5];             // C1, C3, C5, C7, C9 = Gamma((L + 2t + 1)/2) / 4
+  int maxab, maxcd;
+  int seg_begin, seg_end;   // beta sources of a block that writes none (RcSeg)
+  int fresh;                // 1: its first KQN combination writes its betas
+  int pad;
+};
+
+// Where a block that writes no beta integrals reads them (the reference's
+// COMMON /BETA/ keeps the last writer's values): reads at levels
+// max(NX, NY) == nv and flat pair index (IJ-1)*MAXCD+M below fend come from
+// block src's integrals at that flat index; src < 0: never written (zero).
+struct RcSeg {
+  int nv, src;
+  int64_t fend;
+};
+
+// One KQN combination (K4AD) of a block that survives ANGCLM1 (25865).
+struct RcCombo {
+  int block;
+  int inu;                  // bit t: tensor order nus[t] in use
+  double tq[4];             // l + kappa + 1 of A, B, C, D (RKCLM1's T factors)
+  int entry_begin, entry_end;
+};
+
+// One (|m|, sign) combination: its angular factors DKAB * DKCD per tensor
+// order (0 where screened) and the digestion terms it feeds.
+struct RcEntry {
+  double ang[kRcMaxNu];
+  int op_begin, op_end;
+};
+
+// One digestion term: M[dst] += f * X_xt(ij, kl) * D[src] (the real part of
+// D on the axial path), rows and columns = base + one of (i, j, k, l).
+struct RcOp {
+  int dst_r, dst_c, src_r, src_c;
+  uint8_t sdr, sdc, ssr, ssc;  // index of each: 0 i, 1 j, 2 k, 3 l
+  uint8_t mat;                 // 0 GDIR, 1 GXCH
+  uint8_t xt;                  // 0 LLLL, 1 LLSS, 2 SSLL, 3 SSSS
+  uint8_t uniform;             // dst indexed by i and j only
+  uint8_t pad;
+  double f;
+};
+
+struct RcItem {
+  int combo, ij;
+};
+
+struct RacahPlan {
+  std::vector<RcBlock> blocks;
+  std::vector<RcSeg> segs;
+  std::vector<RcCombo> combos;
+  std::vector<RcEntry> entries;
+  std::vector<RcOp> ops;
+  std::vector<RcItem> items;
+  int64_t uncovered = 0;    // beta reads no earlier block wrote
+  double terms = 0.0;       // digestion terms (ij, kl, op) of one build
+};
+
 // everything the host plan produces
 struct Plan {
   int ndim = 0, nskp = 0;

@@ -129,6 +129,25 @@ constexpr size_t kStaticScratchDoubles = (size_t(256) << 20) / sizeof(double);
 constexpr int kMaxDevices = 64;
 // device address of that scratch on `device`, which must be current; looked
 // up once per device (hipGetSymbolAddress) and kept
+// one-centre Racah Coulomb build (racah.hip)
+struct RacahArgs {
+  const RcBlock *blocks;
+  const RcSeg *segs;
+  const RcCombo *combos;
+  const RcEntry *entries;
+  const RcOp *ops;
+  const RcItem *items;
+  int nitems;
+  const double *exps, *normL, *normS;
+  const double2 *dent;
+  double2 *gdir, *gxch;
+  int ndim;
+  int isym;
+  int exact;  // 1: every block's own beta integrals
+};
+// the racah kernel, then the lower triangles set to the conjugate upper ones
+hipError_t launch_racah(const RacahArgs &a, hipStream_t s);
+
 hipError_t static_scratch(int device, double **ptr);
 
 hipError_t upload_constants();

@@ -73,8 +73,10 @@ typedef struct {
    the engine's own options. */
 typedef struct {
   int intsym;          /* INTSYM: permutational symmetry of RR batches (1) */
-  int racah1;          /* RACAH1: skip one-centre quartets (reference default 1;
-                          this engine computes them by MD when 0) */
+  int racah1;          /* RACAH1: bdf_coulomb/bdf_breit skip one-centre
+                          quartets, which bdf_coulomb1 then adds as
+                          COULOMB1 does (reference default 1); 0: this engine
+                          computes them by MD inside the same build */
   int precision;       /* BDF_PREC_* */
   double split_threshold; /* MIXED: a work item (class, primitive pair ij) whose
                              largest quartet contribution bound
@@ -141,6 +143,38 @@ int bdf_coulomb_device(bdf_engine *e, int ilev, const void *d_dent,
    stream, not synchronised, never captured as a graph. */
 int bdf_coulomb_device_static(bdf_engine *e, int ilev, const void *d_dent,
                               void *d_gdir, void *d_gxch);
+
+/* Replaces the per-centre one-centre Coulomb calls of HFSCF with RACAH1 on,
+   DO IZ=1,NCNT: CALL COULOMB1(IZ) (bertha-scf.f:5415-5421; routine at
+   :21540): adds every centre's one-centre Coulomb contributions, evaluated
+   by Racah algebra and beta integrals (TNSCLM1, ANGCLM1, BTACLM1, RKCLM1,
+   CLMMT1Z/CLMMT1G), to GDIR and GXCH, which hold the CLMFAST result of an
+   engine created with racah1 = 1 (one-centre quartets skipped), and then
+   sets every lower-triangle element to the conjugate of its upper partner,
+   as COULOMB1 does on return (:22074-22081).  All component types enter at
+   every stage, as in the reference.  exact_beta = 0 reproduces the
+   reference exactly, including its beta integrals for LQN blocks whose
+   first KQN combination the triangle rule drops (those read the integrals
+   the previous block left in COMMON /BETA/); exact_beta = 1 evaluates
+   every block's own beta integrals instead (the exact one-centre
+   integrals).  Host arrays, ld >= ndim, gdir/gxch updated in place;
+   deno/qdir/qxch must be NULL (closed shells). */
+int bdf_coulomb1(bdf_engine *e, int exact_beta, const double _Complex *dent,
+                 const double _Complex *deno, int ld, double _Complex *gdir,
+                 double _Complex *gxch, double _Complex *qdir,
+                 double _Complex *qxch);
+
+/* Same on device-resident arrays (ld = ndim), enqueued on the engine's
+   stream and not synchronised. */
+int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent,
+                        void *d_gdir, void *d_gxch);
+
+/* Size of the one-centre Racah build: digestion terms (ij, kl, term) of one
+   build, work items, and beta-integral reads that no earlier block of the
+   build writes (the reference reads what an earlier build or a fresh
+   process left there; here they are zero).  Builds the plan on first use.
+   No counterpart in the reference. */
+int bdf_racah_info(bdf_engine *e, double *terms, long *items, long *uncovered);
 
 /* Test hook mirroring ERIFAST's output (bertha-scf.f:12269-13132) for one
    block quartet: centres icnt[4] (1-based), kappa indices kidx[4] (1-based

@@ -63,6 +63,32 @@ module bertha_amd
       integer(c_int), value :: ilev
     end function
 
+    ! in place of DO IZ=1,NCNT: CALL COULOMB1(IZ) (bertha-scf.f:5415-5421)
+    integer(c_int) function bdf_coulomb1(eng, exact_beta, dent, deno, ld, gdir, gxch, qdir, qxch) &
+        bind(c, name='bdf_coulomb1')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), value :: exact_beta, ld
+      complex(c_double_complex), intent(in) :: dent(ld, *)
+      type(c_ptr), value :: deno, qdir, qxch
+      complex(c_double_complex), intent(inout) :: gdir(ld, *), gxch(ld, *)
+    end function
+
+    integer(c_int) function bdf_coulomb1_device(eng, exact_beta, d_dent, d_gdir, d_gxch) &
+        bind(c, name='bdf_coulomb1_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, d_dent, d_gdir, d_gxch
+      integer(c_int), value :: exact_beta
+    end function
+
+    integer(c_int) function bdf_racah_info(eng, terms, items, uncovered) &
+        bind(c, name='bdf_racah_info')
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: eng
+      real(c_double), intent(out) :: terms
+      integer(c_long), intent(out) :: items, uncovered
+    end function
+
     integer(c_int) function bdf_eri_batch(eng, icnt, kidx, midx, ibas, jbas, itn, ilev, rr, maxcd) &
         bind(c, name='bdf_eri_batch')
       import :: c_int, c_ptr, c_double_complex

@@ -81,7 +81,7 @@ static const double TWLG = 0.6931471805599453;
 
 static int IABC[ML4 + 1][ML4 + 1][ML4 + 1];
 static int IAX[MRC + 1], IBX[MRC + 1], ICX[MRC + 1], ILAM[MRC + 1];
-static double RFACT[81], GAMLOG[301];
+static double RFACT[81], GAMLOG[301], GAMHLF[301];
 static int g_init = 0;
 
 static int ntuv(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
@@ -106,10 +106,14 @@ static void init_tables(void) {
   for (int i = 2; i <= 80; ++i) RFACT[i] = (double)i * RFACT[i - 1];
   GAMLOG[1] = PILG * 0.5;
   GAMLOG[2] = 0.0;
+  GAMHLF[1] = PI12;
+  GAMHLF[2] = 1.0;
   double f1 = 0.5, f2 = 1.0;
   for (int k = 4; k <= 300; k += 2) {
     GAMLOG[k - 1] = GAMLOG[k - 3] + log(f1);
     GAMLOG[k] = GAMLOG[k - 2] + log(f2);
+    GAMHLF[k - 1] = GAMHLF[k - 3] * f1;
+    GAMHLF[k] = GAMHLF[k - 2] * f2;
     f1 += 1.0; f2 += 1.0;
   }
   g_init = 1;
@@ -2752,6 +2756,521 @@ void orc_oneel(orc_sys *s, const double *znuc, const double *xi, double cv, doub
             }
           free(sll); free(sss); free(vll); free(vss); free(tls); free(tsl);
         }
+}
+
+/* ------------------------------------------------------------------------ */
+/* one-centre Coulomb by Racah algebra (COULOMB1 21540-22107)               */
+/* ------------------------------------------------------------------------ */
+#define C1MNU 16 /* tensor orders and beta levels per LQN block (any l <= 6) */
+
+/* SYM3J 29215: Wigner 3j symbol (j m, L M, j' m') with j, m, j', m' doubled */
+static double sym3j(int ja, int l, int jb, int ma, int m, int mb) {
+  if (abs(m) > l || 2 * m + ma + mb != 0 || abs(ma) > ja || abs(mb) > jb) return 0.0;
+  double phs = m1pow((ja - mb) / 2 - l);
+  double sym = sqrt(RFACT[(ja + ma) / 2] * RFACT[(ja - ma) / 2] * RFACT[(jb - mb) / 2] *
+                    RFACT[(jb + mb) / 2] * RFACT[l + m] * RFACT[l - m]);
+  double trp = sqrt(RFACT[(ja - jb) / 2 + l] * RFACT[(ja + jb) / 2 - l] *
+                    RFACT[(jb - ja) / 2 + l] / RFACT[(ja + jb) / 2 + l + 1]);
+  int nui = l - (jb + ma) / 2, nuf = (ja - ma) / 2;
+  if (m + (ja - jb) / 2 > nui) nui = m + (ja - jb) / 2;
+  if (nui < 0) nui = 0;
+  if (l + (ja - jb) / 2 < nuf) nuf = l + (ja - jb) / 2;
+  if (l + m < nuf) nuf = l + m;
+  double x = 0.0;
+  for (int nu = nui; nu <= nuf; ++nu)
+    x += m1pow(nu) / (RFACT[(ja - ma) / 2 - nu] * RFACT[(jb + ma) / 2 - l + nu] *
+                      RFACT[(jb - ja) / 2 - m + nu] * RFACT[(ja - jb) / 2 + l - nu] *
+                      RFACT[l + m - nu] * RFACT[nu]);
+  return phs * trp * sym * x;
+}
+
+/* DK 29166: angular factor of a tensor of order nu between two spinors */
+static double dk_coef(int ja, int ma, int jb, int mb, int nu) {
+  double phs = m1pow((ma + 1) / 2);
+  double rtj = sqrt((double)((ja + 1) * (jb + 1)));
+  return phs * rtj * sym3j(ja, nu, jb, 1, 0, -1) * sym3j(ja, nu, jb, -ma, (ma - mb) / 2, mb);
+}
+
+/* BTACLM1 25430-25530: one incomplete beta integral for XJ = x */
+static double c1_beta(double x, int ia, int ib) {
+  if (ib > 1) {
+    double xv = (double)ia + 0.5;
+    double bta1 = powi(sqrt(x), 2 * ia + 1) / xv;
+    double ra = xv, rb = (double)(1 - ib), rc = xv + 1.0, rd = 1.0;
+    double trm = ra * rb / (rc * rd) * x, bin = 1.0 + trm;
+    ra += 1.0; rb += 1.0; rc += 1.0; rd += 1.0;
+    for (int it = 2; it <= ib - 1; ++it) {
+      trm = ra * rb / (rc * rd) * trm * x;
+      bin += trm;
+      ra += 1.0; rb += 1.0; rc += 1.0; rd += 1.0;
+    }
+    return bta1 * bin;
+  }
+  if (ib == 1) return powi(sqrt(x), 2 * ia + 1) / ((double)ia + 0.5);
+  double xr = sqrt(x);
+  double bta1 = log((1.0 + xr) / (1.0 - xr));
+  if (ia == 0) return bta1;
+  double bin = 1.0, trm = x;
+  for (int k = 2; k <= ia; ++k) {
+    bin += (1.0 / (double)(2 * k - 1)) * trm;
+    trm *= x;
+  }
+  return bta1 - 2.0 * xr * bin;
+}
+
+/* a block's basis lists (IJSET1 29301 / KLSET1 29394): exponents, the
+   products of L and S normalisations, and (e_i + e_j)^(-(la+lb+p)/2) for
+   p = -NUF .. NUF+5 */
+typedef struct {
+  int n;
+  double *ei, *ej, *rn; /* rn[m*4 + 0..3]: LL, LS, SL, SS */
+  double *ep;           /* ep[m*(2*C1MNU+6) + p + C1MNU] */
+} c1_pairs;
+
+static void c1_pairs_set(c1_pairs *q, const double *ea, int na, int la, const double *eb, int nb,
+                         int lb, int nuf) {
+  q->n = na * nb;
+  q->ei = calloc(q->n, sizeof(double));
+  q->ej = calloc(q->n, sizeof(double));
+  q->rn = calloc(4L * q->n, sizeof(double));
+  q->ep = calloc((long)q->n * (2 * C1MNU + 6), sizeof(double));
+  const int nmx = na > nb ? na : nb;
+  double rnl[2][nmx], rns[2][nmx];
+  const double *ex[2] = {ea, eb};
+  const int nn[2] = {na, nb}, ll[2] = {la, lb};
+  for (int s = 0; s < 2; ++s) {
+    double g1 = TWLG - GAMLOG[2 * ll[s] + 3], g2 = TWLG - GAMLOG[2 * ll[s] + 5];
+    double r1 = ll[s] + 1.5, r2 = ll[s] + 0.5;
+    for (int i = 0; i < nn[s]; ++i) {
+      double elog = log(2.0 * ex[s][i]);
+      rnl[s][i] = exp(0.5 * (g1 + r1 * elog));
+      rns[s][i] = exp(0.5 * (g2 + r2 * elog));
+    }
+  }
+  int ipower = la + lb - nuf;
+  for (int i = 0, m = 0; i < na; ++i)
+    for (int j = 0; j < nb; ++j, ++m) {
+      q->ei[m] = ea[i];
+      q->ej[m] = eb[j];
+      q->rn[4 * m + 0] = rnl[0][i] * rnl[1][j];
+      q->rn[4 * m + 1] = rnl[0][i] * rns[1][j];
+      q->rn[4 * m + 2] = rns[0][i] * rnl[1][j];
+      q->rn[4 * m + 3] = rns[0][i] * rns[1][j];
+      double r = sqrt(ea[i] + eb[j]);
+      double *e = q->ep + (long)m * (2 * C1MNU + 6) + C1MNU;
+      e[-nuf] = ipower > 0 ? 1.0 / powi(r, ipower) : powi(r, -ipower);
+      for (int p = -nuf + 1; p <= nuf + 5; ++p) e[p] = e[p - 1] / r;
+    }
+}
+
+static void c1_pairs_free(c1_pairs *q) {
+  free(q->ei); free(q->ej); free(q->rn); free(q->ep);
+}
+
+/* the reference's COMMON /BETA/: upper and lower beta integrals by flat
+   pair index (IJ-1)*MAXCD+M and levels (NX, NY); written only for the
+   first KQN combination of an LQN block (K4AD = 1) and read by every
+   combination, so blocks whose first combination the KQN triangle rule
+   skips read the values of the last block that wrote them */
+typedef struct {
+  long cap;
+  double *bdu, *bdl; /* [(nx*C1MNU + ny) * cap + flat] */
+} c1_beta_store;
+
+typedef struct {
+  orc_sys *s;
+  const zc *D;
+  zc *G, *X;
+  int isym;
+  /* quantum numbers and offsets of the current (MQN, sign) combination */
+  int kqn[4], ikl, ikr;
+  int mqn[4], mma, mmb, mmc, mmd;
+  double pab, pcd;
+  int nal, nbl, ncl, ndl, nas, nbs, ncs, nds, kal, kbl, kcl, kdl, kas, kbs, kcs, kds;
+  int nbas[4], ibas, jbas;
+  const double *x[4]; /* XLLLL, XLLSS, XSSLL, XSSSS over M = (k, l) */
+} c1_ctx;
+
+/* density element as the digestion reads it: the real part on the axial
+   path (CLMMT1Z), the complex value on the general one (CLMMT1G) */
+static zc c1_d(const c1_ctx *c, int r, int col) {
+  zc v = c->D[(long)col * c->s->ndim + r];
+  return c->isym ? (zc)creal(v) : v;
+}
+
+#define C1G(M, r, col) (c->M[(long)(col) * c->s->ndim + (r)])
+#define C1LOOP for (int k = 0, m = 0; k < c->nbas[2]; ++k) for (int l = 0; l < c->nbas[3]; ++l, ++m)
+#define XL c->x[0][m]
+#define XLS c->x[1][m]
+#define XSL c->x[2][m]
+#define XS c->x[3][m]
+
+/* CLMMT1Z 22809-23760 / CLMMT1G 24443-25377, closed-shell part: the eight
+   index permutations of (AB|CD) into GDIR and of (AD|CB) into GXCH, each with
+   its variable swap (CD|AB), upper-triangle block guards and the axial path's
+   extra MQN-sign rules */
+static void c1_digest(c1_ctx *c) {
+  const int i = c->ibas, j = c->jbas, z = c->isym != 0;
+  const int mja = c->mqn[0] * (c->mma == 1 ? -1 : 1), mjb = c->mqn[1] * (c->mmb == 1 ? -1 : 1);
+  const int mjc = c->mqn[2] * (c->mmc == 1 ? -1 : 1), mjd = c->mqn[3] * (c->mmd == 1 ? -1 : 1);
+  const int swap = c->ikl != c->ikr; /* PRM1HC on */
+  const int k12 = c->kqn[0] == c->kqn[1], k34 = c->kqn[2] == c->kqn[3];
+  const double pab = c->pab, pcd = c->pcd, pp = pab * pcd;
+  /* GDIR (AB|CD) */
+  if (!(z && (c->mma != c->mmb || c->mmc != c->mmd)) && mja - mjb == mjd - mjc) {
+    if (c->nal <= c->nbl) C1LOOP {
+      C1G(G, c->nal + i, c->nbl + j) += XL * c1_d(c, c->ncl + k, c->ndl + l) + XLS * c1_d(c, c->ncs + k, c->nds + l);
+      C1G(G, c->nas + i, c->nbs + j) += XSL * c1_d(c, c->ncl + k, c->ndl + l) + XS * c1_d(c, c->ncs + k, c->nds + l);
+    }
+    if (swap && c->ncl <= c->ndl) C1LOOP {
+      C1G(G, c->ncl + k, c->ndl + l) += XL * c1_d(c, c->nal + i, c->nbl + j) + XSL * c1_d(c, c->nas + i, c->nbs + j);
+      C1G(G, c->ncs + k, c->nds + l) += XLS * c1_d(c, c->nal + i, c->nbl + j) + XS * c1_d(c, c->nas + i, c->nbs + j);
+    }
+  }
+  /* GDIR (BA|CD) = PAB (AB|CD) */
+  if (!(z && (c->mmb != c->mma || c->mmc != c->mmd)) && mjb - mja == mjd - mjc) {
+    if (!k12 && c->kbl <= c->kal) C1LOOP {
+      C1G(G, c->kbl + j, c->kal + i) += pab * XL * c1_d(c, c->ncl + k, c->ndl + l) + pab * XLS * c1_d(c, c->ncs + k, c->nds + l);
+      C1G(G, c->kbs + j, c->kas + i) += pab * XSL * c1_d(c, c->ncl + k, c->ndl + l) + pab * XS * c1_d(c, c->ncs + k, c->nds + l);
+    }
+    if (swap && !k34 && c->ncl <= c->ndl) C1LOOP {
+      C1G(G, c->ncl + k, c->ndl + l) += pab * XL * c1_d(c, c->kbl + j, c->kal + i) + pab * XSL * c1_d(c, c->kbs + j, c->kas + i);
+      C1G(G, c->ncs + k, c->nds + l) += pab * XLS * c1_d(c, c->kbl + j, c->kal + i) + pab * XS * c1_d(c, c->kbs + j, c->kas + i);
+    }
+  }
+  /* GDIR (AB|DC) = PCD (AB|CD) */
+  if (!(z && (c->mma != c->mmb || c->mmd != c->mmc)) && mja - mjb == mjc - mjd) {
+    if (!k34 && c->nal <= c->nbl) C1LOOP {
+      C1G(G, c->nal + i, c->nbl + j) += pcd * XL * c1_d(c, c->kdl + l, c->kcl + k) + pcd * XLS * c1_d(c, c->kds + l, c->kcs + k);
+      C1G(G, c->nas + i, c->nbs + j) += pcd * XSL * c1_d(c, c->kdl + l, c->kcl + k) + pcd * XS * c1_d(c, c->kds + l, c->kcs + k);
+    }
+    if (swap && !k12 && c->kdl <= c->kcl) C1LOOP {
+      C1G(G, c->kdl + l, c->kcl + k) += pcd * XL * c1_d(c, c->nal + i, c->nbl + j) + pcd * XSL * c1_d(c, c->nas + i, c->nbs + j);
+      C1G(G, c->kds + l, c->kcs + k) += pcd * XLS * c1_d(c, c->nal + i, c->nbl + j) + pcd * XS * c1_d(c, c->nas + i, c->nbs + j);
+    }
+  }
+  /* GDIR (BA|DC) = PAB PCD (AB|CD); the axial rule here pairs B with C and
+     D with A, as the reference writes it */
+  if (!k12 && !k34 && !(z && (c->mmb != c->mmc || c->mmd != c->mma)) && mjb - mja == mjc - mjd) {
+    if (c->kbl <= c->kal) C1LOOP {
+      C1G(G, c->kbl + j, c->kal + i) += pp * XL * c1_d(c, c->kdl + l, c->kcl + k) + pp * XLS * c1_d(c, c->kds + l, c->kcs + k);
+      C1G(G, c->kbs + j, c->kas + i) += pp * XSL * c1_d(c, c->kdl + l, c->kcl + k) + pp * XS * c1_d(c, c->kds + l, c->kcs + k);
+    }
+    if (swap && c->kdl <= c->kcl) C1LOOP {
+      C1G(G, c->kdl + l, c->kcl + k) += pp * XL * c1_d(c, c->kbl + j, c->kal + i) + pp * XSL * c1_d(c, c->kbs + j, c->kas + i);
+      C1G(G, c->kds + l, c->kcs + k) += pp * XLS * c1_d(c, c->kbl + j, c->kal + i) + pp * XS * c1_d(c, c->kbs + j, c->kas + i);
+    }
+  }
+  /* GXCH (AD|CB) */
+  if (!(z && (c->mma != c->mmd || c->mmc != c->mmb)) && mja - mjd == mjb - mjc) {
+    if (c->nal <= c->ndl) C1LOOP {
+      C1G(X, c->nal + i, c->ndl + l) += XL * c1_d(c, c->ncl + k, c->nbl + j);
+      C1G(X, c->nas + i, c->nds + l) += XS * c1_d(c, c->ncs + k, c->nbs + j);
+    }
+    if (c->nal <= c->nds) C1LOOP C1G(X, c->nal + i, c->nds + l) += XLS * c1_d(c, c->ncs + k, c->nbl + j);
+    if (swap) {
+      if (c->ncl <= c->nbl) C1LOOP {
+        C1G(X, c->ncl + k, c->nbl + j) += XL * c1_d(c, c->nal + i, c->ndl + l);
+        C1G(X, c->ncs + k, c->nbs + j) += XS * c1_d(c, c->nas + i, c->nds + l);
+      }
+      if (c->ncl <= c->nbs) C1LOOP C1G(X, c->ncl + k, c->nbs + j) += XSL * c1_d(c, c->nas + i, c->ndl + l);
+    }
+  }
+  /* GXCH (BD|CA) = PAB (AD|CB); the axial rule asks for unequal signs */
+  if (!k12 && !(z && (c->mmb == c->mmd || c->mmc == c->mma)) && -mjb - mjd == -mja - mjc) {
+    if (c->kbl <= c->ndl) C1LOOP {
+      C1G(X, c->kbl + j, c->ndl + l) += pab * XL * c1_d(c, c->ncl + k, c->kal + i);
+      C1G(X, c->kbs + j, c->nds + l) += pab * XS * c1_d(c, c->ncs + k, c->kas + i);
+    }
+    if (c->kbl <= c->nds) C1LOOP C1G(X, c->kbl + j, c->nds + l) += pab * XLS * c1_d(c, c->ncs + k, c->kal + i);
+    if (swap) {
+      if (c->ncl <= c->kal) C1LOOP {
+        C1G(X, c->ncl + k, c->kal + i) += pab * XL * c1_d(c, c->kbl + j, c->ndl + l);
+        C1G(X, c->ncs + k, c->kas + i) += pab * XS * c1_d(c, c->kbs + j, c->nds + l);
+      }
+      if (c->ncl < c->kas) C1LOOP C1G(X, c->ncl + k, c->kas + i) += pab * XSL * c1_d(c, c->kbs + j, c->ndl + l);
+    }
+  }
+  /* GXCH (AC|DB) = PCD (AD|CB) */
+  if (!k34 && !(z && (c->mma == c->mmc || c->mmd == c->mmb)) && mja + mjc == mjb + mjd) {
+    if (c->nal <= c->kcl) C1LOOP {
+      C1G(X, c->nal + i, c->kcl + k) += pcd * XL * c1_d(c, c->kdl + l, c->nbl + j);
+      C1G(X, c->nas + i, c->kcs + k) += pcd * XS * c1_d(c, c->kds + l, c->nbs + j);
+    }
+    if (c->nal <= c->kcs) C1LOOP C1G(X, c->nal + i, c->kcs + k) += pcd * XLS * c1_d(c, c->kds + l, c->nbl + j);
+    if (swap) {
+      if (c->kdl <= c->nbl) C1LOOP {
+        C1G(X, c->kdl + l, c->nbl + j) += pcd * XL * c1_d(c, c->nal + i, c->kcl + k);
+        C1G(X, c->kds + l, c->nbs + j) += pcd * XS * c1_d(c, c->nas + i, c->kcs + k);
+      }
+      if (c->kdl <= c->nbs) C1LOOP C1G(X, c->kdl + l, c->nbs + j) += pcd * XSL * c1_d(c, c->nas + i, c->kcl + k);
+    }
+  }
+  /* GXCH (BC|DA) = PAB PCD (AD|CB) */
+  if (!k12 && !k34 && !(z && (c->mmb != c->mmc || c->mmd != c->mma)) && -mjb + mjc == -mja + mjd) {
+    if (c->kbl <= c->kcl) C1LOOP {
+      C1G(X, c->kbl + j, c->kcl + k) += pp * XL * c1_d(c, c->kdl + l, c->kal + i);
+      C1G(X, c->kbs + j, c->kcs + k) += pp * XS * c1_d(c, c->kds + l, c->kas + i);
+    }
+    if (c->kbl <= c->kcs) C1LOOP C1G(X, c->kbl + j, c->kcs + k) += pp * XLS * c1_d(c, c->kds + l, c->kal + i);
+    if (swap) {
+      if (c->kdl <= c->kal) C1LOOP {
+        C1G(X, c->kdl + l, c->kal + i) += pp * XL * c1_d(c, c->kbl + j, c->kcl + k);
+        C1G(X, c->kds + l, c->kas + i) += pp * XS * c1_d(c, c->kbs + j, c->kcs + k);
+      }
+      if (c->kdl <= c->kas) C1LOOP C1G(X, c->kdl + l, c->kas + i) += pp * XSL * c1_d(c, c->kbs + j, c->kcl + k);
+    }
+  }
+}
+#undef C1G
+#undef C1LOOP
+#undef XL
+#undef XLS
+#undef XSL
+#undef XS
+
+/* COULOMB1(IZ) for one centre: adds its one-centre Coulomb contributions to
+   gdir/gxch (upper-triangle blocks) */
+static void coulomb1(orc_sys *s, int iz, const zc *D, zc *G, zc *Xm, c1_beta_store *bs) {
+  int nkap = s->nkap[iz], lmx = (nkap - 1) / 2;
+  int ikms[MKPX + 1][MKPX + 1][3];
+  int nkms = 0;
+  for (int kn = 1; kn <= nkap; ++kn)
+    for (int mj = 1; mj <= abs(KAPA(s, iz, kn)); ++mj)
+      for (int mm = 1; mm <= 2; ++mm) ikms[kn][mj][mm] = ++nkms;
+  c1_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s; c.D = D; c.G = G; c.X = Xm; c.isym = s->isym;
+  static double dkab[C1MNU][2 * MKPX + 2][2 * MKPX + 2], dkcd[C1MNU][2 * MKPX + 2][2 * MKPX + 2];
+  for (int la = 0; la <= lmx; ++la)
+  for (int lb = 0; lb <= lmx; ++lb)
+  for (int lc = 0; lc <= lmx; ++lc)
+  for (int ld = 0; ld <= lmx; ++ld) {
+    if (la > lb || lc > ld) continue;
+    if (la * (la + 1) / 2 + lb > lc * (lc + 1) / 2 + ld) continue;
+    /* TNSCLM1 25551: LQN parity and triangle rules, tensor orders */
+    if ((la + lb) % 2 != (lc + ld) % 2) continue;
+    int nui = abs(la - lb) > abs(lc - ld) ? abs(la - lb) : abs(lc - ld);
+    int nuf = la + lb < lc + ld ? la + lb : lc + ld;
+    if (nui > nuf) continue;
+    int nus[C1MNU], nunum = 0;
+    for (int nu = nui; nu <= nuf; ++nu)
+      if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) nus[nunum++] = nu;
+    if (nunum == 0) continue;
+    nui = nus[0]; nuf = nus[nunum - 1];
+    int inu[C1MNU][17];
+    memset(inu, 0, sizeof(inu));
+    const int lq[4] = {la, lb, lc, ld};
+    int nb[4];
+    for (int q = 0; q < 4; ++q) nb[q] = NFNC(s, iz, lq[q]);
+    c1_pairs ab, cd;
+    c1_pairs_set(&ab, &BEXL(s, iz, la, 0), nb[0], la, &BEXL(s, iz, lb, 0), nb[1], lb, nuf);
+    c1_pairs_set(&cd, &BEXL(s, iz, lc, 0), nb[2], lc, &BEXL(s, iz, ld, 0), nb[3], ld, nuf);
+    const int maxab = ab.n, maxcd = cd.n;
+    int lsum = la + lb + lc + ld;
+    double cc[5];
+    for (int t = 0; t < 5; ++t) cc[t] = 0.25 * GAMHLF[lsum + 2 * t + 1]; /* C1..C9 */
+    double *rj = calloc(4L * maxcd * nunum, sizeof(double));
+    double *xs = calloc(4L * maxcd, sizeof(double));
+    for (int na = 1; na >= (la == 0); --na)
+    for (int nb_ = 1; nb_ >= (lb == 0); --nb_)
+    for (int nc = 1; nc >= (lc == 0); --nc)
+    for (int nd = 1; nd >= (ld == 0); --nd) {
+      int ka = 2 * la + na, kb = 2 * lb + nb_, kc = 2 * lc + nc, kd = 2 * ld + nd;
+      if (ka > kb || kc > kd) continue;
+      int ikl = ka * (ka + 1) / 2 + kb, ikr = kc * (kc + 1) / 2 + kd;
+      if (ikl > ikr) continue;
+      int k4ad = 16 - (8 * na + 4 * nb_ + 2 * nc + nd);
+      int kq[4] = {KAPA(s, iz, ka), KAPA(s, iz, kb), KAPA(s, iz, kc), KAPA(s, iz, kd)};
+      double tq[4];
+      for (int q = 0; q < 4; ++q) tq[q] = (double)(lq[q] + kq[q] + 1);
+      /* ANGCLM1 25865: KQN triangle and parity rules, DK tables */
+      int jq[4];
+      for (int q = 0; q < 4; ++q) jq[q] = 2 * abs(kq[q]) - 1;
+      int nuit = abs(jq[0] - jq[1]) / 2 > abs(jq[2] - jq[3]) / 2 ? abs(jq[0] - jq[1]) / 2
+                                                                  : abs(jq[2] - jq[3]) / 2;
+      int nuft = (jq[0] + jq[1]) / 2 < (jq[2] + jq[3]) / 2 ? (jq[0] + jq[1]) / 2
+                                                            : (jq[2] + jq[3]) / 2;
+      if (nuit > nuft) continue;
+      int nulng = 0;
+      for (int nu = nuit; nu <= nuft; ++nu)
+        if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) {
+          int kf = 0;
+          for (int t = 0; t < nunum; ++t) if (nus[t] == nu) { kf = t + 1; break; }
+          if (kf) inu[kf - 1][k4ad] = 1;
+          ++nulng;
+        }
+      if (nulng == 0) continue;
+      for (int ma = 1; ma <= abs(kq[0]); ++ma)
+        for (int mb = 1; mb <= abs(kq[1]); ++mb)
+          for (int t = 0; t < nunum; ++t) {
+            if (!inu[t][k4ad]) continue;
+            int mja = 2 * ma - 1, mjb = 2 * mb - 1, nu = nus[t];
+            dkab[t][mja][mjb] = dk_coef(jq[0], -mja, jq[1], -mjb, nu);
+            dkab[t][mja + 1][mjb] = dk_coef(jq[0], mja, jq[1], -mjb, nu);
+            dkab[t][mja][mjb + 1] = dk_coef(jq[0], -mja, jq[1], mjb, nu);
+            dkab[t][mja + 1][mjb + 1] = dk_coef(jq[0], mja, jq[1], mjb, nu);
+          }
+      for (int mc = 1; mc <= abs(kq[2]); ++mc)
+        for (int md = 1; md <= abs(kq[3]); ++md)
+          for (int t = 0; t < nunum; ++t) {
+            if (!inu[t][k4ad]) continue;
+            int mjc = 2 * mc - 1, mjd = 2 * md - 1, nu = nus[t];
+            dkcd[t][mjc][mjd] = dk_coef(jq[3], -mjd, jq[2], -mjc, nu);
+            dkcd[t][mjc + 1][mjd] = dk_coef(jq[3], -mjd, jq[2], mjc, nu);
+            dkcd[t][mjc][mjd + 1] = dk_coef(jq[3], mjd, jq[2], -mjc, nu);
+            dkcd[t][mjc + 1][mjd + 1] = dk_coef(jq[3], mjd, jq[2], mjc, nu);
+          }
+      for (int q = 0; q < 4; ++q) { c.kqn[q] = kq[q]; c.nbas[q] = nb[q]; }
+      c.ikl = ikl; c.ikr = ikr;
+      for (int ij = 0; ij < maxab; ++ij) {
+        c.ibas = ij / nb[1];
+        c.jbas = ij % nb[1];
+        double tij0 = ab.ei[ij] + ab.ej[ij];
+        /* BTACLM1 for the block's first KQN combination only (21842) */
+        if (k4ad == 1) {
+          int nvals = (nuf - nui) / 2 + 3;
+          for (int nx = 1; nx <= nvals; ++nx)
+            for (int ny = 1; ny <= nvals; ++ny)
+              for (int ib = 0; ib < 2; ++ib) {
+                int iaa = ib == 0 ? la + lb + nui + 2 * nx - 1 : lc + ld + nui + 2 * nx - 1;
+                int ibb = ib == 0 ? lc + ld - nuf + 2 * ny - 2 : la + lb - nuf + 2 * ny - 2;
+                double *dst = (ib == 0 ? bs->bdu : bs->bdl) + ((long)nx * C1MNU + ny) * bs->cap +
+                              (long)ij * maxcd;
+                for (int m = 0; m < maxcd; ++m) {
+                  double tkl0 = cd.ei[m] + cd.ej[m];
+                  double xj = (ib == 0 ? tij0 : tkl0) / (tij0 + tkl0);
+                  dst[m] = c1_beta(xj, (iaa - 1) / 2, ibb / 2);
+                }
+              }
+        }
+        /* RKCLM1 25664: radial integrals per tensor order */
+        memset(rj, 0, sizeof(double) * 4 * maxcd * nunum);
+        const double *eij = ab.ep + (long)ij * (2 * C1MNU + 6) + C1MNU;
+        double e_i = ab.ei[ij], e_j = ab.ej[ij];
+        for (int m = 0; m < maxcd; ++m) {
+          const double *ekl = cd.ep + (long)m * (2 * C1MNU + 6) + C1MNU;
+          double e_k = cd.ei[m], e_l = cd.ej[m];
+          long f = (long)ij * maxcd + m;
+          for (int t = 0; t < nunum; ++t) {
+            if (!inu[t][k4ad]) continue;
+            int nten = nunum - (t + 1), nu = nus[t], lt = t + 1;
+#define BU(a, b) bs->bdu[((long)(a) * C1MNU + (b)) * bs->cap + f]
+#define BL(a, b) bs->bdl[((long)(a) * C1MNU + (b)) * bs->cap + f]
+            double b10 = eij[-nu] * ekl[nu + 1] * BL(lt, nten + 1) + eij[nu + 1] * ekl[-nu] * BU(lt, nten + 1);
+            double b12 = eij[-nu + 2] * ekl[nu + 1] * BL(lt, nten + 2) + eij[nu + 3] * ekl[-nu] * BU(lt + 1, nten + 1);
+            double b14 = eij[-nu + 4] * ekl[nu + 1] * BL(lt, nten + 3) + eij[nu + 5] * ekl[-nu] * BU(lt + 2, nten + 1);
+            double b30 = eij[-nu] * ekl[nu + 3] * BL(lt + 1, nten + 1) + eij[nu + 1] * ekl[-nu + 2] * BU(lt, nten + 2);
+            double b32 = eij[-nu + 2] * ekl[nu + 3] * BL(lt + 1, nten + 2) + eij[nu + 3] * ekl[-nu + 2] * BU(lt + 1, nten + 2);
+            double b34 = eij[-nu + 4] * ekl[nu + 3] * BL(lt + 1, nten + 3) + eij[nu + 5] * ekl[-nu + 2] * BU(lt + 2, nten + 2);
+            double b50 = eij[-nu] * ekl[nu + 5] * BL(lt + 2, nten + 1) + eij[nu + 1] * ekl[-nu + 4] * BU(lt, nten + 3);
+            double b52 = eij[-nu + 2] * ekl[nu + 5] * BL(lt + 2, nten + 2) + eij[nu + 3] * ekl[-nu + 4] * BU(lt + 1, nten + 3);
+            double b54 = eij[-nu + 4] * ekl[nu + 5] * BL(lt + 2, nten + 3) + eij[nu + 5] * ekl[-nu + 4] * BU(lt + 2, nten + 3);
+#undef BU
+#undef BL
+            const double ti = tq[0], tj = tq[1], tk = tq[2], tl = tq[3];
+            double *r = rj + (long)t * 4 * maxcd;
+            r[m] = cc[2] * b32;
+            r[maxcd + m] = 4.0 * e_k * e_l * cc[3] * b52 - 2.0 * tl * e_k * cc[2] * b32 -
+                           2.0 * tk * e_l * cc[2] * b32 + tk * tl * cc[1] * b12;
+            r[2 * maxcd + m] = 4.0 * e_i * e_j * cc[3] * b34 - 2.0 * tj * e_i * cc[2] * b32 -
+                               2.0 * ti * e_j * cc[2] * b32 + ti * tj * cc[1] * b30;
+            r[3 * maxcd + m] =
+                16.0 * e_i * e_j * e_k * e_l * cc[4] * b54 - 8.0 * tl * e_i * e_j * e_k * cc[3] * b34 -
+                8.0 * tk * e_i * e_j * e_l * cc[3] * b34 - 8.0 * tj * e_i * e_k * e_l * cc[3] * b52 -
+                8.0 * ti * e_j * e_k * e_l * cc[3] * b52 + 4.0 * ti * tj * e_k * e_l * cc[2] * b50 +
+                4.0 * tk * tl * e_i * e_j * cc[2] * b14 + 4.0 * ti * tl * e_j * e_k * cc[2] * b32 +
+                4.0 * tj * tk * e_i * e_l * cc[2] * b32 + 4.0 * tj * tl * e_i * e_k * cc[2] * b32 +
+                4.0 * ti * tk * e_j * e_l * cc[2] * b32 - 2.0 * ti * tj * tl * e_k * cc[1] * b30 -
+                2.0 * tj * tk * tl * e_i * cc[1] * b12 - 2.0 * ti * tj * tk * e_l * cc[1] * b30 -
+                2.0 * ti * tk * tl * e_j * cc[1] * b12 + ti * tj * tk * tl * cc[0] * b10;
+            const double *ra = ab.rn + 4L * ij, *rc = cd.rn + 4L * m;
+            r[m] *= ra[0] * rc[0];
+            r[maxcd + m] *= ra[0] * rc[3];
+            r[2 * maxcd + m] *= ra[3] * rc[0];
+            r[3 * maxcd + m] *= ra[3] * rc[3];
+          }
+        }
+        /* MQN magnitudes and signs (21900-22060) */
+        for (int ma = 1; ma <= abs(kq[0]); ++ma)
+        for (int mb = 1; mb <= abs(kq[1]); ++mb)
+        for (int mc = 1; mc <= abs(kq[2]); ++mc)
+        for (int md = 1; md <= abs(kq[3]); ++md) {
+          int mq[4] = {2 * ma - 1, 2 * mb - 1, 2 * mc - 1, 2 * md - 1};
+          if (s->isym == 1 || s->isym == 2) {
+            if (!((mq[0] == mq[1] && mq[2] == mq[3]) || (mq[0] == mq[2] && mq[1] == mq[3]) ||
+                  (mq[0] == mq[3] && mq[1] == mq[2])))
+              continue;
+          }
+          for (int mma = 1; mma <= 2; ++mma)
+          for (int mmb = 1; mmb <= 2; ++mmb)
+          for (int mmc = 1; mmc <= 2; ++mmc)
+          for (int mmd = 1; mmd <= 2; ++mmd) {
+            int mja = mma == 1 ? -mq[0] : mq[0], mjb = mmb == 1 ? -mq[1] : mq[1];
+            int mjc = mmc == 1 ? -mq[2] : mq[2], mjd = mmd == 1 ? -mq[3] : mq[3];
+            if (ma == mb && ikms[ka][ma][mma] > ikms[kb][mb][mmb]) continue;
+            if (mc == md && ikms[kc][mc][mmc] > ikms[kd][md][mmd]) continue;
+            if ((s->isym == 1 || s->isym == 2) && mja - mjb != mjd - mjc) continue;
+            for (int q = 0; q < 4; ++q) c.mqn[q] = mq[q];
+            c.mma = mma; c.mmb = mmb; c.mmc = mmc; c.mmd = mmd;
+            c.pab = isgn(kq[0] * kq[1]) * m1pow((mja - mjb) / 2);
+            c.pcd = isgn(kq[2] * kq[3]) * m1pow((mjc - mjd) / 2);
+            c.nal = LRGE(s, iz, ka, 2 * ma - 2 + mma); c.kal = LRGE(s, iz, ka, 2 * ma + 1 - mma);
+            c.nbl = LRGE(s, iz, kb, 2 * mb - 2 + mmb); c.kbl = LRGE(s, iz, kb, 2 * mb + 1 - mmb);
+            c.ncl = LRGE(s, iz, kc, 2 * mc - 2 + mmc); c.kcl = LRGE(s, iz, kc, 2 * mc + 1 - mmc);
+            c.ndl = LRGE(s, iz, kd, 2 * md - 2 + mmd); c.kdl = LRGE(s, iz, kd, 2 * md + 1 - mmd);
+            c.nas = c.nal + s->nskp; c.nbs = c.nbl + s->nskp; c.ncs = c.ncl + s->nskp; c.nds = c.ndl + s->nskp;
+            c.kas = c.kal + s->nskp; c.kbs = c.kbl + s->nskp; c.kcs = c.kcl + s->nskp; c.kds = c.kdl + s->nskp;
+            /* effective interaction strengths, angular screening SENS = 1e-10 */
+            memset(xs, 0, sizeof(double) * 4 * maxcd);
+            int imja = mq[0] + mma - 1, imjb = mq[1] + mmb - 1, imjc = mq[2] + mmc - 1, imjd = mq[3] + mmd - 1;
+            for (int t = 0; t < nunum; ++t) {
+              if (!inu[t][k4ad]) continue;
+              double angfac = dkab[t][imja][imjb] * dkcd[t][imjc][imjd];
+              if (fabs(angfac) <= 1.0e-10) continue;
+              const double *r = rj + (long)t * 4 * maxcd;
+              for (int w = 0; w < 4 * maxcd; ++w) xs[w] += angfac * r[w];
+            }
+            for (int q = 0; q < 4; ++q) c.x[q] = xs + (long)q * maxcd;
+            c1_digest(&c);
+          }
+        }
+      }
+    }
+    free(rj); free(xs);
+    c1_pairs_free(&ab); c1_pairs_free(&cd);
+  }
+}
+
+/* every centre's COULOMB1 after a CLMFAST build (HFSCF 5415-5421, RACAH1):
+   adds the one-centre contributions and, as COULOMB1 ends (22074-22081),
+   overwrites the lower triangles with the conjugate upper ones */
+void orc_coulomb1(orc_sys *s, const double *dent, double *gdir, double *gxch) {
+  init_tables();
+  int ndim = s->ndim;
+  c1_beta_store bs;
+  long cap = 0;
+  for (int iz = 0; iz < s->ncnt; ++iz)
+    for (int l1 = 0; l1 <= s->mel; ++l1)
+      for (int l2 = 0; l2 <= s->mel; ++l2) {
+        long n = (long)NFNC(s, iz, l1) * NFNC(s, iz, l2);
+        long nn = 0;
+        for (int l3 = 0; l3 <= s->mel; ++l3)
+          for (int l4 = 0; l4 <= s->mel; ++l4)
+            if ((long)NFNC(s, iz, l3) * NFNC(s, iz, l4) > nn) nn = (long)NFNC(s, iz, l3) * NFNC(s, iz, l4);
+        if (n * nn > cap) cap = n * nn;
+      }
+  bs.cap = cap;
+  bs.bdu = calloc((long)C1MNU * C1MNU * cap, sizeof(double));
+  bs.bdl = calloc((long)C1MNU * C1MNU * cap, sizeof(double));
+  zc *G = (zc *)gdir, *X = (zc *)gxch;
+  for (int iz = 0; iz < s->ncnt; ++iz) coulomb1(s, iz, (const zc *)dent, G, X, &bs);
+  for (int i = 0; i < ndim; ++i)
+    for (int j = i + 1; j < ndim; ++j) {
+      G[(long)i * ndim + j] = conj(G[(long)j * ndim + i]);
+      X[(long)i * ndim + j] = conj(X[(long)j * ndim + i]);
+    }
+  free(bs.bdu); free(bs.bdl);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,13 @@ int orc_erifast(orc_sys *s, const int icnt[4], const int kidx[4],
 void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
                  double *gxch);
 
+/* COULOMB1 (bertha-scf.f:21540) for every centre, as HFSCF calls it after
+   CLMFAST with RACAH1 on (5415-5421): adds the one-centre Racah-algebra
+   Coulomb contributions to gdir/gxch (a CLMFAST result of an oracle created
+   with racah1 = 1) and overwrites their lower triangles with the conjugate
+   upper ones, as COULOMB1 does on return. */
+void orc_coulomb1(orc_sys *s, const double *dent, double *gdir, double *gxch);
+
 /* same but with OpenMP over the outer centre-pair loop when nthreads > 1
    (digestion goes to thread-private matrices that are summed at the end) */
 void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,

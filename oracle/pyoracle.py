@@ -46,6 +46,7 @@ def _oracle_lib():
         lib.orc_erifast.argtypes = [P, P, P, P, C.c_int, C.c_int, P, C.c_int, P, P]
         lib.orc_clmfast.argtypes = [P, C.c_int, P, P, P]
         lib.orc_clmfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
+        lib.orc_coulomb1.argtypes = [P, P, P, P]
         lib.orc_brtfast_sample.restype = C.c_double
         lib.orc_brtfast_sample.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_int]
         lib.orc_clmfast_sample.restype = C.c_double
@@ -238,6 +239,21 @@ class Oracle:
         else:
             _oracle_lib().orc_clmfast(self._h, ilev, d.ctypes.data, g.ctypes.data, x.ctypes.data)
         return g, x
+
+    def coulomb1(self, dent, g, x):
+        """COULOMB1 for every centre on top of a CLMFAST result g, x (built by
+        an Oracle with racah1=True); returns the completed matrices."""
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        g = np.array(g, dtype=np.complex128, order="F")
+        x = np.array(x, dtype=np.complex128, order="F")
+        _oracle_lib().orc_coulomb1(self._h, d.ctypes.data, g.ctypes.data, x.ctypes.data)
+        return g, x
+
+    def clmfast_racah(self, dent, ilev=3, nthreads=1):
+        """HFSCF's Coulomb build with RACAH1 on: CLMFAST without one-centre
+        quartets, then COULOMB1 per centre (construct with racah1=True)."""
+        g, x = self.clmfast(dent, ilev, nthreads)
+        return self.coulomb1(dent, g, x)
 
 
 _rlib = None

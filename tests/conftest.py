@@ -64,5 +64,31 @@ def small_systems(bdf):
                       [ev(0.08, 3.0, 2), ev(0.3, 2.5, 1)],
                       [ev(0.1, 2.8, 3), ev(0.2, 2.6, 1)],
                       [ev(0.08, 3.0, 2)]], shape="NOSYMM")
+    # single atoms with d and f shells: the one-centre Racah path's LQN blocks
+    # up to (ff|ff), including those that read another block's beta integrals
+    datom = S.System("d-atom", np.zeros((1, 3)),
+                     [[ev(0.1, 2.7, 6), ev(0.15, 2.6, 4), ev(0.3, 2.5, 3)]], shape="ATOMIC")
+    fatom = S.System("f-atom", np.zeros((1, 3)),
+                     [[ev(0.2, 3.0, 3), ev(0.3, 3.0, 2), ev(0.4, 3.0, 2), ev(0.5, 3.0, 2)]],
+                     shape="ATOMIC")
     return {"ne": ne, "diat": diat, "off": off, "tri": tri, "fsh": fsh, "fdiat": fdiat,
-            "quad": quad, "ring4": ring4}
+            "quad": quad, "ring4": ring4, "datom": datom, "fatom": fatom}
+
+
+def sd_axial_density(labels, ndim, seed=1):
+    """A real symmetric density coupling only s and d functions of equal m on
+    each component (the form an atom polarised along its axis has): its
+    one-centre Coulomb contributions reach the (pp|sd) LQN block, whose beta
+    integrals COULOMB1 reads from the block before it."""
+    icn, kqn, mqn = labels
+    ns = ndim // 2
+    lq = [k if k > 0 else -k - 1 for k in kqn]
+    rng = np.random.default_rng(seed)
+    d = np.zeros((ndim, ndim))
+    for comp in (0, ns):
+        for i in range(ns):
+            for j in range(ns):
+                if lq[i] == 0 and lq[j] == 2 and mqn[i] == mqn[j] and icn[i] == icn[j]:
+                    v = rng.normal()
+                    d[comp + i, comp + j] = d[comp + j, comp + i] = v
+    return d
