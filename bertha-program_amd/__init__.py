@@ -168,6 +168,7 @@ class Engine:
                  workgroups: int = 0, tuned: bool = True, graph: bool = False):
         lib = library()
         self.system = system
+        self.racah1 = bool(racah1)
         xyz, lmax, nfnc, exps, lstride, mbs = system.arrays()
         self._keep = (xyz, lmax, nfnc, exps)
         s = _System(system.ncnt, xyz.ctypes.data_as(_C.POINTER(_C.c_double)),

@@ -242,7 +242,12 @@ def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, wfn_dir=N
     With wfn_dir, the wavefunction file of the deck is used as the reference
     uses it: a RESUME deck (READIN) whose file exists starts from the density
     of its spinors (stage 3 from the first iteration, bertha-scf.f:534-536,
-    597-604), and the converged eigenvalues and spinors are written back."""
+    597-604), and the converged eigenvalues and spinors are written back.
+
+    racah1=True (engine_kw) runs the reference's default one-centre path:
+    CLMFAST without one-centre quartets plus COULOMB1 (bdf_coulomb1) in every
+    Coulomb build; DHFB decks need racah1=False (no BREIT1), and DHFP's Breit
+    build takes its one-centre terms from MD."""
     import torch
 
     from . import scf as _scf
@@ -264,7 +269,8 @@ def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, wfn_dir=N
     if deck.hmlt != "DHFP":
         return res, None
     sb = dataclasses.replace(deck.system, hmlt="DHFB")
-    bb = _scf.EngineBackend(pkg, sb, **engine_kw)
+    # one-centre Breit terms by MD: the engine has no BREIT1
+    bb = _scf.EngineBackend(pkg, sb, **dict(engine_kw, racah1=False))
     try:
         bdir, bxch = bb.breit(res.density, 3)
         e = 0.5 * float(torch.sum(res.density * (bdir - bxch)).real)

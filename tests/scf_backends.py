@@ -6,10 +6,11 @@ import torch
 
 
 class OracleBackend:
-    def __init__(self, pkg, orc, system, nthreads: int = 8):
+    def __init__(self, pkg, orc, system, nthreads: int = 8, racah1: bool = False):
         self.pkg = pkg
         self.system = system
-        self.o = orc.Oracle(system)
+        self.racah1 = racah1
+        self.o = orc.Oracle(system, racah1=racah1)
         self.n, self.nskp = self.o.ndim, self.o.nskp
         self.dev = torch.device("cpu")
         self.nthreads = nthreads
@@ -21,7 +22,10 @@ class OracleBackend:
         return [torch.from_numpy(np.ascontiguousarray(x)) for x in mats]
 
     def coulomb(self, dens, ilev):
-        g, x = self.o.clmfast(dens.numpy(), ilev, nthreads=self.nthreads)
+        if self.racah1:  # CLMFAST without one-centre quartets, then COULOMB1
+            g, x = self.o.clmfast_racah(dens.numpy(), ilev, nthreads=self.nthreads)
+        else:
+            g, x = self.o.clmfast(dens.numpy(), ilev, nthreads=self.nthreads)
         return torch.from_numpy(np.ascontiguousarray(g)), torch.from_numpy(np.ascontiguousarray(x))
 
     def breit(self, dens, ilev):

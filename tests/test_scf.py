@@ -99,6 +99,19 @@ def test_ne_scf_on_oracle(bdf, orc, scf):
     assert np.abs(eig - ref).max() < 0.05
 
 
+def test_ne_scf_with_racah_one_centre_terms(bdf, orc, scf):
+    """With RACAH1 (the reference's default) every Ne quartet is one-centre
+    and comes from COULOMB1; for the closed-shell atom's spherical density
+    those equal the McMurchie-Davidson integrals, so the SCF lands on the
+    same energy."""
+    s = bdf.systems.ne_atom()
+    r = scf.run_scf(s, 10, OracleBackend(bdf, orc, s, racah1=True))
+    assert r.converged
+    assert abs(r.energy - (-128.6586819)) < 1e-6
+    r0 = scf.run_scf(s, 10, OracleBackend(bdf, orc, s))
+    assert abs(r.energy - r0.energy) < 1e-9, (r.energy, r0.energy)
+
+
 def test_converged_state_is_stationary(bdf, orc, scf):
     """At convergence the Fock matrix built from the final density commutes
     with it (the DIIS error F P S - S P F, P = conj(D), bertha-scf.f:3883)
@@ -199,10 +212,11 @@ def test_superposed_density_places_atom_blocks(bdf, orc, scf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("hmlt", ["DHFR", "DHFB"])
+@pytest.mark.parametrize("hmlt", ["DHFR", "DHFB", "DHFR-racah"])
 def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
     print(f"[scf] {hmlt}", flush=True)
-    if hmlt == "DHFR":
+    racah1 = hmlt == "DHFR-racah"
+    if hmlt.startswith("DHFR"):
         s = bdf.systems.ne_atom()
         nocc = 10
     else:
@@ -213,8 +227,8 @@ def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
         s = bdf.systems.System("h-h", base.xyz, base.shells, base.shape, hmlt="DHFB",
                                charges=[1.0, 1.0])
         nocc = 2
-    ro = scf.run_scf(s, nocc, OracleBackend(bdf, orc, s))
-    be = scf.EngineBackend(bdf, s)
+    ro = scf.run_scf(s, nocc, OracleBackend(bdf, orc, s, racah1=racah1))
+    be = scf.EngineBackend(bdf, s, racah1=racah1)
     try:
         rg = scf.run_scf(s, nocc, be)
     finally:

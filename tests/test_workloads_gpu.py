@@ -181,3 +181,27 @@ def test_serving_workload_same_alone_and_in_mix(bdf):
         assert abs(a.energy - m.energy) < 1e-8, (a.name, a.energy, m.energy)
     assert sum(1 for m in mix if m.converged) == len(mix)
     assert any(r.system.ncnt == 2 for r in reqs) and any(r.system.ncnt == 1 for r in reqs)
+
+
+def test_tlf_racah_matches_oracle_samples(bdf, tlf_gold, tlf_density):
+    """bdf_coulomb1 on the heavy diatomic (Tl f shells, NDIM 1340) from zero
+    matrices: the one-centre contributions alone, against the oracle's at
+    the sampled elements (tests/golden/make_tlf_racah.py), for the
+    superposed atomic density and a random one."""
+    z = np.load(os.path.join(HERE, "golden", "tlf_racah.npz"))
+    s = bdf.systems.config_system("tlf", seed=0, hmlt="DHFR")
+    n = s.ndim
+    i, j = tlf_gold["idx"][:, 0], tlf_gold["idx"][:, 1]
+    zero = np.zeros((n, n), dtype=np.complex128)
+    dens = {"atoms": tlf_density,
+            "random": bdf.systems.random_density(n, seed=int(z["random_seed"]), real=True)}
+    with bdf.Engine(s, racah1=True) as eng:
+        for tag, d in dens.items():
+            print(f"[tlf racah] {tag}", flush=True)
+            g, x = eng.coulomb1(d, zero, zero)
+            tol = FOCK_ABS * max(1.0, 1e-3 * np.abs(g).max())
+            assert np.abs(g[i, j] - z[f"{tag}_g"]).max() < tol, tag
+            assert np.abs(x[i, j] - z[f"{tag}_x"]).max() < tol, tag
+            ej, ek = 0.5 * np.real(np.sum(d * g)), 0.5 * np.real(np.sum(d * x))
+            assert abs(ej - z[f"{tag}_ej"]) < 1e-12 * max(1.0, abs(z[f"{tag}_ej"])), tag
+            assert abs(ek - z[f"{tag}_ek"]) < 1e-12 * max(1.0, abs(z[f"{tag}_ek"])), tag
