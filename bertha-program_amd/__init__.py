@@ -108,8 +108,10 @@ def library() -> _C.CDLL:
         lib.bdf_coulomb_device_static.argtypes = [P, _C.c_int, P, P, P]
         lib.bdf_coulomb1.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
         lib.bdf_coulomb1_device.argtypes = [P, _C.c_int, P, P, P]
-        lib.bdf_racah_info.argtypes = [P, _C.POINTER(_C.c_double), _C.POINTER(_C.c_long),
-                                       _C.POINTER(_C.c_long)]
+        lib.bdf_breit1.argtypes = [P, _C.c_int, P, P, _C.c_int, P, P, P, P]
+        lib.bdf_breit1_device.argtypes = [P, _C.c_int, P, P, P]
+        lib.bdf_racah_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_double),
+                                       _C.POINTER(_C.c_long), _C.POINTER(_C.c_long)]
         lib.bdf_eri_batch.argtypes = [P, P, P, P, _C.c_int, _C.c_int, P, _C.c_int, P, _C.c_int]
         lib.bdf_etable.argtypes = [P, _C.c_int, _C.c_int, _C.c_int, _C.c_int, _C.c_int,
                                    _C.c_int, _C.c_int, P, _C.c_long, _C.POINTER(_C.c_long)]
@@ -267,11 +269,38 @@ class Engine:
         g, x = self.clmfast(dent, ilev)
         return self.coulomb1(dent, g, x, exact_beta)
 
-    def racah_info(self):
+    def breit1(self, dent: _np.ndarray, bdir: _np.ndarray, bxch: _np.ndarray,
+               exact_beta: bool = False):
+        """bdf_breit1: every centre's BREIT1 on top of a BRTFAST result
+        (bdir, bxch of an engine with racah1=True); returns the matrices."""
+        n = self.ndim
+        d = _np.asfortranarray(dent, dtype=_np.complex128)
+        b = _np.array(bdir, dtype=_np.complex128, order="F")
+        x = _np.array(bxch, dtype=_np.complex128, order="F")
+        if d.shape != (n, n) or b.shape != (n, n) or x.shape != (n, n):
+            raise ValueError(f"matrices must be {n}x{n}")
+        _check(library().bdf_breit1(self._h, 1 if exact_beta else 0, d.ctypes.data, None, n,
+                                    b.ctypes.data, x.ctypes.data, None, None))
+        return b, x
+
+    def breit1_device(self, d_dent: int, d_bdir: int, d_bxch: int,
+                      exact_beta: bool = False) -> None:
+        _check(library().bdf_breit1_device(self._h, 1 if exact_beta else 0, d_dent, d_bdir,
+                                           d_bxch))
+
+    def breit_racah(self, dent: _np.ndarray, ilev: int = 3, exact_beta: bool = False):
+        """HFSCF's Breit build with RACAH1 on: BRTFAST without one-centre
+        quartets, then BREIT1 for every centre (engine built with
+        racah1=True)."""
+        b, x = self.breit(dent, ilev)
+        return self.breit1(dent, b, x, exact_beta)
+
+    def racah_info(self, breit: bool = False):
         """(digestion terms per build, work items, uncovered beta reads) of
-        the one-centre Racah build."""
+        the one-centre Racah build of the Coulomb or Breit interaction."""
         t, i, u = _C.c_double(), _C.c_long(), _C.c_long()
-        _check(library().bdf_racah_info(self._h, _C.byref(t), _C.byref(i), _C.byref(u)))
+        _check(library().bdf_racah_info(self._h, 1 if breit else 0, _C.byref(t), _C.byref(i),
+                                        _C.byref(u)))
         return t.value, i.value, u.value
 
     def erifast(self, icnt, kidx, midx, ibas: int, jbas: int, itn, ilev: int = 3):

@@ -88,6 +88,7 @@ struct DevRacah {
   DevBuf<RcSeg> segs;
   DevBuf<RcCombo> combos;
   DevBuf<RcEntry> entries;
+  DevBuf<double> ang;
   DevBuf<RcOp> ops;
   DevBuf<RcItem> items;
 };
@@ -139,7 +140,7 @@ struct bdf_engine {
   int breit = 0;  // DHFB: LS tables and the Breit plans
   int gaunt = 0;  // GAUNT2
   DevBreitPlan bplans[3];
-  DevRacah racah;  // COULOMB1 replacement (bdf_coulomb1)
+  DevRacah racah[2];  // COULOMB1 (bdf_coulomb1) and BREIT1 (bdf_breit1) replacements
   DevBuf<float4> E32;
   DevBuf<double4> Ecd;  // masked, signed E(CD) of the second contraction
   DevBuf<double4> Eab;  // E with EPSAB applied: the Breit kernel's AB side
@@ -272,12 +273,12 @@ static int make_plan(bdf_engine *e, int ilev) {
   return BDF_OK;
 }
 
-static int make_racah(bdf_engine *e) {
-  DevRacah &r = e->racah;
+static int make_racah(bdf_engine *e, int breit) {
+  DevRacah &r = e->racah[breit];
   if (r.built) return BDF_OK;
   std::string err;
   try {
-    err = build_racah_plan(e->B, e->isym, r.P);
+    err = build_racah_plan(e->B, e->isym, breit, r.P);
   } catch (const std::bad_alloc &) {
     err = "racah plan: out of host memory";
   }
@@ -286,6 +287,7 @@ static int make_racah(bdf_engine *e) {
   HIPCHK(r.segs.upload(r.P.segs, e->stream));
   HIPCHK(r.combos.upload(r.P.combos, e->stream));
   HIPCHK(r.entries.upload(r.P.entries, e->stream));
+  HIPCHK(r.ang.upload(r.P.ang, e->stream));
   HIPCHK(r.ops.upload(r.P.ops, e->stream));
   HIPCHK(r.items.upload(r.P.items, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
@@ -847,15 +849,17 @@ int bdf_coulomb(bdf_engine *e, int ilev, const double _Complex *dent, const doub
   return BDF_OK;
 }
 
-int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent, void *d_gdir,
-                        void *d_gxch) {
-  if (!e || !d_dent || !d_gdir || !d_gxch) return fail(BDF_EINVAL, "bdf_coulomb1_device: null argument");
+// one-centre Racah build of the Coulomb (breit 0) or Breit (1) interaction
+static int racah_device(bdf_engine *e, int breit, int exact_beta, const void *d_dent,
+                        void *d_gdir, void *d_gxch, const char *name) {
+  if (!e || !d_dent || !d_gxch || (!breit && !d_gdir))
+    return fail(BDF_EINVAL, std::string(name) + ": null argument");
   if (exact_beta != 0 && exact_beta != 1)
-    return fail(BDF_EINVAL, "bdf_coulomb1_device: exact_beta must be 0 or 1");
+    return fail(BDF_EINVAL, std::string(name) + ": exact_beta must be 0 or 1");
   HIPCHK(hipSetDevice(e->device));
-  int rc = make_racah(e);
+  int rc = make_racah(e, breit);
   if (rc != BDF_OK) return rc;
-  const DevRacah &r = e->racah;
+  const DevRacah &r = e->racah[breit];
   RacahArgs a;
   a.blocks = r.blocks.p;
   a.segs = r.segs.p;
@@ -863,6 +867,7 @@ int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent, void 
   a.entries = r.entries.p;
   a.ops = r.ops.p;
   a.items = r.items.p;
+  a.ang = r.ang.p;
   a.nitems = (int)r.P.items.size();
   a.exps = e->exps.p;
   a.normL = e->normL.p;
@@ -873,10 +878,21 @@ int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent, void 
   a.ndim = e->B.ndim;
   a.isym = e->isym;
   a.exact = exact_beta;
+  a.breit = breit;
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   HIPCHK(launch_racah(a, e->stream));
   HIPCHK(hipEventRecord(e->ev1, e->stream));
   return BDF_OK;
+}
+
+int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent, void *d_gdir,
+                        void *d_gxch) {
+  return racah_device(e, 0, exact_beta, d_dent, d_gdir, d_gxch, "bdf_coulomb1_device");
+}
+
+int bdf_breit1_device(bdf_engine *e, int exact_beta, const void *d_dent, void *d_bdir,
+                      void *d_bxch) {
+  return racah_device(e, 1, exact_beta, d_dent, d_bdir, d_bxch, "bdf_breit1_device");
 }
 
 int bdf_coulomb1(bdf_engine *e, int exact_beta, const double _Complex *dent,
@@ -902,14 +918,36 @@ int bdf_coulomb1(bdf_engine *e, int exact_beta, const double _Complex *dent,
   return BDF_OK;
 }
 
-int bdf_racah_info(bdf_engine *e, double *terms, long *items, long *uncovered) {
-  if (!e) return fail(BDF_EINVAL, "bdf_racah_info: null engine");
+int bdf_breit1(bdf_engine *e, int exact_beta, const double _Complex *dent,
+               const double _Complex *deno, int ld, double _Complex *bdir, double _Complex *bxch,
+               double _Complex *wdir, double _Complex *wxch) {
+  if (!e || !dent || !bdir || !bxch) return fail(BDF_EINVAL, "bdf_breit1: null argument");
+  if (deno || wdir || wxch) return fail(BDF_EUNSUPP, "bdf_breit1: open shells are not supported");
+  int nd = e->B.ndim;
+  if (ld < nd) return fail(BDF_EINVAL, "bdf_breit1: ld < ndim");
   HIPCHK(hipSetDevice(e->device));
-  int rc = make_racah(e);
+  HIPCHK(e->dent.alloc((size_t)nd * nd));
+  HIPCHK(e->gxch.alloc((size_t)nd * nd));
+  const size_t row = nd * sizeof(double2), lrow = ld * sizeof(double2);
+  HIPCHK(hipMemcpy2DAsync(e->dent.p, row, dent, lrow, row, nd, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpy2DAsync(e->gxch.p, row, bxch, lrow, row, nd, hipMemcpyHostToDevice, e->stream));
+  int rc = bdf_breit1_device(e, exact_beta, e->dent.p, nullptr, e->gxch.p);
   if (rc != BDF_OK) return rc;
-  if (terms) *terms = e->racah.P.terms;
-  if (items) *items = (long)e->racah.P.items.size();
-  if (uncovered) *uncovered = (long)e->racah.P.uncovered;
+  HIPCHK(hipMemcpy2DAsync(bxch, lrow, e->gxch.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BDF_OK;
+}
+
+int bdf_racah_info(bdf_engine *e, int breit, double *terms, long *items, long *uncovered) {
+  if (!e) return fail(BDF_EINVAL, "bdf_racah_info: null engine");
+  if (breit != 0 && breit != 1) return fail(BDF_EINVAL, "bdf_racah_info: breit must be 0 or 1");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_racah(e, breit);
+  if (rc != BDF_OK) return rc;
+  const RacahPlan &P = e->racah[breit].P;
+  if (terms) *terms = P.terms;
+  if (items) *items = (long)P.items.size();
+  if (uncovered) *uncovered = (long)P.uncovered;
   return BDF_OK;
 }
 

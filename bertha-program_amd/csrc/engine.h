@@ -157,11 +157,12 @@ struct BreitPlan {
   double flops = 0.0;
 };
 
-// ---- one-centre Racah plan (COULOMB1, bertha-scf.f:21540) ----------------
-// Tensor orders of one LQN block: NU runs over one parity class of
-// [max(|la-lb|,|lc-ld|), min(la+lb,lc+ld)], at most kMaxL+1 values.
-constexpr int kRcMaxNu = kMaxL + 1;
-constexpr int kRcMaxNuf = 2 * kMaxL;  // largest tensor order
+// ---- one-centre Racah plans (COULOMB1, bertha-scf.f:21540; BREIT1, 26014) --
+// Tensor orders of one LQN block: for the Coulomb interaction one parity
+// class of [max(|la-lb|,|lc-ld|), min(la+lb,lc+ld)], for the Breit
+// interaction the orders of the other class one step beyond it.
+constexpr int kRcMaxNu = kMaxL + 2;
+constexpr int kRcMaxNuf = 2 * kMaxL + 1;  // largest tensor order
 
 // One LQN block (la <= lb, lc <= ld, la(la+1)/2+lb <= lc(lc+1)/2+ld) of one
 // centre, with the tensor orders TNSCLM1 allows (25551).
@@ -173,7 +174,7 @@ struct RcBlock {
   int maxab, maxcd;
   int seg_begin, seg_end;   // beta sources of a block that writes none (RcSeg)
   int fresh;                // 1: its first KQN combination writes its betas
-  int pad;
+  int breit;                // 1: BREIT1's tensor orders and beta integrals
 };
 
 // Where a block that writes no beta integrals reads them (the reference's
@@ -193,10 +194,11 @@ struct RcCombo {
   int entry_begin, entry_end;
 };
 
-// One (|m|, sign) combination: its angular factors DKAB * DKCD per tensor
-// order (0 where screened) and the digestion terms it feeds.
+// One (|m|, sign) combination: the offset of its angular factors (0 where
+// screened) and the digestion terms it feeds.  Coulomb: DKAB DKCD per tensor
+// order, nunum values; Breit: the eight BCOF factors per order, 8 nunum.
 struct RcEntry {
-  double ang[kRcMaxNu];
+  int ang;
   int op_begin, op_end;
 };
 
@@ -205,8 +207,9 @@ struct RcEntry {
 struct RcOp {
   int dst_r, dst_c, src_r, src_c;
   uint8_t sdr, sdc, ssr, ssc;  // index of each: 0 i, 1 j, 2 k, 3 l
-  uint8_t mat;                 // 0 GDIR, 1 GXCH
-  uint8_t xt;                  // 0 LLLL, 1 LLSS, 2 SSLL, 3 SSSS
+  uint8_t mat;                 // 0 GDIR, 1 GXCH (Breit: BXCH)
+  uint8_t xt;                  // Coulomb 0 LLLL, 1 LLSS, 2 SSLL, 3 SSSS;
+                               // Breit 0 LSLS, 1 LSSL, 2 SLLS, 3 SLSL
   uint8_t uniform;             // dst indexed by i and j only
   uint8_t pad;
   double f;
@@ -221,6 +224,7 @@ struct RacahPlan {
   std::vector<RcSeg> segs;
   std::vector<RcCombo> combos;
   std::vector<RcEntry> entries;
+  std::vector<double> ang;
   std::vector<RcOp> ops;
   std::vector<RcItem> items;
   int64_t uncovered = 0;    // beta reads no earlier block wrote

@@ -137,15 +137,18 @@ struct RacahArgs {
   const RcEntry *entries;
   const RcOp *ops;
   const RcItem *items;
+  const double *ang;  // angular factors of the entries
   int nitems;
   const double *exps, *normL, *normS;
   const double2 *dent;
-  double2 *gdir, *gxch;
+  double2 *gdir, *gxch;  // Breit: gxch is BXCH, gdir unused
   int ndim;
   int isym;
   int exact;  // 1: every block's own beta integrals
+  int breit;  // 0: COULOMB1, 1: BREIT1
 };
-// the racah kernel, then the lower triangles set to the conjugate upper ones
+// the racah kernel, then the lower triangles of the matrices it builds set
+// to the conjugate upper ones
 hipError_t launch_racah(const RacahArgs &a, hipStream_t s);
 
 hipError_t static_scratch(int device, double **ptr);

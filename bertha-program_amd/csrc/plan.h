@@ -55,8 +55,9 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
                              int isym, int intsym, int racah1, int ilev, int gaunt,
                              BreitPlan &P);
 
-// the one-centre Racah plan of every centre (racah.cpp); isym: 0 general
-// molecule (complex density), 1, 2 axial molecule or atom
-std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P);
+// the one-centre Racah plan of every centre (racah.cpp) of the Coulomb
+// (breit = 0, COULOMB1) or Breit (breit = 1, BREIT1) interaction; isym: 0
+// general molecule (complex density), 1, 2 axial molecule or atom
+std::string build_racah_plan(const Basis &B, int isym, int breit, RacahPlan &P);
 
 }  // namespace bdf

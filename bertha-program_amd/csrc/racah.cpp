@@ -75,6 +75,35 @@ double dk(int ja, int ma, int jb, int mb, int nu) {
          three_j(ja, 2 * nu, jb, -ma, ma - mb, mb);
 }
 
+// BRCOEF1 (29092): coefficients of the eight (LS|LS)-type radial integrals
+// in the orders nu-1 (column 0) and nu+1 (column 1) of an even-parity order
+// nu (Grant and Pyper 1976, Table 2)
+void breit_coefficients(const int kq[4], int nu, double co[8][2]) {
+  const double kca = kq[1] - kq[0], kdb = kq[3] - kq[2], n = nu;
+  if (nu >= 1) {
+    const double b = n * (n + 1) / (2 * n * (2 * n - 1)), c = (n - 2) / (2 * n * (2 * n - 1));
+    co[0][0] = (n + kca) * (b - c * kdb);
+    co[1][0] = (n + kdb) * (b - c * kca);
+    co[2][0] = (n - kca) * (b + c * kdb);
+    co[3][0] = (n - kdb) * (b + c * kca);
+    co[4][0] = -(n + kca) * (b + c * kdb);
+    co[5][0] = -(n - kdb) * (b - c * kca);
+    co[6][0] = -(n - kca) * (b - c * kdb);
+    co[7][0] = -(n + kdb) * (b + c * kca);
+  } else {
+    for (int u = 0; u < 8; ++u) co[u][0] = 0.0;
+  }
+  const double b = n * (n + 1) / ((2 * n + 2) * (2 * n + 3)), c = (n + 3) / ((2 * n + 2) * (2 * n + 3));
+  co[0][1] = -(n + 1 - kdb) * (b + c * kca);
+  co[1][1] = -(n + 1 - kca) * (b + c * kdb);
+  co[2][1] = -(n + 1 + kdb) * (b - c * kca);
+  co[3][1] = -(n + 1 + kca) * (b - c * kdb);
+  co[4][1] = (n + 1 + kdb) * (b + c * kca);
+  co[5][1] = (n + 1 - kca) * (b - c * kdb);
+  co[6][1] = (n + 1 - kdb) * (b - c * kca);
+  co[7][1] = (n + 1 + kca) * (b + c * kdb);
+}
+
 // A FOCKIND block offset named as the reference names it: N/K (the sign slot
 // or its Kramers partner), A..D, L/S (large or small component).
 struct Label {
@@ -181,6 +210,44 @@ const Section kSections[8] = {
       {true, lab("KDL"), lab("KAS"), false, 1, {T("KDL", "KAS", 2, "KBS", "KCL")}}}},
 };
 
+// BRTMT1Z 26553-27460 / BRTMT1G 27462-28350, closed-shell part: BXCH from
+// the four permutations of (AD|CB), with the same sign rules as COULOMB1's
+// exchange sections (section_allows 4..7)
+const Section kBreitSections[4] = {
+    // BXCH (AD|CB)
+    {1, 0, false, false, 4,
+     {{false, lab("NAL"), lab("NDL"), false, 2,
+       {T("NAL", "NDL", 1, "NCS", "NBS"), T("NAS", "NDS", 2, "NCL", "NBL")}},
+      {false, lab("NAL"), lab("NDS"), false, 1, {T("NAL", "NDS", 0, "NCL", "NBS")}},
+      {true, lab("NCL"), lab("NBL"), false, 2,
+       {T("NCL", "NBL", 2, "NAS", "NDS"), T("NCS", "NBS", 1, "NAL", "NDL")}},
+      {true, lab("NCL"), lab("NBS"), false, 1, {T("NCL", "NBS", 0, "NAL", "NDS")}}}},
+    // BXCH (BD|CA) = PAB (AD|CB)
+    {1, 1, true, false, 4,
+     {{false, lab("KBL"), lab("NDL"), false, 2,
+       {T("KBL", "NDL", 3, "NCS", "KAS"), T("KBS", "NDS", 0, "NCL", "KAL")}},
+      {false, lab("KBL"), lab("NDS"), false, 1, {T("KBL", "NDS", 2, "NCL", "KAS")}},
+      {true, lab("NCL"), lab("KAL"), false, 2,
+       {T("NCL", "KAL", 0, "KBS", "NDS"), T("NCS", "KAS", 3, "KBL", "NDL")}},
+      {true, lab("NCL"), lab("KAS"), false, 1, {T("NCL", "KAS", 2, "KBL", "NDS")}}}},
+    // BXCH (AC|DB) = PCD (AD|CB)
+    {1, 2, false, true, 4,
+     {{false, lab("NAL"), lab("KCL"), false, 2,
+       {T("NAL", "KCL", 0, "KDS", "NBS"), T("NAS", "KCS", 3, "KDL", "NBL")}},
+      {false, lab("NAL"), lab("KCS"), false, 1, {T("NAL", "KCS", 1, "KDL", "NBS")}},
+      {true, lab("KDL"), lab("NBL"), false, 2,
+       {T("KDL", "NBL", 3, "NAS", "KCS"), T("KDS", "NBS", 0, "NAL", "KCL")}},
+      {true, lab("KDL"), lab("NBS"), false, 1, {T("KDL", "NBS", 1, "NAL", "KCS")}}}},
+    // BXCH (BC|DA) = PAB PCD (AD|CB)
+    {1, 3, true, true, 4,
+     {{false, lab("KBL"), lab("KCL"), false, 2,
+       {T("KBL", "KCL", 2, "KDS", "KAS"), T("KBS", "KCS", 1, "KDL", "KAL")}},
+      {false, lab("KBL"), lab("KCS"), false, 1, {T("KBL", "KCS", 3, "KDL", "KAS")}},
+      {true, lab("KDL"), lab("KAL"), false, 2,
+       {T("KDL", "KAL", 1, "KBS", "KCS"), T("KDS", "KAS", 2, "KBL", "KCL")}},
+      {true, lab("KDL"), lab("KAS"), false, 1, {T("KDL", "KAS", 3, "KBL", "KCS")}}}},
+};
+
 // MQN-sign rule of each section: the sum rule both paths apply, and the
 // axial path's (CLMMT1Z) extra conditions on the sign indices, as written
 // there (sections 5 and 6 ask for unequal signs, section 3 pairs B with C)
@@ -207,20 +274,21 @@ struct EntryCtx {
   int mj[4], mm[4];
 };
 
-void emit_entry(const EntryCtx &c, bool axial, std::vector<RcOp> &ops) {
+void emit_entry(const EntryCtx &c, bool axial, bool breit, std::vector<RcOp> &ops) {
   const bool k12 = c.kq[0] == c.kq[1], k34 = c.kq[2] == c.kq[3];
   auto off = [&](Label l) { return c.base[l.letter][l.kram][l.small]; };
-  for (int s = 0; s < 8; ++s) {
-    const Section &S = kSections[s];
+  const int nsec = breit ? 4 : 8;
+  for (int s = 0; s < nsec; ++s) {
+    const Section &S = breit ? kBreitSections[s] : kSections[s];
     if ((S.skip12 && k12) || (S.skip34 && k34)) continue;
-    if (!section_allows(s, c.mj, c.mm, axial)) continue;
+    if (!section_allows(breit ? s + 4 : s, c.mj, c.mm, axial)) continue;
     const double f = S.phase == 0 ? 1.0 : S.phase == 1 ? c.pab : S.phase == 2 ? c.pcd : c.pab * c.pcd;
     for (int g = 0; g < S.ng; ++g) {
       const Group &G = S.g[g];
       if (G.swap && c.ikl == c.ikr) continue;
       // the direct (BA|CD) and (AB|DC) sections skip one half on equal kappas
-      if (s == 1 && ((!G.swap && k12) || (G.swap && k34))) continue;
-      if (s == 2 && ((!G.swap && k34) || (G.swap && k12))) continue;
+      if (!breit && s == 1 && ((!G.swap && k12) || (G.swap && k34))) continue;
+      if (!breit && s == 2 && ((!G.swap && k34) || (G.swap && k12))) continue;
       const int gr = off(G.gr), gc = off(G.gc);
       if (G.strict ? !(gr < gc) : !(gr <= gc)) continue;
       for (int t = 0; t < G.nt; ++t) {
@@ -242,7 +310,7 @@ void emit_entry(const EntryCtx &c, bool axial, std::vector<RcOp> &ops) {
 
 }  // namespace
 
-std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
+std::string build_racah_plan(const Basis &B, int isym, int breit, RacahPlan &P) {
   P = RacahPlan();
   const bool axial = isym != 0;
   const int nskp = B.nskp;
@@ -260,17 +328,43 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
     for (int ld = 0; ld <= lmx; ++ld) {
       if (la > lb || lc > ld) continue;
       if (la * (la + 1) / 2 + lb > lc * (lc + 1) / 2 + ld) continue;
-      // TNSCLM1: parity of the pairs, triangle rules, allowed orders
+      // parity of the pairs and triangle rules (TNSCLM1 25551, TNSBRT1 28691)
       if ((la + lb) % 2 != (lc + ld) % 2) continue;
       int nui = std::max(std::abs(la - lb), std::abs(lc - ld));
       int nuf = std::min(la + lb, lc + ld);
+      if (nui > nuf) continue;
       RcBlock bk{};
+      bk.breit = breit;
       bk.nunum = 0;
-      for (int nu = nui; nu <= nuf; ++nu)
-        if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) {
-          if (bk.nunum == kRcMaxNu) return "racah plan: too many tensor orders";
-          bk.nus[bk.nunum++] = nu;
+      if (!breit) {
+        // Coulomb: the orders of even parity
+        for (int nu = nui; nu <= nuf; ++nu)
+          if ((la + lb + nu) % 2 == 0) {
+            if (bk.nunum == kRcMaxNu) return "racah plan: too many tensor orders";
+            bk.nus[bk.nunum++] = nu;
+          }
+      } else {
+        // Breit: an odd-parity order nu > 0 itself, an even one nu-1 and
+        // nu+1 (nu-1 dropped at nu = 0), as TNSBRT1 lists them
+        int lt = 0;
+        bool any = false;
+        int nus[2 * kRcMaxNu + 2] = {};
+        for (int nu = nui; nu <= nuf; ++nu) {
+          if ((la + lb + nu) % 2 == 1 && nu != 0) {
+            any = true;
+            nus[lt] = nu;
+          } else if ((la + lb + nu) % 2 == 0) {
+            any = true;
+            nus[lt] = nu - 1;
+            if (nu > 0) ++lt;
+            nus[lt] = nu + 1;
+          }
         }
+        if (!any) continue;
+        if (lt + 1 > kRcMaxNu) return "racah plan: too many tensor orders";
+        bk.nunum = lt + 1;
+        for (int t = 0; t < bk.nunum; ++t) bk.nus[t] = nus[t];
+      }
       if (bk.nunum == 0) continue;
       const int lq[4] = {la, lb, lc, ld};
       for (int q = 0; q < 4; ++q) {
@@ -280,7 +374,8 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
       }
       bk.nui = bk.nus[0];
       bk.nuf = bk.nus[bk.nunum - 1];
-      bk.nvals = (bk.nuf - bk.nui) / 2 + 3;
+      // beta levels BTACLM1 / BTABRT1 write
+      bk.nvals = (bk.nuf - bk.nui) / 2 + (breit ? 2 : 3);
       const int lsum = la + lb + lc + ld;
       for (int t = 0; t < 5; ++t) bk.cc[t] = 0.25 * std::tgamma(0.5 * (lsum + 2 * t + 1));
       bk.maxab = bk.nb[0] * bk.nb[1];
@@ -330,20 +425,73 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
           kq[q] = B.kappa_of(c, kidx[q]);
           jq[q] = 2 * std::abs(kq[q]) - 1;
         }
-        // ANGCLM1: J triangle and parity select the tensor orders in use
+        // J triangle and parity select the tensor orders in use (ANGCLM1
+        // 25865, ANGBRT1 28813)
         const int nuit = std::max(std::abs(jq[0] - jq[1]) / 2, std::abs(jq[2] - jq[3]) / 2);
         const int nuft = std::min((jq[0] + jq[1]) / 2, (jq[2] + jq[3]) / 2);
         if (nuit > nuft) continue;
         RcCombo cb{};
         cb.block = bi;
         cb.inu = 0;
+        auto tensor_index = [&](int nu) {
+          for (int t = 0; t < bk.nunum; ++t)
+            if (bk.nus[t] == nu) return t;
+          return -1;
+        };
+        // Breit: BCOF(t, u, a, b, c, d) over the combination's |m| and signs,
+        // a..d = doubled |m| index + sign (0-based 2|m|-2 + sign)
+        const int na4[4] = {2 * std::abs(kq[0]), 2 * std::abs(kq[1]), 2 * std::abs(kq[2]),
+                            2 * std::abs(kq[3])};
+        std::vector<double> bcof;
+        auto bidx = [&](int t, int u, int a, int b2, int c2, int d) {
+          return ((((size_t)t * 8 + u) * na4[0] + a) * na4[1] + b2) * (size_t)na4[2] * na4[3] +
+                 (size_t)c2 * na4[3] + d;
+        };
+        // doubled signed m of sign-index position p (0: -|m|, 1: +|m|) of |m| index mi
+        auto mjs = [](int mi, int sgn) { return sgn ? 2 * mi - 1 : -(2 * mi - 1); };
         int nulng = 0;
-        for (int nu = nuit; nu <= nuft; ++nu)
-          if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) {
-            for (int t = 0; t < bk.nunum; ++t)
-              if (bk.nus[t] == nu) cb.inu |= 1 << t;
+        if (!breit) {
+          for (int nu = nuit; nu <= nuft; ++nu)
+            if ((la + lb + nu) % 2 == 0 && (lc + ld + nu) % 2 == 0) {
+              int t = tensor_index(nu);
+              if (t >= 0) cb.inu |= 1 << t;
+              ++nulng;
+            }
+        } else {
+          bcof.assign((size_t)bk.nunum * 8 * na4[0] * na4[1] * na4[2] * na4[3], 0.0);
+          for (int nu = nuit; nu <= nuft; ++nu) {
+            const bool odd = (la + lb + nu) % 2 == 1;
+            if (odd && nu == 0) continue;
             ++nulng;
+            int tk[2] = {-1, -1};
+            double co[8][2];
+            if (odd) {
+              tk[0] = tensor_index(nu);
+              const double r = -(double)((kq[0] + kq[1]) * (kq[2] + kq[3])) / (double)(nu * (nu + 1));
+              for (int u = 0; u < 8; ++u) co[u][0] = r;
+            } else {
+              breit_coefficients(kq, nu, co);
+              if (nu > 0) tk[0] = tensor_index(nu - 1);
+              tk[1] = tensor_index(nu + 1);
+            }
+            for (int h = 0; h < 2; ++h) {
+              const int t = tk[h];
+              if (t < 0) continue;
+              cb.inu |= 1 << t;
+              for (int a = 0; a < na4[0]; ++a)
+                for (int b2 = 0; b2 < na4[1]; ++b2) {
+                  const double dab = dk(jq[0], mjs(a / 2 + 1, a % 2), jq[1], mjs(b2 / 2 + 1, b2 % 2), nu);
+                  if (dab == 0.0) continue;
+                  for (int c2 = 0; c2 < na4[2]; ++c2)
+                    for (int d = 0; d < na4[3]; ++d) {
+                      const double dcd =
+                          dk(jq[3], mjs(d / 2 + 1, d % 2), jq[2], mjs(c2 / 2 + 1, c2 % 2), nu);
+                      for (int u = 0; u < 8; ++u) bcof[bidx(t, u, a, b2, c2, d)] += dab * dcd * co[u][h];
+                    }
+                }
+            }
           }
+        }
         if (nulng == 0) continue;
         for (int q = 0; q < 4; ++q) cb.tq[q] = (double)(lq[q] + kq[q] + 1);
         cb.entry_begin = (int)P.entries.size();
@@ -356,9 +504,10 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
         for (int mc = 1; mc <= std::abs(kq[2]); ++mc)
         for (int md = 1; md <= std::abs(kq[3]); ++md) {
           const int mq[4] = {2 * ma - 1, 2 * mb - 1, 2 * mc - 1, 2 * md - 1};
-          // |m| pairing conserved on the axis (isym 1, 2)
-          if (axial && !((mq[0] == mq[1] && mq[2] == mq[3]) || (mq[0] == mq[2] && mq[1] == mq[3]) ||
-                         (mq[0] == mq[3] && mq[1] == mq[2])))
+          // |m| pairing conserved on the axis (isym 1, 2); BREIT1 admits the
+          // (AB)(CD) pairing for open shells only (26370)
+          if (axial && !((!breit && mq[0] == mq[1] && mq[2] == mq[3]) ||
+                         (mq[0] == mq[2] && mq[1] == mq[3]) || (mq[0] == mq[3] && mq[1] == mq[2])))
             continue;
           const int mabs[4] = {ma, mb, mc, md};
           for (int s = 0; s < 16; ++s) {
@@ -368,24 +517,46 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
               mm[q] = ((s >> (3 - q)) & 1) + 1;
               mj[q] = mm[q] == 1 ? -mq[q] : mq[q];
             }
-            if (ma == mb && ikms[ka][ma][mm[0] - 1] > ikms[kb][mb][mm[1] - 1]) continue;
-            if (mc == md && ikms[kc][mc][mm[2] - 1] > ikms[kd][md][mm[3] - 1]) continue;
+            // minimal block permutations: COULOMB1 on equal |m| (21960),
+            // BREIT1 on |m_A| < |m_B| (26405)
+            if ((breit ? ma < mb : ma == mb) && ikms[ka][ma][mm[0] - 1] > ikms[kb][mb][mm[1] - 1])
+              continue;
+            if ((breit ? mc < md : mc == md) && ikms[kc][mc][mm[2] - 1] > ikms[kd][md][mm[3] - 1])
+              continue;
             if (axial && mj[0] - mj[1] != mj[3] - mj[2]) continue;
             RcEntry en{};
+            en.ang = (int)P.ang.size();
+            const int per = breit ? 8 : 1;
+            P.ang.resize(P.ang.size() + (size_t)per * bk.nunum, 0.0);
             bool any = false;
             for (int t = 0; t < bk.nunum; ++t) {
-              en.ang[t] = 0.0;
               if (!(cb.inu >> t & 1)) continue;
-              const int nu = bk.nus[t];
-              // DKAB (AB pair) and DKCD (arguments reversed, 25990-26005)
-              double dab = dk(jq[0], mj[0], jq[1], mj[1], nu);
-              double dcd = dk(jq[3], mj[3], jq[2], mj[2], nu);
-              double a = dab * dcd;
-              if (std::fabs(a) <= 1.0e-10) continue;  // SENS of COULOMB1
-              en.ang[t] = a;
+              double *a = &P.ang[en.ang + (size_t)per * t];
+              if (!breit) {
+                const int nu = bk.nus[t];
+                // DKAB (AB pair) and DKCD (arguments reversed, 25990-26005)
+                const double v = dk(jq[0], mj[0], jq[1], mj[1], nu) * dk(jq[3], mj[3], jq[2], mj[2], nu);
+                if (std::fabs(v) <= 1.0e-10) continue;  // SENS of COULOMB1
+                a[0] = v;
+              } else {
+                double sum = 0.0;
+                const int ia = 2 * (ma - 1) + mm[0] - 1, ib = 2 * (mb - 1) + mm[1] - 1;
+                const int ic = 2 * (mc - 1) + mm[2] - 1, id = 2 * (md - 1) + mm[3] - 1;
+                for (int u = 0; u < 8; ++u) {
+                  a[u] = bcof[bidx(t, u, ia, ib, ic, id)];
+                  sum += std::fabs(a[u]);
+                }
+                if (sum <= 1.0e-10) {  // SENS of BREIT1, on the sum of the eight
+                  for (int u = 0; u < 8; ++u) a[u] = 0.0;
+                  continue;
+                }
+              }
               any = true;
             }
-            if (!any) continue;
+            if (!any) {
+              P.ang.resize(en.ang);
+              continue;
+            }
             for (int q = 0; q < 4; ++q) {
               ec.mj[q] = mj[q];
               ec.mm[q] = mm[q];
@@ -398,12 +569,17 @@ std::string build_racah_plan(const Basis &B, int isym, RacahPlan &P) {
             }
             auto sgn = [](int x) { return x >= 0 ? 1.0 : -1.0; };
             auto par = [](int x) { return (x / 2) % 2 == 0 ? 1.0 : -1.0; };
-            ec.pab = sgn(kq[0] * kq[1]) * par(mj[0] - mj[1]);
-            ec.pcd = sgn(kq[2] * kq[3]) * par(mj[2] - mj[3]);
+            // PAB, PCD; BREIT1's carry the opposite sign (26430)
+            const double ph = breit ? -1.0 : 1.0;
+            ec.pab = ph * sgn(kq[0] * kq[1]) * par(mj[0] - mj[1]);
+            ec.pcd = ph * sgn(kq[2] * kq[3]) * par(mj[2] - mj[3]);
             en.op_begin = (int)P.ops.size();
-            emit_entry(ec, axial, P.ops);
+            emit_entry(ec, axial, breit != 0, P.ops);
             en.op_end = (int)P.ops.size();
-            if (en.op_end == en.op_begin) continue;
+            if (en.op_end == en.op_begin) {
+              P.ang.resize(en.ang);
+              continue;
+            }
             P.entries.push_back(en);
             P.terms += (double)bk.maxab * bk.maxcd * (en.op_end - en.op_begin);
           }

@@ -245,9 +245,8 @@ def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, wfn_dir=N
     597-604), and the converged eigenvalues and spinors are written back.
 
     racah1=True (engine_kw) runs the reference's default one-centre path:
-    CLMFAST without one-centre quartets plus COULOMB1 (bdf_coulomb1) in every
-    Coulomb build; DHFB decks need racah1=False (no BREIT1), and DHFP's Breit
-    build takes its one-centre terms from MD."""
+    CLMFAST and BRTFAST without one-centre quartets plus COULOMB1 and BREIT1
+    (bdf_coulomb1, bdf_breit1) in every build."""
     import torch
 
     from . import scf as _scf
@@ -269,8 +268,7 @@ def solve(pkg, deck: Deck, max_iter: int = 100, verbose: bool = False, wfn_dir=N
     if deck.hmlt != "DHFP":
         return res, None
     sb = dataclasses.replace(deck.system, hmlt="DHFB")
-    # one-centre Breit terms by MD: the engine has no BREIT1
-    bb = _scf.EngineBackend(pkg, sb, **dict(engine_kw, racah1=False))
+    bb = _scf.EngineBackend(pkg, sb, **engine_kw)
     try:
         bdir, bxch = bb.breit(res.density, 3)
         e = 0.5 * float(torch.sum(res.density * (bdir - bxch)).real)

@@ -79,11 +79,6 @@ class EngineBackend:
         # the engine's HIP stream orders its kernels; torch's work runs on
         # the calling thread's current stream and is joined by events
         self.system = system
-        if engine_kw.get("racah1") and system.hmlt == "DHFB":
-            # the engine skips one-centre Breit quartets under RACAH1 and has
-            # no BREIT1 (bertha-scf.f:26014) to add them
-            raise ValueError("racah1 with the Breit interaction needs BREIT1, which this "
-                             "engine does not provide: build DHFB systems with racah1=False")
         self.eng = pkg.Engine(system, **engine_kw)
         self.dev = torch.device(device)
         self.n = self.eng.ndim
@@ -120,6 +115,9 @@ class EngineBackend:
         self._d.copy_(dens.T)
         self.stream.wait_stream(torch.cuda.current_stream(self.dev))
         self.eng.breit_device(self._d.data_ptr(), self._a.data_ptr(), self._b.data_ptr(), ilev)
+        if self.eng.racah1:
+            # RACAH1 (bertha-scf.f:5447-5453): one-centre terms by BREIT1
+            self.eng.breit1_device(self._d.data_ptr(), self._a.data_ptr(), self._b.data_ptr())
         self._sync()
         return self._a.T.clone(), self._b.T.clone()
 

@@ -169,12 +169,31 @@ int bdf_coulomb1(bdf_engine *e, int exact_beta, const double _Complex *dent,
 int bdf_coulomb1_device(bdf_engine *e, int exact_beta, const void *d_dent,
                         void *d_gdir, void *d_gxch);
 
-/* Size of the one-centre Racah build: digestion terms (ij, kl, term) of one
-   build, work items, and beta-integral reads that no earlier block of the
-   build writes (the reference reads what an earlier build or a fresh
-   process left there; here they are zero).  Builds the plan on first use.
-   No counterpart in the reference. */
-int bdf_racah_info(bdf_engine *e, double *terms, long *items, long *uncovered);
+/* Replaces HFSCF's per-centre one-centre Breit calls with RACAH1 on,
+   DO IZ=1,NCNT: CALL BREIT1(IZ) (bertha-scf.f:5447-5453; routine at
+   :26014): adds every centre's one-centre Breit contributions (TNSBRT1,
+   ANGBRT1 with BRCOEF1, BTABRT1, RKBRT1, BRTMT1Z/BRTMT1G) to BXCH, which
+   holds the BRTFAST result of an engine created with racah1 = 1, and sets
+   its lower triangle to the conjugate upper one as BREIT1 does on return.
+   BDIR receives nothing for closed shells and is not touched.  exact_beta
+   as for bdf_coulomb1.  Host arrays, ld >= ndim; deno/wdir/wxch must be
+   NULL. */
+int bdf_breit1(bdf_engine *e, int exact_beta, const double _Complex *dent,
+               const double _Complex *deno, int ld, double _Complex *bdir,
+               double _Complex *bxch, double _Complex *wdir, double _Complex *wxch);
+
+/* Same on device-resident arrays (ld = ndim; d_bdir may be NULL), enqueued
+   on the engine's stream and not synchronised. */
+int bdf_breit1_device(bdf_engine *e, int exact_beta, const void *d_dent,
+                      void *d_bdir, void *d_bxch);
+
+/* Size of the one-centre Racah build of the Coulomb (breit = 0) or Breit
+   (breit = 1) interaction: digestion terms (ij, kl, term) of one build,
+   work items, and beta-integral reads that no earlier block of the build
+   writes (the reference reads what an earlier build or a fresh process
+   left there; here they are zero).  Builds the plan on first use.  No
+   counterpart in the reference. */
+int bdf_racah_info(bdf_engine *e, int breit, double *terms, long *items, long *uncovered);
 
 /* Test hook mirroring ERIFAST's output (bertha-scf.f:12269-13132) for one
    block quartet: centres icnt[4] (1-based), kappa indices kidx[4] (1-based

@@ -81,10 +81,29 @@ module bertha_amd
       integer(c_int), value :: exact_beta
     end function
 
-    integer(c_int) function bdf_racah_info(eng, terms, items, uncovered) &
+    ! in place of DO IZ=1,NCNT: CALL BREIT1(IZ) (bertha-scf.f:5447-5453)
+    integer(c_int) function bdf_breit1(eng, exact_beta, dent, deno, ld, bdir, bxch, wdir, wxch) &
+        bind(c, name='bdf_breit1')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: eng
+      integer(c_int), value :: exact_beta, ld
+      complex(c_double_complex), intent(in) :: dent(ld, *)
+      type(c_ptr), value :: deno, wdir, wxch
+      complex(c_double_complex), intent(inout) :: bdir(ld, *), bxch(ld, *)
+    end function
+
+    integer(c_int) function bdf_breit1_device(eng, exact_beta, d_dent, d_bdir, d_bxch) &
+        bind(c, name='bdf_breit1_device')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng, d_dent, d_bdir, d_bxch
+      integer(c_int), value :: exact_beta
+    end function
+
+    integer(c_int) function bdf_racah_info(eng, breit, terms, items, uncovered) &
         bind(c, name='bdf_racah_info')
       import :: c_int, c_ptr, c_double, c_long
       type(c_ptr), value :: eng
+      integer(c_int), value :: breit
       real(c_double), intent(out) :: terms
       integer(c_long), intent(out) :: items, uncovered
     end function

@@ -3274,6 +3274,406 @@ void orc_coulomb1(orc_sys *s, const double *dent, double *gdir, double *gxch) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* one-centre Breit by Racah algebra (BREIT1 26014-26550)                   */
+/* ------------------------------------------------------------------------ */
+
+/* BRCOEF1 29092: the eight (LS|LS)-type coefficients of the orders nu-1
+   (column 0) and nu+1 (column 1) (Grant and Pyper 1976) */
+static void brcoef1(double sc[8][2], const int kq[4], int nu) {
+  int kca = kq[1] - kq[0], kdb = kq[3] - kq[2];
+  if (nu >= 1) {
+    double b1 = (double)(nu * (nu + 1)) / (double)(2 * nu * (2 * nu - 1));
+    double c1 = (double)(nu - 2) / (double)(2 * nu * (2 * nu - 1));
+    sc[0][0] = (double)(nu + kca) * (b1 - c1 * kdb);
+    sc[1][0] = (double)(nu + kdb) * (b1 - c1 * kca);
+    sc[2][0] = (double)(nu - kca) * (b1 + c1 * kdb);
+    sc[3][0] = (double)(nu - kdb) * (b1 + c1 * kca);
+    sc[4][0] = -(double)(nu + kca) * (b1 + c1 * kdb);
+    sc[5][0] = -(double)(nu - kdb) * (b1 - c1 * kca);
+    sc[6][0] = -(double)(nu - kca) * (b1 - c1 * kdb);
+    sc[7][0] = -(double)(nu + kdb) * (b1 + c1 * kca);
+  } else {
+    for (int i = 0; i < 8; ++i) sc[i][0] = 0.0;
+  }
+  double b2 = (double)(nu * (nu + 1)) / (double)((2 * nu + 2) * (2 * nu + 3));
+  double c2 = (double)(nu + 3) / (double)((2 * nu + 2) * (2 * nu + 3));
+  sc[0][1] = -(double)(nu + 1 - kdb) * (b2 + c2 * kca);
+  sc[1][1] = -(double)(nu + 1 - kca) * (b2 + c2 * kdb);
+  sc[2][1] = -(double)(nu + 1 + kdb) * (b2 - c2 * kca);
+  sc[3][1] = -(double)(nu + 1 + kca) * (b2 - c2 * kdb);
+  sc[4][1] = (double)(nu + 1 + kdb) * (b2 + c2 * kca);
+  sc[5][1] = (double)(nu + 1 - kca) * (b2 - c2 * kdb);
+  sc[6][1] = (double)(nu + 1 - kdb) * (b2 - c2 * kca);
+  sc[7][1] = (double)(nu + 1 + kca) * (b2 + c2 * kdb);
+}
+
+#define B1G(r, col) (c->X[(long)(col) * c->s->ndim + (r)])
+#define C1LOOP for (int k = 0, m = 0; k < c->nbas[2]; ++k) for (int l = 0; l < c->nbas[3]; ++l, ++m)
+#define XLSLS c->x[0][m]
+#define XLSSL c->x[1][m]
+#define XSLLS c->x[2][m]
+#define XSLSL c->x[3][m]
+
+/* BRTMT1Z 26553-27460 / BRTMT1G 27462-28350, closed-shell part: BXCH from
+   the four index permutations of (AD|CB) with their variable swaps */
+static void b1_digest(c1_ctx *c) {
+  const int i = c->ibas, j = c->jbas, z = c->isym != 0;
+  const int mja = c->mqn[0] * (c->mma == 1 ? -1 : 1), mjb = c->mqn[1] * (c->mmb == 1 ? -1 : 1);
+  const int mjc = c->mqn[2] * (c->mmc == 1 ? -1 : 1), mjd = c->mqn[3] * (c->mmd == 1 ? -1 : 1);
+  const int swap = c->ikl != c->ikr;
+  const int k12 = c->kqn[0] == c->kqn[1], k34 = c->kqn[2] == c->kqn[3];
+  const double pab = c->pab, pcd = c->pcd, pp = pab * pcd;
+  /* (AD|CB) */
+  if (!(z && (c->mma != c->mmd || c->mmc != c->mmb)) && mja - mjd == mjb - mjc) {
+    if (c->nal <= c->ndl) C1LOOP {
+      B1G(c->nal + i, c->ndl + l) += XLSSL * c1_d(c, c->ncs + k, c->nbs + j);
+      B1G(c->nas + i, c->nds + l) += XSLLS * c1_d(c, c->ncl + k, c->nbl + j);
+    }
+    if (c->nal <= c->nds) C1LOOP B1G(c->nal + i, c->nds + l) += XLSLS * c1_d(c, c->ncl + k, c->nbs + j);
+    if (swap) {
+      if (c->ncl <= c->nbl) C1LOOP {
+        B1G(c->ncl + k, c->nbl + j) += XSLLS * c1_d(c, c->nas + i, c->nds + l);
+        B1G(c->ncs + k, c->nbs + j) += XLSSL * c1_d(c, c->nal + i, c->ndl + l);
+      }
+      if (c->ncl <= c->nbs) C1LOOP B1G(c->ncl + k, c->nbs + j) += XLSLS * c1_d(c, c->nal + i, c->nds + l);
+    }
+  }
+  /* (BD|CA) = PAB (AD|CB) */
+  if (!k12 && !(z && (c->mmb == c->mmd || c->mmc == c->mma)) && -mjb - mjd == -mja - mjc) {
+    if (c->kbl <= c->ndl) C1LOOP {
+      B1G(c->kbl + j, c->ndl + l) += pab * XSLSL * c1_d(c, c->ncs + k, c->kas + i);
+      B1G(c->kbs + j, c->nds + l) += pab * XLSLS * c1_d(c, c->ncl + k, c->kal + i);
+    }
+    if (c->kbl <= c->nds) C1LOOP B1G(c->kbl + j, c->nds + l) += pab * XSLLS * c1_d(c, c->ncl + k, c->kas + i);
+    if (swap) {
+      if (c->ncl <= c->kal) C1LOOP {
+        B1G(c->ncl + k, c->kal + i) += pab * XLSLS * c1_d(c, c->kbs + j, c->nds + l);
+        B1G(c->ncs + k, c->kas + i) += pab * XSLSL * c1_d(c, c->kbl + j, c->ndl + l);
+      }
+      if (c->ncl <= c->kas) C1LOOP B1G(c->ncl + k, c->kas + i) += pab * XSLLS * c1_d(c, c->kbl + j, c->nds + l);
+    }
+  }
+  /* (AC|DB) = PCD (AD|CB) */
+  if (!k34 && !(z && (c->mma == c->mmc || c->mmd == c->mmb)) && mja + mjc == mjb + mjd) {
+    if (c->nal <= c->kcl) C1LOOP {
+      B1G(c->nal + i, c->kcl + k) += pcd * XLSLS * c1_d(c, c->kds + l, c->nbs + j);
+      B1G(c->nas + i, c->kcs + k) += pcd * XSLSL * c1_d(c, c->kdl + l, c->nbl + j);
+    }
+    if (c->nal <= c->kcs) C1LOOP B1G(c->nal + i, c->kcs + k) += pcd * XLSSL * c1_d(c, c->kdl + l, c->nbs + j);
+    if (swap) {
+      if (c->kdl <= c->nbl) C1LOOP {
+        B1G(c->kdl + l, c->nbl + j) += pcd * XSLSL * c1_d(c, c->nas + i, c->kcs + k);
+        B1G(c->kds + l, c->nbs + j) += pcd * XLSLS * c1_d(c, c->nal + i, c->kcl + k);
+      }
+      if (c->kdl <= c->nbs) C1LOOP B1G(c->kdl + l, c->nbs + j) += pcd * XLSSL * c1_d(c, c->nal + i, c->kcs + k);
+    }
+  }
+  /* (BC|DA) = PAB PCD (AD|CB) */
+  if (!k12 && !k34 && !(z && (c->mmb != c->mmc || c->mmd != c->mma)) && -mjb + mjc == -mja + mjd) {
+    if (c->kbl <= c->kcl) C1LOOP {
+      B1G(c->kbl + j, c->kcl + k) += pp * XSLLS * c1_d(c, c->kds + l, c->kas + i);
+      B1G(c->kbs + j, c->kcs + k) += pp * XLSSL * c1_d(c, c->kdl + l, c->kal + i);
+    }
+    if (c->kbl <= c->kcs) C1LOOP B1G(c->kbl + j, c->kcs + k) += pp * XSLSL * c1_d(c, c->kdl + l, c->kas + i);
+    if (swap) {
+      if (c->kdl <= c->kal) C1LOOP {
+        B1G(c->kdl + l, c->kal + i) += pp * XLSSL * c1_d(c, c->kbs + j, c->kcs + k);
+        B1G(c->kds + l, c->kas + i) += pp * XSLLS * c1_d(c, c->kbl + j, c->kcl + k);
+      }
+      if (c->kdl <= c->kas) C1LOOP B1G(c->kdl + l, c->kas + i) += pp * XSLSL * c1_d(c, c->kbl + j, c->kcs + k);
+    }
+  }
+}
+#undef B1G
+#undef C1LOOP
+#undef XLSLS
+#undef XLSSL
+#undef XSLLS
+#undef XSLSL
+
+/* BREIT1(IZ) for one centre: adds its one-centre Breit contributions to
+   bxch (upper-triangle blocks); BDIR has none for closed shells */
+static void breit1(orc_sys *s, int iz, const zc *D, zc *Xm, c1_beta_store *bs) {
+  int nkap = s->nkap[iz], lmx = (nkap - 1) / 2;
+  int ikms[MKPX + 1][MKPX + 1][3];
+  int nkms = 0;
+  for (int kn = 1; kn <= nkap; ++kn)
+    for (int mj = 1; mj <= abs(KAPA(s, iz, kn)); ++mj)
+      for (int mm = 1; mm <= 2; ++mm) ikms[kn][mj][mm] = ++nkms;
+  c1_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s; c.D = D; c.G = NULL; c.X = Xm; c.isym = s->isym;
+  const int M2 = 2 * MKPX + 2;
+  /* BCOF(LTEN, IMU, IMJA, IMJB, IMJC, IMJD) of the current KQN combination */
+  double *bcof = malloc(sizeof(double) * C1MNU * 8 * M2 * M2 * M2 * M2);
+#define BCOF(t, u, a, b, cc, d) bcof[(((((long)(t) * 8 + (u)) * M2 + (a)) * M2 + (b)) * M2 + (cc)) * M2 + (d)]
+  static double dkab[C1MNU + 2][2 * MKPX + 2][2 * MKPX + 2], dkcd[C1MNU + 2][2 * MKPX + 2][2 * MKPX + 2];
+  for (int la = 0; la <= lmx; ++la)
+  for (int lb = 0; lb <= lmx; ++lb)
+  for (int lc = 0; lc <= lmx; ++lc)
+  for (int ld = 0; ld <= lmx; ++ld) {
+    if (la > lb || lc > ld) continue;
+    if (la * (la + 1) / 2 + lb > lc * (lc + 1) / 2 + ld) continue;
+    /* TNSBRT1 28691: orders NU of odd parity, and NU-1, NU+1 of the even ones */
+    if ((la + lb) % 2 != (lc + ld) % 2) continue;
+    int nui = abs(la - lb) > abs(lc - ld) ? abs(la - lb) : abs(lc - ld);
+    int nuf = la + lb < lc + ld ? la + lb : lc + ld;
+    if (nui > nuf) continue;
+    int nus[C1MNU], lten = 0, isel = 0;
+    memset(nus, 0, sizeof(nus));
+    for (int nu = nui; nu <= nuf; ++nu) {
+      int pab_ = (la + lb + nu) % 2, pcd_ = (lc + ld + nu) % 2;
+      if (pab_ == 1 && pcd_ == 1 && nu != 0) { isel = 1; nus[lten] = nu; }
+      if (pab_ == 0 && pcd_ == 0) {
+        isel = 1;
+        nus[lten] = nu - 1;
+        if (nu > 0) ++lten;
+        nus[lten] = nu + 1;
+      }
+    }
+    if (!isel) continue;
+    int nunum = lten + 1;
+    nui = nus[0]; nuf = nus[lten];
+    int inu[C1MNU][17];
+    memset(inu, 0, sizeof(inu));
+    const int lq[4] = {la, lb, lc, ld};
+    int nb[4];
+    for (int q = 0; q < 4; ++q) nb[q] = NFNC(s, iz, lq[q]);
+    c1_pairs ab, cd;
+    c1_pairs_set(&ab, &BEXL(s, iz, la, 0), nb[0], la, &BEXL(s, iz, lb, 0), nb[1], lb, nuf);
+    c1_pairs_set(&cd, &BEXL(s, iz, lc, 0), nb[2], lc, &BEXL(s, iz, ld, 0), nb[3], ld, nuf);
+    const int maxab = ab.n, maxcd = cd.n;
+    const int lsum = la + lb + lc + ld;
+    const double c3 = 0.25 * GAMHLF[lsum + 3], c5 = 0.25 * GAMHLF[lsum + 5], c7 = 0.25 * GAMHLF[lsum + 7];
+    /* rj[t][type][ul][m]: types LSLS, SLSL, LSSL, SLLS; ul 0 upper, 1 lower */
+    double *rj = calloc(8L * maxcd * nunum, sizeof(double));
+    double *xs = calloc(4L * maxcd, sizeof(double));
+    for (int na = 1; na >= (la == 0); --na)
+    for (int nb_ = 1; nb_ >= (lb == 0); --nb_)
+    for (int nc = 1; nc >= (lc == 0); --nc)
+    for (int nd = 1; nd >= (ld == 0); --nd) {
+      int ka = 2 * la + na, kb = 2 * lb + nb_, kc = 2 * lc + nc, kd = 2 * ld + nd;
+      if (ka > kb || kc > kd) continue;
+      int ikl = ka * (ka + 1) / 2 + kb, ikr = kc * (kc + 1) / 2 + kd;
+      if (ikl > ikr) continue;
+      int k4ad = 16 - (8 * na + 4 * nb_ + 2 * nc + nd);
+      int kq[4] = {KAPA(s, iz, ka), KAPA(s, iz, kb), KAPA(s, iz, kc), KAPA(s, iz, kd)};
+      double tq[4];
+      for (int q = 0; q < 4; ++q) tq[q] = (double)(lq[q] + kq[q] + 1);
+      /* ANGBRT1 28813 */
+      int jq[4];
+      for (int q = 0; q < 4; ++q) jq[q] = 2 * abs(kq[q]) - 1;
+      int nuit = abs(jq[0] - jq[1]) / 2 > abs(jq[2] - jq[3]) / 2 ? abs(jq[0] - jq[1]) / 2
+                                                                  : abs(jq[2] - jq[3]) / 2;
+      int nuft = (jq[0] + jq[1]) / 2 < (jq[2] + jq[3]) / 2 ? (jq[0] + jq[1]) / 2
+                                                            : (jq[2] + jq[3]) / 2;
+      if (nuit > nuft) continue;
+      int ma_n = abs(kq[0]), mb_n = abs(kq[1]), mc_n = abs(kq[2]), md_n = abs(kq[3]);
+      for (int t = 0; t < C1MNU; ++t)
+        for (int u = 0; u < 8; ++u)
+          for (int a = 1; a <= 2 * ma_n; ++a)
+            for (int b = 1; b <= 2 * mb_n; ++b)
+              for (int cc = 1; cc <= 2 * mc_n; ++cc)
+                for (int d = 1; d <= 2 * md_n; ++d) BCOF(t, u, a, b, cc, d) = 0.0;
+      for (int ma = 1; ma <= ma_n; ++ma)
+        for (int mb = 1; mb <= mb_n; ++mb)
+          for (int nu = nuit; nu <= nuft; ++nu) {
+            int mja = 2 * ma - 1, mjb = 2 * mb - 1;
+            dkab[nu][mja][mjb] = dk_coef(jq[0], -mja, jq[1], -mjb, nu);
+            dkab[nu][mja + 1][mjb] = dk_coef(jq[0], mja, jq[1], -mjb, nu);
+            dkab[nu][mja][mjb + 1] = dk_coef(jq[0], -mja, jq[1], mjb, nu);
+            dkab[nu][mja + 1][mjb + 1] = dk_coef(jq[0], mja, jq[1], mjb, nu);
+          }
+      for (int mc = 1; mc <= mc_n; ++mc)
+        for (int md = 1; md <= md_n; ++md)
+          for (int nu = nuit; nu <= nuft; ++nu) {
+            int mjc = 2 * mc - 1, mjd = 2 * md - 1;
+            dkcd[nu][mjc][mjd] = dk_coef(jq[3], -mjd, jq[2], -mjc, nu);
+            dkcd[nu][mjc + 1][mjd] = dk_coef(jq[3], -mjd, jq[2], mjc, nu);
+            dkcd[nu][mjc][mjd + 1] = dk_coef(jq[3], mjd, jq[2], -mjc, nu);
+            dkcd[nu][mjc + 1][mjd + 1] = dk_coef(jq[3], mjd, jq[2], mjc, nu);
+          }
+      int aisel = 0;
+      for (int nu = nuit; nu <= nuft; ++nu) {
+        int pab_ = (la + lb + nu) % 2, pcd_ = (lc + ld + nu) % 2;
+        int kf[2] = {-1, -1};
+        double coef[8][2];
+        if (pab_ == 1 && pcd_ == 1 && nu != 0) {
+          aisel = 1;
+          for (int t = 0; t < nunum; ++t) if (nus[t] == nu) kf[0] = t;
+          double rcoeff = -(double)((kq[0] + kq[1]) * (kq[2] + kq[3])) / (double)(nu * (nu + 1));
+          for (int u = 0; u < 8; ++u) coef[u][0] = rcoeff;
+        } else if (pab_ == 0 && pcd_ == 0) {
+          aisel = 1;
+          brcoef1(coef, kq, nu);
+          if (nu > 0) for (int t = 0; t < nunum; ++t) if (nus[t] == nu - 1) kf[0] = t;
+          for (int t = 0; t < nunum; ++t) if (nus[t] == nu + 1) kf[1] = t;
+        } else {
+          continue;
+        }
+        for (int h = 0; h < 2; ++h) {
+          if (kf[h] < 0) continue;
+          inu[kf[h]][k4ad] = 1;
+          for (int u = 0; u < 8; ++u)
+            for (int a = 1; a <= 2 * ma_n; ++a)
+              for (int b = 1; b <= 2 * mb_n; ++b)
+                for (int cc = 1; cc <= 2 * mc_n; ++cc)
+                  for (int d = 1; d <= 2 * md_n; ++d)
+                    BCOF(kf[h], u, a, b, cc, d) += dkab[nu][a][b] * dkcd[nu][cc][d] * coef[u][h];
+        }
+      }
+      if (!aisel) continue;
+      for (int q = 0; q < 4; ++q) { c.kqn[q] = kq[q]; c.nbas[q] = nb[q]; }
+      c.ikl = ikl; c.ikr = ikr;
+      for (int ij = 0; ij < maxab; ++ij) {
+        c.ibas = ij / nb[1];
+        c.jbas = ij % nb[1];
+        double tij0 = ab.ei[ij] + ab.ej[ij];
+        /* BTABRT1 for the block's first KQN combination only (26300) */
+        if (k4ad == 1) {
+          int nvals = (nuf - nui) / 2 + 2;
+          for (int nx = 1; nx <= nvals; ++nx)
+            for (int ny = 1; ny <= nvals; ++ny)
+              for (int ib = 0; ib < 2; ++ib) {
+                int iaa = ib == 0 ? la + lb + nui + 2 * nx : lc + ld + nui + 2 * nx;
+                int ibb = ib == 0 ? lc + ld - nuf + 2 * ny - 1 : la + lb - nuf + 2 * ny - 1;
+                double *dst = (ib == 0 ? bs->bdu : bs->bdl) + ((long)nx * C1MNU + ny) * bs->cap +
+                              (long)ij * maxcd;
+                for (int m = 0; m < maxcd; ++m) {
+                  double tkl0 = cd.ei[m] + cd.ej[m];
+                  double xj = (ib == 0 ? tij0 : tkl0) / (tij0 + tkl0);
+                  dst[m] = c1_beta(xj, (iaa - 1) / 2, ibb / 2);
+                }
+              }
+        }
+        /* RKBRT1 28525 */
+        memset(rj, 0, sizeof(double) * 8 * maxcd * nunum);
+        const double *eij = ab.ep + (long)ij * (2 * C1MNU + 6) + C1MNU;
+        const double e_i = ab.ei[ij], e_j = ab.ej[ij];
+        for (int m = 0; m < maxcd; ++m) {
+          const double *ekl = cd.ep + (long)m * (2 * C1MNU + 6) + C1MNU;
+          const double e_k = cd.ei[m], e_l = cd.ej[m];
+          long f = (long)ij * maxcd + m;
+          for (int t = 0; t < nunum; ++t) {
+            if (!inu[t][k4ad]) continue;
+            int nten = nunum - (t + 1), nu = nus[t], lt = t + 1;
+#define BU(a, b) bs->bdu[((long)(a) * C1MNU + (b)) * bs->cap + f]
+#define BL(a, b) bs->bdl[((long)(a) * C1MNU + (b)) * bs->cap + f]
+            double b21l = eij[-nu + 1] * ekl[nu + 2] * BL(lt, nten + 1);
+            double b21u = eij[nu + 2] * ekl[-nu + 1] * BU(lt, nten + 1);
+            double b23l = eij[-nu + 3] * ekl[nu + 2] * BL(lt, nten + 2);
+            double b41u = eij[nu + 4] * ekl[-nu + 1] * BU(lt + 1, nten + 1);
+            double b41l = eij[-nu + 1] * ekl[nu + 4] * BL(lt + 1, nten + 1);
+            double b23u = eij[nu + 2] * ekl[-nu + 3] * BU(lt, nten + 2);
+            double b43l = eij[-nu + 3] * ekl[nu + 4] * BL(lt + 1, nten + 2);
+            double b43u = eij[nu + 4] * ekl[-nu + 3] * BU(lt + 1, nten + 2);
+#undef BU
+#undef BL
+            /* (x, y) label pairs: LSLS (j, l), SLSL (i, k), LSSL (j, k), SLLS (i, l) */
+            const double ex[4] = {e_j, e_i, e_j, e_i}, ey[4] = {e_l, e_k, e_k, e_l};
+            const double tx[4] = {tq[1], tq[0], tq[1], tq[0]}, ty[4] = {tq[3], tq[2], tq[2], tq[3]};
+            const double *ra = ab.rn + 4L * ij, *rc = cd.rn + 4L * m;
+            /* RNIJ(2) = L_a S_b, RNIJ(3) = S_a L_b */
+            const double rn[4] = {ra[1] * rc[1], ra[2] * rc[2], ra[1] * rc[2], ra[2] * rc[1]};
+            for (int q = 0; q < 4; ++q) {
+              double *r = rj + (((long)t * 4 + q) * 2) * maxcd;
+              r[m] = (4.0 * ex[q] * ey[q] * c7 * b43u - 2.0 * ty[q] * ex[q] * c5 * b41u -
+                      2.0 * tx[q] * ey[q] * c5 * b23u + tx[q] * ty[q] * c3 * b21u) * rn[q];
+              r[maxcd + m] = (4.0 * ex[q] * ey[q] * c7 * b43l - 2.0 * ty[q] * ex[q] * c5 * b23l -
+                              2.0 * tx[q] * ey[q] * c5 * b41l + tx[q] * ty[q] * c3 * b21l) * rn[q];
+            }
+          }
+        }
+        for (int ma = 1; ma <= ma_n; ++ma)
+        for (int mb = 1; mb <= mb_n; ++mb)
+        for (int mc = 1; mc <= mc_n; ++mc)
+        for (int md = 1; md <= md_n; ++md) {
+          int mq[4] = {2 * ma - 1, 2 * mb - 1, 2 * mc - 1, 2 * md - 1};
+          if (s->isym == 1 || s->isym == 2) {
+            /* closed shells: the (AB)(CD) pairing is not admitted (26370) */
+            if (!((mq[0] == mq[2] && mq[1] == mq[3]) || (mq[0] == mq[3] && mq[1] == mq[2]))) continue;
+          }
+          for (int mma = 1; mma <= 2; ++mma)
+          for (int mmb = 1; mmb <= 2; ++mmb)
+          for (int mmc = 1; mmc <= 2; ++mmc)
+          for (int mmd = 1; mmd <= 2; ++mmd) {
+            int mja = mma == 1 ? -mq[0] : mq[0], mjb = mmb == 1 ? -mq[1] : mq[1];
+            int mjc = mmc == 1 ? -mq[2] : mq[2], mjd = mmd == 1 ? -mq[3] : mq[3];
+            if (ma < mb && ikms[ka][ma][mma] > ikms[kb][mb][mmb]) continue;
+            if (mc < md && ikms[kc][mc][mmc] > ikms[kd][md][mmd]) continue;
+            if ((s->isym == 1 || s->isym == 2) && mja - mjb != mjd - mjc) continue;
+            for (int q = 0; q < 4; ++q) c.mqn[q] = mq[q];
+            c.mma = mma; c.mmb = mmb; c.mmc = mmc; c.mmd = mmd;
+            c.pab = -isgn(kq[0] * kq[1]) * m1pow((mja - mjb) / 2);
+            c.pcd = -isgn(kq[2] * kq[3]) * m1pow((mjc - mjd) / 2);
+            c.nal = LRGE(s, iz, ka, 2 * ma - 2 + mma); c.kal = LRGE(s, iz, ka, 2 * ma + 1 - mma);
+            c.nbl = LRGE(s, iz, kb, 2 * mb - 2 + mmb); c.kbl = LRGE(s, iz, kb, 2 * mb + 1 - mmb);
+            c.ncl = LRGE(s, iz, kc, 2 * mc - 2 + mmc); c.kcl = LRGE(s, iz, kc, 2 * mc + 1 - mmc);
+            c.ndl = LRGE(s, iz, kd, 2 * md - 2 + mmd); c.kdl = LRGE(s, iz, kd, 2 * md + 1 - mmd);
+            c.nas = c.nal + s->nskp; c.nbs = c.nbl + s->nskp; c.ncs = c.ncl + s->nskp; c.nds = c.ndl + s->nskp;
+            c.kas = c.kal + s->nskp; c.kbs = c.kbl + s->nskp; c.kcs = c.kcl + s->nskp; c.kds = c.kdl + s->nskp;
+            memset(xs, 0, sizeof(double) * 4 * maxcd);
+            int imja = mq[0] + mma - 1, imjb = mq[1] + mmb - 1, imjc = mq[2] + mmc - 1, imjd = mq[3] + mmd - 1;
+            for (int t = 0; t < nunum; ++t) {
+              if (!inu[t][k4ad]) continue;
+              double af[8], asum = 0.0;
+              for (int u = 0; u < 8; ++u) {
+                af[u] = BCOF(t, u, imja, imjb, imjc, imjd);
+                asum += fabs(af[u]);
+              }
+              if (fabs(asum) <= 1.0e-10) continue;
+              /* X: LSLS, SLSL, LSSL, SLLS from ANGFAC 1-2, 3-4, 5-6, 7-8 */
+              for (int q = 0; q < 4; ++q) {
+                const double *r = rj + (((long)t * 4 + q) * 2) * maxcd;
+                /* digestion order: LSLS, LSSL, SLLS, SLSL */
+                int slot = q == 0 ? 0 : q == 1 ? 3 : q == 2 ? 1 : 2;
+                double *xq = xs + (long)slot * maxcd;
+                for (int m = 0; m < maxcd; ++m) xq[m] += af[2 * q] * r[m] + af[2 * q + 1] * r[maxcd + m];
+              }
+            }
+            for (int q = 0; q < 4; ++q) c.x[q] = xs + (long)q * maxcd;
+            b1_digest(&c);
+          }
+        }
+      }
+    }
+    free(rj); free(xs);
+    c1_pairs_free(&ab); c1_pairs_free(&cd);
+  }
+#undef BCOF
+  free(bcof);
+}
+
+/* every centre's BREIT1 after a BRTFAST build (HFSCF 5447-5453, RACAH1):
+   adds the one-centre Breit contributions to bxch and, as BREIT1 ends
+   (26520-26525), overwrites its lower triangle with the conjugate upper */
+void orc_breit1(orc_sys *s, const double *dent, double *bdir, double *bxch) {
+  init_tables();
+  (void)bdir;
+  int ndim = s->ndim;
+  c1_beta_store bs;
+  long cap = 0;
+  for (int iz = 0; iz < s->ncnt; ++iz)
+    for (int l1 = 0; l1 <= s->mel; ++l1)
+      for (int l2 = 0; l2 <= s->mel; ++l2) {
+        long n = (long)NFNC(s, iz, l1) * NFNC(s, iz, l2);
+        long nn = 0;
+        for (int l3 = 0; l3 <= s->mel; ++l3)
+          for (int l4 = 0; l4 <= s->mel; ++l4)
+            if ((long)NFNC(s, iz, l3) * NFNC(s, iz, l4) > nn) nn = (long)NFNC(s, iz, l3) * NFNC(s, iz, l4);
+        if (n * nn > cap) cap = n * nn;
+      }
+  bs.cap = cap;
+  bs.bdu = calloc((long)C1MNU * C1MNU * cap, sizeof(double));
+  bs.bdl = calloc((long)C1MNU * C1MNU * cap, sizeof(double));
+  zc *X = (zc *)bxch;
+  for (int iz = 0; iz < s->ncnt; ++iz) breit1(s, iz, (const zc *)dent, X, &bs);
+  for (int i = 0; i < ndim; ++i)
+    for (int j = i + 1; j < ndim; ++j) X[(long)i * ndim + j] = conj(X[(long)j * ndim + i]);
+  free(bs.bdu); free(bs.bdl);
+}
+
+/* ------------------------------------------------------------------------ */
 /* set-up and accessors                                                     */
 /* ------------------------------------------------------------------------ */
 orc_sys *orc_create(int ncnt, const double *xyz, const int *lmax, const int *nfnc, int lstride,

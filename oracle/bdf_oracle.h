@@ -68,6 +68,13 @@ void orc_clmfast(orc_sys *s, int ilev, const double *dent, double *gdir,
    upper ones, as COULOMB1 does on return. */
 void orc_coulomb1(orc_sys *s, const double *dent, double *gdir, double *gxch);
 
+/* BREIT1 (bertha-scf.f:26014) for every centre, as HFSCF calls it after
+   BRTFAST with RACAH1 on (5447-5453): adds the one-centre Racah-algebra
+   Breit contributions to bxch (a BRTFAST result of an oracle created with
+   racah1 = 1) and overwrites its lower triangle with the conjugate upper
+   one; bdir (no closed-shell contributions) is left alone. */
+void orc_breit1(orc_sys *s, const double *dent, double *bdir, double *bxch);
+
 /* same but with OpenMP over the outer centre-pair loop when nthreads > 1
    (digestion goes to thread-private matrices that are summed at the end) */
 void orc_clmfast_mt(orc_sys *s, int ilev, const double *dent, double *gdir,

@@ -47,6 +47,7 @@ def _oracle_lib():
         lib.orc_clmfast.argtypes = [P, C.c_int, P, P, P]
         lib.orc_clmfast_mt.argtypes = [P, C.c_int, P, P, P, C.c_int]
         lib.orc_coulomb1.argtypes = [P, P, P, P]
+        lib.orc_breit1.argtypes = [P, P, P, P]
         lib.orc_brtfast_sample.restype = C.c_double
         lib.orc_brtfast_sample.argtypes = [P, C.c_int, P, P, P, C.c_int, C.c_int]
         lib.orc_clmfast_sample.restype = C.c_double
@@ -249,6 +250,21 @@ class Oracle:
         _oracle_lib().orc_coulomb1(self._h, d.ctypes.data, g.ctypes.data, x.ctypes.data)
         return g, x
 
+    def breit1(self, dent, b, x):
+        """BREIT1 for every centre on top of a BRTFAST result (b, x) of an
+        Oracle with racah1=True; returns the completed matrices."""
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        b = np.array(b, dtype=np.complex128, order="F")
+        x = np.array(x, dtype=np.complex128, order="F")
+        _oracle_lib().orc_breit1(self._h, d.ctypes.data, b.ctypes.data, x.ctypes.data)
+        return b, x
+
+    def brtfast_racah(self, dent, ilev=3, nthreads=1):
+        """HFSCF's Breit build with RACAH1 on: BRTFAST without one-centre
+        quartets, then BREIT1 per centre (construct with racah1=True)."""
+        b, x = self.brtfast(dent, ilev, nthreads)
+        return self.breit1(dent, b, x)
+
     def clmfast_racah(self, dent, ilev=3, nthreads=1):
         """HFSCF's Coulomb build with RACAH1 on: CLMFAST without one-centre
         quartets, then COULOMB1 per centre (construct with racah1=True)."""
@@ -440,6 +456,19 @@ class Reference:
         _ref_lib().ref_clmfast(C.c_int(ilev), C.c_void_p(dre.ctypes.data),
                                C.c_void_p(dim.ctypes.data),
                                *[C.c_void_p(o.ctypes.data) for o in out])
+        return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+    def brtfast_racah(self, dent, ilev=3):
+        """BRTFAST followed by BREIT1 for every centre, as HFSCF runs them
+        with RACAH1 on (construct the Reference with racah1=True)."""
+        n = self.ndim
+        d = np.asfortranarray(dent, dtype=np.complex128)
+        dre = np.asfortranarray(d.real)
+        dim = np.asfortranarray(d.imag)
+        out = [np.zeros((n, n), order="F") for _ in range(4)]
+        _ref_lib().ref_brtfast_racah(C.c_int(ilev), C.c_void_p(dre.ctypes.data),
+                                     C.c_void_p(dim.ctypes.data),
+                                     *[C.c_void_p(o.ctypes.data) for o in out])
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
 
     def clmfast_racah(self, dent, ilev=3):

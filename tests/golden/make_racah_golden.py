@@ -1,14 +1,15 @@
 #!/usr/bin/env python3
 """Writes tests/golden/racah_vectors.npz from the reference's own routines.
 
-HFSCF's Coulomb build with RACAH1 on (bertha-scf.f:5401-5421): CLMFAST
-without one-centre quartets, then COULOMB1 for every centre, run by the
-reference compiled from the unmodified sources by oracle/Makefile
+HFSCF's Coulomb and Breit builds with RACAH1 on (bertha-scf.f:5401-5453):
+CLMFAST / BRTFAST without one-centre quartets, then COULOMB1 / BREIT1 for
+every centre, run by the reference compiled from the unmodified sources by oracle/Makefile
 (oracle/_ref/libbertha_ref.so, ref_clmfast_racah in oracle/ref/ref_driver.f)
 on seeded random densities of three small systems of tests/conftest.py: an
 atom with s, p and d shells (ATOMIC, CLMMT1Z), the f-shell diatomic on z
-(DIATOM, CLMMT1Z) and the off-axis two-centre system (NOSYMM, CLMMT1G).  Of
-each GDIR/GXCH pair it keeps 600 elements: the 300 largest in magnitude and
+(DIATOM, CLMMT1Z) and the off-axis two-centre system (NOSYMM, CLMMT1G); the
+Breit build on the same system as DHFB.  Of GDIR, GXCH and BXCH it keeps 600
+elements: the 300 largest in magnitude and
 300 drawn at random.  The file holds inputs (system name, density seed) and
 outputs only.  Run:  python tests/golden/make_racah_golden.py
 """
@@ -37,8 +38,10 @@ def main():
         s = sy[name]
         d = pkg.systems.random_density(s.ndim, seed=seed, real=(s.shape != "NOSYMM"))
         g, x = pyoracle.Reference(s, racah1=True).clmfast_racah(d, 3)
+        sb = pkg.systems.System(s.name, s.xyz, s.shells, s.shape, hmlt="DHFB")
+        _, bx = pyoracle.Reference(sb, racah1=True).brtfast_racah(d, 3)
         n = s.ndim
-        mag = np.abs(g) + np.abs(x)
+        mag = np.abs(g) + np.abs(x) + np.abs(bx)
         big = np.argsort(mag.ravel())[-300:]
         rnd = rng.choice(n * n, size=300, replace=False)
         flat = np.unique(np.concatenate([big, rnd]))
@@ -48,6 +51,7 @@ def main():
         out[f"{name}_cols"] = c.astype(np.int32)
         out[f"{name}_gdir"] = g[r, c]
         out[f"{name}_gxch"] = x[r, c]
+        out[f"{name}_bxch"] = bx[r, c]
         print(name, n, len(flat), flush=True)
     np.savez_compressed(os.path.join(HERE, "racah_vectors.npz"), **out)
 

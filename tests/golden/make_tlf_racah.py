@@ -1,9 +1,11 @@
 #!/usr/bin/env python3
 """Writes tests/golden/tlf_racah.npz: the oracle's one-centre Racah Coulomb
-contributions (COULOMB1 for both centres, bertha-scf.f:21540, followed by
-its lower-triangle overwrite) for the heavy-diatomic workload (Tl
+and Breit contributions (COULOMB1 and BREIT1 for both centres,
+bertha-scf.f:21540, :26014, each followed by its lower-triangle overwrite)
+for the heavy-diatomic workload (Tl
 28s24p18d12f + F 14s9p4d on z, NDIM 1340) at the 2000 matrix elements
-tests/golden/tlf_samples.npz samples, with E_J and E_K of the contributions,
+tests/golden/tlf_samples.npz samples, with E_J, E_K and E_B of the
+contributions,
 for two densities: the superposed atomic density stored in tlf_samples.npz
 (spherical atoms) and a seeded random real density (which also reaches the
 LQN blocks that read another block's beta integrals).
@@ -12,7 +14,7 @@ COULOMB1 starts from zero matrices here, so the values are the one-centre
 part alone.  The reference build holds 26 exponents per shell, fewer than
 Tl's 28 s functions, so the vectors are the oracle's (restated in
 oracle/bdf_oracle.c, pinned to the compiled reference on smaller atoms and
-molecules).  About 30 s of one core.
+molecules).  About a minute of one core.
 Run:  python tests/golden/make_tlf_racah.py
 """
 import os
@@ -45,6 +47,7 @@ def main():
     idx = gold["idx"]
     out = {"random_seed": np.array(RANDOM_SEED)}
     o = pyoracle.Oracle(s, racah1=True)
+    ob = pyoracle.Oracle(pkg.systems.config_system("tlf", seed=0, hmlt="DHFB"), racah1=True)
     zero = np.zeros((n, n), dtype=np.complex128)
     for tag, d in (("atoms", density(scf, s, occs)),
                    ("random", pkg.systems.random_density(n, seed=RANDOM_SEED, real=True))):
@@ -56,6 +59,12 @@ def main():
         out[f"{tag}_x"] = x[idx[:, 0], idx[:, 1]]
         out[f"{tag}_ej"] = 0.5 * np.real(np.sum(d * g))
         out[f"{tag}_ek"] = 0.5 * np.real(np.sum(d * x))
+        t = time.time()
+        _, bx = ob.breit1(d, zero, zero)
+        print(f"{tag}: oracle BREIT1 {time.time() - t:.1f} s, max|BXCH| {np.abs(bx).max():.3e}",
+              flush=True)
+        out[f"{tag}_bx"] = bx[idx[:, 0], idx[:, 1]]
+        out[f"{tag}_eb"] = -0.5 * np.real(np.sum(d * bx))
     np.savez_compressed(os.path.join(HERE, "tlf_racah.npz"), **out)
 
 

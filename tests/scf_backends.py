@@ -29,5 +29,8 @@ class OracleBackend:
         return torch.from_numpy(np.ascontiguousarray(g)), torch.from_numpy(np.ascontiguousarray(x))
 
     def breit(self, dens, ilev):
-        b, x = self.o.brtfast(dens.numpy(), ilev, nthreads=self.nthreads)
+        if self.racah1:  # BRTFAST without one-centre quartets, then BREIT1
+            b, x = self.o.brtfast_racah(dens.numpy(), ilev, nthreads=self.nthreads)
+        else:
+            b, x = self.o.brtfast(dens.numpy(), ilev, nthreads=self.nthreads)
         return torch.from_numpy(np.ascontiguousarray(b)), torch.from_numpy(np.ascontiguousarray(x))

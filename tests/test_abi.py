@@ -35,7 +35,9 @@ def test_null_arguments_return_einval(bdf):
     assert lib.bdf_coulomb_device(None, 3, None, None, None) == bdf.BDF_EINVAL
     assert lib.bdf_coulomb1(None, 0, None, None, 0, None, None, None, None) == bdf.BDF_EINVAL
     assert lib.bdf_coulomb1_device(None, 0, None, None, None) == bdf.BDF_EINVAL
-    assert lib.bdf_racah_info(None, None, None, None) == bdf.BDF_EINVAL
+    assert lib.bdf_racah_info(None, 0, None, None, None) == bdf.BDF_EINVAL
+    assert lib.bdf_breit1(None, 0, None, None, 0, None, None, None, None) == bdf.BDF_EINVAL
+    assert lib.bdf_breit1_device(None, 0, None, None, None) == bdf.BDF_EINVAL
     lib.bdf_destroy(None)
 
 

@@ -1,6 +1,7 @@
-"""HIP one-centre Racah Coulomb build (bdf_coulomb1, the reference's
-COULOMB1 per centre, bertha-scf.f:21540 called at 5415-5421) against the
-oracle, which is pinned to the reference's CLMFAST + COULOMB1 sequence."""
+"""HIP one-centre Racah builds (bdf_coulomb1 and bdf_breit1, the reference's
+COULOMB1 and BREIT1 per centre, bertha-scf.f:21540 and :26014, called at
+5415-5421 and 5447-5453) against the oracle, which is pinned to the
+reference's CLMFAST + COULOMB1 and BRTFAST + BREIT1 sequences."""
 import numpy as np
 import pytest
 
@@ -73,4 +74,47 @@ def test_workload_racah_reads_only_written_betas(bdf, name):
     with bdf.Engine(s, racah1=True) as eng:
         terms, items, uncovered = eng.racah_info()
     print(f"[racah plan] {name}: {terms:.3e} terms, {items} items", flush=True)
+    assert items > 0 and uncovered == 0
+
+
+def _dhfb(bdf, s):
+    return bdf.systems.System(s.name, s.xyz, s.shells, s.shape, hmlt="DHFB")
+
+
+@pytest.mark.parametrize("name", ["ne", "datom", "fatom", "diat", "off", "fdiat", "tri"])
+def test_breit1_matches_oracle(bdf, orc, name):
+    s = _dhfb(bdf, small_systems(bdf)[name])
+    print(f"[breit1] {name}", flush=True)
+    d = bdf.systems.random_density(s.ndim, seed=15, real=(s.shape != "NOSYMM"))
+    bo, xo = orc.Oracle(s, racah1=True).brtfast_racah(d, 3)
+    with bdf.Engine(s, racah1=True) as eng:
+        b, x = eng.breit_racah(d, 3)
+        assert eng.racah_info(breit=True)[2] == 0
+    assert np.abs(b).max() == 0.0
+    assert np.abs(x - xo).max() < FOCK_ABS, (name, np.abs(x - xo).max())
+    il = np.tril_indices(s.ndim, -1)
+    assert np.array_equal(x[il], np.conj(x.T[il]))
+
+
+def test_breit1_exact_beta_on_axial_density(bdf, orc):
+    """exact_beta for BREIT1 on the s-d density: recorded against the MD
+    one-centre Breit build (the oracle pins exact_beta = 0)."""
+    s = _dhfb(bdf, small_systems(bdf)["datom"])
+    d = sd_axial_density(orc.Oracle(s).labels(), s.ndim)
+    with bdf.Engine(s) as md:
+        _, x_md = md.breit(d, 3)
+    with bdf.Engine(s, racah1=True) as eng:
+        _, x_ex = eng.breit_racah(d, 3, exact_beta=True)
+        _, x_rf = eng.breit_racah(d, 3)
+    _, xo = orc.Oracle(s, racah1=True).brtfast_racah(d, 3)
+    print(f"[breit1 sd] exact vs MD {np.abs(x_ex - x_md).max():.3e}, reference-faithful vs MD "
+          f"{np.abs(x_rf - x_md).max():.3e}", flush=True)
+    assert np.abs(x_rf - xo).max() < FOCK_ABS
+
+
+def test_tlf_breit1_reads_only_written_betas(bdf):
+    s = bdf.systems.config_system("tlf", seed=0, hmlt="DHFB")
+    with bdf.Engine(s, racah1=True) as eng:
+        terms, items, uncovered = eng.racah_info(breit=True)
+    print(f"[racah plan] tlf breit: {terms:.3e} terms, {items} items", flush=True)
     assert items > 0 and uncovered == 0

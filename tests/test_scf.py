@@ -212,10 +212,10 @@ def test_superposed_density_places_atom_blocks(bdf, orc, scf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("hmlt", ["DHFR", "DHFB", "DHFR-racah"])
+@pytest.mark.parametrize("hmlt", ["DHFR", "DHFB", "DHFR-racah", "DHFB-racah"])
 def test_engine_scf_matches_oracle_scf(bdf, orc, scf, hmlt):
     print(f"[scf] {hmlt}", flush=True)
-    racah1 = hmlt == "DHFR-racah"
+    racah1 = hmlt.endswith("-racah")
     if hmlt.startswith("DHFR"):
         s = bdf.systems.ne_atom()
         nocc = 10

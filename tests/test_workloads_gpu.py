@@ -184,10 +184,10 @@ def test_serving_workload_same_alone_and_in_mix(bdf):
 
 
 def test_tlf_racah_matches_oracle_samples(bdf, tlf_gold, tlf_density):
-    """bdf_coulomb1 on the heavy diatomic (Tl f shells, NDIM 1340) from zero
-    matrices: the one-centre contributions alone, against the oracle's at
-    the sampled elements (tests/golden/make_tlf_racah.py), for the
-    superposed atomic density and a random one."""
+    """bdf_coulomb1 and bdf_breit1 on the heavy diatomic (Tl f shells, NDIM
+    1340) from zero matrices: the one-centre contributions alone, against
+    the oracle's at the sampled elements (tests/golden/make_tlf_racah.py),
+    for the superposed atomic density and a random one."""
     z = np.load(os.path.join(HERE, "golden", "tlf_racah.npz"))
     s = bdf.systems.config_system("tlf", seed=0, hmlt="DHFR")
     n = s.ndim
@@ -205,3 +205,12 @@ def test_tlf_racah_matches_oracle_samples(bdf, tlf_gold, tlf_density):
             ej, ek = 0.5 * np.real(np.sum(d * g)), 0.5 * np.real(np.sum(d * x))
             assert abs(ej - z[f"{tag}_ej"]) < 1e-12 * max(1.0, abs(z[f"{tag}_ej"])), tag
             assert abs(ek - z[f"{tag}_ek"]) < 1e-12 * max(1.0, abs(z[f"{tag}_ek"])), tag
+    sb = bdf.systems.config_system("tlf", seed=0, hmlt="DHFB")
+    with bdf.Engine(sb, racah1=True) as eng:
+        for tag, d in dens.items():
+            print(f"[tlf racah] breit {tag}", flush=True)
+            _, bx = eng.breit1(d, zero, zero)
+            tol = FOCK_ABS * max(1.0, 1e-3 * np.abs(bx).max())
+            assert np.abs(bx[i, j] - z[f"{tag}_bx"]).max() < tol, tag
+            eb = -0.5 * np.real(np.sum(d * bx))
+            assert abs(eb - z[f"{tag}_eb"]) < 1e-12 * max(1.0, abs(z[f"{tag}_eb"])), tag
