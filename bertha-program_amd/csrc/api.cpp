@@ -192,6 +192,16 @@ static int plan_xacc(const Plan &P, int isym) {
   return n + (n & 1);  // even: 16-byte alignment of what follows
 }
 
+// the same for a Breit plan (its exchange batches use the same layout)
+static int breit_xacc(const BreitPlan &P, int isym) {
+  int n = 0;
+  for (const ClassDesc &cl : P.classes) {
+    const int nC = cl.nbas[2], nD = cl.nbas[3];
+    n = std::max(n, isym != 0 ? 6 * nD + 8 * nC : 2 * (12 * nD + 16 * nC));
+  }
+  return n + (n & 1);
+}
+
 // doubles per workgroup of kernel kind (0 Coulomb, 1 Breit)
 static int64_t slab_stride(const bdf_engine *e, int kind) {
   return (int64_t)e->slab_rows[kind] * e->slab_ld;
@@ -227,7 +237,7 @@ static FockLaunch fock_config(const bdf_engine *e, const DevPlan &dp) {
   FockLaunch fl = fock_launch_config(dp.xacc_n, e->isym);
   if (e->tune_waves) fl.waves = e->tune_waves;
   if (e->tune_cap) fl.cap = e->tune_cap;
-  if (e->tune_waves == 3 && !e->tune_cap) fl.cap = 1;
+  if (e->tune_waves >= 3 && !e->tune_cap) fl.cap = 1;
   return fl;
 }
 
@@ -238,13 +248,23 @@ static int size_launch(bdf_engine *e, DevPlan &dp) {
   return ensure_slab(e, 0, dp.P.max_maxcd, 4 * dp.fl.cap * ntuv(kMaxLab), dp.blocks);
 }
 
-static int breit_blocks(const bdf_engine *e) {
-  return blocks_for(e, e->tune_breit_per_cu ? e->tune_breit_per_cu : 8);
+static int breit_per_cu(const bdf_engine *e) {
+  return e->tune_breit_per_cu ? e->tune_breit_per_cu : 16;
 }
+
+static int breit_blocks(const bdf_engine *e) { return blocks_for(e, breit_per_cu(e)); }
 
 // AB combinations the Breit kernel contracts together
 static int breit_group(const bdf_engine *e) {
   return e->tune_breit_group ? e->tune_breit_group : 1;
+}
+
+// waves per SIMD the Breit kernel is compiled for: one per four workgroups
+// per CU (its workgroups are one wave), 2..4; grouped combinations need the
+// registers of two
+static int breit_waves(const bdf_engine *e) {
+  if (breit_group(e) > 1) return 2;
+  return std::max(2, std::min(4, breit_per_cu(e) / 4));
 }
 
 static int make_plan(bdf_engine *e, int ilev) {
@@ -632,6 +652,8 @@ static int enqueue_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent,
   a.slab_stride = slab_stride(e, 1);
   a.ld = e->slab_ld;
   a.group = rr_out ? 1 : breit_group(e);
+  a.waves = rr_out ? 2 : breit_waves(e);
+  a.xacc_n = breit_xacc(dp.P, e->isym);
   a.isym = e->isym;
   a.intsym = e->intsym;
   a.gaunt = e->gaunt;
@@ -1449,11 +1471,11 @@ int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgrou
 
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
   if (!e || !l) return fail(BDF_EINVAL, "bdf_set_launch: null argument");
-  if (l->waves != 0 && l->waves != 2 && l->waves != 3)
-    return fail(BDF_EINVAL, "bdf_set_launch: waves must be 0, 2 or 3");
+  if (l->waves != 0 && (l->waves < 2 || l->waves > 4))
+    return fail(BDF_EINVAL, "bdf_set_launch: waves must be 0, 2, 3 or 4");
   if (l->cap < 0 || l->cap > 2) return fail(BDF_EINVAL, "bdf_set_launch: cap must be 0, 1 or 2");
-  if (l->waves == 3 && l->cap == 2)
-    return fail(BDF_EINVAL, "bdf_set_launch: three waves run at capacity 1 only");
+  if (l->waves >= 3 && l->cap == 2)
+    return fail(BDF_EINVAL, "bdf_set_launch: three or four waves run at capacity 1 only");
   if (l->per_cu < 0 || l->per_cu > 32 || l->breit_per_cu < 0 || l->breit_per_cu > 32)
     return fail(BDF_EINVAL, "bdf_set_launch: workgroups per CU outside 0..32");
   if (l->breit_group != 0 && l->breit_group != 1 && l->breit_group != 2 && l->breit_group != 4)

@@ -100,6 +100,8 @@ struct BreitArgs {
   const double4 *Ecd;  // E(CD) with column masks and ERIFAST's Z sign applied
   const double4 *Eab;  // E with EPSAB applied (launch_eps_filter), the AB side
   int group;           // AB combinations contracted together: 1, 2 or 4
+  int waves;           // waves per SIMD the kernel is built for: 2, 3 or 4 (group 1), 2
+  int xacc_n;          // exchange-accumulator doubles (dynamic LDS) of the largest class
 };
 
 struct OneArgs {

@@ -1157,16 +1157,15 @@ __global__ void __launch_bounds__(64) breit_flags_kernel(BreitArgs a) {
   }
 }
 
-// LDS of a Breit workgroup contracting GB AB combinations together; at
-// GB = 1 eight workgroups fit a CU's 160 KiB
+// static LDS of a Breit workgroup contracting GB AB combinations together;
+// the exchange accumulators (XView, BreitArgs::xacc_n doubles) follow in
+// dynamic LDS, sized by the plan's largest class
 template <int GB>
 struct BreitShared {
   uint8_t cflag[ntuv(kMaxLfl)];
   uint8_t gflag[ntuv(kMaxLfl)];
   uint8_t gm[GB][ntuv(2 * kMaxL + 1) * 12];
-  double xacc[kXAccMax];  // exchange accumulators (XView)
 };
-static_assert(8 * sizeof(BreitShared<1>) <= 160 * 1024, "Breit LDS limits occupancy");
 
 __device__ __forceinline__ int any4(const double *e) {
   return e[0] != 0.0 || e[1] != 0.0 || e[2] != 0.0 || e[3] != 0.0;
@@ -1179,7 +1178,7 @@ __device__ __forceinline__ int any4(const double *e) {
 // (:18830-19150) and the exchange digestion into BXCH.
 template <int kBreitGroup>
 __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
-                           BreitShared<kBreitGroup> &sh, int lane) {
+                           BreitShared<kBreitGroup> &sh, double *xacc, int lane) {
   const ClassDesc &cl = a.classes[it.cls];
   const int ld = cl.maxcd;  // slab rows packed at the class's kl count
   double *R = slab;
@@ -1188,7 +1187,7 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   const double2 *Dm = a.dent;
   const int nd = a.ndim;
   const int nB = cl.nbas[1], nC = cl.nbas[2], nD = cl.nbas[3];
-  const XView xv{sh.xacc, cl.nbas[2], nD};
+  const XView xv{xacc, cl.nbas[2], nD};
   const int ib = it.ij / nB, jb = it.ij % nB;
   const double ea = a.exps[cl.exp[0] + ib], eb = a.exps[cl.exp[1] + jb];
   const double p = ea + eb;
@@ -1451,13 +1450,18 @@ __device__ void breit_item(const BreitArgs &a, const WorkItem &it, double *slab,
   }
 }
 
-template <int GB>
-__global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
+// W waves per SIMD: the register budget 512 / W (at W = 4 the compiler
+// spills some registers to scratch; the occupancy still pays, the kernel
+// being latency-bound on its R-slab loads)
+template <int GB, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, W)))
+breit_kernel(BreitArgs a) {
   __shared__ BreitShared<GB> sh;
   __shared__ int s_item;
+  extern __shared__ double xacc[];  // a.xacc_n doubles (launch_breit)
   const int lane = threadIdx.x;
   double *slab = a.slab + (int64_t)blockIdx.x * a.slab_stride;
-  for (int w = lane; w < kXAccMax; w += 64) sh.xacc[w] = 0.0;
+  for (int w = lane; w < a.xacc_n; w += 64) xacc[w] = 0.0;
   __syncthreads();
   for (;;) {
     if (lane == 0) s_item = atomicAdd(a.counter, 1);
@@ -1465,7 +1469,7 @@ __global__ void __launch_bounds__(64) breit_kernel(BreitArgs a) {
     int idx = __builtin_amdgcn_readfirstlane(s_item);  // the same in every lane
     __syncthreads();
     if (idx >= a.nitems) break;
-    breit_item<GB>(a, a.items[idx], slab, sh, lane);
+    breit_item<GB>(a, a.items[idx], slab, sh, xacc, lane);
   }
 }
 
@@ -1872,12 +1876,14 @@ static hipError_t launch_fock_w(const FockArgs &a, int nblocks, hipStream_t s, h
 }
 
 bool fock_launch_valid(FockLaunch c) {
-  return (c.waves == 3 && c.cap == 1) || (c.waves == 2 && (c.cap == 1 || c.cap == 2));
+  return ((c.waves == 3 || c.waves == 4) && c.cap == 1) ||
+         (c.waves == 2 && (c.cap == 1 || c.cap == 2));
 }
 
 hipError_t launch_fock(const FockArgs &a, FockLaunch c, int nblocks, hipStream_t s,
                        hipStream_t s32, hipEvent_t fork, hipEvent_t join) {
   if (!fock_launch_valid(c)) return hipErrorInvalidValue;
+  if (c.waves == 4) return launch_fock_w<4, 1>(a, nblocks, s, s32, fork, join);
   if (c.waves == 3) return launch_fock_w<3, 1>(a, nblocks, s, s32, fork, join);
   if (c.cap == 2) return launch_fock_w<2, 2>(a, nblocks, s, s32, fork, join);
   return launch_fock_w<2, 1>(a, nblocks, s, s32, fork, join);
@@ -1895,12 +1901,18 @@ hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(breit_flags_kernel, dim3(nf), dim3(64), 0, s, a);
   if (a.nitems > 0) {
     const dim3 grid(std::max(1, std::min(nblocks, a.nitems)));
-    if (a.group == 1)
-      hipLaunchKernelGGL(breit_kernel<1>, grid, dim3(64), 0, s, a);
-    else if (a.group == 2)
-      hipLaunchKernelGGL(breit_kernel<2>, grid, dim3(64), 0, s, a);
-    else if (a.group == 4)
-      hipLaunchKernelGGL(breit_kernel<4>, grid, dim3(64), 0, s, a);
+    if (a.xacc_n < 0 || a.xacc_n > kXAccMax) return hipErrorInvalidValue;
+    const size_t lds = (size_t)a.xacc_n * sizeof(double);
+    if (a.group == 1 && a.waves == 2)
+      hipLaunchKernelGGL((breit_kernel<1, 2>), grid, dim3(64), lds, s, a);
+    else if (a.group == 1 && a.waves == 3)
+      hipLaunchKernelGGL((breit_kernel<1, 3>), grid, dim3(64), lds, s, a);
+    else if (a.group == 1 && a.waves == 4)
+      hipLaunchKernelGGL((breit_kernel<1, 4>), grid, dim3(64), lds, s, a);
+    else if (a.group == 2 && a.waves == 2)
+      hipLaunchKernelGGL((breit_kernel<2, 2>), grid, dim3(64), lds, s, a);
+    else if (a.group == 4 && a.waves == 2)
+      hipLaunchKernelGGL((breit_kernel<4, 2>), grid, dim3(64), lds, s, a);
     else
       return hipErrorInvalidValue;
   }

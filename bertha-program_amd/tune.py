@@ -29,9 +29,10 @@ TABLE = os.path.join(HERE, "tuned_launch.json")
 # (waves, cap, workgroups per CU) of fock_kernel: the instantiated shapes
 # (three waves at capacity 1, two at capacity 1 or 2) at the occupancies
 # their registers allow and below
-COULOMB_SHAPES = [(3, 1, 12), (3, 1, 8), (2, 2, 8), (2, 1, 8), (2, 2, 6), (2, 1, 6),
-                  (2, 2, 4)]
-BREIT_SHAPES = [(8, 1), (6, 1), (8, 2), (4, 4)]  # (workgroups per CU, AB group)
+COULOMB_SHAPES = [(4, 1, 16), (3, 1, 12), (3, 1, 8), (2, 2, 8), (2, 1, 8), (2, 2, 6),
+                  (2, 1, 6), (2, 2, 4)]
+# (workgroups per CU, AB group); one wave per SIMD for each four workgroups
+BREIT_SHAPES = [(16, 1), (12, 1), (8, 1), (8, 2)]
 
 
 def signature(system) -> str:
@@ -104,9 +105,9 @@ def tune(pkg, system, reps: int = 3, log=print) -> dict:
                 f"{ms:.2f} ms")
         bbest = min(breit, key=breit.get)
         entry.update({"breit_per_cu": bbest[0], "breit_group": bbest[1], "breit_ms": breit[bbest]})
-        entry["default"]["breit_per_cu"] = 8
+        entry["default"]["breit_per_cu"] = 16
         entry["default"]["breit_group"] = 1
-        entry["default"]["breit_ms"] = breit.get((8, 1))
+        entry["default"]["breit_ms"] = breit.get((16, 1))
     eng.close()
     return entry
 

@@ -292,12 +292,14 @@ int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long
    buffers) as a HIP graph and replayed on later calls with the same
    buffers (bdf_coulomb and bdf_breit reuse one set of buffers); the
    captured launches are the uncaptured ones, so results agree to the
-   rounding of the kernels' atomic accumulation.  waves (2 or 3 waves per
-   SIMD), cap (group capacity 1 or 2; three waves run at capacity 1) and
+   rounding of the kernels' atomic accumulation.  waves (2, 3 or 4 waves per
+   SIMD), cap (group capacity 1 or 2; three or four waves run at capacity 1) and
    per_cu (persistent workgroups per compute unit) set the Coulomb kernel's
-   launch shape, breit_per_cu the Breit kernel's workgroups per CU and
-   breit_group the number of AB combinations it contracts together (1, 2 or
-   4; its scratch grows with it); 0 keeps the engine's choice.  Takes effect
+   launch shape, breit_per_cu the Breit kernel's workgroups per CU (one
+   wave each; the kernel is built for one wave per SIMD per four of them,
+   2 to 4 waves; default 16) and breit_group the number of AB combinations
+   it contracts together (1, 2 or 4, the last two at two waves per SIMD;
+   its scratch grows with it); 0 keeps the engine's choice.  Takes effect
    from the next build; captured builds are dropped. */
 typedef struct {
   int graph;

@@ -337,7 +337,7 @@ def test_graph_builds_equal_stream_builds(bdf, systems, name, kw):
         assert np.abs(g - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
 
 
-@pytest.mark.parametrize("shape", [(3, 1, 12), (2, 1, 8), (2, 2, 8), (2, 1, 4)])
+@pytest.mark.parametrize("shape", [(4, 1, 16), (3, 1, 12), (2, 1, 8), (2, 2, 8), (2, 1, 4)])
 def test_forced_launch_shapes_match_oracle(bdf, orc, systems, shape):
     """Every launch shape the autotuner may choose gives the oracle's build,
     on the complex path (three-centre) and the real path (f-shell diatomic)."""
@@ -355,6 +355,7 @@ def test_forced_launch_shapes_match_oracle(bdf, orc, systems, shape):
 
 def test_set_launch_rejects_bad_shapes(bdf, systems):
     with bdf.Engine(systems["ne"]) as eng:
-        for bad in ({"waves": 4}, {"cap": 3}, {"waves": 3, "cap": 2}, {"per_cu": 40}):
+        for bad in ({"waves": 5}, {"cap": 3}, {"waves": 3, "cap": 2}, {"waves": 4, "cap": 2},
+                    {"per_cu": 40}):
             with pytest.raises(bdf.BerthaError):
                 eng.set_launch(**bad)
