@@ -88,8 +88,13 @@ def atoms_density(pkg, system):
                                       shape="ATOMIC", charges=[z])
             eng = pkg.Engine(atom)
             try:
-                ovlp, hnuc, hkin = (torch.from_numpy(np.ascontiguousarray(m))
-                                    for m in eng.one_electron(xi=xi))
+                n = eng.ndim
+                mats = [torch.zeros((n, n), dtype=torch.complex128, device="cuda")
+                        for _ in range(3)]
+                eng.one_electron_device(*[m.data_ptr() for m in mats], xi=xi)
+                torch.cuda.synchronize()
+                # the device arrays hold the column-major matrices, i.e. M^T
+                ovlp, hnuc, hkin = (m.T.cpu().contiguous() for m in mats)
                 nskp = eng.nskp
             finally:
                 eng.close()

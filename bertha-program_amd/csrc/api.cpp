@@ -1378,12 +1378,12 @@ int bdf_one_electron(bdf_engine *e, const double *znuc, const double *xi, double
   HIPCHK(k.alloc((size_t)nd * nd));
   int rc = bdf_one_electron_device(e, znuc, xi, cv, o.p, h.p, k.p);
   if (rc != BDF_OK) return rc;
-  HIPCHK(hipMemcpy2D(ovlp, ld * sizeof(double2), o.p, nd * sizeof(double2), nd * sizeof(double2),
-                     nd, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy2D(hnuc, ld * sizeof(double2), h.p, nd * sizeof(double2), nd * sizeof(double2),
-                     nd, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy2D(hkin, ld * sizeof(double2), k.p, nd * sizeof(double2), nd * sizeof(double2),
-                     nd, hipMemcpyDeviceToHost));
+  // on the engine's stream, as every other host entry point copies
+  const size_t row = nd * sizeof(double2), lrow = ld * sizeof(double2);
+  HIPCHK(hipMemcpy2DAsync(ovlp, lrow, o.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpy2DAsync(hnuc, lrow, h.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpy2DAsync(hkin, lrow, k.p, row, row, nd, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   return BDF_OK;
 }
 
