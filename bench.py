@@ -23,9 +23,11 @@ Legs:
             of every class, classes as OpenMP work items) on all host cores
             and on one core, scaled by the fraction of the build's ERIFAST/
             BIIFAST calls the sample made
-  serve     Poisson-arriving DHF solves of the serving workload through the
-            continuous-batching server: solves/s with arrivals above
-            capacity, then p50/p99 latency with arrivals below it
+  serve     the serving workload's stream of 512 Poisson-arriving DHF
+            solves (atoms and diatomics, NDIM 40-748) through the
+            continuous-batching server at 3 arrivals/s, near its capacity:
+            p50/p99 latency from arrival to converged energy and solves/s;
+            then 32 solves at 1.2/s, well below capacity
 
 Other workloads: --workload water|ne|ring (Dirac-Coulomb only), e.g. the
 12-centre ring with Schwarz screening at SENS 1e-12:
@@ -153,10 +155,10 @@ def main() -> int:
                     help="oracle sample: every S-th primitive pair (0: by workload)")
     ap.add_argument("--split", type=float, default=0.1,
                     help="Schwarz split threshold of the mixed-precision leg (0: skip the leg)")
-    ap.add_argument("--serve", type=int, default=64,
+    ap.add_argument("--serve", type=int, default=512,
                     help="requests of the served-SCF leg (0: skip the leg)")
-    ap.add_argument("--serve-rate", type=float, default=8.0,
-                    help="Poisson arrivals per second of the served-SCF leg (above capacity)")
+    ap.add_argument("--serve-rate", type=float, default=3.0,
+                    help="Poisson arrivals per second of the served-SCF leg (near capacity)")
     ap.add_argument("--serve-sustained-rate", type=float, default=1.2,
                     help="arrival rate of a second served-SCF leg below capacity (0: skip)")
     ap.add_argument("--serve-sustained", type=int, default=32,

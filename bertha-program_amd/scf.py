@@ -218,18 +218,27 @@ def orthogonaliser(ovlp):
                                          upper=False)
 
 
-# largest NDIM whose transformed Fock matrix is diagonalised on the host (the
-# serving mix, NDIM 40-748: 1.59 -> 2.26 solves/s against MAGMA on the device)
+# largest NDIM whose complex transformed Fock matrix is diagonalised on the
+# host (NDIM 40-748: 1.59 -> 2.26 solves/s against MAGMA on the device)
 HOST_EIGH_MAX = 768
 
 
-def solve_fock(fock, li):
+def solve_fock(fock, li, real: bool = False):
     """ZHEGV(1,'V','L'): F C = S C e from the lower triangle of F, with
     li = orthogonaliser(S).  Returns (e, C, finite) where finite is a device
-    flag telling whether every eigenvector is finite."""
+    flag telling whether every eigenvector is finite.
+
+    real: the matrices of an atom or axial molecule (SHAPE ATOMIC, DIATOM,
+    LINEAR), whose one- and two-electron matrices the real path builds with
+    zero imaginary parts: the transformed F is real symmetric and is
+    diagonalised on the device in real arithmetic (rocSOLVER through torch),
+    with no copy to the host."""
     f = torch.tril(fock) + torch.tril(fock, -1).conj().T
     fp = li @ f @ li.conj().T
-    if fp.is_cuda and fp.shape[0] <= HOST_EIGH_MAX:
+    if real and fp.is_cuda:
+        w, v = torch.linalg.eigh(fp.real.contiguous())
+        v = v.to(fp.dtype)
+    elif fp.is_cuda and fp.shape[0] <= HOST_EIGH_MAX:
         # small matrices: the host's LAPACK, as the reference's ZHEGV; MAGMA's
         # hybrid solver has a fixed cost that dominates at these sizes
         w, v = torch.linalg.eigh(fp.cpu())
@@ -313,7 +322,7 @@ def run_scf(system: System, nocc: int, backend, *, shifts=(1.0, 1.0, 1.0), ilev:
             shifted = shlv
         nmlev[ilev] += 1
         fock = diis(nmlev[ilev], fock, dens, ovlp)
-        eig, coef, finite = solve_fock(fock, li)
+        eig, coef, finite = solve_fock(fock, li, real=system.shape != "NOSYMM")
         dprev = dens
         dens = density(coef, nskp, nocc)
         if it > 1 and diis.last_errmax <= ERRCOM:
