@@ -313,7 +313,7 @@ typedef struct {
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l);
 
 /* How the stage-ilev Coulomb build is launched: waves per SIMD of its
-   quartet kernel (2 or 3), the capacity multiple of its AB-combination
+   quartet kernel (2, 3 or 4), the capacity multiple of its AB-combination
    groups (1 or 2), persistent workgroups, and the engine's scratch in
    bytes.  No counterpart in the reference. */
 int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgroups,
