@@ -62,6 +62,7 @@ class _Launch(_C.Structure):
         ("per_cu", _C.c_int),
         ("breit_per_cu", _C.c_int),
         ("breit_group", _C.c_int),
+        ("breit_kernel", _C.c_int),
     ]
 
 
@@ -191,7 +192,7 @@ class Engine:
         # launch settings (bdf_set_launch), with the autotuner's shapes for
         # this kind of system (tune.py) unless tuned is False
         self._launch = {"graph": False, "waves": 0, "cap": 0, "per_cu": 0, "breit_per_cu": 0,
-                        "breit_group": 0}
+                        "breit_group": 0, "breit_kernel": 0}
         self.tuning = None
         upd = {"graph": bool(graph)}
         if tuned:
@@ -402,14 +403,15 @@ class Engine:
     def set_launch(self, **kw) -> None:
         """bdf_set_launch: graph (HIP-graph replay of builds), waves, cap,
         per_cu, breit_per_cu, breit_group (the quartet kernels' launch
-        shapes, 0: the engine's choice); settings not named keep their
+        shapes, 0: the engine's choice), breit_kernel (1 vector-ALU, 2
+        matrix-core, 0 the engine's choice); settings not named keep their
         current values."""
         bad = set(kw) - set(self._launch)
         if bad:
             raise TypeError(f"unknown launch settings {sorted(bad)}")
         new = dict(self._launch, **kw)
         lo = _Launch(1 if new["graph"] else 0, new["waves"], new["cap"], new["per_cu"],
-                     new["breit_per_cu"], new["breit_group"])
+                     new["breit_per_cu"], new["breit_group"], new["breit_kernel"])
         _check(library().bdf_set_launch(self._h, _C.byref(lo)))
         self._launch = new
 

@@ -78,6 +78,14 @@ struct DevBreitPlan {
   DevBuf<uint8_t> masks;
   DevBuf<WorkItem> items;
   DevBuf<uint8_t> stale;
+  // matrix-core kernel (real path)
+  DevBuf<BrTile> tiles;
+  DevBuf<int> tile_cd;
+  DevBuf<WorkItem> titems;
+  DevBuf<int> sig, csig;
+  DevBuf<int> tile_pair;
+  DevBuf<BrComp> comp;
+  DevBuf<WorkItem> rest_items;
 };
 
 // device copy of the one-centre Racah plan
@@ -108,6 +116,7 @@ struct bdf_engine {
   Etabs T;
   int isym = 0, intsym = 1, racah1 = 0;
   std::vector<uint8_t> eflags;
+  std::vector<uint8_t> zflags;  // DHFB: exact-nonzero flags of the E tables
   DevBuf<double> exps, normL, normS;
   DevBuf<EPair> pairs;
   DevBuf<ETerm> terms;
@@ -153,6 +162,7 @@ struct bdf_engine {
   // launch settings (bdf_set_launch): 0 = the engine's choice
   int tune_waves = 0, tune_cap = 0, tune_per_cu = 0, tune_breit_per_cu = 0;
   int tune_breit_group = 0;
+  int tune_breit_kernel = 0;
   int use_graph = 0;
   std::vector<GraphEntry> graphs;
   // this device's static scratch (bdf_coulomb_device_static)
@@ -257,6 +267,22 @@ static int breit_blocks(const bdf_engine *e) { return blocks_for(e, breit_per_cu
 // AB combinations the Breit kernel contracts together
 static int breit_group(const bdf_engine *e) {
   return e->tune_breit_group ? e->tune_breit_group : 1;
+}
+
+// the matrix-core Breit kernel for this plan: on the real path, where its
+// largest class's LDS fits a compute unit, unless the vector-ALU kernel is
+// asked for
+static bool breit_tc(const bdf_engine *e, const BreitPlan &P) {
+  return e->isym != 0 && e->tune_breit_kernel != 1 && P.tc_ok && !P.titems.empty() &&
+         P.tc_lds <= (size_t)kTcLdsMax;
+}
+
+// workgroups (kTcWaves waves each) per CU of the matrix-core kernel: one, its
+// two waves per SIMD sharing the LDS tables
+static int breit_tc_blocks(const bdf_engine *e, const BreitPlan &P) {
+  (void)P;
+  const int per_cu = 1;
+  return e->user_blocks ? e->nblocks : e->cus * per_cu;
 }
 
 // waves per SIMD the Breit kernel is compiled for: one per four workgroups
@@ -606,7 +632,7 @@ static int make_breit_plan(bdf_engine *e, int ilev) {
   std::string err;
   try {
     err = build_breit_plan(e->B, e->T, e->eflags, e->isym, e->intsym, e->racah1, ilev, e->gaunt,
-                           dp.P);
+                           dp.P, e->zflags.empty() ? nullptr : &e->zflags);
   } catch (const std::out_of_range &) {
     err = "Breit plan: missing E-coefficient block";
   }
@@ -617,8 +643,23 @@ static int make_breit_plan(bdf_engine *e, int ilev) {
   HIPCHK(dp.masks.upload(dp.P.masks, e->stream));
   HIPCHK(dp.items.upload(dp.P.items, e->stream));
   HIPCHK(dp.stale.alloc((size_t)std::max(dp.P.nflag, 1)));
+  if (dp.P.tc_ok) {
+    HIPCHK(dp.tiles.upload(dp.P.tiles, e->stream));
+    HIPCHK(dp.tile_cd.upload(dp.P.tile_cd, e->stream));
+    HIPCHK(dp.tile_pair.upload(dp.P.tile_pair, e->stream));
+    HIPCHK(dp.comp.upload(dp.P.comp, e->stream));
+    HIPCHK(dp.titems.upload(dp.P.titems, e->stream));
+    HIPCHK(dp.rest_items.upload(dp.P.rest_items, e->stream));
+    HIPCHK(dp.sig.upload(dp.P.tc_sig, e->stream));
+    HIPCHK(dp.csig.upload(dp.P.tc_csig, e->stream));
+  }
+  if (e->tune_breit_kernel == 2 && !breit_tc(e, dp.P))
+    return fail(BDF_EUNSUPP, "matrix-core Breit kernel: not a real-path system, or its LDS "
+                             "exceeds a compute unit");
   int rc = ensure_slab(e, 1, dp.P.max_maxcd, 12 * breit_group(e) * std::max(dp.P.max_ntg, 1),
                        breit_blocks(e));
+  if (rc == BDF_OK && breit_tc(e, dp.P))
+    rc = ensure_slab(e, 1, dp.P.max_maxcd, 0, breit_tc_blocks(e, dp.P));
   if (rc != BDF_OK) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
   dp.built = 1;
@@ -662,7 +703,44 @@ static int enqueue_breit(bdf_engine *e, DevBreitPlan &dp, const double2 *d_dent,
   a.Eab = e->Eab.p;
   a.rr_out = rr_out;
   if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
-  HIPCHK(launch_breit(a, breit_blocks(e), e->stream));
+  if (!rr_out && breit_tc(e, dp.P)) {
+    BreitTcArgs t;
+    t.classes = dp.classes.p;
+    t.abs = dp.abs.p;
+    t.cds = dp.cds.p;
+    t.masks = dp.masks.p;
+    t.tiles = dp.tiles.p;
+    t.tile_cd = dp.tile_cd.p;
+    t.tile_pair = dp.tile_pair.p;
+    t.comp = dp.comp.p;
+    t.max_npair = dp.P.max_npair;
+    t.items = dp.titems.p;
+    t.nitems = (int)dp.P.titems.size();
+    t.counter = e->counter.p;
+    t.exps = e->exps.p;
+    t.Eab = e->Eab.p;
+    t.Ecd = e->Ecd.p;
+    t.dent = d_dent;
+    t.bxch = d_bxch;
+    t.ndim = nd;
+    t.slab = e->slab.p;
+    t.slab_stride = slab_stride(e, 1);
+    t.intsym = e->intsym;
+    t.gaunt = e->gaunt;
+    t.stale = dp.stale.p;
+    t.sig = dp.sig.p;
+    t.csig = dp.csig.p;
+    BreitArgs rest = a;
+    rest.items = dp.rest_items.p;
+    rest.nitems = (int)dp.P.rest_items.size();
+    rest.counter = e->counter.p + 1;
+    rest.group = 1;
+    rest.waves = 4;
+    HIPCHK(launch_breit_tc(a, t, breit_tc_blocks(e, dp.P), dp.P.tc_lds, dp.P.max_nu, rest,
+                           blocks_for(e, 16), e->stream));
+  } else {
+    HIPCHK(launch_breit(a, breit_blocks(e), e->stream));
+  }
   if (timing) HIPCHK(hipEventRecord(e->ev2, e->stream));
   return BDF_OK;
 }
@@ -773,10 +851,22 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     if (e->mixed) HIPCHK(e->Ecd32.alloc((size_t)e->T.total));
     fa.Ecd = e->Ecd.p;
     fa.Ecd32 = e->mixed ? e->Ecd32.p : nullptr;
+    DevBuf<uint8_t> zf;
+    fa.zflags = nullptr;
+    if (e->breit) {
+      HIPCHK(zf.alloc((size_t)e->T.nflag));
+      fa.zflags = zf.p;
+    }
     HIPCHK(launch_eflag(fa, e->stream));
     e->eflags.resize(e->T.nflag);
     HIPCHK(hipMemcpyAsync(e->eflags.data(), e->flags.p, e->T.nflag, hipMemcpyDeviceToHost,
                           e->stream));
+    if (e->breit) {
+      e->zflags.resize(e->T.nflag);
+      HIPCHK(hipMemcpyAsync(e->zflags.data(), zf.p, e->T.nflag, hipMemcpyDeviceToHost,
+                            e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+    }
     HIPCHK(e->icn.upload(e->B.labicn, e->stream));
     HIPCHK(e->kqn.upload(e->B.labkqn, e->stream));
     HIPCHK(e->mqn.upload(e->B.labmqn, e->stream));
@@ -1480,6 +1570,8 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
     return fail(BDF_EINVAL, "bdf_set_launch: workgroups per CU outside 0..32");
   if (l->breit_group != 0 && l->breit_group != 1 && l->breit_group != 2 && l->breit_group != 4)
     return fail(BDF_EINVAL, "bdf_set_launch: breit_group must be 0, 1, 2 or 4");
+  if (l->breit_kernel < 0 || l->breit_kernel > 2)
+    return fail(BDF_EINVAL, "bdf_set_launch: breit_kernel must be 0, 1 or 2");
   HIPCHK(hipSetDevice(e->device));
   HIPCHK(hipStreamSynchronize(e->stream));
   drop_graphs(e);
@@ -1489,6 +1581,7 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
   e->tune_per_cu = l->per_cu;
   e->tune_breit_per_cu = l->breit_per_cu;
   e->tune_breit_group = l->breit_group;
+  e->tune_breit_kernel = l->breit_kernel;
   for (DevPlan &dp : e->plans)
     if (dp.built) {
       int rc = size_launch(e, dp);
@@ -1496,8 +1589,13 @@ int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
     }
   for (DevBreitPlan &bp : e->bplans)
     if (bp.built) {
+      if (e->tune_breit_kernel == 2 && !breit_tc(e, bp.P))
+        return fail(BDF_EUNSUPP, "matrix-core Breit kernel: not a real-path system, or its LDS "
+                                 "exceeds a compute unit");
       int rc = ensure_slab(e, 1, bp.P.max_maxcd,
                            12 * breit_group(e) * std::max(bp.P.max_ntg, 1), breit_blocks(e));
+      if (rc == BDF_OK && breit_tc(e, bp.P))
+        rc = ensure_slab(e, 1, bp.P.max_maxcd, 0, breit_tc_blocks(e, bp.P));
       if (rc != BDF_OK) return rc;
     }
   return BDF_OK;

@@ -145,6 +145,103 @@ struct BrCD {
   int8_t pab[2], pcd[2];
 };
 
+// ---- matrix-core Breit kernel (real path, ISYM 1 and 2) -------------------
+// On the real path every component c of an LS table E^j (j = x, y, z) is
+// nonzero for bra Hermite indices t of at most one parity class (on the
+// axes where all centres sit at 0), and so is every component of G^ix for a
+// given parity class of t'.  The kernel's 16 x 16 x 4 FP64 matrix products
+// therefore take as rows sixteen AB combinations of one class, each at the
+// component that class selects (kap), and as output the one component of
+// G^ix that lives in the class of t' (cout); the plan checks this structure
+// from the tables' exact-zero flags and falls back to the vector-ALU kernel
+// where it does not hold.
+//
+// A tile holds up to kTcRows combinations; its calls are listed once
+// (tile_cd[(tile * kTcMaxU + u) * kTcRows + g]: the BrCD index of
+// combination g's call u, -1 where g has no such call) so that one E(CD)
+// load serves every combination having the call, and its (call,
+// combination) pairs are numbered for the partial sums (tile_pair, u | g <<
+// 8, from pair_begin).
+constexpr int kTcRows = 16;
+constexpr int kTcMaxU = 16;
+constexpr int kTcNone = 4;  // no component
+constexpr int kTcWaves = 8;  // waves of a workgroup (two per SIMD)
+constexpr int kTcThreads = 64 * kTcWaves;
+// the 16 columns of the first contraction: kTcQ ket pairs kl of a sub-tile
+// times kTcR ket Hermite indices t' of a group
+constexpr int kTcQ = 2, kTcR = 8;
+struct BrTile {
+  int ab[kTcRows];  // BrAB index of row g, -1 = empty
+  int nu;           // calls of the tile
+  int fresh;        // 1: the class's first combination, which reads this pair's
+                    // own R-column flags (BIIFAST's IRC); every other tile
+                    // reads the class's stale flags
+  int pair_begin, npair;
+  // the component of G^ix living in ket class c, common to the tile's rows
+  // (kTcNone: G^ix vanishes there for every row)
+  uint8_t cout[3][8];
+};
+// per BrAB: kap[j][c] (component of E^j living in bra class c) and
+// cout[ix][c] (component of G^ix living in ket class c), kTcNone if none
+struct BrComp {
+  uint8_t kap[3][8];
+  uint8_t cout[3][8];
+};
+
+// Hermite indices h < N(L) by parity class under an axis mask m (the class
+// of h is (h_x & 1 | (h_y & 1) << 1 | (h_z & 1) << 2) & m): sz[c] members of
+// class c, in the Hermite order of CARTIND.
+inline void hermite_class_sizes(int m, int L, int sz[8]) {
+  for (int c = 0; c < 8; ++c) sz[c] = 0;
+  for (int n = 0; n <= L; ++n)
+    for (int t = 0; t <= n; ++t)
+      for (int u = 0; u <= n - t; ++u) {
+        const int v = n - t - u;
+        ++sz[((t & 1) | ((u & 1) << 1) | ((v & 1) << 2)) & m];
+      }
+}
+
+// Dynamic LDS of one workgroup of the matrix-core Breit kernel for a class,
+// byte offsets of its arrays: nab = N(L_AB) bra Hermite rows, wrows rows per
+// gauge table, nr0 class-0 R rows, nt = N(L) R columns, nw tables (10
+// Breit, 1 Gaunt), ngrp t' groups, ntg = N(L_CD), npair partial-sum slots.
+struct TcLds {
+  int et, w, wrec, rt, gbuf, rawb, dig, cdoff, kp, pidx, comp, cls0, tpl, tco, rowtab, cf, gf, gmk,
+      bytes;
+};
+// digestion constants of one (call, combination) pair of a tile
+struct TcDig {
+  int fa[2], fb[2], fc[2], fd[2];
+  int iflg;
+  int8_t pab[2], pcd[2];
+};
+__host__ __device__ inline TcLds tc_lds_layout(int nab, int wrows, int nr0, int nt, int nw,
+                                               int ngrp, int ntg, int npair) {
+  TcLds o;
+  int b = 0;
+  o.et = b;     b += 3 * (nab + 1) * kTcRows * 8;  // E(AB) [3][nab + 1][16 g]
+  o.w = b;      b += nw * wrows * kTcQ * 8;        // tables [nw][wrows][Q]
+  o.wrec = b;   b += nw * wrows * 8;               // their recipes (one per row)
+  o.rt = b;     b += nr0 * kTcQ * 8;               // class-0 R rows [nr0][Q]
+  o.gbuf = b;   b += kTcWaves * kTcRows * 16 * 8;  // G^ix per wave [Q][R][16 g]
+  o.rawb = b;   b += npair * kTcQ * 4 * 8;         // partial sums [pair][Q][4]
+  o.dig = b;    b += npair * (int)sizeof(TcDig);   // digestion constants [pair]
+  o.cdoff = b;  b += kTcMaxU * 3 * 8;              // E(CD) offsets [u][3]
+  o.kp = b;     b += 5 * kTcQ * 8;                 // 1/(2 alpha) [Q], PQ [3][Q], prefactor [Q]
+  o.pidx = b;   b += kTcMaxU * kTcRows * 2;        // pair of (u, g), -1: none
+  o.comp = b;   b += kTcRows * 48;                 // BrComp of each row
+  o.cls0 = b;   b += (nr0 * 2 + 7) & ~7;           // class-0 Hermite indices [nr0]
+  o.tpl = b;    b += (ntg * 2 + 7) & ~7;           // ket t' in class order [ntg]
+  o.tco = b;    b += 24;                           // the tile's cout [3][8]
+  o.rowtab = b; b += (ngrp * kTcR * (nab + 1) * 2 + 7) & ~7;  // rows [grp][R][nab + 1]
+  o.cf = b;     b += (nt + 7) & ~7;                // this pair's R-column flags
+  o.gf = b;     b += (nt + 7) & ~7;                // the class's stale flags
+  o.gmk = b;    b += (kTcRows * ntg * 3 + 7) & ~7; // G masks [16 g][ntg][3] (4 bits)
+  o.bytes = b;
+  return o;
+}
+constexpr int kTcLdsMax = 160 * 1024 - 256;
+
 struct BreitPlan {
   std::vector<ClassDesc> classes;
   std::vector<BrAB> abs;
@@ -155,6 +252,24 @@ struct BreitPlan {
   int nflag = 0;          // gauge-term flag bytes over all classes
   double quartets = 0.0;
   double flops = 0.0;
+  // matrix-core kernel (real path): tiles, their calls and pairs, the
+  // components of each combination, and work items (class, ij, tiles
+  // [ab_lo, ab_hi)) over the classes whose LDS fits a compute unit; the
+  // vector-ALU kernel takes rest_items, the items of the other classes.
+  // tc_sig holds one block per (axis mask, L_AB, L_CD) signature (layout at
+  // tc_signature), tc_csig each class's block offset.  tc_ok is 0 where the
+  // one-component structure does not hold (the vector-ALU kernel then takes
+  // the whole build).
+  std::vector<int> ab_pair;  // 3 LS table pairs per BrAB
+  std::vector<BrTile> tiles;
+  std::vector<int> tile_cd;
+  std::vector<int> tile_pair;
+  std::vector<BrComp> comp;
+  std::vector<WorkItem> titems, rest_items;
+  std::vector<int> tc_sig;
+  std::vector<int> tc_csig;
+  size_t tc_lds = 0;
+  int max_nu = 0, max_npair = 0, tc_ok = 0;
 };
 
 // ---- one-centre Racah plans (COULOMB1, bertha-scf.f:21540; BREIT1, 26014) --

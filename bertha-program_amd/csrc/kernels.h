@@ -25,6 +25,7 @@ struct EflagArgs {
   uint8_t *flags;
   double4 *Ecd;   // if set: masked, signed E(CD) copy (and its FP32 copy Ecd32)
   float4 *Ecd32;
+  uint8_t *zflags;  // if set: per (pair, tuv, component) 1 where any entry is nonzero
 };
 
 struct FockArgs {
@@ -102,6 +103,35 @@ struct BreitArgs {
   int group;           // AB combinations contracted together: 1, 2 or 4
   int waves;           // waves per SIMD the kernel is built for: 2, 3 or 4 (group 1), 2
   int xacc_n;          // exchange-accumulator doubles (dynamic LDS) of the largest class
+};
+
+// matrix-core Breit kernel of the real path (breit_tc.inc): items are
+// (class, ij, tiles [ab_lo, ab_hi)) of BreitPlan::titems
+struct BreitTcArgs {
+  const ClassDesc *classes;
+  const BrAB *abs;
+  const BrCD *cds;
+  const uint8_t *masks;
+  const BrTile *tiles;
+  const int *tile_cd;
+  const int *tile_pair;
+  const BrComp *comp;   // per BrAB
+  int max_npair;        // partial-sum slots the LDS layout reserves
+  const WorkItem *items;
+  int nitems;
+  int *counter;
+  const double *exps;
+  const double4 *Eab;   // E with EPSAB applied (AB side)
+  const double4 *Ecd;   // E(CD) with column masks and ERIFAST's Z sign
+  const double2 *dent;
+  double2 *bxch;
+  int ndim;
+  double *slab;         // per workgroup: R rows and Boys rows of the class
+  int64_t slab_stride;
+  int intsym, gaunt;
+  const uint8_t *stale; // per class: column flags of the last R batch
+  const int *sig;       // signature blocks (BreitPlan::tc_sig)
+  const int *csig;      // per class: offset of its block
 };
 
 struct OneArgs {
@@ -188,6 +218,13 @@ hipError_t launch_diag(const DiagArgs &a, hipStream_t s);
 // (BreitArgs::group: 1, 2 or 4; its scratch holds 12 N(L_CD) G rows for each)
 constexpr int kMaxBreitGroup = 4;
 hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s);
+// the matrix-core kernel for the same build (real path): the stale-flag pass
+// of `flags`, then breit_tc_kernel with `lds` bytes of dynamic LDS per
+// workgroup and call slots for `nu` calls per tile, then the vector-ALU
+// kernel on the items of `rest` (classes beyond the LDS; its own work
+// counter), then the SL completion
+hipError_t launch_breit_tc(const BreitArgs &flags, const BreitTcArgs &a, int nblocks, size_t lds,
+                           int nu, const BreitArgs &rest, int rest_blocks, hipStream_t s);
 hipError_t launch_oneel(const OneArgs &a, hipStream_t s);
 
 }  // namespace bdf

@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(128) etab_kernel(EtabArgs a) {
   }
 }
 
-// per (pair, tuv, component): sum over ij of |E| > 1e-10 (E0LLCD0 EPSCD)
+// per (pair, tuv, component): sum over ij of |E| > 1e-10 (E0LLCD0 EPSCD);
+// with zflags also whether any entry is nonzero at all
 __global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
   int p = blockIdx.x;
   const EPair P = a.pairs[p];
@@ -203,6 +204,10 @@ __global__ void __launch_bounds__(64) eflag_kernel(EflagArgs a) {
     }
     uint8_t *f = a.flags + a.flag_off[p] + 4 * n;
     f[0] = s0 > 1.0e-10; f[1] = s1 > 1.0e-10; f[2] = s2 > 1.0e-10; f[3] = s3 > 1.0e-10;
+    if (a.zflags) {
+      uint8_t *z = a.zflags + a.flag_off[p] + 4 * n;
+      z[0] = s0 > 0.0; z[1] = s1 > 0.0; z[2] = s2 > 0.0; z[3] = s3 > 0.0;
+    }
   }
 }
 
@@ -1664,6 +1669,8 @@ __global__ void zero_kernel(double2 *a, int64_t n, double2 *b, int *counter,
   if (g == 0 && dmax) *dmax = 0ull;
 }
 
+#include "breit_tc.inc"
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -1745,6 +1752,31 @@ hipError_t upload_constants() {
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_tperm), perm, sizeof(perm));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_coff), coff, sizeof(coff));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_hsp), hsp, sizeof(hsp));
+    if (e != hipSuccess) return e;
+  }
+  // parity-class ranks, sizes and class-0 members of all h < N(kMaxLfl)
+  // (the matrix-core Breit kernel's compact tables)
+  {
+    static int16_t crank[8][ntuv(kMaxLfl)], cls0[8][ntuv(kMaxLfl)];
+    static int16_t csz[8][kMaxLfl + 1][8];
+    for (int m = 0; m < 8; ++m) {
+      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int h = 0; h < ntuv(kMaxLfl); ++h) {
+        const uint32_t x = tuv[h];
+        const int c = (int)(((x >> 10) & 1) | (((x >> 5) & 1) << 1) | ((x & 1) << 2)) & m;
+        if (c == 0) cls0[m][cnt[0]] = (int16_t)h;
+        crank[m][h] = (int16_t)cnt[c]++;
+      }
+      for (int h = cnt[0]; h < ntuv(kMaxLfl); ++h) cls0[m][h] = 0;
+      for (int L = 0; L <= kMaxLfl; ++L) {
+        int sz[8];
+        hermite_class_sizes(m, L, sz);
+        for (int c = 0; c < 8; ++c) csz[m][L][c] = (int16_t)sz[c];
+      }
+    }
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_crank), crank, sizeof(crank));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_cls0), cls0, sizeof(cls0));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_csz), csz, sizeof(csz));
     if (e != hipSuccess) return e;
   }
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
@@ -1922,6 +1954,31 @@ hipError_t launch_breit(const BreitArgs &a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(breit_complete_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s,
                        a.bdir, a.bxch, a.ndim, a.nskp);
   }
+  return hipGetLastError();
+}
+hipError_t launch_breit_tc(const BreitArgs &flags, const BreitTcArgs &a, int nblocks, size_t lds,
+                           int nu, const BreitArgs &rest, int rest_blocks, hipStream_t s) {
+  int nf = std::max(1, std::min(nblocks, flags.nclasses));
+  if (flags.nclasses > 0)
+    hipLaunchKernelGGL(breit_flags_kernel, dim3(nf), dim3(64), 0, s, flags);
+  if (a.nitems > 0) {
+    if (lds > (size_t)kTcLdsMax || nu > kTcMaxU) return hipErrorInvalidValue;
+    const dim3 grid(std::max(1, std::min(nblocks, a.nitems)));
+    if (nu <= 8)
+      hipLaunchKernelGGL(breit_tc_kernel<8>, grid, dim3(kTcThreads), lds, s, a);
+    else
+      hipLaunchKernelGGL(breit_tc_kernel<kTcMaxU>, grid, dim3(kTcThreads), lds, s, a);
+  }
+  if (rest.nitems > 0) {
+    if (rest.xacc_n < 0 || rest.xacc_n > kXAccMax) return hipErrorInvalidValue;
+    const dim3 grid(std::max(1, std::min(rest_blocks, rest.nitems)));
+    hipLaunchKernelGGL((breit_kernel<1, 4>), grid, dim3(64), (size_t)rest.xacc_n * sizeof(double),
+                       s, rest);
+  }
+  int64_t n = (int64_t)flags.nskp * (flags.ndim - flags.nskp);
+  int bs = 256;
+  hipLaunchKernelGGL(breit_complete_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s,
+                     flags.bdir, flags.bxch, flags.ndim, flags.nskp);
   return hipGetLastError();
 }
 hipError_t launch_oneel(const OneArgs &a, hipStream_t s) {

@@ -693,12 +693,359 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
   return check_offsets(P);
 }
 
+// Tiles of the matrix-core Breit kernel (real path): per class, the first
+// combination alone (it reads this pair's own R-column flags), then the
+// others sorted by the set of CD blocks they call and packed four to a tile
+// while the tile's calls stay within kTcMaxU, so that combinations sharing
+// their calls share one E(CD) load.  Work items are runs of a class's tiles
+// for one primitive pair ij, costed like the VALU kernel's items.
+// Hermite bookkeeping of the matrix-core kernel's signature blocks, in
+// CARTIND order: components of index h, index of (t,u,v), parity class.
+namespace {
+struct HermiteOrder {
+  std::vector<int> t, u, v;
+  HermiteOrder() {
+    for (int L = 0; L <= kMaxLfl; ++L)
+      for (int a = 0; a <= L; ++a)
+        for (int b = 0; b <= L - a; ++b) {
+          t.push_back(a);
+          u.push_back(b);
+          v.push_back(L - a - b);
+        }
+  }
+  static int index(int a, int b, int c) {
+    const int L = a + b + c;
+    return L * (L + 1) * (L + 2) / 6 + a * (L + 1) - a * (a - 1) / 2 + b;
+  }
+  int cls(int h, int m) const { return ((t[h] & 1) | ((u[h] & 1) << 1) | ((v[h] & 1) << 2)) & m; }
+  // h < N(L) class by class, ascending inside a class
+  std::vector<int> by_class(int m, int L, std::vector<int> &coff) const {
+    std::vector<int> out;
+    coff.assign(9, 0);
+    for (int c = 0; c < 8; ++c) {
+      coff[c] = (int)out.size();
+      for (int h = 0; h < ntuv(L); ++h)
+        if (cls(h, m) == c) out.push_back(h);
+    }
+    coff[8] = (int)out.size();
+    return out;
+  }
+  // rank of h inside its class
+  int rank(int h, int m) const {
+    int r = 0;
+    for (int x = 0; x < h; ++x) r += cls(x, m) == cls(h, m);
+    return r;
+  }
+};
+}  // namespace
+
+// One signature block of the matrix-core Breit kernel for (axis mask m,
+// L_AB, L_CD): the ket's Hermite indices t' in groups of up to kTcR of one
+// parity class, dealt to the kTcWaves waves of a workgroup by their matrix-
+// product count (largest first to the least loaded wave), and for each
+// group the compact table row of t + t' for every bra t (class order) and
+// group member r; and for each (ix, ket class) the first contraction's
+// chunks of four bra indices, table by table.  Layout, in ints: ngrp, wave
+// ranges gw[0..kTcWaves], the chunk region's offset, then per group (ctp,
+// p0, n, chunks), then rows [ngrp][kTcR][nab + 1] as 16-bit pairs, then the
+// chunk region: 25 offsets (ix * 8 + class, and the end) and the chunks,
+// tab | src << 4 | pos << 6 | count << 14 (tab: W table, src: E^j table,
+// pos: first bra position, count: positions of the chunk, 1..4).
+static void tc_signature(const HermiteOrder &H, int m, int lab, int lcd, int gaunt,
+                         std::vector<int> &out, int &ngrp_out) {
+  std::vector<int> cab, ccd;
+  const std::vector<int> tab = H.by_class(m, lab, cab), tcd = H.by_class(m, lcd, ccd);
+  const int nab = ntuv(lab), Lw = lab + lcd;
+  int sz[8];
+  hermite_class_sizes(m, Lw, sz);
+  const int zrow = *std::max_element(sz, sz + 8);
+  struct Grp { int ctp, p0, n, cost; };
+  std::vector<Grp> gs;
+  for (int c = 0; c < 8; ++c)
+    for (int p = ccd[c]; p < ccd[c + 1]; p += kTcR) {
+      Grp g{c, p, std::min(kTcR, ccd[c + 1] - p), 0};
+      for (int ix = 0; ix < 3; ++ix)
+        for (int ty = 0; ty < (gaunt ? 1 : 4); ++ty) {
+          const int sh = ty == 0 ? 0 : (((1 << ix) ^ (1 << (ty - 1))) & m);
+          const int tc = c ^ sh;
+          g.cost += (cab[tc + 1] - cab[tc] + 3) / 4;
+        }
+      gs.push_back(g);
+    }
+  std::vector<int> order(gs.size());
+  for (size_t i = 0; i < gs.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gs[x].cost > gs[y].cost; });
+  std::vector<std::vector<int>> wv(kTcWaves);
+  int load[kTcWaves] = {};
+  for (int gi : order) {
+    const int w = (int)(std::min_element(load, load + kTcWaves) - load);
+    wv[w].push_back(gi);
+    load[w] += gs[gi].cost;
+  }
+  std::vector<int> seq;
+  out.clear();
+  out.push_back((int)gs.size());
+  int acc = 0;
+  for (int w = 0; w < kTcWaves; ++w) {
+    out.push_back(acc);
+    for (int gi : wv[w]) seq.push_back(gi);
+    acc += (int)wv[w].size();
+  }
+  out.push_back(acc);
+  const size_t choff_at = out.size();
+  out.push_back(0);
+  for (int gi : seq) {
+    out.push_back(gs[gi].ctp);
+    out.push_back(gs[gi].p0);
+    out.push_back(gs[gi].n);
+    out.push_back(gs[gi].cost);
+  }
+  std::vector<uint16_t> rows;
+  for (int gi : seq)
+    for (int r = 0; r < kTcR; ++r)
+      for (int pos = 0; pos <= nab; ++pos) {
+        int row = zrow;
+        if (pos < nab && r < gs[gi].n) {
+          const int a = tab[pos], b = tcd[gs[gi].p0 + r];
+          const int h = HermiteOrder::index(H.t[a] + H.t[b], H.u[a] + H.u[b], H.v[a] + H.v[b]);
+          row = H.rank(h, m);
+        }
+        rows.push_back((uint16_t)row);
+      }
+  if (rows.size() & 1) rows.push_back(0);
+  for (size_t i = 0; i < rows.size(); i += 2) out.push_back((int)(rows[i] | ((uint32_t)rows[i + 1] << 16)));
+  out[choff_at] = (int)out.size();
+  const size_t hdr = out.size();
+  out.resize(hdr + 25, 0);
+  std::vector<int> ch;
+  for (int ix = 0; ix < 3; ++ix)
+    for (int c = 0; c < 8; ++c) {
+      out[hdr + ix * 8 + c] = (int)ch.size();
+      for (int ty = 0; ty < (gaunt ? 1 : 4); ++ty) {
+        const int src = ty == 0 ? ix : ty - 1;
+        const int tb = ty == 0 ? 0 : 1 + 3 * ix + (ty - 1);
+        const int sh = ty == 0 ? 0 : (((1 << ix) ^ (1 << (ty - 1))) & m);
+        const int tc = c ^ sh;
+        for (int p = cab[tc]; p < cab[tc + 1]; p += 4)
+          ch.push_back(tb | (src << 4) | (p << 6) | (std::min(4, cab[tc + 1] - p) << 14));
+      }
+    }
+  out[hdr + 24] = (int)ch.size();
+  out.insert(out.end(), ch.begin(), ch.end());
+  ngrp_out = (int)gs.size();
+}
+
+static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
+                              const std::vector<uint8_t> &zflags) {
+  P.tiles.clear();
+  P.tile_cd.clear();
+  P.tile_pair.clear();
+  P.comp.clear();
+  P.titems.clear();
+  P.rest_items.clear();
+  P.tc_sig.clear();
+  P.tc_csig.assign(P.classes.size(), 0);
+  P.max_nu = P.max_npair = 0;
+  P.tc_lds = 0;
+  P.tc_ok = 0;
+  static const HermiteOrder H;
+  // components: kap from the exact-zero flags of each LS table, cout from kap
+  P.comp.resize(P.abs.size());
+  for (const ClassDesc &cl : P.classes) {
+    const int m = cl.sym;
+    int lab = 0;
+    while (ntuv(lab) < P.abs[cl.ab_begin].ntab) ++lab;
+    for (int a = cl.ab_begin; a < cl.ab_end; ++a) {
+      BrComp &bc = P.comp[a];
+      for (int j = 0; j < 3; ++j) {
+        const int pr = P.ab_pair[3 * a + j];
+        int bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < ntuv(lab); ++t)
+          for (int c = 0; c < 4; ++c)
+            if (zflags[T.flag_off[pr] + 4 * t + c]) bits[H.cls(t, m)] |= 1 << c;
+        for (int c = 0; c < 8; ++c) {
+          if (__builtin_popcount(bits[c]) > 1) return;  // not one component per class
+          bc.kap[j][c] = bits[c] ? (uint8_t)__builtin_ctz(bits[c]) : (uint8_t)kTcNone;
+        }
+      }
+      for (int ix = 0; ix < 3; ++ix)
+        for (int c = 0; c < 8; ++c) {
+          int out = bc.kap[ix][c];
+          if (!gaunt)
+            for (int j = 0; j < 3; ++j) {
+              const int k = bc.kap[j][c ^ (((1 << ix) ^ (1 << j)) & m)];
+              if (k == kTcNone) continue;
+              if (out != kTcNone && out != k) return;  // G^ix with two components
+              out = k;
+            }
+          bc.cout[ix][c] = (uint8_t)out;
+        }
+    }
+  }
+  std::vector<double> tcost;
+  std::vector<std::pair<int, int>> ctile(P.classes.size());
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> sigs;  // -> (offset, ngrp)
+  std::vector<char> fits(P.classes.size(), 0);
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    auto calls = [&](int a) {
+      std::vector<int64_t> s;
+      for (int k = P.abs[a].cd_begin; k < P.abs[a].cd_end; ++k) s.push_back(P.cds[k].eoff[0]);
+      std::sort(s.begin(), s.end());
+      return s;
+    };
+    // rows of one tile share the component of G^ix per ket class (cout,
+    // where defined): combinations are sorted by cout, then by their calls,
+    // and a tile closes when it is full, its calls would exceed the slots,
+    // or the next combination's cout conflicts with the tile's
+    std::vector<std::vector<int>> groups;
+    groups.push_back({cl.ab_begin});
+    auto csig = [&](int a) {
+      return std::vector<uint8_t>(&P.comp[a].cout[0][0], &P.comp[a].cout[0][0] + 24);
+    };
+    std::vector<std::tuple<std::vector<uint8_t>, std::vector<int64_t>, int>> rest;
+    for (int a = cl.ab_begin + 1; a < cl.ab_end; ++a) rest.push_back({csig(a), calls(a), a});
+    std::stable_sort(rest.begin(), rest.end());
+    std::vector<int> cur;
+    std::vector<int64_t> uni;
+    uint8_t tco[24];
+    for (const auto &r : rest) {
+      const std::vector<uint8_t> &co = std::get<0>(r);
+      std::vector<int64_t> u2 = uni;
+      for (int64_t x : std::get<1>(r))
+        if (std::find(u2.begin(), u2.end(), x) == u2.end()) u2.push_back(x);
+      bool conflict = false;
+      if (!cur.empty())
+        for (int k = 0; k < 24; ++k)
+          if (co[k] != kTcNone && tco[k] != kTcNone && co[k] != tco[k]) conflict = true;
+      if (!cur.empty() &&
+          (cur.size() == (size_t)kTcRows || u2.size() > (size_t)kTcMaxU || conflict)) {
+        groups.push_back(cur);
+        cur.clear();
+        uni.clear();
+        u2 = std::get<1>(r);
+      }
+      if (cur.empty())
+        for (int k = 0; k < 24; ++k) tco[k] = (uint8_t)kTcNone;
+      for (int k = 0; k < 24; ++k)
+        if (co[k] != kTcNone) tco[k] = co[k];
+      cur.push_back(std::get<2>(r));
+      uni = u2;
+    }
+    if (!cur.empty()) groups.push_back(cur);
+    ctile[c].first = (int)P.tiles.size();
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      BrTile t;
+      std::vector<int64_t> ids;
+      for (int g = 0; g < kTcRows; ++g) t.ab[g] = g < (int)groups[gi].size() ? groups[gi][g] : -1;
+      for (int g = 0; g < kTcRows; ++g)
+        if (t.ab[g] >= 0)
+          for (int k = P.abs[t.ab[g]].cd_begin; k < P.abs[t.ab[g]].cd_end; ++k)
+            if (std::find(ids.begin(), ids.end(), P.cds[k].eoff[0]) == ids.end())
+              ids.push_back(P.cds[k].eoff[0]);
+      if (ids.size() > (size_t)kTcMaxU) return;  // one combination beyond the call slots
+      t.nu = (int)ids.size();
+      t.fresh = gi == 0;
+      for (int ix = 0; ix < 3; ++ix)
+        for (int cc = 0; cc < 8; ++cc) {
+          t.cout[ix][cc] = (uint8_t)kTcNone;
+          for (int g = 0; g < kTcRows; ++g)
+            if (t.ab[g] >= 0 && P.comp[t.ab[g]].cout[ix][cc] != kTcNone)
+              t.cout[ix][cc] = P.comp[t.ab[g]].cout[ix][cc];
+        }
+      t.pair_begin = (int)P.tile_pair.size();
+      std::vector<int> cdi((size_t)kTcMaxU * kTcRows, -1);
+      double w = 0.0;
+      for (int g = 0; g < kTcRows; ++g) {
+        if (t.ab[g] < 0) continue;
+        const BrAB &ab = P.abs[t.ab[g]];
+        w += 30.0 * ab.ntab * ab.ntg * cl.maxcd / 4.0;
+        for (int k = ab.cd_begin; k < ab.cd_end; ++k) {
+          const int u = (int)(std::find(ids.begin(), ids.end(), P.cds[k].eoff[0]) - ids.begin());
+          cdi[(size_t)u * kTcRows + g] = k;
+          w += 6.0 * P.cds[k].ntcd * cl.maxcd + 64.0 * cl.maxcd;
+        }
+      }
+      for (int u = 0; u < kTcMaxU; ++u)
+        for (int g = 0; g < kTcRows; ++g)
+          if (cdi[(size_t)u * kTcRows + g] >= 0) P.tile_pair.push_back(u | (g << 8));
+      t.npair = (int)P.tile_pair.size() - t.pair_begin;
+      P.max_nu = std::max(P.max_nu, t.nu);
+      P.max_npair = std::max(P.max_npair, t.npair);
+      P.tiles.push_back(t);
+      P.tile_cd.insert(P.tile_cd.end(), cdi.begin(), cdi.end());
+      tcost.push_back(w);
+    }
+    ctile[c].second = (int)P.tiles.size();
+  }
+  // signature blocks and LDS per class; classes beyond a compute unit's LDS
+  // go to the vector-ALU kernel
+  std::vector<std::pair<double, WorkItem>> runs, rrest;
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    const BrAB &ab0 = P.abs[cl.ab_begin];
+    int lab = 0, lcd = 0;
+    while (ntuv(lab) < ab0.ntab) ++lab;
+    while (ntuv(lcd) < ab0.ntg) ++lcd;
+    auto key = std::make_tuple(cl.sym, lab, lcd);
+    if (!sigs.count(key)) {
+      std::vector<int> blk;
+      int ng = 0;
+      tc_signature(H, cl.sym, lab, lcd, gaunt, blk, ng);
+      sigs[key] = {(int)P.tc_sig.size(), ng};
+      P.tc_sig.insert(P.tc_sig.end(), blk.begin(), blk.end());
+    }
+    P.tc_csig[c] = sigs[key].first;
+    int sz[8], szr[8];
+    hermite_class_sizes(cl.sym, lab + lcd, sz);
+    hermite_class_sizes(cl.sym, cl.lamfl, szr);
+    const int wrows = *std::max_element(sz, sz + 8) + 1;
+    const TcLds lo = tc_lds_layout(ab0.ntab, wrows, szr[0], ntuv(cl.lamfl), gaunt ? 1 : 10,
+                                   sigs[key].second, ab0.ntg, P.max_npair);
+    fits[c] = lo.bytes <= kTcLdsMax;
+    if (fits[c]) P.tc_lds = std::max(P.tc_lds, (size_t)lo.bytes);
+  }
+  // items: runs of consecutive tiles of one class, about 1/8192 of the build each
+  double total = 0.0;
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    if (!fits[c]) continue;
+    const ClassDesc &cl = P.classes[c];
+    double r = (double)ntuv(cl.lamfl) * cl.maxcd;
+    for (int t = ctile[c].first; t < ctile[c].second; ++t) r += tcost[t];
+    total += r * cl.nbas[0] * cl.nbas[1];
+  }
+  const double target = total / 8192.0;
+  for (size_t c = 0; c < P.classes.size(); ++c) {
+    const ClassDesc &cl = P.classes[c];
+    if (!fits[c]) continue;
+    const double rc = (double)ntuv(cl.lamfl) * cl.maxcd;
+    int lo = ctile[c].first;
+    while (lo < ctile[c].second) {
+      double w = rc + tcost[lo];
+      int hi = lo + 1;
+      while (hi < ctile[c].second && w + tcost[hi] <= target) w += tcost[hi++];
+      for (int ij = 0; ij < cl.nbas[0] * cl.nbas[1]; ++ij)
+        runs.push_back({w, WorkItem{(int)c, ij, lo, hi}});
+      lo = hi;
+    }
+  }
+  std::stable_sort(runs.begin(), runs.end(),
+                   [](const std::pair<double, WorkItem> &x, const std::pair<double, WorkItem> &y) {
+                     return x.first > y.first;
+                   });
+  P.titems.reserve(runs.size());
+  for (const auto &r : runs) P.titems.push_back(r.second);
+  for (const WorkItem &w : P.items)
+    if (!fits[w.cls]) P.rest_items.push_back(w);
+  P.tc_ok = 1;
+}
+
 // ---------------------------------------------------------------------------
 // the Breit plan: BRTFAST's loop nest (bertha-scf.f:17185-17600)
 // ---------------------------------------------------------------------------
 std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                              int isym, int intsym, int racah1, int ilev, int gaunt,
-                             BreitPlan &P) {
+                             BreitPlan &P, const std::vector<uint8_t> *zflags) {
   P = BreitPlan();
   std::map<std::tuple<int, int, int>, int> iqidx;
   int cnt = 0;
@@ -761,8 +1108,11 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
                   for (int mb = 1; mb <= std::abs(kqb); ++mb) {
                     int mqa = 2 * ma - 1, mqb = 2 * mb - 1;
                     BrAB ab;
-                    for (int ix = 0; ix < 3; ++ix)
-                      ab.eoff[ix] = T.pairs[ls_pair(ca, cb, ka, kb, ma, mb, ix)].off;
+                    int abpr[3];
+                    for (int ix = 0; ix < 3; ++ix) {
+                      abpr[ix] = ls_pair(ca, cb, ka, kb, ma, mb, ix);
+                      ab.eoff[ix] = T.pairs[abpr[ix]].off;
+                    }
                     ab.ntab = ntuv(lamab);
                     ab.ntg = ntcd;
                     // IT1 = 2: A large, B small (:17432-17440)
@@ -849,6 +1199,7 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
                       }
                     P.max_ntg = std::max(P.max_ntg, ab.ntg);
                     P.abs.push_back(ab);
+                    P.ab_pair.insert(P.ab_pair.end(), abpr, abpr + 3);
                   }
               }
             }
@@ -897,6 +1248,7 @@ std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<u
     }
   }
   make_items(P.classes, abcost, rcost, P.items);
+  if (isym != 0 && zflags) build_breit_tiles(P, gaunt, T, *zflags);
   return check_offsets(P);
 }
 

@@ -51,9 +51,11 @@ int axis_symmetry(const ClassDesc &cl);
 void build_etables(const Basis &B, Etabs &T, bool with_ls);
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                        int isym, int intsym, int racah1, int ilev, Plan &P);
+// zflags (exact-nonzero flags of the E tables, or null) enable the tiles of
+// the matrix-core kernel on the real path
 std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
                              int isym, int intsym, int racah1, int ilev, int gaunt,
-                             BreitPlan &P);
+                             BreitPlan &P, const std::vector<uint8_t> *zflags = nullptr);
 
 // the one-centre Racah plan of every centre (racah.cpp) of the Coulomb
 // (breit = 0, COULOMB1) or Breit (breit = 1, BREIT1) interaction; isym: 0

@@ -299,8 +299,13 @@ int bdf_plan_info(bdf_engine *e, int ilev, double *flops, double *quartets, long
    wave each; the kernel is built for one wave per SIMD per four of them,
    2 to 4 waves; default 16) and breit_group the number of AB combinations
    it contracts together (1, 2 or 4, the last two at two waves per SIMD;
-   its scratch grows with it); 0 keeps the engine's choice.  Takes effect
-   from the next build; captured builds are dropped. */
+   its scratch grows with it); 0 keeps the engine's choice.  breit_kernel
+   picks the Breit kernel: 1 the vector-ALU kernel (every shape; the
+   per-CU and group settings above apply to it), 2 the matrix-core kernel
+   (axial molecules and atoms, ISYM 1 and 2; builds that it cannot take
+   fail with BDF_EUNSUPP), 0 the engine's choice (the matrix-core kernel
+   wherever it applies).  Takes effect from the next build; captured builds
+   are dropped. */
 typedef struct {
   int graph;
   int waves;
@@ -308,6 +313,7 @@ typedef struct {
   int per_cu;
   int breit_per_cu;
   int breit_group;
+  int breit_kernel;
 } bdf_launch;
 
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l);

@@ -27,7 +27,7 @@ module bertha_amd
   end type bdf_options
 
   type, bind(c) :: bdf_launch
-    integer(c_int) :: graph, waves, cap, per_cu, breit_per_cu, breit_group
+    integer(c_int) :: graph, waves, cap, per_cu, breit_per_cu, breit_group, breit_kernel
   end type bdf_launch
 
   interface
