@@ -176,6 +176,12 @@ struct BrTile {
   int fresh;        // 1: the class's first combination, which reads this pair's
                     // own R-column flags (BIIFAST's IRC); every other tile
                     // reads the class's stale flags
+  // the class's first combination also sits in a stale tile as row
+  // fresh_row (-1: not this tile); on the fresh tile fresh_row is that row
+  // when there is one.  A pair whose own flags keep the same gauge terms as
+  // the stale flags takes the combination there and skips the fresh tile;
+  // any other pair takes the fresh tile and leaves that row out.
+  int fresh_row;
   int pair_begin, npair;
   // the component of G^ix living in ket class c, common to the tile's rows
   // (kTcNone: G^ix vanishes there for every row)
@@ -206,8 +212,8 @@ inline void hermite_class_sizes(int m, int L, int sz[8]) {
 // gauge table, nr0 class-0 R rows, nt = N(L) R columns, nw tables (10
 // Breit, 1 Gaunt), ngrp t' groups, ntg = N(L_CD), npair partial-sum slots.
 struct TcLds {
-  int et, w, wrec, rt, gbuf, rawb, dig, cdoff, kp, pidx, comp, cls0, tpl, tco, rowtab, cf, gf, gmk,
-      bytes;
+  int et, w, wrec, rt, gbuf, rawb, dig, cdoff, kp, pidx, comp, cls0, tpl, tco, rowtab, chl, cf, gf,
+      gmk, bytes;
 };
 // digestion constants of one (call, combination) pair of a tile
 struct TcDig {
@@ -216,7 +222,7 @@ struct TcDig {
   int8_t pab[2], pcd[2];
 };
 __host__ __device__ inline TcLds tc_lds_layout(int nab, int wrows, int nr0, int nt, int nw,
-                                               int ngrp, int ntg, int npair) {
+                                               int ngrp, int ntg, int npair, int nchk) {
   TcLds o;
   int b = 0;
   o.et = b;     b += 3 * (nab + 1) * kTcRows * 8;  // E(AB) [3][nab + 1][16 g]
@@ -234,9 +240,10 @@ __host__ __device__ inline TcLds tc_lds_layout(int nab, int wrows, int nr0, int 
   o.tpl = b;    b += (ntg * 2 + 7) & ~7;           // ket t' in class order [ntg]
   o.tco = b;    b += 24;                           // the tile's cout [3][8]
   o.rowtab = b; b += (ngrp * kTcR * (nab + 1) * 2 + 7) & ~7;  // rows [grp][R][nab + 1]
+  o.chl = b;    b += ((25 + nchk) * 4 + 7) & ~7;   // chunk offsets [ix * 8 + class], end; chunks
   o.cf = b;     b += (nt + 7) & ~7;                // this pair's R-column flags
   o.gf = b;     b += (nt + 7) & ~7;                // the class's stale flags
-  o.gmk = b;    b += (kTcRows * ntg * 3 + 7) & ~7; // G masks [16 g][ntg][3] (4 bits)
+  o.gmk = b;    b += (ntg * 3 * 4 * 2 + 7) & ~7;   // G masks [ntg][3][4 c] (16 bits: g)
   o.bytes = b;
   return o;
 }

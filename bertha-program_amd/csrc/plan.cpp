@@ -884,7 +884,8 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
   }
   std::vector<double> tcost;
   std::vector<std::pair<int, int>> ctile(P.classes.size());
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> sigs;  // -> (offset, ngrp)
+  // -> (offset, groups, chunks)
+  std::map<std::tuple<int, int, int>, std::tuple<int, int, int>> sigs;
   std::vector<char> fits(P.classes.size(), 0);
   for (size_t c = 0; c < P.classes.size(); ++c) {
     const ClassDesc &cl = P.classes[c];
@@ -933,6 +934,27 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
       uni = u2;
     }
     if (!cur.empty()) groups.push_back(cur);
+    // the first combination also joins a stale tile with a free row, no
+    // conflicting cout and room for its calls
+    int mg = -1;
+    {
+      const std::vector<uint8_t> fco = csig(cl.ab_begin);
+      const std::vector<int64_t> fca = calls(cl.ab_begin);
+      for (size_t gi = 1; gi < groups.size() && mg < 0; ++gi) {
+        if (groups[gi].size() >= (size_t)kTcRows) continue;
+        bool ok = true;
+        std::vector<int64_t> u = fca;
+        for (int a : groups[gi]) {
+          const std::vector<uint8_t> co = csig(a);
+          for (int k = 0; k < 24; ++k)
+            if (fco[k] != kTcNone && co[k] != kTcNone && fco[k] != co[k]) ok = false;
+          for (int64_t x : calls(a))
+            if (std::find(u.begin(), u.end(), x) == u.end()) u.push_back(x);
+        }
+        if (ok && u.size() <= (size_t)kTcMaxU) mg = (int)gi;
+      }
+      if (mg > 0) groups[mg].push_back(cl.ab_begin);
+    }
     ctile[c].first = (int)P.tiles.size();
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       BrTile t;
@@ -946,6 +968,8 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
       if (ids.size() > (size_t)kTcMaxU) return;  // one combination beyond the call slots
       t.nu = (int)ids.size();
       t.fresh = gi == 0;
+      t.fresh_row = gi == 0 ? (mg > 0 ? (int)groups[mg].size() - 1 : -1)
+                            : ((int)gi == mg ? (int)groups[gi].size() - 1 : -1);
       for (int ix = 0; ix < 3; ++ix)
         for (int cc = 0; cc < 8; ++cc) {
           t.cout[ix][cc] = (uint8_t)kTcNone;
@@ -992,16 +1016,19 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
       std::vector<int> blk;
       int ng = 0;
       tc_signature(H, cl.sym, lab, lcd, gaunt, blk, ng);
-      sigs[key] = {(int)P.tc_sig.size(), ng};
+      for (int w = 0; w < kTcWaves; ++w)
+        if (3 * (blk[2 + w] - blk[1 + w]) > 63) return;  // a wave's steps beyond one lane each
+      sigs[key] = {(int)P.tc_sig.size(), ng, blk[blk[2 + kTcWaves] + 24]};
       P.tc_sig.insert(P.tc_sig.end(), blk.begin(), blk.end());
     }
-    P.tc_csig[c] = sigs[key].first;
+    P.tc_csig[c] = std::get<0>(sigs[key]);
     int sz[8], szr[8];
     hermite_class_sizes(cl.sym, lab + lcd, sz);
     hermite_class_sizes(cl.sym, cl.lamfl, szr);
     const int wrows = *std::max_element(sz, sz + 8) + 1;
     const TcLds lo = tc_lds_layout(ab0.ntab, wrows, szr[0], ntuv(cl.lamfl), gaunt ? 1 : 10,
-                                   sigs[key].second, ab0.ntg, P.max_npair);
+                                   std::get<1>(sigs[key]), ab0.ntg, P.max_npair,
+                                   std::get<2>(sigs[key]));
     fits[c] = lo.bytes <= kTcLdsMax;
     if (fits[c]) P.tc_lds = std::max(P.tc_lds, (size_t)lo.bytes);
   }
