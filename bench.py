@@ -16,6 +16,7 @@ Legs:
   mixed     the same with the Coulomb build's Schwarz split (FP32 below the
             split, FP64 above); the headline value is this leg when its
             two-electron energy is within 1 microhartree of the FP64 leg's
+            and its step is the faster
   kernels   average duration of each quartet kernel (HIP events bracketing
             it on the engine's stream) -> roofline of the dominant one
   cpu       the oracle (the C restatement of the reference path) on the
@@ -264,6 +265,10 @@ def main() -> int:
     e64 = e2(g, x, b, bx)
     launch = {"hip_graph": graph, "settings": eng.launch,
               "tuned": eng.tuning is not None}
+    # the Breit build's quartet kernel: 2 matrix-core, 1 vector-ALU
+    bkern = eng.breit_launch_info(args.stage)[0] if breit else 0
+    if breit:
+        launch["breit_kernel"] = "breit_tc_kernel" if bkern == 2 else "breit_kernel"
     eng.close()
 
     mixed = None
@@ -284,9 +289,11 @@ def main() -> int:
 
     # roofline of the dominant kernel of the FP64 step
     if breit and kb64 > kc64:
-        kname, kms, kflops = "breit_kernel", kb64, flops_b
+        kname = "breit_tc_kernel" if bkern == 2 else "breit_kernel"
+        kms, kflops = kb64, flops_b
     else:
         kname, kms, kflops = "fock_kernel", kc64, flops_c
+    bound = "mfma" if kname == "breit_tc_kernel" else "valu"
     achieved = kflops / (kms * 1e-3) / 1e12
     traffic, tsrc = profiled_traffic(kname, args.workload)
 
@@ -366,7 +373,9 @@ def main() -> int:
         if args.serve_sustained_rate > 0 and args.serve_sustained > 0:
             serving_sustained = serve_leg(args.serve_sustained, args.serve_sustained_rate)
 
-    headline = mixed if (mixed is not None and mixed["abs_dE2e_vs_fp64_Eh"] < 1e-6) else None
+    # the mixed leg is the headline where it keeps the energy and is faster
+    headline = mixed if (mixed is not None and mixed["abs_dE2e_vs_fp64_Eh"] < 1e-6
+                         and mixed["ms_per_step"] < ms64) else None
     value = headline["value"] if headline else 1000.0 / ms64
     out = {
         "metric": METRIC,
@@ -396,13 +405,17 @@ def main() -> int:
                  "breit_kernel_ms": kb64, "two_electron_energy_Eh": e64,
                  "coulomb_tflops": flops_c / (kc64 * 1e-3) / 1e12,
                  "breit_tflops": (flops_b / (kb64 * 1e-3) / 1e12) if breit else None},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": bound, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic, "traffic_source": tsrc,
                      "kernel": f"{kname} (FP64 step)", "kernel_ms": kms,
                      "flops_per_launch": kflops,
-                     "note": "FP64 vector-ALU bound (no MFMA); the MI355X FP64 vector peak "
-                             "equals its FP64 matrix peak"},
+                     "note": ("FP64 matrix-core kernel (16x16x4 FP64 MFMA); achieved counts "
+                              "BIIFAST's statements (the plan's flops), of which the kernel "
+                              "executes fewer: its gauge tables are shared by the rows of a "
+                              "tile" if bound == "mfma" else
+                              "FP64 vector-ALU kernel (no MFMA); the MI355X FP64 vector peak "
+                              "equals its FP64 matrix peak")},
         "cpu_baseline": cpu,
         "parity": parity,
         "mixed": mixed,

@@ -131,6 +131,8 @@ def library() -> _C.CDLL:
         lib.bdf_set_launch.argtypes = [P, _C.POINTER(_Launch)]
         lib.bdf_launch_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_int), _C.POINTER(_C.c_int),
                                         _C.POINTER(_C.c_int), _C.POINTER(_C.c_size_t)]
+        lib.bdf_breit_launch_info.argtypes = [P, _C.c_int, _C.POINTER(_C.c_int),
+                                              _C.POINTER(_C.c_int)]
         lib.bdf_stream.argtypes = [P]
         lib.bdf_stream.restype = P
         lib.bdf_destroy.argtypes = [P]
@@ -201,7 +203,8 @@ class Engine:
             if entry:
                 upd.update(waves=entry["waves"], cap=entry["cap"], per_cu=entry["per_cu"],
                            breit_per_cu=entry.get("breit_per_cu", 0),
-                           breit_group=entry.get("breit_group", 0))
+                           breit_group=entry.get("breit_group", 0),
+                           breit_kernel=entry.get("breit_kernel", 0))
                 self.tuning = entry
         if any(upd.values()):
             self.set_launch(**upd)
@@ -427,6 +430,13 @@ class Engine:
         _check(library().bdf_launch_info(self._h, ilev, _C.byref(w), _C.byref(c), _C.byref(g),
                                          _C.byref(b)))
         return w.value, c.value, g.value, b.value
+
+    def breit_launch_info(self, ilev: int = 3):
+        """(kernel, workgroups) of the stage's Breit build: kernel 1 is the
+        vector-ALU breit_kernel, 2 the matrix-core breit_tc_kernel."""
+        k, g = _C.c_int(), _C.c_int()
+        _check(library().bdf_breit_launch_info(self._h, ilev, _C.byref(k), _C.byref(g)))
+        return k.value, g.value
 
     def device_bytes(self) -> int:
         """Device memory held by the engine."""

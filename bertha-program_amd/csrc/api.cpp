@@ -1559,6 +1559,20 @@ int bdf_launch_info(bdf_engine *e, int ilev, int *waves, int *cap, int *workgrou
   return BDF_OK;
 }
 
+int bdf_breit_launch_info(bdf_engine *e, int ilev, int *kernel, int *workgroups) {
+  if (!e) return fail(BDF_EINVAL, "bdf_breit_launch_info: null engine");
+  if (!e->breit) return fail(BDF_EINVAL, "bdf_breit_launch_info: engine built without DHFB");
+  if (ilev < 1 || ilev > 3) return fail(BDF_EINVAL, "bdf_breit_launch_info: ilev outside 1..3");
+  HIPCHK(hipSetDevice(e->device));
+  int rc = make_breit_plan(e, ilev);
+  if (rc != BDF_OK) return rc;
+  const BreitPlan &P = e->bplans[ilev - 1].P;
+  const bool tc = breit_tc(e, P);
+  if (kernel) *kernel = tc ? 2 : 1;
+  if (workgroups) *workgroups = tc ? breit_tc_blocks(e, P) : breit_blocks(e);
+  return BDF_OK;
+}
+
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l) {
   if (!e || !l) return fail(BDF_EINVAL, "bdf_set_launch: null argument");
   if (l->waves != 0 && (l->waves < 2 || l->waves > 4))

@@ -2,9 +2,11 @@
 
 For a system, times the stage-3 Coulomb build under every launch shape of
 fock_kernel (waves per SIMD, group capacity, persistent workgroups per CU)
-and, for Dirac-Coulomb-Breit systems, the Breit build under every workgroup
-count and AB-combination group of breit_kernel, and records the fastest in tuned_launch.json beside
-this file, keyed by the system's signature: integral path (real for
+and, for Dirac-Coulomb-Breit systems, the Breit build on the matrix-core
+kernel (where the system's path has it) and on the vector-ALU breit_kernel
+under every workgroup count and AB-combination group, and records the
+fastest in tuned_launch.json beside this file, keyed by the system's
+signature: integral path (real for
 ATOMIC/DIATOM/LINEAR, complex otherwise), Hamiltonian, largest primitive
 count per shell and largest l.  Engine(..., tuned=True) applies the entry
 of its system's signature through bdf_set_launch.
@@ -94,20 +96,27 @@ def tune(pkg, system, reps: int = 3, log=print) -> dict:
              "workload": system.name, "ndim": n,
              "tuned": time.strftime("%Y-%m-%d")}
     if system.hmlt.upper() == "DHFB":
+        # (kernel, workgroups per CU, AB group): the matrix-core kernel has
+        # one launch shape; the vector-ALU kernel's shapes follow
         breit = {}
-        for per_cu, group in BREIT_SHAPES:
-            eng.set_launch(waves=best[0], cap=best[1], per_cu=best[2], breit_per_cu=per_cu,
-                           breit_group=group)
+        shapes = [(2, 0, 0)] + [(1, per_cu, group) for per_cu, group in BREIT_SHAPES]
+        for kern, per_cu, group in shapes:
+            try:
+                eng.set_launch(waves=best[0], cap=best[1], per_cu=best[2], breit_per_cu=per_cu,
+                               breit_group=group, breit_kernel=kern)
+            except pkg.BerthaError:
+                continue  # no matrix-core kernel on this path
             ms = _time(eng, lambda: eng.breit_device(d.data_ptr(), g.data_ptr(), x.data_ptr(), 3),
                        max(1, reps - 1))
-            breit[(per_cu, group)] = ms
-            log(f"tune {signature(system)}: breit_kernel per CU {per_cu} group {group}: "
+            breit[(kern, per_cu, group)] = ms
+            log(f"tune {signature(system)}: Breit kernel {kern} per CU {per_cu} group {group}: "
                 f"{ms:.2f} ms")
         bbest = min(breit, key=breit.get)
-        entry.update({"breit_per_cu": bbest[0], "breit_group": bbest[1], "breit_ms": breit[bbest]})
-        entry["default"]["breit_per_cu"] = 16
-        entry["default"]["breit_group"] = 1
-        entry["default"]["breit_ms"] = breit.get((16, 1))
+        entry.update({"breit_kernel": bbest[0], "breit_per_cu": bbest[1], "breit_group": bbest[2],
+                      "breit_ms": breit[bbest]})
+        dkey = (2, 0, 0) if (2, 0, 0) in breit else (1, 16, 1)
+        entry["default"].update({"breit_kernel": dkey[0], "breit_per_cu": dkey[1],
+                                 "breit_group": dkey[2], "breit_ms": breit.get(dkey)})
     eng.close()
     return entry
 

@@ -318,6 +318,12 @@ typedef struct {
 
 int bdf_set_launch(bdf_engine *e, const bdf_launch *l);
 
+/* How the stage-ilev Breit build is launched: its quartet kernel (1
+   vector-ALU breit_kernel, 2 matrix-core breit_tc_kernel) under the current
+   launch settings, and its persistent workgroups.  No counterpart in the
+   reference. */
+int bdf_breit_launch_info(bdf_engine *e, int ilev, int *kernel, int *workgroups);
+
 /* How the stage-ilev Coulomb build is launched: waves per SIMD of its
    quartet kernel (2, 3 or 4), the capacity multiple of its AB-combination
    groups (1 or 2), persistent workgroups, and the engine's scratch in

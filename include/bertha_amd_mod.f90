@@ -224,6 +224,14 @@ module bertha_amd
       type(bdf_launch), intent(in) :: l
     end function
 
+    integer(c_int) function bdf_breit_launch_info(eng, ilev, kernel, workgroups) &
+        bind(c, name='bdf_breit_launch_info')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int), value :: ilev
+      integer(c_int), intent(out) :: kernel, workgroups
+    end function
+
     integer(c_int) function bdf_launch_info(eng, ilev, waves, cap, workgroups, scratch_bytes) &
         bind(c, name='bdf_launch_info')
       import :: c_int, c_ptr, c_size_t

@@ -92,16 +92,16 @@ def test_gaunt_matches_oracle(bdf, orc, systems):
 @pytest.mark.parametrize("group,per_cu", [(1, 16), (1, 12), (1, 8), (2, 4), (4, 4)])
 def test_breit_groups_match_oracle(bdf, orc, systems, group, per_cu):
     """Every AB-combination group size and waves-per-SIMD instance of the
-    Breit kernel gives the oracle's build (Breit and Gaunt), on the complex
-    three-centre system and the real f-shell system, where classes hold many
-    combinations."""
+    vector-ALU Breit kernel gives the oracle's build (Breit and Gaunt), on
+    the complex three-centre system and the real f-shell system, where
+    classes hold many combinations."""
     for name, gaunt in (("tri", False), ("fsh", False), ("off", True)):
         s = systems[name]
         print(f"[breit group] {group} per CU {per_cu} {name} gaunt={gaunt}", flush=True)
         d = bdf.systems.random_density(s.ndim, seed=7, real=(s.shape != "NOSYMM"))
         bo, xo = orc.Oracle(s).brtfast(d, 3, gaunt=gaunt)
         with bdf.Engine(s, tuned=False, gaunt=gaunt) as eng:
-            eng.set_launch(breit_group=group, breit_per_cu=per_cu)
+            eng.set_launch(breit_group=group, breit_per_cu=per_cu, breit_kernel=1)
             b, x = eng.breit(d, 3)
             # a second build on the same scratch (the exchange blocks are
             # summed by atomics, so only to rounding)

@@ -45,15 +45,19 @@ def test_matrix_core_breit_is_the_default_on_the_real_path(bdf, systems):
     s = systems["fdiat"]
     d = bdf.systems.random_density(s.ndim, seed=12, real=True)
     with bdf.Engine(s, tuned=False) as eng:
+        assert eng.breit_launch_info(3)[0] == 2
         _, x0 = eng.breit(d, 3)
         eng.set_launch(breit_kernel=2)
         _, x2 = eng.breit(d, 3)
+        eng.set_launch(breit_kernel=1)
+        assert eng.breit_launch_info(3)[0] == 1
     assert np.abs(x0 - x2).max() <= 1e-12 * np.abs(x2).max()
 
 
 def test_matrix_core_breit_refuses_the_complex_path(bdf, systems):
     with bdf.Engine(systems["off"], tuned=False) as eng:
         d = bdf.systems.random_density(eng.ndim, seed=13)
+        assert eng.breit_launch_info(3)[0] == 1
         eng.breit(d, 3)  # the vector-ALU kernel
         with pytest.raises(bdf.BerthaError):
             eng.set_launch(breit_kernel=2)
