@@ -164,6 +164,7 @@ struct BrCD {
 // 8, from pair_begin).
 constexpr int kTcRows = 16;
 constexpr int kTcMaxU = 16;
+constexpr int kTcGbLd = 17;  // row stride of a wave's G^ix buffer (doubles)
 constexpr int kTcNone = 4;  // no component
 constexpr int kTcWaves = 8;  // waves of a workgroup (two per SIMD)
 constexpr int kTcThreads = 64 * kTcWaves;
@@ -229,7 +230,7 @@ __host__ __device__ inline TcLds tc_lds_layout(int nab, int wrows, int nr0, int 
   o.w = b;      b += nw * wrows * kTcQ * 8;        // tables [nw][wrows][Q]
   o.wrec = b;   b += nw * wrows * 8;               // their recipes (one per row)
   o.rt = b;     b += nr0 * kTcQ * 8;               // class-0 R rows [nr0][Q]
-  o.gbuf = b;   b += kTcWaves * kTcRows * 16 * 8;  // G^ix per wave [Q][R][16 g]
+  o.gbuf = b;   b += kTcWaves * 16 * kTcGbLd * 8;  // G^ix per wave [Q][R][16 g], padded rows
   o.rawb = b;   b += npair * kTcQ * 4 * 8;         // partial sums [pair][Q][4]
   o.dig = b;    b += npair * (int)sizeof(TcDig);   // digestion constants [pair]
   o.cdoff = b;  b += kTcMaxU * 3 * 8;              // E(CD) offsets [u][3]
