@@ -741,16 +741,18 @@ struct HermiteOrder {
 
 // One signature block of the matrix-core Breit kernel for (axis mask m,
 // L_AB, L_CD): the ket's Hermite indices t' in groups of up to kTcR of one
-// parity class, dealt to the kTcWaves waves of a workgroup by their matrix-
-// product count (largest first to the least loaded wave), and for each
-// group the compact table row of t + t' for every bra t (class order) and
-// group member r; and for each (ix, ket class) the first contraction's
-// chunks of four bra indices, table by table.  Layout, in ints: ngrp, wave
-// ranges gw[0..kTcWaves], the chunk region's offset, then per group (ctp,
-// p0, n, chunks), then rows [ngrp][kTcR][nab + 1] as 16-bit pairs, then the
-// chunk region: 25 offsets (ix * 8 + class, and the end) and the chunks,
-// tab | src << 4 | pos << 6 | count << 14 (tab: W table, src: E^j table,
-// pos: first bra position, count: positions of the chunk, 1..4).
+// parity class; the steps (group, ix) dealt to the kTcWaves waves of a
+// workgroup by their matrix-product count (largest first to the least
+// loaded wave); for each group the compact table row of t + t' for every
+// bra t (class order) and group member r; and for each (ix, ket class) the
+// first contraction's chunks of four bra indices, table by table.  Layout,
+// in ints: ngrp, wave ranges gw[0..kTcWaves] into the step list, the chunk
+// region's offset, then per group (ctp, p0, n, chunks), then the step list
+// (3 ngrp codes group * 3 + ix, wave by wave), then rows [ngrp][kTcR][nab +
+// 1] as 16-bit pairs, then the chunk region: 25 offsets (ix * 8 + class, and
+// the end) and the chunks, tab | src << 4 | pos << 6 | count << 14 (tab: W
+// table, src: E^j table, pos: first bra position, count: positions of the
+// chunk, 1..4).
 static void tc_signature(const HermiteOrder &H, int m, int lab, int lcd, int gaunt,
                          std::vector<int> &out, int &ngrp_out) {
   std::vector<int> cab, ccd;
@@ -772,34 +774,48 @@ static void tc_signature(const HermiteOrder &H, int m, int lab, int lcd, int gau
         }
       gs.push_back(g);
     }
-  std::vector<int> order(gs.size());
-  for (size_t i = 0; i < gs.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gs[x].cost > gs[y].cost; });
+  // steps (group, ix) dealt to the waves, largest first to the least loaded:
+  // cost = the first contraction's chunk products plus the second's
+  struct Step { int code, cost; };
+  std::vector<Step> steps;
+  for (size_t gi = 0; gi < gs.size(); ++gi)
+    for (int ix = 0; ix < 3; ++ix) {
+      int cost = kTcQ * kTcR / 4;
+      for (int ty = 0; ty < (gaunt ? 1 : 4); ++ty) {
+        const int sh = ty == 0 ? 0 : (((1 << ix) ^ (1 << (ty - 1))) & m);
+        const int tc = gs[gi].ctp ^ sh;
+        cost += (cab[tc + 1] - cab[tc] + 3) / 4;
+      }
+      steps.push_back({(int)gi * 3 + ix, cost});
+    }
+  std::stable_sort(steps.begin(), steps.end(),
+                   [](const Step &x, const Step &y) { return x.cost > y.cost; });
   std::vector<std::vector<int>> wv(kTcWaves);
   int load[kTcWaves] = {};
-  for (int gi : order) {
+  for (const Step &st : steps) {
     const int w = (int)(std::min_element(load, load + kTcWaves) - load);
-    wv[w].push_back(gi);
-    load[w] += gs[gi].cost;
+    wv[w].push_back(st.code);
+    load[w] += st.cost;
   }
-  std::vector<int> seq;
   out.clear();
   out.push_back((int)gs.size());
   int acc = 0;
   for (int w = 0; w < kTcWaves; ++w) {
     out.push_back(acc);
-    for (int gi : wv[w]) seq.push_back(gi);
     acc += (int)wv[w].size();
   }
   out.push_back(acc);
   const size_t choff_at = out.size();
   out.push_back(0);
-  for (int gi : seq) {
-    out.push_back(gs[gi].ctp);
-    out.push_back(gs[gi].p0);
-    out.push_back(gs[gi].n);
-    out.push_back(gs[gi].cost);
+  for (const Grp &g : gs) {
+    out.push_back(g.ctp);
+    out.push_back(g.p0);
+    out.push_back(g.n);
+    out.push_back(g.cost);
   }
+  for (int w = 0; w < kTcWaves; ++w) out.insert(out.end(), wv[w].begin(), wv[w].end());
+  std::vector<int> seq(gs.size());
+  for (size_t i = 0; i < gs.size(); ++i) seq[i] = (int)i;
   std::vector<uint16_t> rows;
   for (int gi : seq)
     for (int r = 0; r < kTcR; ++r)
@@ -1017,7 +1033,7 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
       int ng = 0;
       tc_signature(H, cl.sym, lab, lcd, gaunt, blk, ng);
       for (int w = 0; w < kTcWaves; ++w)
-        if (3 * (blk[2 + w] - blk[1 + w]) > 63) return;  // a wave's steps beyond one lane each
+        if (blk[2 + w] - blk[1 + w] > 63) return;  // a wave's steps beyond one lane each
       sigs[key] = {(int)P.tc_sig.size(), ng, blk[blk[2 + kTcWaves] + 24]};
       P.tc_sig.insert(P.tc_sig.end(), blk.begin(), blk.end());
     }
