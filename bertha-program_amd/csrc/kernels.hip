@@ -1778,6 +1778,21 @@ hipError_t upload_constants() {
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_cls0), cls0, sizeof(cls0));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_csz), csz, sizeof(csz));
     if (e != hipSuccess) return e;
+    // closed form of R on the z axis: v! / (j! (v - 2j)! 2^j) and (t - 1)!!
+    static double hz[kMaxLfl + 1][kMaxLfl / 2 + 1], dfac[kMaxLfl + 1];
+    double fact[kMaxLfl + 1];
+    fact[0] = 1.0;
+    for (int n = 1; n <= kMaxLfl; ++n) fact[n] = fact[n - 1] * n;
+    for (int v = 0; v <= kMaxLfl; ++v)
+      for (int j = 0; j <= kMaxLfl / 2; ++j)
+        hz[v][j] = 2 * j <= v ? fact[v] / (fact[j] * fact[v - 2 * j] * std::ldexp(1.0, j)) : 0.0;
+    for (int t = 0; t <= kMaxLfl; ++t) {
+      dfac[t] = 1.0;
+      for (int k = t - 1; k > 0; k -= 2) dfac[t] *= k;
+    }
+    e = hipMemcpyToSymbol(HIP_SYMBOL(c_hz), hz, sizeof(hz));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_dfac), dfac, sizeof(dfac));
+    if (e != hipSuccess) return e;
   }
   double rcp[2 * kMaxLfl + 2 * 60 + 4];
   float rcpf[2 * kMaxLfl + 2 * 60 + 4];
