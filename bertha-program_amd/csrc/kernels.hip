@@ -381,6 +381,46 @@ __device__ void boys_ref(T t, int L, T *f, int ld) {
 // The seeds (-2a)^n F_n are formed in the precision of f (FP64 for every
 // item: the FP32 items of the mixed build run the recursion in FP32 from
 // FP64 Boys values).
+// rank of Hermite index h within its parity class under axis mask m (classes
+// enumerated in CARTIND order); class sizes of h < N(L); the class-0 members
+__constant__ int16_t c_crank[8][ntuv(kMaxLfl)];
+__constant__ int16_t c_csz[8][kMaxLfl + 1][8];
+__constant__ int16_t c_cls0[8][ntuv(kMaxLfl)];
+// R on the z axis (X = Y = 0): R_tuv = (t-1)!! (u-1)!! sum_j c_hz[v][j]
+// Z^(v-2j) A_((t+u)/2+v-j), A_k = (-2 alpha)^k F_k, for even t and u (else
+// zero): c_hz[v][j] = v! / (j! (v-2j)! 2^j), c_dfac[t] = (t-1)!!
+__constant__ double c_hz[kMaxLfl + 1][kMaxLfl / 2 + 1];
+__constant__ double c_dfac[kMaxLfl + 1];
+
+__device__ __forceinline__ int hcls(int n, int m) { return hpar(n) & m; }
+
+// RMAKE's R for a class on the z axis (axis mask m holds x and y, so P - Q
+// lies on z): the class-0 values (parity class 0 under m) from the Boys
+// values F_0..F_L at stride ld, which it scales in place to A_k; the other
+// classes vanish exactly and are not written.  Each value is the closed form
+// above, a sum of at most L/2 + 1 terms with no chain through r.
+template <typename T>
+__device__ void rints_z(T *r, double *f, int ld, int L, double alpha, double Z, int m) {
+  double pw = 1.0;
+  for (int k = 0; k <= L; ++k) {
+    f[(int64_t)k * ld] *= pw;
+    pw *= -2.0 * alpha;
+  }
+  const int nr0 = c_csz[m][L][0];
+  for (int w = 0; w < nr0; ++w) {
+    const int h = c_cls0[m][w];
+    const uint32_t tv = c_tuv[h];
+    const int t = (int)(tv >> 10), u = (int)((tv >> 5) & 31), v = (int)(tv & 31);
+    const int k0 = (t + u) / 2 + v;
+    double zp = (v & 1) ? Z : 1.0, sum = 0.0;
+    for (int j = v / 2; j >= 0; --j) {  // Z^(v - 2j) rises as j falls
+      sum += c_hz[v][j] * zp * f[(int64_t)(k0 - j) * ld];
+      zp *= Z * Z;
+    }
+    r[(int64_t)h * ld] = (T)(c_dfac[t] * c_dfac[u] * sum);
+  }
+}
+
 template <typename T, typename FT>
 __device__ void rints(T *r, const FT *f, int ld, int L, FT alpha, T X, T Y, T Z) {
   const FT m2a = -2 * alpha;
@@ -817,6 +857,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
   const int L = cl.lamfl, nt = ntuv(L), maxcd = cl.maxcd;
   const int maxab = cl.nbas[0] * nB;
   const int sym = cl.sym;
+  const bool zaxis = (sym & 3) == 3;  // P - Q on the z axis for every kl
 
   // R-integrals for every kl of this ij, then the reference's column test
   // (sum over kl of |R| > 1e-10 before the prefactor, :12660-12675)
@@ -833,11 +874,18 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       double X = Px - Qx, Y = Py - Qy, Z = Pz - Qz;
       double Targ = alpha * (X * X + Y * Y + Z * Z);
       boys_ref<double>(Targ, L, F + kl, ld);
-      rints<T, double>(R + kl, F + kl, ld, L, alpha, (T)X, (T)Y, (T)Z);
+      if (zaxis)
+        rints_z<T>(R + kl, F + kl, ld, L, alpha, Z, sym);
+      else
+        rints<T, double>(R + kl, F + kl, ld, L, alpha, (T)X, (T)Y, (T)Z);
     }
   }
   __syncthreads();
   for (int col = lane; col < nt; col += 64) {
+    if (zaxis && hcls(col, sym) != 0) {  // exactly zero off class 0
+      sh.cflag[col] = 0;
+      continue;
+    }
     T s = 0;
     const T *row = R + (int64_t)col * ld;
     for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
@@ -852,7 +900,10 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       T pre = (T)(2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
       for (int col = 0; col < nt; ++col) {
         T *x = R + (int64_t)col * ld + kl;
-        *x = sh.cflag[col] ? pre * *x : (T)0;
+        if (sh.cflag[col])
+          *x = pre * *x;
+        else
+          *x = (T)0;
       }
     }
   }
