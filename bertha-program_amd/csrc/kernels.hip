@@ -1016,12 +1016,14 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
 #pragma unroll
         for (int g = 0; g < kGroup; ++g) {
           if (g >= na) break;
+          // masked components are neither stored nor read (second
+          // contraction)
           const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
           T *gp = G + (int64_t)(4 * (g * ntg + tp)) * ld + kl;
-          gp[0] = m[0] ? acc[g][0] : (T)0;
-          gp[ld] = m[1] ? acc[g][1] : (T)0;
-          gp[2 * ld] = m[2] ? acc[g][2] : (T)0;
-          gp[3 * ld] = m[3] ? acc[g][3] : (T)0;
+          if (m[0]) gp[0] = acc[g][0];
+          if (m[1]) gp[ld] = acc[g][1];
+          if (m[2]) gp[2 * ld] = acc[g][2];
+          if (m[3]) gp[3 * ld] = acc[g][3];
         }
       }
     }
@@ -1054,7 +1056,11 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
             T e11r, e11i, e21r, e21i;
             ldcd(a, cd.eoff + (int64_t)tp * maxcd + kl, e11r, e11i, e21r, e21i);
             const T *gp = Gg + (int64_t)(4 * tp) * ld + kl;
-            T g11r = gp[0], g11i = gp[ld], g21r = gp[2 * ld], g21i = gp[3 * ld];
+            // components masked off in the first contraction were stored as
+            // zeros: not read again
+            const uint8_t *gmk = &sh.gm[4 * (g * ntg + tp)];
+            T g11r = gmk[0] ? gp[0] : (T)0, g11i = gmk[1] ? gp[ld] : (T)0;
+            T g21r = gmk[2] ? gp[2 * ld] : (T)0, g21i = gmk[3] ? gp[3 * ld] : (T)0;
             // set 0: E11cd x G11, set 1: E21cd x G11, set 2: E11cd x G21,
             // set 3: E21cd x G21
             ar[0] += e11r * g11r - e11i * g11i; ai[0] += e11r * g11i + e11i * g11r;
