@@ -116,7 +116,8 @@ struct bdf_engine {
   Etabs T;
   int isym = 0, intsym = 1, racah1 = 0;
   std::vector<uint8_t> eflags;
-  std::vector<uint8_t> zflags;  // DHFB: exact-nonzero flags of the E tables
+  std::vector<uint8_t> zflags;  // DHFB or axial: exact-nonzero flags of the E tables
+  DevBuf<uint8_t> ecomp;        // axial: the nonzero component of each (E pair, t)
   DevBuf<double> exps, normL, normS;
   DevBuf<EPair> pairs;
   DevBuf<ETerm> terms;
@@ -296,7 +297,8 @@ static int breit_waves(const bdf_engine *e) {
 static int make_plan(bdf_engine *e, int ilev) {
   DevPlan &dp = e->plans[ilev - 1];
   if (dp.built) return BDF_OK;
-  std::string err = build_plan(e->B, e->T, e->eflags, e->isym, e->intsym, e->racah1, ilev, dp.P);
+  std::string err = build_plan(e->B, e->T, e->eflags, e->isym, e->intsym, e->racah1, ilev, dp.P,
+                               e->zflags.empty() ? nullptr : &e->zflags);
   if (!err.empty()) return fail(BDF_EINVAL, err);
   HIPCHK(dp.classes.upload(dp.P.classes, e->stream));
   HIPCHK(dp.abs.upload(dp.P.abs, e->stream));
@@ -519,7 +521,7 @@ static int enqueue_fock(bdf_engine *e, DevPlan &dp, const double2 *d_dent, doubl
   if (mixed) HIPCHK(launch_dmax(d_dent, (int64_t)nd * nd, e->dmax.p, e->stream));
   if (e->schwarz && !rr_out)
     HIPCHK(launch_slot_tables(nullptr, d_dent, e->partner.p, nd, e->dslot.p, e->stream));
-  FockArgs a;
+  FockArgs a{};
   a.classes = dp.classes.p;
   a.abs = dp.abs.p;
   a.cds = dp.cds.p;
@@ -553,6 +555,7 @@ static int enqueue_fock(bdf_engine *e, DevPlan &dp, const double2 *d_dent, doubl
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
   a.xacc_n = plan_xacc(dp.P, e->isym);
+  a.ecomp = dp.P.one_comp ? e->ecomp.p : nullptr;
   const FockLaunch fl = dp.fl;
   const int nb = std::min(sv.blocks, std::max(1, a.nitems));
   if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
@@ -853,7 +856,8 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     fa.Ecd32 = e->mixed ? e->Ecd32.p : nullptr;
     DevBuf<uint8_t> zf;
     fa.zflags = nullptr;
-    if (e->breit) {
+    const bool want_z = e->breit || e->isym != 0;
+    if (want_z) {
       HIPCHK(zf.alloc((size_t)e->T.nflag));
       fa.zflags = zf.p;
     }
@@ -861,10 +865,22 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     e->eflags.resize(e->T.nflag);
     HIPCHK(hipMemcpyAsync(e->eflags.data(), e->flags.p, e->T.nflag, hipMemcpyDeviceToHost,
                           e->stream));
-    if (e->breit) {
+    if (want_z) {
       e->zflags.resize(e->T.nflag);
       HIPCHK(hipMemcpyAsync(e->zflags.data(), zf.p, e->T.nflag, hipMemcpyDeviceToHost,
                             e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+    }
+    if (e->isym != 0) {
+      // the one nonzero component of each (pair, t), 4 for none (plans use
+      // it only where no entry has two)
+      std::vector<uint8_t> comp((size_t)e->T.nflag / 4);
+      for (size_t i = 0; i < comp.size(); ++i) {
+        int b = 0;
+        for (int c = 0; c < 4; ++c) b |= (e->zflags[4 * i + c] != 0) << c;
+        comp[i] = b ? (uint8_t)__builtin_ctz(b) : (uint8_t)4;
+      }
+      HIPCHK(e->ecomp.upload(comp, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
     }
     HIPCHK(e->icn.upload(e->B.labicn, e->stream));
@@ -1501,6 +1517,7 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   add(e->Eab.n, sizeof(double4));
   add(e->Ecd32.n, sizeof(float4));
   add(e->flags.n, 1);
+  add(e->ecomp.n, 1);
   add(e->slab.n, sizeof(double));
   add(e->gdsc.n + e->qbnd.n + e->gslot.n + e->dslot.n, sizeof(double));
   add(e->partner.n, sizeof(int));

@@ -82,6 +82,9 @@ struct ABCombo {
   int cd_begin, cd_end;
   int nb;           // primitives of B (ij -> i, j)
   int lab;          // L_AB: ntab = N(lab)
+  // the one component of G in each parity class c of t' (the class's axis
+  // mask), 4 bits at 4 c (4: none); used where the plan's one_comp is set
+  uint32_t gcomp;
 };
 
 // One ERIFAST call (CD combination and IT2) with its digestion flags.
@@ -91,6 +94,7 @@ struct CDCall {
   int fc[2], fd[2]; // Fock offsets of C, D for sign slots
   int iflg;         // bit b set: batch b+1 active
   int8_t pab[2], pcd[2];  // [0] same-slot phase (PAB1), [1] PAB2
+  int ecf;          // this table's (t' = 0) entry of the component table
 };
 
 // One block pair of the Schwarz tables: the self-pair integrals
@@ -366,6 +370,10 @@ struct Plan {
   int max_lfl = 0, max_maxcd = 0, max_ntg = 0;
   double quartets = 0.0;  // primitive quartet count (ij,kl) x calls
   double flops = 0.0;     // algorithmic FP64 work of one build (see plan.cpp)
+  // every E(CD) entry and every parity class of every E(AB) table has at
+  // most one nonzero component: the second contraction reads one value of
+  // each and forms one product per t'
+  int one_comp = 0;
 };
 
 }  // namespace bdf

@@ -65,6 +65,9 @@ struct FockArgs {
   // exchange accumulators in dynamic LDS: 2 (12 nD + 16 nC) doubles for the
   // plan's largest class
   int xacc_n;
+  // the nonzero component (0..3, 4: none) of each (E pair, t) where the
+  // plan's one_comp is set, else null
+  const uint8_t *ecomp;
 };
 
 struct DiagArgs {

@@ -741,6 +741,13 @@ __device__ __forceinline__ void lde(const FockArgs &a, int64_t i, float &x, floa
   float4 e = a.E32[i];
   x = e.x; y = e.y; z = e.z; w = e.w;
 }
+// component c of an E(CD) entry alone (FP64 or FP32 table)
+__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, int c, double &x) {
+  x = reinterpret_cast<const double *>(a.Ecd + i)[c];
+}
+__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, int c, float &x) {
+  x = reinterpret_cast<const float *>(a.Ecd32 + i)[c];
+}
 // the same for the masked, signed E(CD) table of the second contraction
 __device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, double &x, double &y, double &z,
                                      double &w) {
@@ -1042,6 +1049,18 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
         const CDCall cd = a.cds[ci];
         const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
+        // the call's E(CD) component of t' = 64 r + lane, loaded while every
+        // lane is active (the kl loop below leaves the tail lanes out, and
+        // readlane reads a lane's register whatever its state)
+        int cev[3] = {4, 4, 4};
+        static_assert(ntuv(2 * kMaxL + 2) <= 3 * 64, "E(CD) components: three registers");
+        if (a.ecomp) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const int tq = 64 * r + lane;
+            cev[r] = tq < cd.ntcd ? a.ecomp[cd.ecf + tq] : 4;
+          }
+        }
         const double pabcd = pab1 * pcd1;
         for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
           int kl = kl0 + lane;
@@ -1050,6 +1069,44 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
           T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
           T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
+          if (a.ecomp) {
+            // one nonzero component of E(CD) per t' and of G per class of
+            // t': one product per t', into P[ce][cg] = sum E_ce G_cg, from
+            // which the sums below are formed
+            T pp[4][4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+              for (int y = 0; y < 4; ++y) pp[x][y] = 0;
+            for (int tp = 0; tp < cd.ntcd; ++tp) {
+              // readlane straight from the registers written with every lane
+              // active, then a choice among the (uniform) results
+              const int c0 = __builtin_amdgcn_readlane(cev[0], tp & 63);
+              const int c1 = __builtin_amdgcn_readlane(cev[1], tp & 63);
+              const int c2 = __builtin_amdgcn_readlane(cev[2], tp & 63);
+              const int ce = tp < 64 ? c0 : tp < 128 ? c1 : c2;
+              const int cg = (int)((ab.gcomp >> (4 * (hpar(tp) & sym))) & 15);
+              if ((unsigned)ce > 3u || (unsigned)cg > 3u || !sh.gm[4 * (g * ntg + tp) + cg]) continue;
+              T e;
+              ldcd1(a, cd.eoff + (int64_t)tp * maxcd + kl, ce, e);
+              const T pv = e * Gg[(int64_t)(4 * tp + cg) * ld + kl];
+#pragma unroll
+              for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                  if (ce == x && cg == y) pp[x][y] += pv;
+            }
+            // set s = (E11 or E21) x (G11 or G21): components 0, 1 (2, 3) are
+            // the real and imaginary parts of E11 (E21), G11 (G21)
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+              const int x = 2 * (st & 1), y = 2 * (st >> 1);
+              ar[st] = pp[x][y] - pp[x + 1][y + 1];
+              ai[st] = pp[x][y + 1] + pp[x + 1][y];
+              br[st] = pp[x][y] + pp[x + 1][y + 1];
+              bi[st] = pp[x + 1][y] - pp[x][y + 1];
+            }
+          } else
   #pragma unroll 4
           for (int tp = 0; tp < cd.ntcd; ++tp) {
             // masked E(CD) times Z (fold_kernel)

@@ -450,8 +450,49 @@ static void make_items(const std::vector<ClassDesc> &classes, const std::vector<
   for (const auto &r : runs) items.push_back(r.second);
 }
 
+// Hermite bookkeeping of the matrix-core kernel's signature blocks, in
+// CARTIND order: components of index h, index of (t,u,v), parity class.
+namespace {
+struct HermiteOrder {
+  std::vector<int> t, u, v;
+  HermiteOrder() {
+    for (int L = 0; L <= kMaxLfl; ++L)
+      for (int a = 0; a <= L; ++a)
+        for (int b = 0; b <= L - a; ++b) {
+          t.push_back(a);
+          u.push_back(b);
+          v.push_back(L - a - b);
+        }
+  }
+  static int index(int a, int b, int c) {
+    const int L = a + b + c;
+    return L * (L + 1) * (L + 2) / 6 + a * (L + 1) - a * (a - 1) / 2 + b;
+  }
+  int cls(int h, int m) const { return ((t[h] & 1) | ((u[h] & 1) << 1) | ((v[h] & 1) << 2)) & m; }
+  // h < N(L) class by class, ascending inside a class
+  std::vector<int> by_class(int m, int L, std::vector<int> &coff) const {
+    std::vector<int> out;
+    coff.assign(9, 0);
+    for (int c = 0; c < 8; ++c) {
+      coff[c] = (int)out.size();
+      for (int h = 0; h < ntuv(L); ++h)
+        if (cls(h, m) == c) out.push_back(h);
+    }
+    coff[8] = (int)out.size();
+    return out;
+  }
+  // rank of h inside its class
+  int rank(int h, int m) const {
+    int r = 0;
+    for (int x = 0; x < h; ++x) r += cls(x, m) == cls(h, m);
+    return r;
+  }
+};
+}  // namespace
+
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
-                       int isym, int intsym, int racah1, int ilev, Plan &P) {
+                       int isym, int intsym, int racah1, int ilev, Plan &P,
+                       const std::vector<uint8_t> *zflags) {
   P = Plan();
   P.ndim = B.ndim;
   P.nskp = B.nskp;
@@ -529,7 +570,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
                     int mqa = 2 * ma - 1, mqb = 2 * mb - 1;
                     for (int it1 = 4; it1 >= 1; it1 -= 3) {
                       int naddab = it1 == 1 ? 0 : B.nskp;
-                      ABCombo ab;
+                      ABCombo ab{};
                       int pab_pair = T.pair_index.at(Etabs::key(ca, cb, ka, kb, ma, mb, it1 == 4));
                       ab.eoff = T.pairs[pab_pair].off;
                       ab.ntab = ntuv(la + lb + (it1 == 4 ? 2 : 0));
@@ -598,7 +639,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
                                 if (ilev == 1 && itt > 1 && mcnt != 1) continue;
                                 if (ilev == 2 && itt > 3 && mcnt != 1) continue;
                                 int naddcd = it2 == 1 ? 0 : B.nskp;
-                                CDCall call;
+                                CDCall call{};
                                 int pcd_pair = T.pair_index.at(Etabs::key(cc, cd, kc, kd, mc, md, it2 == 4));
                                 call.eoff = T.pairs[pcd_pair].off;
                                 call.ntcd = ntuv(lc + ld + (it2 == 4 ? 2 : 0));
@@ -690,54 +731,50 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
     }
   }
   make_items(P.classes, abcost, rcost, P.items);
+  // one nonzero component per E(CD) entry and per parity class of each E(AB)
+  // table (axial systems, from the exact-zero flags): the component of G in
+  // t''s class is E(AB)'s in the same class, since R(t + t') vanishes off
+  // class 0
+  if (zflags && isym != 0) {
+    static const HermiteOrder H;
+    std::map<int64_t, int> pair_of;
+    for (size_t p = 0; p < T.pairs.size(); ++p) pair_of[T.pairs[p].off] = (int)p;
+    auto bits_at = [&](int p, int t) {
+      int b = 0;
+      for (int c = 0; c < 4; ++c) b |= ((*zflags)[T.flag_off[p] + 4 * t + c] != 0) << c;
+      return b;
+    };
+    P.one_comp = 1;
+    for (const ClassDesc &cl : P.classes)
+      for (int a = cl.ab_begin; a < cl.ab_end && P.one_comp; ++a) {
+        ABCombo &ab = P.abs[a];
+        const int pa = pair_of.at(ab.eoff);
+        int cb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < ab.ntab; ++t) cb[H.cls(t, cl.sym)] |= bits_at(pa, t);
+        ab.gcomp = 0;
+        for (int c = 0; c < 8; ++c) {
+          if (__builtin_popcount(cb[c]) > 1) P.one_comp = 0;
+          ab.gcomp |= (uint32_t)(cb[c] ? __builtin_ctz(cb[c]) : 4) << (4 * c);
+        }
+        for (int k = ab.cd_begin; k < ab.cd_end; ++k) {
+          CDCall &cd = P.cds[k];
+          const int pc = pair_of.at(cd.eoff);
+          cd.ecf = (int)(T.flag_off[pc] / 4);
+          for (int t = 0; t < cd.ntcd; ++t)
+            if (__builtin_popcount(bits_at(pc, t)) > 1) P.one_comp = 0;
+        }
+      }
+  }
   return check_offsets(P);
 }
 
 // Tiles of the matrix-core Breit kernel (real path): per class, the first
-// combination alone (it reads this pair's own R-column flags), then the
-// others sorted by the set of CD blocks they call and packed four to a tile
-// while the tile's calls stay within kTcMaxU, so that combinations sharing
-// their calls share one E(CD) load.  Work items are runs of a class's tiles
-// for one primitive pair ij, costed like the VALU kernel's items.
-// Hermite bookkeeping of the matrix-core kernel's signature blocks, in
-// CARTIND order: components of index h, index of (t,u,v), parity class.
-namespace {
-struct HermiteOrder {
-  std::vector<int> t, u, v;
-  HermiteOrder() {
-    for (int L = 0; L <= kMaxLfl; ++L)
-      for (int a = 0; a <= L; ++a)
-        for (int b = 0; b <= L - a; ++b) {
-          t.push_back(a);
-          u.push_back(b);
-          v.push_back(L - a - b);
-        }
-  }
-  static int index(int a, int b, int c) {
-    const int L = a + b + c;
-    return L * (L + 1) * (L + 2) / 6 + a * (L + 1) - a * (a - 1) / 2 + b;
-  }
-  int cls(int h, int m) const { return ((t[h] & 1) | ((u[h] & 1) << 1) | ((v[h] & 1) << 2)) & m; }
-  // h < N(L) class by class, ascending inside a class
-  std::vector<int> by_class(int m, int L, std::vector<int> &coff) const {
-    std::vector<int> out;
-    coff.assign(9, 0);
-    for (int c = 0; c < 8; ++c) {
-      coff[c] = (int)out.size();
-      for (int h = 0; h < ntuv(L); ++h)
-        if (cls(h, m) == c) out.push_back(h);
-    }
-    coff[8] = (int)out.size();
-    return out;
-  }
-  // rank of h inside its class
-  int rank(int h, int m) const {
-    int r = 0;
-    for (int x = 0; x < h; ++x) r += cls(x, m) == cls(h, m);
-    return r;
-  }
-};
-}  // namespace
+// combination alone (it reads this pair's own R-column flags, and also joins
+// a compatible stale tile as an extra row), then the others sorted by their
+// G components and the CD blocks they call and packed up to kTcRows to a
+// tile while the tile's calls stay within kTcMaxU, so that combinations
+// sharing their calls share one E(CD) load.  Work items are runs of a class's
+// tiles for one primitive pair ij, costed like the VALU kernel's items.
 
 // One signature block of the matrix-core Breit kernel for (axis mask m,
 // L_AB, L_CD): the ket's Hermite indices t' in groups of up to kTcR of one

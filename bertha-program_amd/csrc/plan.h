@@ -49,8 +49,11 @@ std::string build_basis(const bdf_system &sys, Basis &B);
 // ClassDesc::sym of a class from its centres' coordinates
 int axis_symmetry(const ClassDesc &cl);
 void build_etables(const Basis &B, Etabs &T, bool with_ls);
+// zflags (exact-nonzero flags of the E tables, or null) let an axial plan
+// take the one-component second contraction (Plan::one_comp)
 std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
-                       int isym, int intsym, int racah1, int ilev, Plan &P);
+                       int isym, int intsym, int racah1, int ilev, Plan &P,
+                       const std::vector<uint8_t> *zflags = nullptr);
 // zflags (exact-nonzero flags of the E tables, or null) enable the tiles of
 // the matrix-core kernel on the real path
 std::string build_breit_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t> &eflags,
