@@ -1049,16 +1049,22 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
         const CDCall cd = a.cds[ci];
         const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
-        // the call's E(CD) component of t' = 64 r + lane, loaded while every
-        // lane is active (the kl loop below leaves the tail lanes out, and
-        // readlane reads a lane's register whatever its state)
-        int cev[3] = {4, 4, 4};
+        // for t' = 64 r + lane: the call's E(CD) component ce and G's cg,
+        // ce | cg << 2, or 255 where E(CD) or G vanishes or G is masked;
+        // formed while every lane is active (the kl loop below leaves the
+        // tail lanes out, and readlane reads a lane's register whatever its
+        // state)
+        int cev[3] = {255, 255, 255};
         static_assert(ntuv(2 * kMaxL + 2) <= 3 * 64, "E(CD) components: three registers");
         if (a.ecomp) {
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             const int tq = 64 * r + lane;
-            cev[r] = tq < cd.ntcd ? a.ecomp[cd.ecf + tq] : 4;
+            if (tq < cd.ntcd) {
+              const int ce = a.ecomp[cd.ecf + tq];
+              const int cg = (int)((ab.gcomp >> (4 * (hpar(tq) & sym))) & 15);
+              if (ce <= 3 && cg <= 3 && sh.gm[4 * (g * ntg + tq) + cg]) cev[r] = ce | cg << 2;
+            }
           }
         }
         const double pabcd = pab1 * pcd1;
@@ -1084,9 +1090,9 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
               const int c0 = __builtin_amdgcn_readlane(cev[0], tp & 63);
               const int c1 = __builtin_amdgcn_readlane(cev[1], tp & 63);
               const int c2 = __builtin_amdgcn_readlane(cev[2], tp & 63);
-              const int ce = tp < 64 ? c0 : tp < 128 ? c1 : c2;
-              const int cg = (int)((ab.gcomp >> (4 * (hpar(tp) & sym))) & 15);
-              if ((unsigned)ce > 3u || (unsigned)cg > 3u || !sh.gm[4 * (g * ntg + tp) + cg]) continue;
+              const int cc = tp < 64 ? c0 : tp < 128 ? c1 : c2;
+              if (cc > 15) continue;
+              const int ce = cc & 3, cg = cc >> 2;
               T e;
               ldcd1(a, cd.eoff + (int64_t)tp * maxcd + kl, ce, e);
               const T pv = e * Gg[(int64_t)(4 * tp + cg) * ld + kl];
