@@ -1071,6 +1071,8 @@ static void build_breit_tiles(BreitPlan &P, int gaunt, const Etabs &T,
       tc_signature(H, cl.sym, lab, lcd, gaunt, blk, ng);
       for (int w = 0; w < kTcWaves; ++w)
         if (blk[2 + w] - blk[1 + w] > 63) return;  // a wave's steps beyond one lane each
+      // a step's code, first t' and size fit the kernel's packed step word
+      if (3 * ng > 256 || ntuv(lcd) > 512) return;
       sigs[key] = {(int)P.tc_sig.size(), ng, blk[blk[2 + kTcWaves] + 24]};
       P.tc_sig.insert(P.tc_sig.end(), blk.begin(), blk.end());
     }
