@@ -1024,13 +1024,20 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         for (int g = 0; g < kGroup; ++g) {
           if (g >= na) break;
           // masked components are neither stored nor read (second
-          // contraction)
+          // contraction); on a one-component plan only G's component in
+          // t''s class is (the others are exact zeros)
           const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
           T *gp = G + (int64_t)(4 * (g * ntg + tp)) * ld + kl;
-          if (m[0]) gp[0] = acc[g][0];
-          if (m[1]) gp[ld] = acc[g][1];
-          if (m[2]) gp[2 * ld] = acc[g][2];
-          if (m[3]) gp[3 * ld] = acc[g][3];
+          if (a.ecomp) {
+            const int cg = (int)((a.abs[a0 + g].gcomp >> (4 * (hpar(tp) & sym))) & 15);
+            if (cg <= 3 && m[cg])
+              gp[cg * ld] = cg == 0 ? acc[g][0] : cg == 1 ? acc[g][1] : cg == 2 ? acc[g][2] : acc[g][3];
+          } else {
+            if (m[0]) gp[0] = acc[g][0];
+            if (m[1]) gp[ld] = acc[g][1];
+            if (m[2]) gp[2 * ld] = acc[g][2];
+            if (m[3]) gp[3 * ld] = acc[g][3];
+          }
         }
       }
     }
