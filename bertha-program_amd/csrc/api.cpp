@@ -117,7 +117,6 @@ struct bdf_engine {
   int isym = 0, intsym = 1, racah1 = 0;
   std::vector<uint8_t> eflags;
   std::vector<uint8_t> zflags;  // DHFB or axial: exact-nonzero flags of the E tables
-  DevBuf<uint8_t> ecomp;        // axial: the nonzero component of each (E pair, t)
   DevBuf<double> exps, normL, normS;
   DevBuf<EPair> pairs;
   DevBuf<ETerm> terms;
@@ -555,7 +554,7 @@ static int enqueue_fock(bdf_engine *e, DevPlan &dp, const double2 *d_dent, doubl
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
   a.xacc_n = plan_xacc(dp.P, e->isym);
-  a.ecomp = dp.P.one_comp ? e->ecomp.p : nullptr;
+  a.one_comp = dp.P.one_comp;
   const FockLaunch fl = dp.fl;
   const int nb = std::min(sv.blocks, std::max(1, a.nitems));
   if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
@@ -869,18 +868,6 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
       e->zflags.resize(e->T.nflag);
       HIPCHK(hipMemcpyAsync(e->zflags.data(), zf.p, e->T.nflag, hipMemcpyDeviceToHost,
                             e->stream));
-      HIPCHK(hipStreamSynchronize(e->stream));
-    }
-    if (e->isym != 0) {
-      // the one nonzero component of each (pair, t), 4 for none (plans use
-      // it only where no entry has two)
-      std::vector<uint8_t> comp((size_t)e->T.nflag / 4);
-      for (size_t i = 0; i < comp.size(); ++i) {
-        int b = 0;
-        for (int c = 0; c < 4; ++c) b |= (e->zflags[4 * i + c] != 0) << c;
-        comp[i] = b ? (uint8_t)__builtin_ctz(b) : (uint8_t)4;
-      }
-      HIPCHK(e->ecomp.upload(comp, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
     }
     HIPCHK(e->icn.upload(e->B.labicn, e->stream));
@@ -1517,7 +1504,6 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   add(e->Eab.n, sizeof(double4));
   add(e->Ecd32.n, sizeof(float4));
   add(e->flags.n, 1);
-  add(e->ecomp.n, 1);
   add(e->slab.n, sizeof(double));
   add(e->gdsc.n + e->qbnd.n + e->gslot.n + e->dslot.n, sizeof(double));
   add(e->partner.n, sizeof(int));

@@ -94,7 +94,9 @@ struct CDCall {
   int fc[2], fd[2]; // Fock offsets of C, D for sign slots
   int iflg;         // bit b set: batch b+1 active
   int8_t pab[2], pcd[2];  // [0] same-slot phase (PAB1), [1] PAB2
-  int ecf;          // this table's (t' = 0) entry of the component table
+  // where the plan's one_comp is set: E(CD)'s one nonzero component in each
+  // parity class of t' (4 bits at 4 c, 4: none)
+  uint32_t ccomp;
 };
 
 // One block pair of the Schwarz tables: the self-pair integrals
@@ -370,9 +372,9 @@ struct Plan {
   int max_lfl = 0, max_maxcd = 0, max_ntg = 0;
   double quartets = 0.0;  // primitive quartet count (ij,kl) x calls
   double flops = 0.0;     // algorithmic FP64 work of one build (see plan.cpp)
-  // every E(CD) entry and every parity class of every E(AB) table has at
-  // most one nonzero component: the second contraction reads one value of
-  // each and forms one product per t'
+  // every parity class of every E(AB) and E(CD) table has at most one
+  // nonzero component: the second contraction reads one value of each and
+  // sums one product per t' class by class
   int one_comp = 0;
 };
 

@@ -65,9 +65,10 @@ struct FockArgs {
   // exchange accumulators in dynamic LDS: 2 (12 nD + 16 nC) doubles for the
   // plan's largest class
   int xacc_n;
-  // the nonzero component (0..3, 4: none) of each (E pair, t) where the
-  // plan's one_comp is set, else null
-  const uint8_t *ecomp;
+  // the plan's one_comp: one nonzero component per parity class of t in
+  // every E table, G's in each class of t' (ABCombo::gcomp) and E(CD)'s
+  // (CDCall::ccomp)
+  int one_comp;
 };
 
 struct DiagArgs {

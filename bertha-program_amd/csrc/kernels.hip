@@ -386,6 +386,11 @@ __device__ void boys_ref(T t, int L, T *f, int ld) {
 __constant__ int16_t c_crank[8][ntuv(kMaxLfl)];
 __constant__ int16_t c_csz[8][kMaxLfl + 1][8];
 __constant__ int16_t c_cls0[8][ntuv(kMaxLfl)];
+// the members of each parity class, ascending, of h < N(kMaxLab) (the ket
+// Hermite indices t' of a pair table): c_cmem[m][c][r], r < c_csz[m][L][c]
+// for those below N(L); 32-bit so that uniform reads go through the scalar
+// cache
+__constant__ int c_cmem[8][8][ntuv(kMaxLab)];
 // R on the z axis (X = Y = 0): R_tuv = (t-1)!! (u-1)!! sum_j c_hz[v][j]
 // Z^(v-2j) A_((t+u)/2+v-j), A_k = (-2 alpha)^k F_k, for even t and u (else
 // zero): c_hz[v][j] = v! / (j! (v-2j)! 2^j), c_dfac[t] = (t-1)!!
@@ -1028,7 +1033,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           // t''s class is (the others are exact zeros)
           const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
           T *gp = G + (int64_t)(4 * (g * ntg + tp)) * ld + kl;
-          if (a.ecomp) {
+          if (a.one_comp) {
             const int cg = (int)((a.abs[a0 + g].gcomp >> (4 * (hpar(tp) & sym))) & 15);
             if (cg <= 3 && m[cg])
               gp[cg * ld] = cg == 0 ? acc[g][0] : cg == 1 ? acc[g][1] : cg == 2 ? acc[g][2] : acc[g][3];
@@ -1056,21 +1061,27 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       for (int ci = ab.cd_begin; ci < ab.cd_end; ++ci) {
         const CDCall cd = a.cds[ci];
         const double pab1 = cd.pab[0], pcd1 = cd.pcd[0];
-        // for t' = 64 r + lane: the call's E(CD) component ce and G's cg,
-        // ce | cg << 2, or 255 where E(CD) or G vanishes or G is masked;
-        // formed while every lane is active (the kl loop below leaves the
-        // tail lanes out, and readlane reads a lane's register whatever its
-        // state)
-        int cev[3] = {255, 255, 255};
-        static_assert(ntuv(2 * kMaxL + 2) <= 3 * 64, "E(CD) components: three registers");
-        if (a.ecomp) {
+        // L_CD of the call (ntcd = N(L_CD)): the class sizes of t' < ntcd
+        int lcd = 0;
+        while (ntuv(lcd) < cd.ntcd) ++lcd;
+        // one-component plans: for parity class c of t' (lane c), bit r set
+        // where the class's r-th t' has G's component stored (not masked),
+        // the terms of the second contraction (a plan's classes hold at most
+        // 64 t'); formed while every lane is active (the kl loop below leaves
+        // the tail lanes out, and readlane reads a lane's register whatever
+        // its state)
+        uint32_t okl = 0, okh = 0;
+        if (a.one_comp) {
 #pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            const int tq = 64 * r + lane;
-            if (tq < cd.ntcd) {
-              const int ce = a.ecomp[cd.ecf + tq];
-              const int cg = (int)((ab.gcomp >> (4 * (hpar(tq) & sym))) & 15);
-              if (ce <= 3 && cg <= 3 && sh.gm[4 * (g * ntg + tq) + cg]) cev[r] = ce | cg << 2;
+          for (int c = 0; c < 8; ++c) {
+            const int n = c_csz[sym][lcd][c];
+            const int cg = (int)((ab.gcomp >> (4 * c)) & 15);
+            bool v = false;
+            if (cg <= 3 && lane < n) v = sh.gm[4 * (g * ntg + c_cmem[sym][c][lane]) + cg] != 0;
+            const uint64_t m = __ballot(v);
+            if (lane == c) {
+              okl = (uint32_t)m;
+              okh = (uint32_t)(m >> 32);
             }
           }
         }
@@ -1082,32 +1093,47 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
           // b_s = sum Z E_CD conj(G) for the four (E_CD, G) pairings
           T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
           T br[4] = {0, 0, 0, 0}, bi[4] = {0, 0, 0, 0};
-          if (a.ecomp) {
-            // one nonzero component of E(CD) per t' and of G per class of
-            // t': one product per t', into P[ce][cg] = sum E_ce G_cg, from
-            // which the sums below are formed
+          if (a.one_comp) {
+            // one nonzero component of E(CD) and of G per parity class of
+            // t' (ce, cg): one product per t', summed class by class into
+            // P[ce][cg] = sum E_ce G_cg, from which the sums below are
+            // formed; four t' at a time, their loads issued together
             T pp[4][4];
 #pragma unroll
             for (int x = 0; x < 4; ++x)
 #pragma unroll
               for (int y = 0; y < 4; ++y) pp[x][y] = 0;
-            for (int tp = 0; tp < cd.ntcd; ++tp) {
-              // readlane straight from the registers written with every lane
-              // active, then a choice among the (uniform) results
-              const int c0 = __builtin_amdgcn_readlane(cev[0], tp & 63);
-              const int c1 = __builtin_amdgcn_readlane(cev[1], tp & 63);
-              const int c2 = __builtin_amdgcn_readlane(cev[2], tp & 63);
-              const int cc = tp < 64 ? c0 : tp < 128 ? c1 : c2;
-              if (cc > 15) continue;
-              const int ce = cc & 3, cg = cc >> 2;
-              T e;
-              ldcd1(a, cd.eoff + (int64_t)tp * maxcd + kl, ce, e);
-              const T pv = e * Gg[(int64_t)(4 * tp + cg) * ld + kl];
+            for (int c = 0; c < 8; ++c) {
+              const int ce = (int)((cd.ccomp >> (4 * c)) & 15);
+              const int cg = (int)((ab.gcomp >> (4 * c)) & 15);
+              // readlane returns an int: both words through uint32_t, or the
+              // low word's bit 31 would be sign-extended over the high word
+              uint64_t m = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(okh, c) << 32 |
+                           (uint32_t)__builtin_amdgcn_readlane(okl, c);
+              if (ce > 3 || cg > 3 || m == 0) continue;
+              const int *mem = c_cmem[sym][c];
+              T sum = 0;
+              while (m) {
+                T ev[4], gv[4];
+                bool ok[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  // the next term's t', or (none left) the class's first
+                  // again, loaded but not summed
+                  ok[i] = m != 0;
+                  const int tq = mem[ok[i] ? __builtin_ctzll(m) : 0];
+                  m &= m - 1;
+                  ldcd1(a, cd.eoff + (int64_t)tq * maxcd + kl, ce, ev[i]);
+                  gv[i] = Gg[(int64_t)(4 * tq + cg) * ld + kl];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sum += ok[i] ? ev[i] * gv[i] : (T)0;
+              }
 #pragma unroll
               for (int x = 0; x < 4; ++x)
 #pragma unroll
                 for (int y = 0; y < 4; ++y)
-                  if (ce == x && cg == y) pp[x][y] += pv;
+                  if (ce == x && cg == y) pp[x][y] += sum;
             }
             // set s = (E11 or E21) x (G11 or G21): components 0, 1 (2, 3) are
             // the real and imaginary parts of E11 (E21), G11 (G21)
@@ -1901,7 +1927,17 @@ hipError_t upload_constants() {
         for (int c = 0; c < 8; ++c) csz[m][L][c] = (int16_t)sz[c];
       }
     }
+    static int cmem[8][8][ntuv(kMaxLab)];
+    for (int m = 0; m < 8; ++m) {
+      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int h = 0; h < ntuv(kMaxLab); ++h) {
+        const uint32_t x = tuv[h];
+        const int c = (int)(((x >> 10) & 1) | (((x >> 5) & 1) << 1) | ((x & 1) << 2)) & m;
+        cmem[m][c][cnt[c]++] = h;
+      }
+    }
     e = hipMemcpyToSymbol(HIP_SYMBOL(c_crank), crank, sizeof(crank));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_cmem), cmem, sizeof(cmem));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_cls0), cls0, sizeof(cls0));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_csz), csz, sizeof(csz));
     if (e != hipSuccess) return e;

@@ -731,7 +731,7 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
     }
   }
   make_items(P.classes, abcost, rcost, P.items);
-  // one nonzero component per E(CD) entry and per parity class of each E(AB)
+  // one nonzero component per parity class of t in each E(AB) and E(CD)
   // table (axial systems, from the exact-zero flags): the component of G in
   // t''s class is E(AB)'s in the same class, since R(t + t') vanishes off
   // class 0
@@ -759,9 +759,18 @@ std::string build_plan(const Basis &B, const Etabs &T, const std::vector<uint8_t
         for (int k = ab.cd_begin; k < ab.cd_end; ++k) {
           CDCall &cd = P.cds[k];
           const int pc = pair_of.at(cd.eoff);
-          cd.ecf = (int)(T.flag_off[pc] / 4);
-          for (int t = 0; t < cd.ntcd; ++t)
-            if (__builtin_popcount(bits_at(pc, t)) > 1) P.one_comp = 0;
+          int cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          for (int t = 0; t < cd.ntcd; ++t) cc[H.cls(t, cl.sym)] |= bits_at(pc, t);
+          // the kernel walks each class of t' through one 64-bit mask
+          int lcd = 0, sz[8];
+          while (ntuv(lcd) < cd.ntcd) ++lcd;
+          hermite_class_sizes(cl.sym, lcd, sz);
+          if (*std::max_element(sz, sz + 8) > 64) P.one_comp = 0;
+          cd.ccomp = 0;
+          for (int c = 0; c < 8; ++c) {
+            if (__builtin_popcount(cc[c]) > 1) P.one_comp = 0;
+            cd.ccomp |= (uint32_t)(cc[c] ? __builtin_ctz(cc[c]) : 4) << (4 * c);
+          }
         }
       }
   }
