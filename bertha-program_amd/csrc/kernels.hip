@@ -972,6 +972,57 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
             any |= m[0] | m[1] | m[2] | m[3];
           }
         }
+        if (any && a.one_comp) {
+          // one-component plan: G(t') lives in one component cg (E(AB)'s in
+          // t''s class, since R(t + t') vanishes off class 0), so only that
+          // component is summed and stored, four R rows in flight
+          const int pc = hpar(tp) & sym;
+          const int tb = c_coff[sym][ab0.lab][pc];
+          const int nc = c_coff[sym][ab0.lab][pc + 1] - tb;
+          const int *hs = &g_hsp[sym][tp][0];
+          const T *eb = s_eab + 4 * tb;
+          int cgv[kGroup];
+          T s1[kGroup];
+#pragma unroll
+          for (int g = 0; g < kGroup; ++g) {
+            cgv[g] = g < na ? (int)((a.abs[a0 + g].gcomp >> (4 * pc)) & 15) : 4;
+            s1[g] = 0;
+          }
+          int t = 0;
+          for (; t + 4 <= nc; t += 4) {
+            const T r0 = R[(int64_t)hs[t] * ld + kl];
+            const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
+            const T r2 = R[(int64_t)hs[t + 2] * ld + kl];
+            const T r3 = R[(int64_t)hs[t + 3] * ld + kl];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              const T *e = eb + 4 * (g * ntab + t) + (cgv[g] & 3);
+              s1[g] += e[0] * r0;
+              s1[g] += e[4] * r1;
+              s1[g] += e[8] * r2;
+              s1[g] += e[12] * r3;
+            }
+          }
+          for (; t < nc; ++t) {
+            const T r = R[(int64_t)hs[t] * ld + kl];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              s1[g] += eb[4 * (g * ntab + t) + (cgv[g] & 3)] * r;
+            }
+          }
+          // masked components are neither stored nor read (second
+          // contraction)
+#pragma unroll
+          for (int g = 0; g < kGroup; ++g) {
+            if (g >= na) break;
+            const int cg = cgv[g];
+            if (cg <= 3 && sh.gm[4 * (g * ntg + tp) + cg])
+              G[(int64_t)(4 * (g * ntg + tp) + cg) * ld + kl] = s1[g];
+          }
+          continue;
+        }
         if (any) {
           // R rows from the sum table, independent loads in flight: four with
           // two waves per SIMD, two with three (more would spill); only the t
@@ -1029,20 +1080,14 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         for (int g = 0; g < kGroup; ++g) {
           if (g >= na) break;
           // masked components are neither stored nor read (second
-          // contraction); on a one-component plan only G's component in
-          // t''s class is (the others are exact zeros)
+          // contraction); a one-component plan stored its items above (here
+          // every mask is off)
           const uint8_t *m = &sh.gm[4 * (g * ntg + tp)];
           T *gp = G + (int64_t)(4 * (g * ntg + tp)) * ld + kl;
-          if (a.one_comp) {
-            const int cg = (int)((a.abs[a0 + g].gcomp >> (4 * (hpar(tp) & sym))) & 15);
-            if (cg <= 3 && m[cg])
-              gp[cg * ld] = cg == 0 ? acc[g][0] : cg == 1 ? acc[g][1] : cg == 2 ? acc[g][2] : acc[g][3];
-          } else {
-            if (m[0]) gp[0] = acc[g][0];
-            if (m[1]) gp[ld] = acc[g][1];
-            if (m[2]) gp[2 * ld] = acc[g][2];
-            if (m[3]) gp[3 * ld] = acc[g][3];
-          }
+          if (m[0]) gp[0] = acc[g][0];
+          if (m[1]) gp[ld] = acc[g][1];
+          if (m[2]) gp[2 * ld] = acc[g][2];
+          if (m[3]) gp[3 * ld] = acc[g][3];
         }
       }
     }
