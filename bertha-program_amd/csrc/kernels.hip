@@ -893,8 +893,13 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
     }
   }
   __syncthreads();
+  // the column test.  On the z axis only the class-0 columns (the others
+  // vanish exactly and are neither written nor read), four at a time with
+  // lanes over kl (coalesced rows) and a wave sum per column; elsewhere one
+  // column per lane (every column is tested, and the wave sums would cost
+  // more than the strided reads)
   for (int col = lane; col < nt; col += 64) {
-    if (zaxis && hcls(col, sym) != 0) {  // exactly zero off class 0
+    if (zaxis) {
       sh.cflag[col] = 0;
       continue;
     }
@@ -903,6 +908,25 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
     for (int kl = 0; kl < maxcd; ++kl) s += fabs(row[kl]);
     sh.cflag[col] = s > (T)1.0e-10;
   }
+  const int ncol = zaxis ? c_csz[sym][L][0] : nt;
+  for (int c0 = 0; zaxis && c0 < ncol; c0 += 4) {
+    int cols[4];
+    T s[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = min(c0 + i, ncol - 1);
+      cols[i] = zaxis ? c_cls0[sym][r] : r;
+      s[i] = 0;
+    }
+    for (int kl = lane; kl < maxcd; kl += 64)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s[i] += fabs(R[(int64_t)cols[i] * ld + kl]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double tot = wave_sum((double)s[i]);
+      if (lane == 0 && c0 + i < ncol) sh.cflag[cols[i]] = tot > 1.0e-10;
+    }
+  }
   __syncthreads();
   for (int kl0 = 0; kl0 < maxcd; kl0 += 64) {
     int kl = kl0 + lane;
@@ -910,7 +934,8 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
       int kb = kl % nC, lb = kl / nC;
       double q = a.exps[cl.exp[2] + kb] + a.exps[cl.exp[3] + lb];
       T pre = (T)(2.0 * 1.7493418327624863e1 / (sqrt(p + q) * p * q));
-      for (int col = 0; col < nt; ++col) {
+      for (int r = 0; r < ncol; ++r) {
+        const int col = zaxis ? c_cls0[sym][r] : r;
         T *x = R + (int64_t)col * ld + kl;
         if (sh.cflag[col])
           *x = pre * *x;
