@@ -154,6 +154,8 @@ struct bdf_engine {
   DevBuf<double4> Ecd;  // masked, signed E(CD) of the second contraction
   DevBuf<double4> Eab;  // E with EPSAB applied: the Breit kernel's AB side
   DevBuf<float4> Ecd32;
+  DevBuf<double> Ecd1;  // axial: each Ecd entry's one nonzero component
+  DevBuf<float> Ecd1_32;
   DevBuf<unsigned long long> dmax;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   hipStream_t stream32 = nullptr;  // FP32 instance of the mixed build
@@ -554,7 +556,9 @@ static int enqueue_fock(bdf_engine *e, DevPlan &dp, const double2 *d_dent, doubl
   a.Ecd = e->Ecd.p;
   a.Ecd32 = e->Ecd32.p;
   a.xacc_n = plan_xacc(dp.P, e->isym);
-  a.one_comp = dp.P.one_comp;
+  a.one_comp = dp.P.one_comp && e->Ecd1.p && (!mixed || e->Ecd1_32.p);
+  a.Ecd1 = e->Ecd1.p;
+  a.Ecd1_32 = e->Ecd1_32.p;
   const FockLaunch fl = dp.fl;
   const int nb = std::min(sv.blocks, std::max(1, a.nitems));
   if (timing) HIPCHK(hipEventRecord(e->ev0, e->stream));
@@ -853,6 +857,16 @@ int bdf_create(const bdf_system *sys, const bdf_options *opt, bdf_engine **out, 
     if (e->mixed) HIPCHK(e->Ecd32.alloc((size_t)e->T.total));
     fa.Ecd = e->Ecd.p;
     fa.Ecd32 = e->mixed ? e->Ecd32.p : nullptr;
+    fa.Ecd1 = nullptr;
+    fa.Ecd1_32 = nullptr;
+    if (e->isym != 0) {
+      HIPCHK(e->Ecd1.alloc((size_t)e->T.total));
+      fa.Ecd1 = e->Ecd1.p;
+      if (e->mixed) {
+        HIPCHK(e->Ecd1_32.alloc((size_t)e->T.total));
+        fa.Ecd1_32 = e->Ecd1_32.p;
+      }
+    }
     DevBuf<uint8_t> zf;
     fa.zflags = nullptr;
     const bool want_z = e->breit || e->isym != 0;
@@ -1503,6 +1517,8 @@ int bdf_memory_info(bdf_engine *e, size_t *device_bytes) {
   add(e->Ecd.n, sizeof(double4));
   add(e->Eab.n, sizeof(double4));
   add(e->Ecd32.n, sizeof(float4));
+  add(e->Ecd1.n, sizeof(double));
+  add(e->Ecd1_32.n, sizeof(float));
   add(e->flags.n, 1);
   add(e->slab.n, sizeof(double));
   add(e->gdsc.n + e->qbnd.n + e->gslot.n + e->dslot.n, sizeof(double));

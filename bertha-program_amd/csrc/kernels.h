@@ -25,6 +25,10 @@ struct EflagArgs {
   uint8_t *flags;
   double4 *Ecd;   // if set: masked, signed E(CD) copy (and its FP32 copy Ecd32)
   float4 *Ecd32;
+  // if set (axial engines): each Ecd entry's one nonzero component, as the
+  // sum of its four (exact where at most one is nonzero), FP64 and FP32
+  double *Ecd1;
+  float *Ecd1_32;
   uint8_t *zflags;  // if set: per (pair, tuv, component) 1 where any entry is nonzero
 };
 
@@ -67,8 +71,11 @@ struct FockArgs {
   int xacc_n;
   // the plan's one_comp: one nonzero component per parity class of t in
   // every E table, G's in each class of t' (ABCombo::gcomp) and E(CD)'s
-  // (CDCall::ccomp)
+  // (CDCall::ccomp); the second contraction then reads E(CD) from the
+  // one-value tables (FP64, FP32)
   int one_comp;
+  const double *Ecd1;
+  const float *Ecd1_32;
 };
 
 struct DiagArgs {

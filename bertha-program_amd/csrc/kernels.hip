@@ -236,6 +236,8 @@ __global__ void __launch_bounds__(256) fold_kernel(EflagArgs a) {
     const int64_t dst = P.off + (int64_t)n * maxab + j * P.nA + i;
     a.Ecd[dst] = o;
     if (a.Ecd32) a.Ecd32[dst] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    if (a.Ecd1) a.Ecd1[dst] = o.x + o.y + o.z + o.w;
+    if (a.Ecd1_32) a.Ecd1_32[dst] = (float)(o.x + o.y + o.z + o.w);
   }
 }
 
@@ -746,13 +748,11 @@ __device__ __forceinline__ void lde(const FockArgs &a, int64_t i, float &x, floa
   float4 e = a.E32[i];
   x = e.x; y = e.y; z = e.z; w = e.w;
 }
-// component c of an E(CD) entry alone (FP64 or FP32 table)
-__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, int c, double &x) {
-  x = reinterpret_cast<const double *>(a.Ecd + i)[c];
-}
-__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, int c, float &x) {
-  x = reinterpret_cast<const float *>(a.Ecd32 + i)[c];
-}
+// the one nonzero component of an E(CD) entry (one-component plans; FP64 or
+// FP32 table of one value per entry, so that consecutive lanes read
+// consecutive values)
+__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, double &x) { x = a.Ecd1[i]; }
+__device__ __forceinline__ void ldcd1(const FockArgs &a, int64_t i, float &x) { x = a.Ecd1_32[i]; }
 // the same for the masked, signed E(CD) table of the second contraction
 __device__ __forceinline__ void ldcd(const FockArgs &a, int64_t i, double &x, double &y, double &z,
                                      double &w) {
@@ -1193,7 +1193,7 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
                   ok[i] = m != 0;
                   const int tq = mem[ok[i] ? __builtin_ctzll(m) : 0];
                   m &= m - 1;
-                  ldcd1(a, cd.eoff + (int64_t)tq * maxcd + kl, ce, ev[i]);
+                  ldcd1(a, cd.eoff + (int64_t)tq * maxcd + kl, ev[i]);
                   gv[i] = Gg[(int64_t)(4 * tq + cg) * ld + kl];
                 }
 #pragma unroll
