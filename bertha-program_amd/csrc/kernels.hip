@@ -1000,7 +1000,8 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
         if (any && a.one_comp) {
           // one-component plan: G(t') lives in one component cg (E(AB)'s in
           // t''s class, since R(t + t') vanishes off class 0), so only that
-          // component is summed and stored, four R rows in flight
+          // component is summed and stored, eight R rows in flight (then
+          // four), the sum still taken in order of t
           const int pc = hpar(tp) & sym;
           const int tb = c_coff[sym][ab0.lab][pc];
           const int nc = c_coff[sym][ab0.lab][pc + 1] - tb;
@@ -1014,6 +1015,18 @@ __device__ void fock_item(const FockArgs &a, const WorkItem &it, double *slab, F
             s1[g] = 0;
           }
           int t = 0;
+          for (; t + 8 <= nc; t += 8) {
+            T r[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) r[i] = R[(int64_t)hs[t + i] * ld + kl];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+              if (g >= na) break;
+              const T *e = eb + 4 * (g * ntab + t) + (cgv[g] & 3);
+#pragma unroll
+              for (int i = 0; i < 8; ++i) s1[g] += e[4 * i] * r[i];
+            }
+          }
           for (; t + 4 <= nc; t += 4) {
             const T r0 = R[(int64_t)hs[t] * ld + kl];
             const T r1 = R[(int64_t)hs[t + 1] * ld + kl];
